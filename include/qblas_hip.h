@@ -1,0 +1,133 @@
+/*
+ * qblas_hip.h -- C ABI of libqblas_hip.so, the MI355X (gfx950) backend for the QuadBLAS binary128
+ * hot path: dot, gemv and gemm.
+ *
+ * Part 1 keeps the reference's C entry points byte-for-byte (same names, argument order and
+ * meaning, same return types); they take HOST pointers to Sleef_quad / __float128 arrays, copy
+ * them to the GPU, run the kernels and copy results back before returning.
+ * Part 2 carries the C++ template API's full-precision entry points (binary128 alpha/beta and a
+ * binary128 dot result), which include/quadblas/quadblas.hpp wraps as QuadBLAS::dot/gemv/gemm
+ * and Vector/Matrix.
+ * Part 3 is new: the same routines on DEVICE pointers, asynchronous on a caller stream, with an
+ * explicit evaluation mode.  It serves data already resident in HBM and kernel timing.
+ *
+ * Binary128 values cross the ABI as 16-byte little-endian images of __float128 ({lo, hi} 64-bit
+ * words), 16-byte aligned.  No function throws; failures are reported through
+ * quadblas_hip_last_error() (thread-local) and leave outputs unchanged.
+ */
+#ifndef QBLAS_HIP_H
+#define QBLAS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- Part 1: the reference C ABI (include/quadblas/interface/c_interface.hpp) ------------- */
+
+/* c_interface.hpp:21-31.  Returns the binary128 dot product rounded to double (round to nearest
+   even).  n <= 0 returns 0.  Evaluation order follows quadblas_get_num_threads() like the
+   reference's OpenMP chunking (level1.hpp:38-137) unless the mode is changed. */
+double quadblas_qdot(int n, void *x, int incx, void *y, int incy);
+
+/* c_interface.hpp:64-90.  layout 'C'/'c' = column-major, anything else row-major; trans in
+   {T,t,C,c} swaps m and n and flips the layout.  alpha, beta are converted exactly. */
+void quadblas_qgemv(char layout, char trans, int m, int n, double alpha, void *A, int lda, void *x, int incx,
+                    double beta, void *y, int incy);
+
+/* c_interface.hpp:95-117.  transa/transb are accepted and ignored, as in the reference. */
+void quadblas_qgemm(char layout, char transa, char transb, int m, int n, int k, double alpha, void *A, int lda,
+                    void *B, int ldb, double beta, void *C, int ldc);
+
+/* c_interface.hpp:122-131.  The thread count fixes the reference's dot chunking (T); the
+   default is the host's OpenMP default (OMP_NUM_THREADS, else the available cores). */
+void quadblas_set_num_threads(int num_threads);
+int quadblas_get_num_threads(void);
+
+/* c_interface.hpp:134-137 and :140-144 (32-byte alignment test, constants.hpp:18). */
+const char *quadblas_get_version(void);
+int quadblas_is_aligned(const void *ptr);
+
+/* ---- Part 2: full-precision entry points behind the C++ template API --------------------- */
+
+/* QuadBLAS::dot (level1.hpp:80-137; Vector<L>::dot, cpp_classes.hpp:66-69): writes the
+   binary128 result to *result. */
+void quadblas_dot_q(size_t n, const void *x, size_t incx, const void *y, size_t incy, void *result);
+
+/* QuadBLAS::gemv (level2.hpp:85-99; Matrix<L>::gemv, cpp_classes.hpp:143-147).
+   layout: 0 = RowMajor, 1 = ColMajor.  alpha, beta point to binary128 values. */
+void quadblas_gemv_q(int layout, size_t m, size_t n, const void *alpha, const void *A, size_t lda,
+                     const void *x, size_t incx, const void *beta, void *y, size_t incy);
+
+/* QuadBLAS::gemm (level3.hpp:215-336; Matrix<L>::gemm, cpp_classes.hpp:150-154). */
+void quadblas_gemm_q(int layout, size_t m, size_t n, size_t k, const void *alpha, const void *A, size_t lda,
+                     const void *B, size_t ldb, const void *beta, void *C, size_t ldc);
+
+/* ---- Part 3: device-pointer API (new; no reference counterpart) --------------------------- */
+
+/* Evaluation modes.
+   QBLAS_MODE_REFERENCE: the reference's summation order and blocking, every operation correctly
+                         rounded -> bit-identical to the reference (dot additionally needs the
+                         same T).  Default for Parts 1 and 2.
+   QBLAS_MODE_EXACT:     dot/gemv: the exact sum of products rounded once to binary128 (correctly
+                         rounded result, independent of order and T).
+   QBLAS_MODE_DD:        double-double arithmetic on FP64 FMA (inputs rounded to 106 bits); faster,
+                         ~2^-104 relative accuracy on well-conditioned data.
+   QBLAS_MODE_OZAKI:     gemm only: int8 slices on MFMA with exact int32 accumulation. */
+enum {
+  QBLAS_MODE_REFERENCE = 0,
+  QBLAS_MODE_EXACT = 1,
+  QBLAS_MODE_DD = 2,
+  QBLAS_MODE_OZAKI = 3
+};
+
+/* Status codes returned by Part 3 (0 = success). */
+enum {
+  QBLAS_OK = 0,
+  QBLAS_ERR_ARG = 1,      /* invalid argument (negative size/stride, misaligned pointer, ...) */
+  QBLAS_ERR_HIP = 2,      /* a HIP runtime call failed */
+  QBLAS_ERR_MODE = 3      /* mode not available for this routine */
+};
+
+/* Mode used by Parts 1 and 2 (default QBLAS_MODE_REFERENCE; environment QBLAS_MODE overrides at
+   load: "reference", "exact", "dd", "ozaki"). */
+int qblas_set_mode(int mode);
+int qblas_get_mode(void);
+
+/* k-block length of the reference's blocked GEMM (BlockingParams::kc, blocking.hpp:39-41):
+   126 on x86-64 Linux (the default), 256 on Apple silicon. */
+void qblas_set_kc(int kc);
+int qblas_get_kc(void);
+
+/* Last error message of the calling thread ("" if none); cleared by qblas_clear_error(). */
+const char *quadblas_hip_last_error(void);
+void qblas_clear_error(void);
+
+/* dot: *d_result (device, 16 bytes) = x . y.  T: the reference thread count for
+   QBLAS_MODE_REFERENCE (ignored otherwise).  stream: hipStream_t or NULL. */
+int qblas_hip_dot(int mode, int64_t n, const void *d_x, int64_t incx, const void *d_y, int64_t incy, int T,
+                  void *d_result, void *stream);
+
+/* gemv on device pointers; alpha, beta are HOST pointers to binary128 values. */
+int qblas_hip_gemv(int mode, int layout, int64_t m, int64_t n, const void *alpha, const void *d_A, int64_t lda,
+                   const void *d_x, int64_t incx, const void *beta, void *d_y, int64_t incy, void *stream);
+
+/* gemm on device pointers; alpha, beta are HOST pointers to binary128 values. */
+int qblas_hip_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, const void *alpha, const void *d_A,
+                   int64_t lda, const void *d_B, int64_t ldb, const void *beta, void *d_C, int64_t ldc,
+                   void *stream);
+
+/* Seeded synthetic inputs written to device memory (definitions shared with the test oracle):
+   kind 0: a + b*2^-60 summed in binary128, a, b uniform [-1, 1) doubles (full 113-bit mantissa);
+   kind 1: random sign, exponent uniform in [lo_exp, hi_exp), 112 random fraction bits;
+   kind 2: the qdot cancellation variant's x: +-(1e20*u) pairs on slots 0,1 of every 8, kind 0
+           elsewhere; kind 3: the matching y (equal values on each pair). */
+int qblas_hip_fill(int kind, uint64_t seed, int64_t n, int lo_exp, int hi_exp, void *d_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QBLAS_HIP_H */

@@ -1,0 +1,424 @@
+// capi.hip -- the C ABI of libqblas_hip.so (declared in include/qblas_hip.h).
+//
+// Host-pointer entry points (the reference's quadblas_q* functions and the full-precision
+// quadblas_*_q functions behind the C++ template API) stage their operands into per-thread device
+// buffers on a per-thread stream, run the device routine and copy the outputs back before
+// returning (synchronous, like the reference).  Device-pointer entry points (qblas_hip_*) only
+// enqueue work on the caller's stream.
+//
+// There is no CPU fallback: when a HIP call fails the function records the error (thread-local,
+// quadblas_hip_last_error), leaves outputs unchanged and returns.
+#include <hip/hip_runtime.h>
+#include <sched.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+#include <atomic>
+#include <cmath>
+#include <string>
+
+#include "../../include/qblas_hip.h"
+#include "kernels.h"
+
+using qb::q128;
+
+namespace {
+
+thread_local std::string t_err;
+
+void set_err(const char *where, hipError_t e) {
+  t_err = std::string(where) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")";
+}
+void set_err_msg(const std::string &m) { t_err = m; }
+
+#define QB_CHECK(call)                 \
+  do {                                 \
+    hipError_t e_ = (call);            \
+    if (e_ != hipSuccess) {            \
+      set_err(#call, e_);              \
+      return QBLAS_ERR_HIP;            \
+    }                                  \
+  } while (0)
+
+std::atomic<int> g_threads{0};  // 0: host default
+std::atomic<int> g_kc{126};
+std::atomic<int> g_mode{-1};    // -1: not yet read from the environment
+
+int default_threads() {
+  // omp_get_max_threads() without an OpenMP runtime: OMP_NUM_THREADS's first entry, else the
+  // cores this process may run on.
+  const char *e = getenv("OMP_NUM_THREADS");
+  if (e) {
+    const int v = atoi(e);
+    if (v > 0) return v;
+  }
+  cpu_set_t set;
+  CPU_ZERO(&set);
+  if (sched_getaffinity(0, sizeof(set), &set) == 0) {
+    const int c = CPU_COUNT(&set);
+    if (c > 0) return c;
+  }
+  const long c = sysconf(_SC_NPROCESSORS_ONLN);
+  return c > 0 ? (int)c : 1;
+}
+int cur_threads() {
+  const int t = g_threads.load();
+  return t > 0 ? t : default_threads();
+}
+
+int env_mode() {
+  const char *e = getenv("QBLAS_MODE");
+  if (!e) return QBLAS_MODE_REFERENCE;
+  if (!strcmp(e, "exact")) return QBLAS_MODE_EXACT;
+  if (!strcmp(e, "dd")) return QBLAS_MODE_DD;
+  if (!strcmp(e, "ozaki")) return QBLAS_MODE_OZAKI;
+  return QBLAS_MODE_REFERENCE;
+}
+int cur_mode() {
+  int m = g_mode.load();
+  if (m < 0) {
+    m = env_mode();
+    int expected = -1;
+    g_mode.compare_exchange_strong(expected, m);
+    m = g_mode.load();
+  }
+  return m;
+}
+
+// ---------------------------------------------------------------------------------------------
+// device workspaces: grow-only per host thread; cross-stream reuse is ordered with an event
+// ---------------------------------------------------------------------------------------------
+struct Workspace {
+  void *p = nullptr;
+  size_t cap = 0;
+  int dev = -1;
+  hipEvent_t ev = nullptr;
+  bool recorded = false;
+  hipStream_t last = nullptr;
+};
+
+int ws_acquire(Workspace &w, size_t bytes, hipStream_t s, void **out) {
+  int dev = 0;
+  QB_CHECK(hipGetDevice(&dev));
+  if (w.p && w.dev != dev) {
+    // the thread moved to another device: drop the old buffer once its work is done
+    if (w.recorded) QB_CHECK(hipEventSynchronize(w.ev));
+    QB_CHECK(hipFree(w.p));
+    QB_CHECK(hipEventDestroy(w.ev));
+    w = Workspace();
+  }
+  if (!w.ev) QB_CHECK(hipEventCreateWithFlags(&w.ev, hipEventDisableTiming));
+  if (bytes > w.cap) {
+    if (w.p) {
+      if (w.recorded) QB_CHECK(hipEventSynchronize(w.ev));
+      QB_CHECK(hipFree(w.p));
+      w.p = nullptr;
+      w.cap = 0;
+    }
+    size_t want = bytes < 4096 ? 4096 : bytes;
+    QB_CHECK(hipMalloc(&w.p, want));
+    w.cap = want;
+    w.recorded = false;
+    w.dev = dev;
+  } else if (w.recorded && w.last != s) {
+    QB_CHECK(hipStreamWaitEvent(s, w.ev, 0));
+  }
+  *out = w.p;
+  return QBLAS_OK;
+}
+int ws_release(Workspace &w, hipStream_t s) {
+  QB_CHECK(hipEventRecord(w.ev, s));
+  w.recorded = true;
+  w.last = s;
+  return QBLAS_OK;
+}
+
+thread_local Workspace t_ws;           // partials / scaled x for the device routines
+thread_local Workspace t_stage[4];     // host-API operand staging
+thread_local hipStream_t t_stream = nullptr;
+
+int host_stream(hipStream_t *s) {
+  if (!t_stream) QB_CHECK(hipStreamCreateWithFlags(&t_stream, hipStreamNonBlocking));
+  *s = t_stream;
+  return QBLAS_OK;
+}
+
+bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
+
+q128 load_q(const void *p) {
+  q128 v;
+  memcpy(&v, p, 16);
+  return v;
+}
+
+// ---------------------------------------------------------------------------------------------
+// device routines
+// ---------------------------------------------------------------------------------------------
+int dev_dot(int mode, int64_t n, const void *x, int64_t incx, const void *y, int64_t incy, int T, void *res,
+            hipStream_t s) {
+  if (n < 0 || incx < 0 || incy < 0 || T < 1) {
+    set_err_msg("qblas_hip_dot: negative size/stride or T < 1");
+    return QBLAS_ERR_ARG;
+  }
+  if (!aligned16(x) || !aligned16(y) || !aligned16(res)) {
+    set_err_msg("qblas_hip_dot: binary128 pointers must be 16-byte aligned");
+    return QBLAS_ERR_ARG;
+  }
+  if (n == 0) {  // level1.hpp:83-84
+    QB_CHECK(qbk::launch_zero((q128 *)res, 1, s));
+    return QBLAS_OK;
+  }
+  if (mode != QBLAS_MODE_REFERENCE) {
+    set_err_msg("qblas_hip_dot: mode not available");
+    return QBLAS_ERR_MODE;
+  }
+  void *part = nullptr;
+  int rc = ws_acquire(t_ws, (size_t)2 * (size_t)T * 16, s, &part);
+  if (rc) return rc;
+  QB_CHECK(qbk::launch_dot_ref((const q128 *)x, incx, (const q128 *)y, incy, n, T, (q128 *)part, (q128 *)res, s));
+  return ws_release(t_ws, s);
+}
+
+int dev_gemv(int mode, int layout, int64_t m, int64_t n, q128 alpha, const void *A, int64_t lda, const void *x,
+             int64_t incx, q128 beta, void *y, int64_t incy, hipStream_t s) {
+  if (m < 0 || n < 0 || lda < 0 || incx < 0 || incy < 0) {
+    set_err_msg("qblas_hip_gemv: negative size or stride");
+    return QBLAS_ERR_ARG;
+  }
+  if (!aligned16(A) || !aligned16(x) || !aligned16(y)) {
+    set_err_msg("qblas_hip_gemv: binary128 pointers must be 16-byte aligned");
+    return QBLAS_ERR_ARG;
+  }
+  if (m == 0 || n == 0) return QBLAS_OK;  // level2.hpp:21-22,59-60
+  if (mode != QBLAS_MODE_REFERENCE) {
+    set_err_msg("qblas_hip_gemv: mode not available");
+    return QBLAS_ERR_MODE;
+  }
+  const int colmajor = layout != 0;
+  void *ax = nullptr;
+  if (colmajor) {
+    int rc = ws_acquire(t_ws, (size_t)n * 16, s, &ax);
+    if (rc) return rc;
+  }
+  QB_CHECK(qbk::launch_gemv_ref(colmajor, m, n, alpha, (const q128 *)A, lda, (const q128 *)x, incx, beta,
+                                (q128 *)y, incy, (q128 *)ax, s));
+  if (colmajor) return ws_release(t_ws, s);
+  return QBLAS_OK;
+}
+
+int dev_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, q128 alpha, const void *A, int64_t lda,
+             const void *B, int64_t ldb, q128 beta, void *C, int64_t ldc, hipStream_t s) {
+  if (m < 0 || n < 0 || k < 0 || lda < 0 || ldb < 0 || ldc < 0) {
+    set_err_msg("qblas_hip_gemm: negative size or leading dimension");
+    return QBLAS_ERR_ARG;
+  }
+  if (!aligned16(A) || !aligned16(B) || !aligned16(C)) {
+    set_err_msg("qblas_hip_gemm: binary128 pointers must be 16-byte aligned");
+    return QBLAS_ERR_ARG;
+  }
+  if (m == 0 || n == 0 || k == 0) return QBLAS_OK;  // level3.hpp:221-222
+  if (mode != QBLAS_MODE_REFERENCE) {
+    set_err_msg("qblas_hip_gemm: mode not available");
+    return QBLAS_ERR_MODE;
+  }
+  const int colmajor = layout != 0;
+  // gemm_simple for m,n,k <= 64 is one chain over all of k; otherwise blocks of kc
+  // (level3.hpp:224-229, :244-246).  For k <= kc both are one block.
+  const int64_t kc = (m <= 64 && n <= 64 && k <= 64) ? k : (int64_t)g_kc.load();
+  QB_CHECK(qbk::launch_gemm_ref(colmajor, m, n, k, kc, alpha, (const q128 *)A, lda, (const q128 *)B, ldb, beta,
+                                (q128 *)C, ldc, s));
+  return QBLAS_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// host-pointer wrappers
+// ---------------------------------------------------------------------------------------------
+// bytes spanned by `count` strided elements
+size_t span_bytes(int64_t count, int64_t inc) { return count <= 0 ? 0 : (size_t)((count - 1) * inc + 1) * 16; }
+
+int stage_in(int slot, const void *host, size_t bytes, hipStream_t s, void **dev) {
+  int rc = ws_acquire(t_stage[slot], bytes ? bytes : 16, s, dev);
+  if (rc) return rc;
+  if (bytes) QB_CHECK(hipMemcpyAsync(*dev, host, bytes, hipMemcpyHostToDevice, s));
+  return QBLAS_OK;
+}
+
+int host_dot(int64_t n, const void *x, int64_t incx, const void *y, int64_t incy, q128 *out) {
+  if (n <= 0) {
+    *out = qb::q_zero(0);
+    return QBLAS_OK;
+  }
+  if (incx < 0 || incy < 0) {
+    set_err_msg("quadblas dot: negative stride");
+    return QBLAS_ERR_ARG;
+  }
+  hipStream_t s;
+  int rc = host_stream(&s);
+  if (rc) return rc;
+  void *dx, *dy, *dr;
+  if ((rc = stage_in(0, x, span_bytes(n, incx), s, &dx))) return rc;
+  if ((rc = stage_in(1, y, span_bytes(n, incy), s, &dy))) return rc;
+  if ((rc = stage_in(2, nullptr, 0, s, &dr))) return rc;
+  if ((rc = dev_dot(cur_mode(), n, dx, incx, dy, incy, cur_threads(), dr, s))) return rc;
+  q128 r;
+  QB_CHECK(hipMemcpyAsync(&r, dr, 16, hipMemcpyDeviceToHost, s));
+  QB_CHECK(hipStreamSynchronize(s));
+  for (int i = 0; i < 3; ++i) ws_release(t_stage[i], s);
+  *out = r;
+  return QBLAS_OK;
+}
+
+int host_gemv(int layout, int64_t m, int64_t n, q128 alpha, const void *A, int64_t lda, const void *x,
+              int64_t incx, q128 beta, void *y, int64_t incy) {
+  if (m <= 0 || n <= 0) return QBLAS_OK;
+  if (lda < 0 || incx < 0 || incy < 0) {
+    set_err_msg("quadblas gemv: negative stride");
+    return QBLAS_ERR_ARG;
+  }
+  hipStream_t s;
+  int rc = host_stream(&s);
+  if (rc) return rc;
+  const size_t abytes = layout == 0 ? span_bytes(m, lda) - 16 + (size_t)n * 16 : span_bytes(n, lda) - 16 + (size_t)m * 16;
+  const size_t xbytes = span_bytes(n, incx), ybytes = span_bytes(m, incy);
+  void *dA, *dx, *dy;
+  if ((rc = stage_in(0, A, abytes, s, &dA))) return rc;
+  if ((rc = stage_in(1, x, xbytes, s, &dx))) return rc;
+  if ((rc = stage_in(2, y, ybytes, s, &dy))) return rc;
+  if ((rc = dev_gemv(cur_mode(), layout, m, n, alpha, dA, lda, dx, incx, beta, dy, incy, s))) return rc;
+  QB_CHECK(hipMemcpyAsync(y, dy, ybytes, hipMemcpyDeviceToHost, s));
+  QB_CHECK(hipStreamSynchronize(s));
+  for (int i = 0; i < 3; ++i) ws_release(t_stage[i], s);
+  return QBLAS_OK;
+}
+
+int host_gemm(int layout, int64_t m, int64_t n, int64_t k, q128 alpha, const void *A, int64_t lda, const void *B,
+              int64_t ldb, q128 beta, void *C, int64_t ldc) {
+  if (m <= 0 || n <= 0 || k <= 0) return QBLAS_OK;
+  if (lda < 0 || ldb < 0 || ldc < 0) {
+    set_err_msg("quadblas gemm: negative leading dimension");
+    return QBLAS_ERR_ARG;
+  }
+  hipStream_t s;
+  int rc = host_stream(&s);
+  if (rc) return rc;
+  size_t abytes, bbytes, cbytes;
+  if (layout == 0) {
+    abytes = (size_t)((m - 1) * lda + k) * 16;
+    bbytes = (size_t)((k - 1) * ldb + n) * 16;
+    cbytes = (size_t)((m - 1) * ldc + n) * 16;
+  } else {
+    abytes = (size_t)((k - 1) * lda + m) * 16;
+    bbytes = (size_t)((n - 1) * ldb + k) * 16;
+    cbytes = (size_t)((n - 1) * ldc + m) * 16;
+  }
+  void *dA, *dB, *dC;
+  if ((rc = stage_in(0, A, abytes, s, &dA))) return rc;
+  if ((rc = stage_in(1, B, bbytes, s, &dB))) return rc;
+  if ((rc = stage_in(2, C, cbytes, s, &dC))) return rc;
+  if ((rc = dev_gemm(cur_mode(), layout, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc, s))) return rc;
+  QB_CHECK(hipMemcpyAsync(C, dC, cbytes, hipMemcpyDeviceToHost, s));
+  QB_CHECK(hipStreamSynchronize(s));
+  for (int i = 0; i < 3; ++i) ws_release(t_stage[i], s);
+  return QBLAS_OK;
+}
+
+bool is_colmajor_char(char c) { return c == 'C' || c == 'c'; }
+bool is_trans_char(char c) { return c == 'T' || c == 't' || c == 'C' || c == 'c'; }
+
+}  // namespace
+
+// =============================================================================================
+// exported C ABI
+// =============================================================================================
+extern "C" {
+
+double quadblas_qdot(int n, void *x, int incx, void *y, int incy) {
+  q128 r;
+  if (host_dot(n, x, incx, y, incy, &r) != QBLAS_OK) return std::nan("");
+  return qb::to_double(r);
+}
+
+void quadblas_qgemv(char layout, char trans, int m, int n, double alpha, void *A, int lda, void *x, int incx,
+                    double beta, void *y, int incy) {
+  int lay = is_colmajor_char(layout) ? 1 : 0;  // c_interface.hpp:76
+  if (is_trans_char(trans)) {                  // c_interface.hpp:79-85
+    const int t = m;
+    m = n;
+    n = t;
+    lay = 1 - lay;
+  }
+  host_gemv(lay, m, n, qb::from_double(alpha), A, lda, x, incx, qb::from_double(beta), y, incy);
+}
+
+void quadblas_qgemm(char layout, char transa, char transb, int m, int n, int k, double alpha, void *A, int lda,
+                    void *B, int ldb, double beta, void *C, int ldc) {
+  (void)transa;  // c_interface.hpp:109-112: ignored
+  (void)transb;
+  host_gemm(is_colmajor_char(layout) ? 1 : 0, m, n, k, qb::from_double(alpha), A, lda, B, ldb,
+            qb::from_double(beta), C, ldc);
+}
+
+void quadblas_set_num_threads(int num_threads) { g_threads.store(num_threads > 0 ? num_threads : 0); }
+int quadblas_get_num_threads(void) { return cur_threads(); }
+
+const char *quadblas_get_version(void) { return "QuadBLAS 1.0.0 - MI355X (gfx950) HIP backend"; }
+int quadblas_is_aligned(const void *ptr) { return ((uintptr_t)ptr % 32u) == 0; }
+
+void quadblas_dot_q(size_t n, const void *x, size_t incx, const void *y, size_t incy, void *result) {
+  q128 r;
+  if (host_dot((int64_t)n, x, (int64_t)incx, y, (int64_t)incy, &r) == QBLAS_OK) memcpy(result, &r, 16);
+}
+void quadblas_gemv_q(int layout, size_t m, size_t n, const void *alpha, const void *A, size_t lda, const void *x,
+                     size_t incx, const void *beta, void *y, size_t incy) {
+  host_gemv(layout != 0, (int64_t)m, (int64_t)n, load_q(alpha), A, (int64_t)lda, x, (int64_t)incx, load_q(beta), y,
+            (int64_t)incy);
+}
+void quadblas_gemm_q(int layout, size_t m, size_t n, size_t k, const void *alpha, const void *A, size_t lda,
+                     const void *B, size_t ldb, const void *beta, void *C, size_t ldc) {
+  host_gemm(layout != 0, (int64_t)m, (int64_t)n, (int64_t)k, load_q(alpha), A, (int64_t)lda, B, (int64_t)ldb,
+            load_q(beta), C, (int64_t)ldc);
+}
+
+int qblas_set_mode(int mode) {
+  if (mode < QBLAS_MODE_REFERENCE || mode > QBLAS_MODE_OZAKI) return QBLAS_ERR_ARG;
+  g_mode.store(mode);
+  return QBLAS_OK;
+}
+int qblas_get_mode(void) { return cur_mode(); }
+void qblas_set_kc(int kc) { g_kc.store(kc > 0 ? kc : 126); }
+int qblas_get_kc(void) { return g_kc.load(); }
+
+const char *quadblas_hip_last_error(void) { return t_err.c_str(); }
+void qblas_clear_error(void) { t_err.clear(); }
+
+int qblas_hip_dot(int mode, int64_t n, const void *d_x, int64_t incx, const void *d_y, int64_t incy, int T,
+                  void *d_result, void *stream) {
+  return dev_dot(mode, n, d_x, incx, d_y, incy, T, d_result, (hipStream_t)stream);
+}
+int qblas_hip_gemv(int mode, int layout, int64_t m, int64_t n, const void *alpha, const void *d_A, int64_t lda,
+                   const void *d_x, int64_t incx, const void *beta, void *d_y, int64_t incy, void *stream) {
+  return dev_gemv(mode, layout, m, n, load_q(alpha), d_A, lda, d_x, incx, load_q(beta), d_y, incy,
+                  (hipStream_t)stream);
+}
+int qblas_hip_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, const void *alpha, const void *d_A,
+                   int64_t lda, const void *d_B, int64_t ldb, const void *beta, void *d_C, int64_t ldc,
+                   void *stream) {
+  return dev_gemm(mode, layout, m, n, k, load_q(alpha), d_A, lda, d_B, ldb, load_q(beta), d_C, ldc,
+                  (hipStream_t)stream);
+}
+int qblas_hip_fill(int kind, uint64_t seed, int64_t n, int lo_exp, int hi_exp, void *d_out, void *stream) {
+  if (n < 0 || kind < 0 || kind > 3 || (kind == 1 && hi_exp <= lo_exp) || !aligned16(d_out)) {
+    set_err_msg("qblas_hip_fill: bad arguments");
+    return QBLAS_ERR_ARG;
+  }
+  if (kind == 1 && (lo_exp < -16382 || hi_exp > 16384)) {
+    set_err_msg("qblas_hip_fill: exponent range outside binary128 normals");
+    return QBLAS_ERR_ARG;
+  }
+  QB_CHECK(qbk::launch_fill(kind, seed, n, lo_exp, hi_exp, (q128 *)d_out, (hipStream_t)stream));
+  return QBLAS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,23 @@
+// kernels.h -- launchers shared between the kernel translation units and the C ABI (capi.hip).
+// All pointers are device pointers to packed binary128 values; all launches are asynchronous on
+// the given stream and return hipGetLastError().
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "q128.hpp"
+
+namespace qbk {
+using qb::q128;
+
+// reference order (kernels_ref.hip)
+hipError_t launch_dot_ref(const q128 *x, int64_t incx, const q128 *y, int64_t incy, int64_t n, int T,
+                          q128 *part /* 2*T */, q128 *out, hipStream_t st);
+hipError_t launch_gemv_ref(int colmajor, int64_t m, int64_t n, q128 alpha, const q128 *A, int64_t lda,
+                           const q128 *x, int64_t incx, q128 beta, q128 *y, int64_t incy,
+                           q128 *ax /* n, colmajor only */, hipStream_t st);
+hipError_t launch_gemm_ref(int colmajor, int64_t m, int64_t n, int64_t k, int64_t kc, q128 alpha, const q128 *A,
+                           int64_t lda, const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc,
+                           hipStream_t st);
+hipError_t launch_zero(q128 *p, int64_t n, hipStream_t st);
+hipError_t launch_fill(int kind, uint64_t seed, int64_t n, int lo_exp, int hi_exp, q128 *out, hipStream_t st);
+}  // namespace qbk
