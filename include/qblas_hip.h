@@ -88,7 +88,8 @@ enum {
   QBLAS_OK = 0,
   QBLAS_ERR_ARG = 1,      /* invalid argument (negative size/stride, misaligned pointer, ...) */
   QBLAS_ERR_HIP = 2,      /* a HIP runtime call failed */
-  QBLAS_ERR_MODE = 3      /* mode not available for this routine */
+  QBLAS_ERR_MODE = 3,     /* mode not available for this routine */
+  QBLAS_ERR_RANGE = 4     /* EXACT form: product exponents too far apart for the window */
 };
 
 /* Mode used by Parts 1 and 2 (default QBLAS_MODE_REFERENCE; environment QBLAS_MODE overrides at
@@ -104,6 +105,11 @@ int qblas_get_kc(void);
 /* Last error message of the calling thread ("" if none); cleared by qblas_clear_error(). */
 const char *quadblas_hip_last_error(void);
 void qblas_clear_error(void);
+
+/* After a device-API call in QBLAS_MODE_EXACT on this thread: synchronizes the stream and returns
+   1 if a product fell outside the accumulation window (the output was set to NaN), 0 if the
+   result is exact, -1 on a HIP error.  Host-pointer entry points check this themselves. */
+int qblas_hip_exact_status(void *stream);
 
 /* dot: *d_result (device, 16 bytes) = x . y.  T: the reference thread count for
    QBLAS_MODE_REFERENCE (ignored otherwise).  stream: hipStream_t or NULL. */

@@ -70,6 +70,8 @@ def lib():
         L.or_exact_gemm_samples.argtypes = [sz, P, P, sz, Q128, P, ctypes.c_int64, ctypes.c_int64,
                                             P, ctypes.c_int64, ctypes.c_int64, Q128, P,
                                             ctypes.c_int64, ctypes.c_int64, P, P, P]
+        L.or_exact_gemv.restype = ctypes.c_int
+        L.or_exact_gemv.argtypes = [ctypes.c_int, sz, sz, Q128, P, sz, P, sz, Q128, P, sz]
         L.or_rand64.restype = ctypes.c_uint64
         L.or_rand64.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
         L.or_fill.argtypes = [ctypes.c_int, ctypes.c_uint64, sz, ctypes.c_int, ctypes.c_int, P]
@@ -232,6 +234,20 @@ def exact_gemm_samples(ii, jj, k, alpha, A, a_strides, B, b_strides, beta, C0, c
     if rc != 0:
         raise ValueError("exact_gemm_samples needs finite inputs")
     return rounded, rel
+
+
+def exact_gemv(colmajor, m, n, alpha, A, lda, x, incx, beta, y, incy):
+    """y <- RN(alpha*A x + beta*y) row by row, each row rounded once (in place)."""
+    _chk(A), _chk(x), _chk(y)
+    if lib().or_exact_gemv(int(colmajor), m, n, q(alpha), _ptr(A), lda, _ptr(x), incx, q(beta), _ptr(y), incy) != 0:
+        raise ValueError("exact_gemv needs finite inputs")
+
+
+def abs_array(a):
+    """|a| elementwise (sign bits cleared)."""
+    out = np.array(a, dtype=np.uint64, copy=True)
+    out[..., 1] &= np.uint64((1 << 63) - 1)
+    return out
 
 
 # ------------------------------------------------------------------------------------------------

@@ -528,12 +528,20 @@ int or_exact_dot(size_t n, const q128 *x, ptrdiff_t incx, const q128 *y, ptrdiff
         if (e < emin) emin = e;
         if (e + 227 > emax) emax = e + 227;
     }
+    int any_term = emin <= emax;
     for (size_t j = 0; j < ncand; ++j) {
         dec D = decode(cand[j]);
         if (D.cls == C_FIN) { int e = D.exp - 112; if (e < emin) emin = e; if (e + 114 > emax) emax = e + 114; }
     }
-    if (emin > emax) {  /* everything is zero */
-        if (rounded) *rounded = QZERO;
+    if (!any_term) {  /* every term is zero */
+        if (rounded) {
+            /* IEEE signed zeros as the reference would produce them: the sum of products starts at
+               +0 (level1.hpp:18, level2.hpp:31), alpha*(+0) has alpha's sign, beta*c is a zero
+               with sign(beta)^sign(c), and -0 + -0 is the only way to a -0 result */
+            int sz = Al.sign & (Be.sign ^ Cc.sign);
+            *rounded = QZERO;
+            rounded->hi = (uint64_t)sz << 63;
+        }
         for (size_t j = 0; j < ncand; ++j) {
             dec D = decode(cand[j]);
             relerr[j] = (D.cls == C_ZERO) ? 0.0 : __builtin_inf();
@@ -689,12 +697,13 @@ void or_fill(int kind, uint64_t seed, size_t n, int lo_exp, int hi_exp, q128 *ou
         }
     }
 }
-/* y vector for the cancellation variant: y_{i+1} = y_i on each pair (p % 8 < 2), kind-0 else */
+/* y vector for the cancellation variant: kind-0 values from the stream seed + 1, with
+   y_{i+1} = y_i on each pair (p % 8 < 2) */
 void or_fill_pair_y(uint64_t seed, size_t n, q128 *out) {
     #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < n; ++i) {
         size_t p = ((i & ~(size_t)1) % 8 < 2) ? (i & ~(size_t)1) : i;
-        double a = u11(or_rand64(seed, 2 * p)), b = u11(or_rand64(seed, 2 * p + 1));
+        double a = u11(or_rand64(seed + 1, 2 * p)), b = u11(or_rand64(seed + 1, 2 * p + 1));
         out[i] = f2q((__float128)a + (__float128)b * 0x1p-60Q);
     }
 }
@@ -702,3 +711,19 @@ void or_fill_pair_y(uint64_t seed, size_t n, q128 *out) {
 /* conversions used by the C ABI check: binary128 -> double, round to nearest even
    (Sleef_cast_to_doubleq1 at c_interface.hpp:30) */
 double or_q2d(q128 a) { return (double)q2f(a); }
+
+/* Exact gemv: y_i <- RN(alpha * sum_j A(i,j) x_j + beta * y_i) for every row (colmajor selects
+   A(i,j) = A[j*lda + i], else A[i*lda + j]).  Returns -1 if an input is Inf/NaN. */
+int or_exact_gemv(int colmajor, size_t m, size_t n, q128 alpha, const q128 *A, size_t lda, const q128 *x,
+                  size_t incx, q128 beta, q128 *y, size_t incy) {
+    int bad = 0;
+    #pragma omp parallel for schedule(dynamic, 8) reduction(|:bad)
+    for (size_t i = 0; i < m; ++i) {
+        const q128 *row = colmajor ? A + i : A + i * lda;
+        ptrdiff_t rs = colmajor ? (ptrdiff_t)lda : 1;
+        q128 r;
+        if (or_exact_dot(n, row, rs, x, (ptrdiff_t)incx, alpha, beta, y[i * incy], NULL, 0, &r, NULL) != 0) bad = 1;
+        else y[i * incy] = r;
+    }
+    return bad ? -1 : 0;
+}

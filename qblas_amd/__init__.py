@@ -81,6 +81,7 @@ def lib():
         "qblas_hip_gemv": (I, [I, I, I64, I64, P, P, I64, P, I64, P, P, I64, P]),
         "qblas_hip_gemm": (I, [I, I, I64, I64, I64, P, P, I64, P, I64, P, P, I64, P]),
         "qblas_hip_fill": (I, [I, ctypes.c_uint64, I64, I, I, P, P]),
+        "qblas_hip_exact_status": (I, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -255,6 +256,11 @@ def dev_gemm(mode, layout, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, stream=
     a, b = _q(alpha), _q(beta)
     _check(lib().qblas_hip_gemm(int(mode), int(layout), m, n, k, _ptr(a), A, lda, B, ldb, _ptr(b), C, ldc, stream),
            "qblas_hip_gemm")
+
+
+def dev_exact_status(stream=0):
+    """1 if the last EXACT-form device call of this thread lost a product (output NaN), else 0."""
+    return lib().qblas_hip_exact_status(stream)
 
 
 FILL_UNIFORM_FULL = 0

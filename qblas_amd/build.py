@@ -19,8 +19,10 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 ARCH = os.environ.get("QBLAS_ARCH", "gfx950")
 
 # -packed-fp32-ops off: the compiler emits no packed FP32 instructions (v_pk_{add,mul,fma}_f32).
+# -ffp-contract=off: HIP's default (fast) fuses a*b + c across statements, which breaks the
+# error-free transforms (TwoProd/TwoSum) of the double-double form.
 COMMON = [
-    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result", "-Wno-unused-function", "-Wno-bitwise-instead-of-logical",
     "-I", os.path.join(HERE, "..", "include"),

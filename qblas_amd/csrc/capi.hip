@@ -136,6 +136,21 @@ int ws_release(Workspace &w, hipStream_t s) {
 }
 
 thread_local Workspace t_ws;           // partials / scaled x for the device routines
+thread_local const uint32_t *t_exact_status = nullptr;  // status word of the last EXACT-form call
+
+// After a synchronous EXACT-form call: report a product that fell outside its window.
+int check_exact_status(hipStream_t s) {
+  if (!t_exact_status) return QBLAS_OK;
+  uint32_t st = 0;
+  QB_CHECK(hipMemcpyAsync(&st, t_exact_status, 4, hipMemcpyDeviceToHost, s));
+  QB_CHECK(hipStreamSynchronize(s));
+  t_exact_status = nullptr;
+  if (st & 1u) {
+    set_err_msg("exact form: a product lies outside the accumulation window; the output was set to NaN");
+    return QBLAS_ERR_RANGE;
+  }
+  return QBLAS_OK;
+}
 thread_local Workspace t_stage[4];     // host-API operand staging
 thread_local hipStream_t t_stream = nullptr;
 
@@ -170,6 +185,21 @@ int dev_dot(int mode, int64_t n, const void *x, int64_t incx, const void *y, int
     QB_CHECK(qbk::launch_zero((q128 *)res, 1, s));
     return QBLAS_OK;
   }
+  if (mode == QBLAS_MODE_DD) {
+    void *ws = nullptr;
+    int rc = ws_acquire(t_ws, qbk::dot_dd_ws(n), s, &ws);
+    if (rc) return rc;
+    QB_CHECK(qbk::launch_dot_dd((const q128 *)x, incx, (const q128 *)y, incy, n, ws, (q128 *)res, s));
+    return ws_release(t_ws, s);
+  }
+  if (mode == QBLAS_MODE_EXACT) {
+    void *ws = nullptr;
+    int rc = ws_acquire(t_ws, qbk::dot_exact_ws(n), s, &ws);
+    if (rc) return rc;
+    QB_CHECK(qbk::launch_dot_exact((const q128 *)x, incx, (const q128 *)y, incy, n, ws, (q128 *)res, s));
+    t_exact_status = (const uint32_t *)ws;
+    return ws_release(t_ws, s);
+  }
   if (mode != QBLAS_MODE_REFERENCE) {
     set_err_msg("qblas_hip_dot: mode not available");
     return QBLAS_ERR_MODE;
@@ -192,11 +222,28 @@ int dev_gemv(int mode, int layout, int64_t m, int64_t n, q128 alpha, const void 
     return QBLAS_ERR_ARG;
   }
   if (m == 0 || n == 0) return QBLAS_OK;  // level2.hpp:21-22,59-60
+  const int colmajor = layout != 0;
+  if (mode == QBLAS_MODE_DD) {
+    void *ws = nullptr;
+    int rc = ws_acquire(t_ws, qbk::gemv_dd_ws(colmajor, m, n), s, &ws);
+    if (rc) return rc;
+    QB_CHECK(qbk::launch_gemv_dd(colmajor, m, n, alpha, (const q128 *)A, lda, (const q128 *)x, incx, beta,
+                                 (q128 *)y, incy, ws, s));
+    return ws_release(t_ws, s);
+  }
+  if (mode == QBLAS_MODE_EXACT) {
+    void *ws = nullptr;
+    int rc = ws_acquire(t_ws, qbk::gemv_exact_ws(colmajor, m, n), s, &ws);
+    if (rc) return rc;
+    QB_CHECK(qbk::launch_gemv_exact(colmajor, m, n, alpha, (const q128 *)A, lda, (const q128 *)x, incx, beta,
+                                    (q128 *)y, incy, ws, s));
+    t_exact_status = (const uint32_t *)ws;
+    return ws_release(t_ws, s);
+  }
   if (mode != QBLAS_MODE_REFERENCE) {
     set_err_msg("qblas_hip_gemv: mode not available");
     return QBLAS_ERR_MODE;
   }
-  const int colmajor = layout != 0;
   void *ax = nullptr;
   if (colmajor) {
     int rc = ws_acquire(t_ws, (size_t)n * 16, s, &ax);
@@ -267,7 +314,7 @@ int host_dot(int64_t n, const void *x, int64_t incx, const void *y, int64_t incy
   QB_CHECK(hipStreamSynchronize(s));
   for (int i = 0; i < 3; ++i) ws_release(t_stage[i], s);
   *out = r;
-  return QBLAS_OK;
+  return check_exact_status(s);
 }
 
 int host_gemv(int layout, int64_t m, int64_t n, q128 alpha, const void *A, int64_t lda, const void *x,
@@ -290,7 +337,7 @@ int host_gemv(int layout, int64_t m, int64_t n, q128 alpha, const void *A, int64
   QB_CHECK(hipMemcpyAsync(y, dy, ybytes, hipMemcpyDeviceToHost, s));
   QB_CHECK(hipStreamSynchronize(s));
   for (int i = 0; i < 3; ++i) ws_release(t_stage[i], s);
-  return QBLAS_OK;
+  return check_exact_status(s);
 }
 
 int host_gemm(int layout, int64_t m, int64_t n, int64_t k, q128 alpha, const void *A, int64_t lda, const void *B,
@@ -391,6 +438,11 @@ void qblas_set_kc(int kc) { g_kc.store(kc > 0 ? kc : 126); }
 int qblas_get_kc(void) { return g_kc.load(); }
 
 const char *quadblas_hip_last_error(void) { return t_err.c_str(); }
+
+int qblas_hip_exact_status(void *stream) {
+  const int rc = check_exact_status((hipStream_t)stream);
+  return rc == QBLAS_ERR_RANGE ? 1 : (rc == QBLAS_OK ? 0 : -1);
+}
 void qblas_clear_error(void) { t_err.clear(); }
 
 int qblas_hip_dot(int mode, int64_t n, const void *d_x, int64_t incx, const void *d_y, int64_t incy, int T,

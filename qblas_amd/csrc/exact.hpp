@@ -1,0 +1,455 @@
+// exact.hpp -- exact (correctly rounded) accumulation of binary128 products.
+//
+// EXACT form of dot and gemv: the result is the exact value of alpha * sum_i a_i b_i + beta * y
+// rounded once to binary128 (round to nearest even) -- the correctly rounded answer, independent of
+// summation order and thread count.  The reference rounds after every fused multiply-add
+// (level1.hpp:20-32); this form has no intermediate rounding at all.
+//
+// Stage 1 (streaming, per lane): every product a_i*b_i is formed exactly (226 bits, mul113) and
+//   added into a 384-bit two's complement fixed-point window W (12 x u32) whose bit 0 has weight
+//   2^A.  A product with lsb exponent e_p lands at bit offset o = e_p - A; it fits when
+//   1 <= o <= 126, leaving 32 bits of headroom for carries (<= 2^32 additions per window).  Each
+//   lane keeps two such windows, 124 bits apart (win2_add_product), anchored on the centre of the
+//   first products of its wave, so products within +-124 binary orders of magnitude of that centre
+//   are exact.  A product outside both windows is not added and sets the LOST flag: the output is
+//   then NaN and the call reports QBLAS_ERR_RANGE.
+// Stage 2 (combine): windows with possibly different anchors are summed exactly into a wide
+//   accumulator (ACCW words), then alpha*S + beta*y is formed exactly as big integers and rounded
+//   once.
+#pragma once
+#include "q128.hpp"
+
+namespace qb {
+
+constexpr int WW = 12;          // window words
+constexpr int WIN_OMIN = 1;     // valid product offsets [WIN_OMIN, WIN_OMAX]
+constexpr int WIN_OMAX = 126;
+constexpr int WIN_CENTER = 63;  // anchor = centre exponent - WIN_CENTER
+
+enum : uint32_t { FL_LOST = 1u, FL_NAN = 2u, FL_PINF = 4u, FL_NINF = 8u };
+
+QB_HD uint32_t alignbit32(uint32_t hi, uint32_t lo, uint32_t s) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_alignbit(hi, lo, s);
+#else
+  return (uint32_t)((((uint64_t)hi << 32) | lo) >> (s & 31));
+#endif
+}
+
+// 113x113 -> 226-bit product as eight 32-bit words (p[7] holds bits 224..225)
+QB_HD void mul113_w(uint32_t (&p)[8], const uq &a, const uq &b) {
+  const uint32_t x[4] = {(uint32_t)a.ml, (uint32_t)(a.ml >> 32), (uint32_t)a.mh, (uint32_t)(a.mh >> 32)};
+  const uint32_t y[4] = {(uint32_t)b.ml, (uint32_t)(b.ml >> 32), (uint32_t)b.mh, (uint32_t)(b.mh >> 32)};
+#pragma unroll
+  for (int i = 0; i < 8; ++i) p[i] = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    uint64_t carry = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint64_t t = (uint64_t)x[i] * y[j] + p[i + j] + carry;
+      p[i + j] = (uint32_t)t;
+      carry = t >> 32;
+    }
+    p[i + 4] = (uint32_t)carry;
+  }
+}
+
+// product lsb exponent: value = P * 2^(a.e + b.e - 224)
+QB_HD int prod_lsb(const uq &a, const uq &b) { return a.e + b.e - 224; }
+
+// W += (-1)^(a.s^b.s) * a.m*b.m * 2^(e_p - A).  Returns false (and adds nothing) if the product
+// does not fit the window.
+QB_HD bool win_add_product(uint32_t (&W)[WW], int A, const uq &a, const uq &b) {
+  const int o = prod_lsb(a, b) - A;
+  if (o < WIN_OMIN || o > WIN_OMAX) return false;
+  uint32_t P[8];
+  mul113_w(P, a, b);
+  const int w = (o - 1) >> 5;          // 0..3
+  const uint32_t sh = 32u - (uint32_t)(o - (w << 5));  // bit offset 1..32 -> funnel 31..0
+  uint32_t T[9];
+  T[0] = alignbit32(P[0], 0u, sh);
+#pragma unroll
+  for (int j = 1; j < 8; ++j) T[j] = alignbit32(P[j], P[j - 1], sh);
+  T[8] = alignbit32(0u, P[7], sh);
+  const uint32_t mask = 0u - ((a.s ^ b.s) & 1u);
+  uint32_t carry = mask & 1u;
+#pragma unroll
+  for (int i = 0; i < WW; ++i) {
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int j = i - k;
+      if (j >= 0 && j < 9) v = (w == k) ? T[j] : v;
+    }
+    v ^= mask;
+    const uint64_t s = (uint64_t)W[i] + v + carry;
+    W[i] = (uint32_t)s;
+    carry = (uint32_t)(s >> 32);
+  }
+  return true;
+}
+
+// Two windows per lane: lo anchored at A, hi at A + WIN_BAND.  Together they hold products with
+// lsb offsets o = e_p - A in [1, WIN_BAND + WIN_OMAX] = [1, 250] exactly, i.e. a spread of 249
+// binary orders of magnitude around the centre (anchor A = centre - WIN2_CENTER).
+constexpr int WIN_BAND = 124;
+constexpr int WIN2_CENTER = 125;
+QB_HD bool win2_add_product(uint32_t (&Wl)[WW], uint32_t (&Wh)[WW], int A, const uq &a, const uq &b) {
+  if (prod_lsb(a, b) - A > WIN_OMAX) return win_add_product(Wh, A + WIN_BAND, a, b);
+  return win_add_product(Wl, A, a, b);
+}
+
+QB_HD void win_add_win(uint32_t (&W)[WW], const uint32_t (&X)[WW]) {
+  uint32_t carry = 0;
+#pragma unroll
+  for (int i = 0; i < WW; ++i) {
+    const uint64_t s = (uint64_t)W[i] + X[i] + carry;
+    W[i] = (uint32_t)s;
+    carry = (uint32_t)(s >> 32);
+  }
+}
+
+// Classify a packed operand pair for the streaming loop.  Returns 0 for a regular pair (both
+// normal nonzero: ua/ub filled), 1 if the product is exactly zero (skip), 2 for a pair involving
+// Inf/NaN (flags updated), 3 if a subnormal is involved (ua/ub filled via the general unpack).
+QB_HD int classify_pair(q128 a, q128 b, uq &ua, uq &ub, uint32_t &flags) {
+  bool spa, spb;
+  ua = unpack_fast(a, spa);
+  ub = unpack_fast(b, spb);
+  if (!(spa | spb)) return 0;
+  if (q_isnan(a) | q_isnan(b)) {
+    flags |= FL_NAN;
+    return 2;
+  }
+  const bool ia = q_isinf(a), ib = q_isinf(b);
+  if (ia | ib) {
+    if (q_iszero(a) | q_iszero(b)) {
+      flags |= FL_NAN;  // Inf * 0
+    } else {
+      flags |= ((a.hi ^ b.hi) >> 63) ? FL_NINF : FL_PINF;
+    }
+    return 2;
+  }
+  if (q_iszero(a) | q_iszero(b)) return 1;
+  ua = unpack_finite(a);
+  ub = unpack_finite(b);
+  return 3;
+}
+
+// ------------------------------------------------------------------------------------------
+// stage 2: wide accumulator and exact epilogue (plain loops; runs once per output)
+// ------------------------------------------------------------------------------------------
+
+constexpr int ACCW = 40;  // accumulator words (1280 bits) for combining windows
+constexpr int BIGW = 56;  // big magnitude words for alpha*S + beta*y
+
+struct wide_acc {
+  uint32_t w[ACCW];  // two's complement, bit 0 weight 2^base
+  int base;
+  bool any;          // a nonzero window has been added
+  bool lost;
+};
+
+QB_HD void wide_init(wide_acc &acc) {
+  for (int i = 0; i < ACCW; ++i) acc.w[i] = 0;
+  acc.base = 0;
+  acc.any = false;
+  acc.lost = false;
+}
+
+QB_HD bool win_is_zero(const uint32_t *W) {
+  uint32_t o = 0;
+  for (int i = 0; i < WW; ++i) o |= W[i];
+  return o == 0;
+}
+
+// Add window W (anchor A) exactly.  The first nonzero window fixes base = A - 320 (room for
+// windows up to 10 words lower and ~16 words higher).  A window that does not fit sets lost.
+QB_HD void wide_add_window(wide_acc &acc, const uint32_t *W, int A) {
+  if (win_is_zero(W)) return;
+  if (!acc.any) {
+    acc.any = true;
+    acc.base = A - 320;
+  }
+  const int off = A - acc.base;
+  if (off < 0 || off + WW * 32 + 32 > ACCW * 32) {
+    acc.lost = true;
+    return;
+  }
+  const int ow = off >> 5, ob = off & 31;
+  const uint32_t sext = (W[WW - 1] >> 31) ? 0xffffffffu : 0u;
+  // shifted, sign-extended copy added word by word
+  uint32_t carry = 0;
+  for (int i = ow; i < ACCW; ++i) {
+    const int j = i - ow;  // source word index of the shifted value
+    const uint32_t cur = (j < WW) ? W[j] : sext;
+    const uint32_t prev = (j == 0) ? 0u : ((j - 1 < WW) ? W[j - 1] : sext);
+    const uint32_t v = ob ? ((cur << ob) | (prev >> (32 - ob))) : cur;
+    const uint64_t s = (uint64_t)acc.w[i] + v + carry;
+    acc.w[i] = (uint32_t)s;
+    carry = (uint32_t)(s >> 32);
+  }
+}
+
+// Big magnitude: value = (-1)^s * m * 2^lsb, m over BIGW words.
+struct bigmag {
+  uint32_t m[BIGW];
+  int lsb;
+  uint32_t s;
+};
+
+QB_HD void big_zero(bigmag &b) {
+  for (int i = 0; i < BIGW; ++i) b.m[i] = 0;
+  b.lsb = 0;
+  b.s = 0;
+}
+QB_HD int big_msb(const bigmag &b) {
+  for (int i = BIGW - 1; i >= 0; --i)
+    if (b.m[i]) return i * 32 + 31 - clz64((uint64_t)b.m[i]) + 32;
+  return -1;
+}
+QB_HD uint32_t big_bit(const bigmag &b, int pos) {
+  if (pos < 0 || pos >= BIGW * 32) return 0;
+  return (b.m[pos >> 5] >> (pos & 31)) & 1u;
+}
+QB_HD bool big_any_below(const bigmag &b, int pos) {
+  if (pos <= 0) return false;
+  const int wi = pos >> 5, bi = pos & 31;
+  for (int i = 0; i < wi && i < BIGW; ++i)
+    if (b.m[i]) return true;
+  if (wi < BIGW && bi && (b.m[wi] & ((1u << bi) - 1u))) return true;
+  return false;
+}
+// 32 bits of b starting at bit pos
+QB_HD uint32_t big_get32(const bigmag &b, int pos) {
+  const int wi = pos >> 5, bi = pos & 31;
+  const uint32_t lo = (wi >= 0 && wi < BIGW) ? b.m[wi] : 0u;
+  const uint32_t hi = (wi + 1 >= 0 && wi + 1 < BIGW) ? b.m[wi + 1] : 0u;
+  return bi ? ((lo >> bi) | (hi << (32 - bi))) : lo;
+}
+
+// Round b to binary128 (round to nearest even), with subnormal and overflow handling.  The
+// magnitude is exact, or rounded to odd at bit 0 with bit 0 at least 2 positions below the
+// rounding position.
+QB_HD q128 big_round(const bigmag &b) {
+  const int msb = big_msb(b);
+  if (msb < 0) return q_zero(b.s);
+  int e = b.lsb + msb;
+  int keep = msb - 112;
+  if (e < EMIN) keep += EMIN - e;
+  // gather 113 bits [keep, keep+112]
+  uint64_t sl = ((uint64_t)big_get32(b, keep + 32) << 32) | big_get32(b, keep);
+  uint64_t sh = ((uint64_t)big_get32(b, keep + 96) << 32) | big_get32(b, keep + 64);
+  sh &= (1ull << 49) - 1;
+  if (keep + 112 > msb) {
+    // fewer than 113 bits kept (subnormal range): bits above msb are zero already
+  }
+  const uint32_t guard = big_bit(b, keep - 1);
+  const bool sticky = big_any_below(b, keep - 1);
+  const uint64_t inc = guard & ((sticky ? 1u : 0u) | (uint32_t)(sl & 1));
+  sl += inc;
+  sh += (sl == 0) & inc;
+  int ulp = b.lsb + keep;
+  if (sh >> 49) {
+    sh >>= 1;
+    ulp += 1;
+  }
+  if (sh >> 48) {
+    const int be = ulp + 112 + QBIAS;
+    if (be >= 0x7fff) return q_inf(b.s);
+    return q_make(sl, ((uint64_t)b.s << 63) | ((uint64_t)be << 48) | (sh & FRAC_HI_MASK));
+  }
+  return q_make(sl, ((uint64_t)b.s << 63) | sh);  // subnormal
+}
+
+// wide accumulator -> signed big magnitude
+QB_HD void wide_to_big(const wide_acc &acc, bigmag &b) {
+  big_zero(b);
+  b.lsb = acc.base;
+  const bool neg = (acc.w[ACCW - 1] >> 31) != 0;
+  b.s = neg ? 1u : 0u;
+  uint32_t carry = 1;
+  for (int i = 0; i < ACCW; ++i) {
+    uint32_t v = acc.w[i];
+    if (neg) {
+      const uint64_t t = (uint64_t)(~v) + carry;
+      v = (uint32_t)t;
+      carry = (uint32_t)(t >> 32);
+    }
+    b.m[i] = v;
+  }
+}
+
+// r = b * a.m (113-bit significand), exponent/sign adjusted; b must fit BIGW-4 words
+QB_HD void big_mul_q(bigmag &r, const bigmag &b, const uq &a) {
+  const uint32_t x[4] = {(uint32_t)a.ml, (uint32_t)(a.ml >> 32), (uint32_t)a.mh, (uint32_t)(a.mh >> 32)};
+  bigmag t;
+  big_zero(t);
+  for (int i = 0; i < BIGW - 4; ++i) {
+    const uint32_t bi = b.m[i];
+    if (!bi) continue;
+    uint64_t carry = 0;
+    for (int j = 0; j < 4; ++j) {
+      const uint64_t v = (uint64_t)bi * x[j] + t.m[i + j] + carry;
+      t.m[i + j] = (uint32_t)v;
+      carry = v >> 32;
+    }
+    for (int k = i + 4; carry && k < BIGW; ++k) {
+      const uint64_t v = (uint64_t)t.m[k] + carry;
+      t.m[k] = (uint32_t)v;
+      carry = v >> 32;
+    }
+  }
+  t.lsb = b.lsb + a.e - 112;
+  t.s = b.s ^ a.s;
+  r = t;
+}
+
+QB_HD bool big_is_zero(const bigmag &b) {
+  for (int i = 0; i < BIGW; ++i)
+    if (b.m[i]) return false;
+  return true;
+}
+
+// dst = src * 2^sh over BIGW words (sh may be negative: bits shifted below bit 0 are dropped and
+// reported by the return value).  Bits shifted above the top word are dropped (callers size the
+// frame so that never happens).
+QB_HD bool big_shift(bigmag &dst, const bigmag &src, int sh) {
+  bool lost = false;
+  for (int i = 0; i < BIGW; ++i) dst.m[i] = 0;
+  if (sh >= 0) {
+    const int ws = sh >> 5, bs = sh & 31;
+    for (int i = BIGW - 1; i >= ws; --i) {
+      const int j = i - ws;
+      const uint32_t cur = src.m[j];
+      const uint32_t prev = (j > 0) ? src.m[j - 1] : 0u;
+      dst.m[i] = bs ? ((cur << bs) | (prev >> (32 - bs))) : cur;
+    }
+  } else {
+    const int r = -sh;
+    const int ws = r >> 5, bs = r & 31;
+    for (int i = 0; i < BIGW; ++i) {
+      const int j = i + ws;
+      const uint32_t cur = (j < BIGW) ? src.m[j] : 0u;
+      const uint32_t next = (j + 1 < BIGW) ? src.m[j + 1] : 0u;
+      dst.m[i] = bs ? ((cur >> bs) | (next << (32 - bs))) : cur;
+    }
+    for (int i = 0; i < ws && i < BIGW; ++i)
+      if (src.m[i]) lost = true;
+    if (ws < BIGW && bs && (src.m[ws] & ((1u << bs) - 1u))) lost = true;
+  }
+  return lost;
+}
+
+// Correctly rounded (X + Y) for two exact big magnitudes.  The larger operand is placed with its
+// leading bit at frame bit BIGW*32-2 (exactly: its width is at most BIGW*32-2 bits); the smaller
+// may lose low bits, which then act as a sticky: the difference is formed as larger - truncated
+// smaller - 1 and bit 0 is set (the exact result rounded to odd at bit 0, far below the rounding
+// position), which keeps the final round-to-nearest-even correct.
+QB_HD q128 big_add_round(const bigmag &X, const bigmag &Y) {
+  if (big_is_zero(Y)) return big_round(X);
+  if (big_is_zero(X)) return big_round(Y);
+  const int mx = big_msb(X), my = big_msb(Y);
+  const int tx = X.lsb + mx, ty = Y.lsb + my;
+  const bool xl = tx >= ty;
+  const bigmag &L = xl ? X : Y;
+  const bigmag &S = xl ? Y : X;
+  const int ml = xl ? mx : my;
+  const int shL = (BIGW * 32 - 2) - ml;
+  const int flsb = L.lsb - shL;
+  bigmag F, G;
+  big_shift(F, L, shL);
+  const bool lostS = big_shift(G, S, S.lsb - flsb);
+  bigmag R;
+  R.lsb = flsb;
+  if (L.s == S.s) {
+    uint64_t c = 0;
+    for (int i = 0; i < BIGW; ++i) {
+      const uint64_t v = (uint64_t)F.m[i] + G.m[i] + c;
+      R.m[i] = (uint32_t)v;
+      c = v >> 32;
+    }
+    R.s = L.s;
+  } else {
+    int cmp = 0;
+    for (int i = BIGW - 1; i >= 0 && !cmp; --i)
+      if (F.m[i] != G.m[i]) cmp = F.m[i] > G.m[i] ? 1 : -1;
+    if (cmp == 0 && !lostS) return q_zero(0);  // exact cancellation: +0
+    if (cmp < 0) {
+      // only possible without loss (the smaller operand lost bits only when far below)
+      uint64_t br = 0;
+      for (int i = 0; i < BIGW; ++i) {
+        const uint64_t v = (uint64_t)G.m[i] - F.m[i] - br;
+        R.m[i] = (uint32_t)v;
+        br = (v >> 63) & 1u;
+      }
+      R.s = S.s;
+    } else {
+      uint64_t br = lostS ? 1u : 0u;
+      for (int i = 0; i < BIGW; ++i) {
+        const uint64_t v = (uint64_t)F.m[i] - G.m[i] - br;
+        R.m[i] = (uint32_t)v;
+        br = (v >> 63) & 1u;
+      }
+      R.s = L.s;
+    }
+  }
+  if (lostS) R.m[0] |= 1u;
+  return big_round(R);
+}
+
+// value of a sum that met Inf/NaN products (IEEE rules for the exact sum)
+QB_HD q128 special_result(uint32_t flags) {
+  if ((flags & FL_NAN) || ((flags & FL_PINF) && (flags & FL_NINF))) return q_nan();
+  if (flags & FL_PINF) return q_inf(0);
+  return q_inf(1);
+}
+
+// Combine a row's / the dot's exact sum S with alpha, beta, y:  RN(alpha*S + beta*y).
+// has_ab = false means the plain rounded S (dot).
+QB_HD q128 finish_value(const wide_acc &acc, uint32_t flags, bool has_ab, q128 alpha, q128 beta, q128 yv) {
+  const bool sflag = (flags & (FL_NAN | FL_PINF | FL_NINF)) != 0;
+  bigmag S;
+  if (acc.any) {
+    wide_to_big(acc, S);
+  } else {
+    big_zero(S);
+  }
+  if (!has_ab) {
+    if (sflag) return special_result(flags);
+    return acc.any ? big_round(S) : q_zero(0);
+  }
+  const bool abspecial = (q_bexp(alpha) == 0x7fffu) || (q_bexp(beta) == 0x7fffu) || (q_bexp(yv) == 0x7fffu);
+  if (sflag || abspecial) {
+    // IEEE special rules through the packed ops: the result is NaN/Inf whatever the rounding
+    const q128 sq = sflag ? special_result(flags) : (acc.any ? big_round(S) : q_zero(0));
+    return fma_packed(alpha, sq, mul_packed(beta, yv));
+  }
+  bigmag T1, T2;
+  big_zero(T1);
+  big_zero(T2);
+  const uq ua = unpack_finite(alpha), ub = unpack_finite(beta), uy = unpack_finite(yv);
+  if (acc.any && (ua.mh | ua.ml)) big_mul_q(T1, S, ua);
+  if ((ub.mh | ub.ml) && (uy.mh | uy.ml)) {
+    bigmag Y;
+    big_zero(Y);
+    Y.m[0] = (uint32_t)uy.ml;
+    Y.m[1] = (uint32_t)(uy.ml >> 32);
+    Y.m[2] = (uint32_t)uy.mh;
+    Y.m[3] = (uint32_t)(uy.mh >> 32);
+    Y.lsb = uy.e - 112;
+    Y.s = uy.s;
+    big_mul_q(T2, Y, ub);
+  }
+  const bool z1 = big_is_zero(T1), z2 = big_is_zero(T2);
+  if (z1 && z2) {
+    // exact zero: IEEE sign of (alpha*S) + (beta*y) with zero terms
+    const uint32_t s1 = (alpha.hi >> 63) ^ (acc.any ? S.s : 0u);
+    const uint32_t s2 = (uint32_t)((beta.hi ^ yv.hi) >> 63);
+    return q_zero(s1 & s2);
+  }
+  return big_add_round(T1, T2);
+}
+
+}  // namespace qb

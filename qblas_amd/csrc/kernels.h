@@ -20,4 +20,23 @@ hipError_t launch_gemm_ref(int colmajor, int64_t m, int64_t n, int64_t k, int64_
                            hipStream_t st);
 hipError_t launch_zero(q128 *p, int64_t n, hipStream_t st);
 hipError_t launch_fill(int kind, uint64_t seed, int64_t n, int lo_exp, int hi_exp, q128 *out, hipStream_t st);
+
+// double-double form (kernels_dd.hip); ws sizes from the *_ws functions
+size_t dot_dd_ws(int64_t n);
+hipError_t launch_dot_dd(const q128 *x, int64_t incx, const q128 *y, int64_t incy, int64_t n, void *ws, q128 *out,
+                         hipStream_t st);
+size_t gemv_dd_ws(int colmajor, int64_t m, int64_t n);
+hipError_t launch_gemv_dd(int colmajor, int64_t m, int64_t n, q128 alpha, const q128 *A, int64_t lda,
+                          const q128 *x, int64_t incx, q128 beta, q128 *y, int64_t incy, void *ws,
+                          hipStream_t st);
+
+// exact (correctly rounded) form (kernels_exact.hip); ws[0] (u32) is the status word:
+// bit 0 set when a product fell outside its window (the output is then NaN)
+size_t dot_exact_ws(int64_t n);
+hipError_t launch_dot_exact(const q128 *x, int64_t incx, const q128 *y, int64_t incy, int64_t n, void *ws,
+                            q128 *out, hipStream_t st);
+size_t gemv_exact_ws(int colmajor, int64_t m, int64_t n);
+hipError_t launch_gemv_exact(int colmajor, int64_t m, int64_t n, q128 alpha, const q128 *A, int64_t lda,
+                             const q128 *x, int64_t incx, q128 beta, q128 *y, int64_t incy, void *ws,
+                             hipStream_t st);
 }  // namespace qbk

@@ -1,0 +1,240 @@
+// kernels_dd.hip -- double-double (FP64 FMA) forms of dot and gemv.
+//
+// Same routines as the reference (include/quadblas/algorithms/level1.hpp:80-137,
+// level2.hpp:15-99), evaluated in double-double: inputs are split to hi+lo (106 bits), products
+// and sums use error-free transforms (dd.hpp).  The summation order is a fixed tree (per-lane
+// strided partial sums, then wave/block/grid reductions in a fixed order), so results are
+// deterministic for a given size.  These kernels stream A (and x, y for dot) once with coalesced
+// 128-bit loads; they are HBM-bound.
+#include "dd.hpp"
+#include "kernels.h"
+
+using namespace qb;
+
+namespace qbk {
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+// streamed operand: read once, no reuse -> nontemporal
+__device__ __forceinline__ q128 ldq(const q128 *p) {
+  const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
+  return q_make(((uint64_t)v.y << 32) | v.x, ((uint64_t)v.w << 32) | v.z);
+}
+__device__ __forceinline__ q128 ldq_cached(const q128 *p) {
+  const uint4 v = *reinterpret_cast<const uint4 *>(p);
+  return q_make(((uint64_t)v.y << 32) | v.x, ((uint64_t)v.w << 32) | v.z);
+}
+__device__ __forceinline__ void stq(q128 *p, q128 v) {
+  uint4 w;
+  w.x = (uint32_t)v.lo;
+  w.y = (uint32_t)(v.lo >> 32);
+  w.z = (uint32_t)v.hi;
+  w.w = (uint32_t)(v.hi >> 32);
+  *reinterpret_cast<uint4 *>(p) = w;
+}
+__device__ __forceinline__ dd ld_dd(const dd *p) {
+  const double2 v = *reinterpret_cast<const double2 *>(p);
+  dd r;
+  r.hi = v.x;
+  r.lo = v.y;
+  return r;
+}
+__device__ __forceinline__ void st_dd(dd *p, dd v) {
+  double2 w;
+  w.x = v.hi;
+  w.y = v.lo;
+  *reinterpret_cast<double2 *>(p) = w;
+}
+
+__device__ __forceinline__ dd shfl_xor_dd(dd v, int m) {
+  dd r;
+  r.hi = __shfl_xor(v.hi, m);
+  r.lo = __shfl_xor(v.lo, m);
+  return r;
+}
+// fixed-order wave reduction (butterfly): every lane ends with the same value
+__device__ __forceinline__ dd wave_sum_dd(dd v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = dd_add(v, shfl_xor_dd(v, m));
+  return v;
+}
+// block of BS threads -> lane 0 of wave 0 holds the block sum
+template <int BS>
+__device__ __forceinline__ dd block_sum_dd(dd v) {
+  __shared__ double2 red[BS / 64];
+  v = wave_sum_dd(v);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) red[w] = make_double2(v.hi, v.lo);
+  __syncthreads();
+  dd s = dd_zero();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < BS / 64; ++i) {
+      dd t;
+      t.hi = red[i].x;
+      t.lo = red[i].y;
+      s = dd_add(s, t);
+    }
+  }
+  return s;
+}
+
+// ------------------------------------------------------------------------------------------
+// dot
+// ------------------------------------------------------------------------------------------
+constexpr int DOT_BS = 256;
+
+__global__ void __launch_bounds__(DOT_BS) k_dot_dd_partial(const q128 *__restrict__ x, int64_t incx,
+                                                           const q128 *__restrict__ y, int64_t incy, int64_t n,
+                                                           dd *__restrict__ part) {
+  const int64_t stride = (int64_t)gridDim.x * DOT_BS;
+  int64_t i = (int64_t)blockIdx.x * DOT_BS + threadIdx.x;
+  dd acc0 = dd_zero(), acc1 = dd_zero();
+  if (incx == 1 && incy == 1) {
+    for (; i + stride < n; i += 2 * stride) {
+      const q128 a0 = ldq(x + i), b0 = ldq(y + i), a1 = ldq(x + i + stride), b1 = ldq(y + i + stride);
+      acc0 = dd_fma(q_to_dd(a0), q_to_dd(b0), acc0);
+      acc1 = dd_fma(q_to_dd(a1), q_to_dd(b1), acc1);
+    }
+    if (i < n) acc0 = dd_fma(q_to_dd(ldq(x + i)), q_to_dd(ldq(y + i)), acc0);
+  } else {
+    for (; i < n; i += stride) acc0 = dd_fma(q_to_dd(ldq(x + i * incx)), q_to_dd(ldq(y + i * incy)), acc0);
+  }
+  const dd s = block_sum_dd<DOT_BS>(dd_add(acc0, acc1));
+  if (threadIdx.x == 0) st_dd(part + blockIdx.x, s);
+}
+
+__global__ void __launch_bounds__(256) k_dot_dd_final(const dd *__restrict__ part, int nparts,
+                                                      q128 *__restrict__ out) {
+  dd acc = dd_zero();
+  for (int i = threadIdx.x; i < nparts; i += 256) acc = dd_add(acc, ld_dd(part + i));
+  const dd s = block_sum_dd<256>(acc);
+  if (threadIdx.x == 0) stq(out, dd_to_q(s));
+}
+
+// ------------------------------------------------------------------------------------------
+// gemv
+// ------------------------------------------------------------------------------------------
+
+// xs[j] = dd(scale * x_j) with the scale applied in binary128 first (ColMajor pre-rounds
+// alpha*x_j like level2.hpp:71; RowMajor uses scale = 1).
+__global__ void k_vec_to_dd(int64_t n, q128 scale, int use_scale, const q128 *__restrict__ x, int64_t incx,
+                            dd *__restrict__ xs) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  q128 v = ldq_cached(x + j * incx);
+  if (use_scale) v = mul_packed(scale, v);
+  st_dd(xs + j, q_to_dd(v));
+}
+
+// RowMajor: one wave per row, lanes stride along the row (coalesced), dd partial per lane,
+// butterfly wave sum, then y_i = alpha*sum + beta*y_i in binary128.
+constexpr int GV_BS = 256;
+__global__ void __launch_bounds__(GV_BS) k_gemv_row_dd(int64_t m, int64_t n, q128 alpha, const q128 *__restrict__ A,
+                                                       int64_t lda, const dd *__restrict__ xs, q128 beta,
+                                                       q128 *__restrict__ y, int64_t incy) {
+  const int64_t row = (int64_t)blockIdx.x * (GV_BS / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= m) return;  // whole waves exit together
+  const q128 *r = A + row * lda;
+  dd acc0 = dd_zero(), acc1 = dd_zero();
+  int64_t j = lane;
+  for (; j + 64 < n; j += 128) {
+    const q128 a0 = ldq(r + j), a1 = ldq(r + j + 64);
+    const dd x0 = ld_dd(xs + j), x1 = ld_dd(xs + j + 64);
+    acc0 = dd_fma(q_to_dd(a0), x0, acc0);
+    acc1 = dd_fma(q_to_dd(a1), x1, acc1);
+  }
+  if (j < n) acc0 = dd_fma(q_to_dd(ldq(r + j)), ld_dd(xs + j), acc0);
+  const dd s = wave_sum_dd(dd_add(acc0, acc1));
+  if (lane == 0) {
+    q128 *yi = y + row * incy;
+    stq(yi, fma_packed(alpha, dd_to_q(s), mul_packed(beta, ldq_cached(yi))));
+  }
+}
+
+// ColMajor: lane = row, blockIdx.y = column chunk; partial dd sums per (chunk, row).
+constexpr int GC_BS = 256;
+__global__ void __launch_bounds__(GC_BS) k_gemv_col_dd_partial(int64_t m, int64_t n, int64_t chunk,
+                                                               const q128 *__restrict__ A, int64_t lda,
+                                                               const dd *__restrict__ axs, dd *__restrict__ part) {
+  const int64_t i = (int64_t)blockIdx.x * GC_BS + threadIdx.x;
+  const int64_t j0 = (int64_t)blockIdx.y * chunk;
+  const int64_t j1 = (j0 + chunk < n) ? j0 + chunk : n;
+  if (i >= m) return;
+  dd acc0 = dd_zero(), acc1 = dd_zero();
+  int64_t j = j0;
+  for (; j + 1 < j1; j += 2) {
+    const q128 a0 = ldq(A + j * lda + i), a1 = ldq(A + (j + 1) * lda + i);
+    acc0 = dd_fma(q_to_dd(a0), ld_dd(axs + j), acc0);
+    acc1 = dd_fma(q_to_dd(a1), ld_dd(axs + j + 1), acc1);
+  }
+  if (j < j1) acc0 = dd_fma(q_to_dd(ldq(A + j * lda + i)), ld_dd(axs + j), acc0);
+  st_dd(part + blockIdx.y * m + i, dd_add(acc0, acc1));
+}
+__global__ void k_gemv_col_dd_final(int64_t m, int nchunks, const dd *__restrict__ part, q128 beta,
+                                    q128 *__restrict__ y, int64_t incy) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  dd s = dd_zero();
+  for (int c = 0; c < nchunks; ++c) s = dd_add(s, ld_dd(part + (int64_t)c * m + i));
+  q128 *yi = y + i * incy;
+  stq(yi, add_packed(mul_packed(beta, ldq_cached(yi)), dd_to_q(s)));
+}
+
+// ------------------------------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------------------------------
+int dot_dd_blocks(int64_t n) {
+  int64_t b = (n + DOT_BS * 8 - 1) / (DOT_BS * 8);
+  if (b < 1) b = 1;
+  if (b > 2048) b = 2048;
+  return (int)b;
+}
+size_t dot_dd_ws(int64_t n) { return (size_t)dot_dd_blocks(n) * sizeof(dd); }
+
+hipError_t launch_dot_dd(const q128 *x, int64_t incx, const q128 *y, int64_t incy, int64_t n, void *ws, q128 *out,
+                         hipStream_t st) {
+  const int blocks = dot_dd_blocks(n);
+  dd *part = (dd *)ws;
+  hipLaunchKernelGGL(k_dot_dd_partial, dim3(blocks), dim3(DOT_BS), 0, st, x, incx, y, incy, n, part);
+  hipLaunchKernelGGL(k_dot_dd_final, dim3(1), dim3(256), 0, st, part, blocks, out);
+  return hipGetLastError();
+}
+
+int gemv_col_chunks(int64_t m, int64_t n) {
+  const int64_t rowblocks = (m + GC_BS - 1) / GC_BS;
+  int64_t c = (2048 + rowblocks - 1) / rowblocks;  // aim for >= 2048 blocks
+  if (c > n) c = n;
+  if (c < 1) c = 1;
+  if (c > 1024) c = 1024;
+  return (int)c;
+}
+size_t gemv_dd_ws(int colmajor, int64_t m, int64_t n) {
+  size_t ws = (size_t)n * sizeof(dd);
+  if (colmajor) ws += (size_t)gemv_col_chunks(m, n) * (size_t)m * sizeof(dd);
+  return ws;
+}
+
+hipError_t launch_gemv_dd(int colmajor, int64_t m, int64_t n, q128 alpha, const q128 *A, int64_t lda,
+                          const q128 *x, int64_t incx, q128 beta, q128 *y, int64_t incy, void *ws,
+                          hipStream_t st) {
+  dd *xs = (dd *)ws;
+  hipLaunchKernelGGL(k_vec_to_dd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, alpha, colmajor, x, incx,
+                     xs);
+  if (!colmajor) {
+    hipLaunchKernelGGL(k_gemv_row_dd, dim3((unsigned)((m + GV_BS / 64 - 1) / (GV_BS / 64))), dim3(GV_BS), 0, st, m,
+                       n, alpha, A, lda, xs, beta, y, incy);
+  } else {
+    const int nch = gemv_col_chunks(m, n);
+    const int64_t chunk = (n + nch - 1) / nch;
+    const int nch_eff = (int)((n + chunk - 1) / chunk);
+    dd *part = xs + n;
+    hipLaunchKernelGGL(k_gemv_col_dd_partial, dim3((unsigned)((m + GC_BS - 1) / GC_BS), (unsigned)nch_eff),
+                       dim3(GC_BS), 0, st, m, n, chunk, A, lda, xs, part);
+    hipLaunchKernelGGL(k_gemv_col_dd_final, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, m, nch_eff, part,
+                       beta, y, incy);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace qbk

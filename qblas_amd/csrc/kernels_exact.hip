@@ -1,0 +1,388 @@
+// kernels_exact.hip -- EXACT form of dot and gemv (correctly rounded results).
+//
+// Computes the same routines as the reference (include/quadblas/algorithms/level1.hpp:80-137,
+// level2.hpp:15-99) but returns RN(alpha * sum_j a_j b_j + beta * y), the exact value rounded once
+// (exact.hpp).  Kernels:
+//   k_dot_exact_partial    grid-stride lanes, wave-uniform window anchor, wave reduction,
+//                          one partial window per wave.
+//   k_gemv_row_exact       one wave per (row, column chunk), wave-uniform anchor, wave reduction.
+//   k_gemv_col_exact       one lane per row, blockIdx.y = column chunk, per-lane anchor.
+//   k_exact_finish_dot     combines all wave partials (common base) and rounds.
+//   k_exact_finish_rows    one thread per row: combines the row's chunk partials, forms
+//                          alpha*S + beta*y exactly and rounds into y.
+// Loads of the streamed operands are coalesced 128-bit loads.  If any product falls outside its
+// window the result is not exact: the output is set to NaN and bit 0 of the status word is set
+// (the host entry points report it as an error).
+#include "exact.hpp"
+#include "kernels.h"
+
+using namespace qb;
+
+namespace qbk {
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ q128 ldq_nt(const q128 *p) {
+  const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
+  return q_make(((uint64_t)v.y << 32) | v.x, ((uint64_t)v.w << 32) | v.z);
+}
+__device__ __forceinline__ q128 ldq_c(const q128 *p) {
+  const uint4 v = *reinterpret_cast<const uint4 *>(p);
+  return q_make(((uint64_t)v.y << 32) | v.x, ((uint64_t)v.w << 32) | v.z);
+}
+__device__ __forceinline__ void stq_c(q128 *p, q128 v) {
+  uint4 w;
+  w.x = (uint32_t)v.lo;
+  w.y = (uint32_t)(v.lo >> 32);
+  w.z = (uint32_t)v.hi;
+  w.w = (uint32_t)(v.hi >> 32);
+  *reinterpret_cast<uint4 *>(p) = w;
+}
+
+struct part_t {
+  int32_t A;
+  uint32_t flags;
+  uint32_t W[WW];
+  uint32_t pad[2];
+};
+static_assert(sizeof(part_t) == 64, "partial record is one 64-byte line");
+
+constexpr int EXP_NONE = 0x3fffffff;
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = min(v, __shfl_xor(v, m));
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m));
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v |= (uint32_t)__shfl_xor((int)v, m);
+  return v;
+}
+__device__ __forceinline__ void wave_sum_win(uint32_t (&W)[WW]) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    uint32_t X[WW];
+#pragma unroll
+    for (int i = 0; i < WW; ++i) X[i] = (uint32_t)__shfl_xor((int)W[i], m);
+    win_add_win(W, X);
+  }
+}
+
+// product lsb exponent of a pair, or EXP_NONE if the pair is not a regular nonzero finite product
+__device__ __forceinline__ int pair_exp(q128 a, q128 b) {
+  uq ua, ub;
+  uint32_t fl = 0;
+  const int c = classify_pair(a, b, ua, ub, fl);
+  if (c == 0 || c == 3) return prod_lsb(ua, ub);
+  return EXP_NONE;
+}
+
+// wave-uniform anchor centred between the smallest and largest first products of the wave
+__device__ __forceinline__ int wave_anchor(int e0) {
+  const int mn = wave_min(e0);
+  const int mx = wave_max(e0 == EXP_NONE ? -EXP_NONE : e0);
+  if (mn == EXP_NONE) return 0;
+  return ((mn + mx) >> 1) - WIN2_CENTER;
+}
+
+__device__ __forceinline__ void accumulate(uint32_t (&Wl)[WW], uint32_t (&Wh)[WW], int A, q128 a, q128 b,
+                                           uint32_t &flags) {
+  uq ua, ub;
+  const int c = classify_pair(a, b, ua, ub, flags);
+  if (c == 0 || c == 3) {
+    if (!win2_add_product(Wl, Wh, A, ua, ub)) flags |= FL_LOST;
+  }
+}
+__device__ __forceinline__ void put_parts(part_t *p, int A, uint32_t flags, const uint32_t (&Wl)[WW],
+                                          const uint32_t (&Wh)[WW]) {
+  p[0].A = A;
+  p[0].flags = flags;
+  p[1].A = A + WIN_BAND;
+  p[1].flags = 0;
+#pragma unroll
+  for (int k = 0; k < WW; ++k) {
+    p[0].W[k] = Wl[k];
+    p[1].W[k] = Wh[k];
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// dot
+// ------------------------------------------------------------------------------------------
+constexpr int XD_BS = 256;
+
+__global__ void __launch_bounds__(XD_BS) k_dot_exact_partial(const q128 *__restrict__ x, int64_t incx,
+                                                             const q128 *__restrict__ y, int64_t incy, int64_t n,
+                                                             part_t *__restrict__ parts) {
+  const int64_t stride = (int64_t)gridDim.x * XD_BS;
+  const int64_t i0 = (int64_t)blockIdx.x * XD_BS + threadIdx.x;
+  // anchor from the first element of every lane of the wave
+  int e0 = EXP_NONE;
+  if (i0 < n) e0 = pair_exp(ldq_c(x + i0 * incx), ldq_c(y + i0 * incy));
+  const int A = wave_anchor(e0);
+  uint32_t Wl[WW], Wh[WW];
+#pragma unroll
+  for (int k = 0; k < WW; ++k) {
+    Wl[k] = 0;
+    Wh[k] = 0;
+  }
+  uint32_t flags = 0;
+  int64_t i = i0;
+  if (incx == 1 && incy == 1) {
+    for (; i + stride < n; i += 2 * stride) {
+      const q128 a0 = ldq_nt(x + i), b0 = ldq_nt(y + i);
+      const q128 a1 = ldq_nt(x + i + stride), b1 = ldq_nt(y + i + stride);
+      accumulate(Wl, Wh, A, a0, b0, flags);
+      accumulate(Wl, Wh, A, a1, b1, flags);
+    }
+    if (i < n) accumulate(Wl, Wh, A, ldq_nt(x + i), ldq_nt(y + i), flags);
+  } else {
+    for (; i < n; i += stride) accumulate(Wl, Wh, A, ldq_nt(x + i * incx), ldq_nt(y + i * incy), flags);
+  }
+  wave_sum_win(Wl);
+  wave_sum_win(Wh);
+  flags = wave_or(flags);
+  if ((threadIdx.x & 63) == 0) put_parts(parts + 2 * ((int64_t)blockIdx.x * (XD_BS / 64) + (threadIdx.x >> 6)), A, flags, Wl, Wh);
+}
+
+// ------------------------------------------------------------------------------------------
+// gemv
+// ------------------------------------------------------------------------------------------
+constexpr int XR_BS = 256;
+
+// RowMajor: wave = (row, chunk).  Lanes stride the chunk's columns (coalesced).
+__global__ void __launch_bounds__(XR_BS) k_gemv_row_exact(int64_t m, int64_t n, int64_t chunk, int nch,
+                                                          const q128 *__restrict__ A, int64_t lda,
+                                                          const q128 *__restrict__ x, int64_t incx,
+                                                          part_t *__restrict__ parts) {
+  const int64_t wid = (int64_t)blockIdx.x * (XR_BS / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (wid >= m * nch) return;  // whole waves
+  const int64_t row = wid / nch;
+  const int c = (int)(wid - row * nch);
+  const int64_t j0 = (int64_t)c * chunk;
+  const int64_t j1 = (j0 + chunk < n) ? j0 + chunk : n;
+  const q128 *r = A + row * lda;
+  int e0 = EXP_NONE;
+  if (j0 + lane < j1) e0 = pair_exp(ldq_c(r + j0 + lane), ldq_c(x + (j0 + lane) * incx));
+  const int An = wave_anchor(e0);
+  uint32_t Wl[WW], Wh[WW];
+#pragma unroll
+  for (int k = 0; k < WW; ++k) {
+    Wl[k] = 0;
+    Wh[k] = 0;
+  }
+  uint32_t flags = 0;
+  int64_t j = j0 + lane;
+  if (incx == 1) {
+    for (; j + 64 < j1; j += 128) {
+      const q128 a0 = ldq_nt(r + j), a1 = ldq_nt(r + j + 64);
+      const q128 b0 = ldq_c(x + j), b1 = ldq_c(x + j + 64);
+      accumulate(Wl, Wh, An, a0, b0, flags);
+      accumulate(Wl, Wh, An, a1, b1, flags);
+    }
+    if (j < j1) accumulate(Wl, Wh, An, ldq_nt(r + j), ldq_c(x + j), flags);
+  } else {
+    for (; j < j1; j += 64) accumulate(Wl, Wh, An, ldq_nt(r + j), ldq_c(x + j * incx), flags);
+  }
+  wave_sum_win(Wl);
+  wave_sum_win(Wh);
+  flags = wave_or(flags);
+  if (lane == 0) put_parts(parts + 2 * ((int64_t)c * m + row), An, flags, Wl, Wh);
+}
+
+// ColMajor: lane = row i, blockIdx.y = column chunk; per-lane anchor.
+constexpr int XC_BS = 256;
+__global__ void __launch_bounds__(XC_BS) k_gemv_col_exact(int64_t m, int64_t n, int64_t chunk,
+                                                          const q128 *__restrict__ A, int64_t lda,
+                                                          const q128 *__restrict__ x, int64_t incx,
+                                                          part_t *__restrict__ parts) {
+  const int64_t i = (int64_t)blockIdx.x * XC_BS + threadIdx.x;
+  const int64_t j0 = (int64_t)blockIdx.y * chunk;
+  const int64_t j1 = (j0 + chunk < n) ? j0 + chunk : n;
+  if (i >= m) return;
+  // lane anchor: centred on the first regular product of the chunk (scan a few columns)
+  int e0 = EXP_NONE;
+  for (int64_t j = j0; j < j1 && j < j0 + 4 && e0 == EXP_NONE; ++j) e0 = pair_exp(ldq_c(A + j * lda + i), ldq_c(x + j * incx));
+  const int An = (e0 == EXP_NONE) ? 0 : e0 - WIN2_CENTER;
+  uint32_t Wl[WW], Wh[WW];
+#pragma unroll
+  for (int k = 0; k < WW; ++k) {
+    Wl[k] = 0;
+    Wh[k] = 0;
+  }
+  uint32_t flags = 0;
+  int64_t j = j0;
+  for (; j + 1 < j1; j += 2) {
+    const q128 a0 = ldq_nt(A + j * lda + i), a1 = ldq_nt(A + (j + 1) * lda + i);
+    const q128 b0 = ldq_c(x + j * incx), b1 = ldq_c(x + (j + 1) * incx);
+    accumulate(Wl, Wh, An, a0, b0, flags);
+    accumulate(Wl, Wh, An, a1, b1, flags);
+  }
+  if (j < j1) accumulate(Wl, Wh, An, ldq_nt(A + j * lda + i), ldq_c(x + j * incx), flags);
+  put_parts(parts + 2 * ((int64_t)blockIdx.y * m + i), An, flags, Wl, Wh);
+}
+
+// One block: combines all wave partials (two windows per wave) on a common base and rounds.
+__global__ void __launch_bounds__(256) k_exact_finish_dot(const part_t *__restrict__ parts, int nparts,
+                                                          q128 *__restrict__ out, uint32_t *__restrict__ status) {
+  // common base: the minimum anchor over nonzero partials
+  __shared__ int s_min[4];
+  __shared__ uint32_t s_fl[4];
+  int amin = EXP_NONE;
+  uint32_t fl = 0;
+  for (int p = threadIdx.x; p < nparts; p += 256) {
+    fl |= parts[p].flags;
+    if (!win_is_zero(parts[p].W)) amin = min(amin, parts[p].A);
+  }
+  amin = wave_min(amin);
+  fl = wave_or(fl);
+  if ((threadIdx.x & 63) == 0) {
+    s_min[threadIdx.x >> 6] = amin;
+    s_fl[threadIdx.x >> 6] = fl;
+  }
+  __syncthreads();
+  amin = min(min(s_min[0], s_min[1]), min(s_min[2], s_min[3]));
+  fl = s_fl[0] | s_fl[1] | s_fl[2] | s_fl[3];
+  wide_acc acc;
+  wide_init(acc);
+  if (amin != EXP_NONE) {
+    acc.any = true;
+    acc.base = amin - 32;
+  }
+  for (int p = threadIdx.x; p < nparts; p += 256) wide_add_window(acc, parts[p].W, parts[p].A);
+  // tree reduction of the 256 accumulators through LDS (same base everywhere)
+  __shared__ uint32_t red[256][ACCW + 1];
+  for (int k = 0; k < ACCW; ++k) red[threadIdx.x][k] = acc.w[k];
+  red[threadIdx.x][ACCW] = acc.lost ? 1u : 0u;
+  __syncthreads();
+  for (int s = 128; s >= 1; s >>= 1) {
+    if (threadIdx.x < s) {
+      uint32_t c = 0;
+      for (int k = 0; k < ACCW; ++k) {
+        const uint64_t v = (uint64_t)red[threadIdx.x][k] + red[threadIdx.x + s][k] + c;
+        red[threadIdx.x][k] = (uint32_t)v;
+        c = (uint32_t)(v >> 32);
+      }
+      red[threadIdx.x][ACCW] |= red[threadIdx.x + s][ACCW];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < ACCW; ++k) acc.w[k] = red[0][k];
+    acc.lost = red[0][ACCW] != 0;
+    if (acc.lost) fl |= FL_LOST;
+    q128 r;
+    if (fl & FL_LOST) {
+      r = q_nan();
+      atomicOr(status, 1u);
+    } else {
+      r = finish_value(acc, fl, false, q_one(), q_zero(0), q_zero(0));
+    }
+    stq_c(out, r);
+  }
+}
+
+// one thread per row: combine nch chunk partials, then RN(alpha*S + beta*y)
+__global__ void k_exact_finish_rows(int64_t m, int nch, const part_t *__restrict__ parts, q128 alpha, q128 beta,
+                                    q128 *__restrict__ y, int64_t incy, uint32_t *__restrict__ status) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  wide_acc acc;
+  wide_init(acc);
+  uint32_t fl = 0;
+  for (int c = 0; c < nch; ++c) {
+    const part_t *p = parts + 2 * ((int64_t)c * m + i);
+    fl |= p[0].flags | p[1].flags;
+    wide_add_window(acc, p[0].W, p[0].A);
+    wide_add_window(acc, p[1].W, p[1].A);
+  }
+  if (acc.lost) fl |= FL_LOST;
+  q128 *yi = y + i * incy;
+  q128 r;
+  if (fl & FL_LOST) {
+    r = q_nan();
+    atomicOr(status, 1u);
+  } else {
+    r = finish_value(acc, fl, true, alpha, beta, ldq_c(yi));
+  }
+  stq_c(yi, r);
+}
+
+// ------------------------------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------------------------------
+static int dot_exact_blocks(int64_t n) {
+  int64_t b = (n + XD_BS * 8 - 1) / (XD_BS * 8);
+  if (b < 1) b = 1;
+  if (b > 2048) b = 2048;
+  return (int)b;
+}
+static void gemv_exact_split(int colmajor, int64_t m, int64_t n, int &nch, int64_t &chunk) {
+  int64_t want;
+  if (!colmajor) {
+    // waves = m * nch >= ~8192, chunk >= 256 columns
+    want = (8192 + m - 1) / m;
+    const int64_t maxc = (n + 255) / 256;
+    if (want > maxc) want = maxc;
+  } else {
+    const int64_t rowblocks = (m + XC_BS - 1) / XC_BS;
+    want = (1024 + rowblocks - 1) / rowblocks;
+    const int64_t maxc = (n + 63) / 64;
+    if (want > maxc) want = maxc;
+  }
+  if (want < 1) want = 1;
+  if (want > 4096) want = 4096;
+  chunk = (n + want - 1) / want;
+  nch = (int)((n + chunk - 1) / chunk);
+}
+size_t dot_exact_ws(int64_t n) { return 256 + 2 * (size_t)dot_exact_blocks(n) * (XD_BS / 64) * sizeof(part_t); }
+size_t gemv_exact_ws(int colmajor, int64_t m, int64_t n) {
+  int nch;
+  int64_t chunk;
+  gemv_exact_split(colmajor, m, n, nch, chunk);
+  return 256 + 2 * (size_t)nch * (size_t)m * sizeof(part_t);
+}
+
+// workspace: [status (u32) | pad to 256 | partials]
+hipError_t launch_dot_exact(const q128 *x, int64_t incx, const q128 *y, int64_t incy, int64_t n, void *ws,
+                            q128 *out, hipStream_t st) {
+  uint32_t *status = (uint32_t *)ws;
+  part_t *parts = (part_t *)((char *)ws + 256);
+  const int blocks = dot_exact_blocks(n);
+  hipError_t e = launch_zero((q128 *)ws, 1, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_dot_exact_partial, dim3(blocks), dim3(XD_BS), 0, st, x, incx, y, incy, n, parts);
+  hipLaunchKernelGGL(k_exact_finish_dot, dim3(1), dim3(256), 0, st, parts, 2 * blocks * (XD_BS / 64), out, status);
+  return hipGetLastError();
+}
+
+hipError_t launch_gemv_exact(int colmajor, int64_t m, int64_t n, q128 alpha, const q128 *A, int64_t lda,
+                             const q128 *x, int64_t incx, q128 beta, q128 *y, int64_t incy, void *ws,
+                             hipStream_t st) {
+  uint32_t *status = (uint32_t *)ws;
+  part_t *parts = (part_t *)((char *)ws + 256);
+  int nch;
+  int64_t chunk;
+  gemv_exact_split(colmajor, m, n, nch, chunk);
+  hipError_t e = launch_zero((q128 *)ws, 1, st);
+  if (e != hipSuccess) return e;
+  if (!colmajor) {
+    const int64_t waves = m * nch;
+    hipLaunchKernelGGL(k_gemv_row_exact, dim3((unsigned)((waves + XR_BS / 64 - 1) / (XR_BS / 64))), dim3(XR_BS), 0,
+                       st, m, n, chunk, nch, A, lda, x, incx, parts);
+  } else {
+    hipLaunchKernelGGL(k_gemv_col_exact, dim3((unsigned)((m + XC_BS - 1) / XC_BS), (unsigned)nch), dim3(XC_BS), 0,
+                       st, m, n, chunk, A, lda, x, incx, parts);
+  }
+  hipLaunchKernelGGL(k_exact_finish_rows, dim3((unsigned)((m + 63) / 64)), dim3(64), 0, st, m, nch, parts, alpha,
+                     beta, y, incy, status);
+  return hipGetLastError();
+}
+
+}  // namespace qbk

@@ -229,10 +229,10 @@ __global__ void k_fill(int kind, uint64_t seed, int64_t n, int lo_exp, int hi_ex
     } else {
       v = full_uniform(seed, (uint64_t)i);
     }
-  } else {  // kind 3: y vector of the cancellation variant
+  } else {  // kind 3: y vector of the cancellation variant (own stream; equal values on each pair)
     const int64_t p0 = i & ~(int64_t)1;
     const int64_t p = ((p0 % 8) < 2) ? p0 : i;
-    v = full_uniform(seed, (uint64_t)p);
+    v = full_uniform(seed + 1, (uint64_t)p);
   }
   st(out + i, v);
 }

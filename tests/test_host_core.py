@@ -58,3 +58,18 @@ def test_host_side_state(Q):
     Q.set_mode("exact")
     assert Q.get_mode() == Q.MODE_EXACT
     Q.set_mode(m)
+
+
+def test_exact_pipeline_matches_oracle(O, tmp_path):
+    """Host run of the EXACT form's arithmetic (qblas_amd/csrc/exact.hpp) with the kernels' wave and
+    partial structure, bit-for-bit against the oracle's exact accumulator, incl. crafted ties that
+    only a far-away beta*y breaks and the signs of exact zeros."""
+    exe = tmp_path / "exact_host_check"
+    src = os.path.join(ROOT, "tests", "cpu", "exact_host_check.cpp")
+    odir = os.path.join(ROOT, "oracle")
+    subprocess.run(["g++", "-O2", "-std=gnu++17", "-o", str(exe), src, "-L" + odir, "-loracle",
+                    "-Wl,-rpath," + odir, "-lquadmath"], check=True)
+    for seed in (1, 2):
+        r = subprocess.run([str(exe), str(seed), "200"], capture_output=True, text=True)
+        assert r.returncode == 0, r.stdout[-3000:]
+        assert " mismatches 0" in r.stdout
