@@ -37,6 +37,8 @@ def parse():
     p.add_argument("--m", type=int, default=8192)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0 = OMP default)")
+    p.add_argument("--gemm", default="", help="also time qgemm 4096^3 in these forms (comma list)")
+    p.add_argument("--gemm-size", type=int, default=4096)
     return p.parse_args()
 
 
@@ -130,6 +132,10 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(n, m, T, args.cpu_threads)
 
+    gemm = {}
+    for gform in [f for f in args.gemm.split(",") if f]:
+        gemm[gform] = bench_gemm(Q, torch, dev, stream, Q.MODES[gform], args.gemm_size)
+
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(step_ms, 4), "higher_is_better": True, "scaling": "weak",
@@ -140,7 +146,33 @@ def main():
         "per_kernel": per_kernel, "roofline": roofline, "cpu_baseline": cpu,
         "wall_s": round(wall, 3),
     }
+    if gemm:
+        line["qgemm"] = gemm
     print(json.dumps(line))
+
+
+def bench_gemm(Q, torch, dev, stream, mode, N, reps=2):
+    """qgemm N^3 RowMajor, alpha = 1, beta = 0, entries with random sign, exponent uniform in
+    [-30, 30) and a full 113-bit mantissa.  Returns time and quad-equiv GFLOP/s (2 N^3 / t)."""
+    sh = stream.cuda_stream
+    A = torch.empty((N * N, 2), dtype=torch.int64, device=dev)
+    B = torch.empty((N * N, 2), dtype=torch.int64, device=dev)
+    C = torch.empty((N * N, 2), dtype=torch.int64, device=dev)
+    Q.dev_fill(Q.FILL_LOG_UNIFORM, 11, N * N, A.data_ptr(), -30, 30, stream=sh)
+    Q.dev_fill(Q.FILL_LOG_UNIFORM, 12, N * N, B.data_ptr(), -30, 30, stream=sh)
+    C.zero_()
+    Q.dev_gemm(mode, 0, N, N, N, 1.0, A.data_ptr(), N, B.data_ptr(), N, 0.0, C.data_ptr(), N, sh)  # warm-up
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(reps):
+        s_ev, e_ev = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s_ev.record(stream)
+        Q.dev_gemm(mode, 0, N, N, N, 1.0, A.data_ptr(), N, B.data_ptr(), N, 0.0, C.data_ptr(), N, sh)
+        e_ev.record(stream)
+        torch.cuda.synchronize()
+        times.append(s_ev.elapsed_time(e_ev))
+    ms = min(times)
+    return {"N": N, "ms": round(ms, 3), "quad_GFLOPs": round(2.0 * N ** 3 / (ms * 1e-3) / 1e9, 2)}
 
 
 def cpu_baseline(n, m, T, threads):

@@ -47,7 +47,7 @@ QB_HD void mul113_w(uint32_t (&p)[8], const uq &a, const uq &b) {
     uint64_t carry = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const uint64_t t = (uint64_t)x[i] * y[j] + p[i + j] + carry;
+      const uint64_t t = (uint64_t)x[i] * y[j] + ((uint64_t)p[i + j] + carry);  // < 2^64
       p[i + j] = (uint32_t)t;
       carry = t >> 32;
     }
@@ -58,22 +58,33 @@ QB_HD void mul113_w(uint32_t (&p)[8], const uq &a, const uq &b) {
 // product lsb exponent: value = P * 2^(a.e + b.e - 224)
 QB_HD int prod_lsb(const uq &a, const uq &b) { return a.e + b.e - 224; }
 
-// W += (-1)^(a.s^b.s) * a.m*b.m * 2^(e_p - A).  Returns false (and adds nothing) if the product
-// does not fit the window.
-QB_HD bool win_add_product(uint32_t (&W)[WW], int A, const uq &a, const uq &b) {
-  const int o = prod_lsb(a, b) - A;
-  if (o < WIN_OMIN || o > WIN_OMAX) return false;
+QB_HD uint32_t addc32(uint32_t a, uint32_t b, uint32_t cin, uint32_t &cout) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  unsigned int co;
+  const unsigned int r = __builtin_addc(a, b, cin, &co);
+  cout = co;
+  return r;
+#else
+  const uint64_t s = (uint64_t)a + b + cin;
+  cout = (uint32_t)(s >> 32);
+  return (uint32_t)s;
+#endif
+}
+
+// The product a.m*b.m shifted to bit offset o (1..126) of a 12-word frame, with the sign applied as
+// one's complement (X ^ mask) plus a carry-in (mask & 1): adding the returned words with that
+// carry-in adds (-1)^(a.s^b.s) * a.m*b.m * 2^o.
+QB_HD void place_product(uint32_t (&X)[WW], uint32_t &mask, int o, const uq &a, const uq &b) {
   uint32_t P[8];
   mul113_w(P, a, b);
-  const int w = (o - 1) >> 5;          // 0..3
+  const int w = (o - 1) >> 5;                          // 0..3
   const uint32_t sh = 32u - (uint32_t)(o - (w << 5));  // bit offset 1..32 -> funnel 31..0
   uint32_t T[9];
   T[0] = alignbit32(P[0], 0u, sh);
 #pragma unroll
   for (int j = 1; j < 8; ++j) T[j] = alignbit32(P[j], P[j - 1], sh);
   T[8] = alignbit32(0u, P[7], sh);
-  const uint32_t mask = 0u - ((a.s ^ b.s) & 1u);
-  uint32_t carry = mask & 1u;
+  mask = 0u - ((a.s ^ b.s) & 1u);
 #pragma unroll
   for (int i = 0; i < WW; ++i) {
     uint32_t v = 0;
@@ -82,11 +93,20 @@ QB_HD bool win_add_product(uint32_t (&W)[WW], int A, const uq &a, const uq &b) {
       const int j = i - k;
       if (j >= 0 && j < 9) v = (w == k) ? T[j] : v;
     }
-    v ^= mask;
-    const uint64_t s = (uint64_t)W[i] + v + carry;
-    W[i] = (uint32_t)s;
-    carry = (uint32_t)(s >> 32);
+    X[i] = v ^ mask;
   }
+}
+
+// W += (-1)^(a.s^b.s) * a.m*b.m * 2^(e_p - A).  Returns false (and adds nothing) if the product
+// does not fit the window.
+QB_HD bool win_add_product(uint32_t (&W)[WW], int A, const uq &a, const uq &b) {
+  const int o = prod_lsb(a, b) - A;
+  if (o < WIN_OMIN || o > WIN_OMAX) return false;
+  uint32_t X[WW], mask;
+  place_product(X, mask, o, a, b);
+  uint32_t c = mask & 1u;
+#pragma unroll
+  for (int i = 0; i < WW; ++i) W[i] = addc32(W[i], X[i], c, c);
   return true;
 }
 
@@ -96,18 +116,30 @@ QB_HD bool win_add_product(uint32_t (&W)[WW], int A, const uq &a, const uq &b) {
 constexpr int WIN_BAND = 124;
 constexpr int WIN2_CENTER = 125;
 QB_HD bool win2_add_product(uint32_t (&Wl)[WW], uint32_t (&Wh)[WW], int A, const uq &a, const uq &b) {
-  if (prod_lsb(a, b) - A > WIN_OMAX) return win_add_product(Wh, A + WIN_BAND, a, b);
-  return win_add_product(Wl, A, a, b);
+  int o = prod_lsb(a, b) - A;
+  const bool hi = o > WIN_OMAX;
+  if (hi) o -= WIN_BAND;
+  if (o < WIN_OMIN || o > WIN_OMAX) return false;
+  uint32_t X[WW], mask;
+  place_product(X, mask, o, a, b);
+  // lo window: always updated, the addend masked off for hi products (both windows stay in
+  // registers: no data-dependent choice between arrays)
+  const uint32_t keep = hi ? 0u : ~0u;
+  uint32_t c = mask & keep & 1u;
+#pragma unroll
+  for (int i = 0; i < WW; ++i) Wl[i] = addc32(Wl[i], X[i] & keep, c, c);
+  if (hi) {  // skipped by every wave whose products all fit the lo window
+    uint32_t ch = mask & 1u;
+#pragma unroll
+    for (int i = 0; i < WW; ++i) Wh[i] = addc32(Wh[i], X[i], ch, ch);
+  }
+  return true;
 }
 
 QB_HD void win_add_win(uint32_t (&W)[WW], const uint32_t (&X)[WW]) {
-  uint32_t carry = 0;
+  uint32_t c = 0;
 #pragma unroll
-  for (int i = 0; i < WW; ++i) {
-    const uint64_t s = (uint64_t)W[i] + X[i] + carry;
-    W[i] = (uint32_t)s;
-    carry = (uint32_t)(s >> 32);
-  }
+  for (int i = 0; i < WW; ++i) W[i] = addc32(W[i], X[i], c, c);
 }
 
 // Classify a packed operand pair for the streaming loop.  Returns 0 for a regular pair (both
@@ -452,4 +484,273 @@ QB_HD q128 finish_value(const wide_acc &acc, uint32_t flags, bool has_ab, q128 a
   return big_add_round(T1, T2);
 }
 
+
+// ------------------------------------------------------------------------------------------
+// register-resident epilogue: fixed-size word arrays, every index a compile-time constant
+// (variable shifts are done with selects), so nothing is spilled to scratch memory
+// ------------------------------------------------------------------------------------------
+
+// r = x << sh (N words, bits above the top dropped), 0 <= sh
+template <int N>
+QB_HD void rshl(uint32_t (&r)[N], const uint32_t (&x)[N], int sh) {
+  uint32_t t[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) t[i] = x[i];
+  const int ws = sh >> 5, bs = sh & 31;
+#pragma unroll
+  for (int step = 32; step >= 1; step >>= 1) {
+    if (step >= N) continue;
+    const bool on = (ws & step) != 0;
+#pragma unroll
+    for (int i = N - 1; i >= 0; --i) t[i] = on ? ((i >= step) ? t[i - step] : 0u) : t[i];
+  }
+  const bool big = ws >= N;
+#pragma unroll
+  for (int i = N - 1; i >= 0; --i) {
+    const uint32_t lo = (i > 0) ? t[i - 1] : 0u;
+    const uint32_t v = bs ? ((t[i] << bs) | (lo >> (32 - bs))) : t[i];
+    r[i] = big ? 0u : v;
+  }
+}
+
+// r = x >> sh (truncating), returns 1 if a nonzero bit was shifted out; any sh >= 0
+template <int N>
+QB_HD uint32_t rshr(uint32_t (&r)[N], const uint32_t (&x)[N], int sh) {
+  if (sh >= 32 * N) {
+    uint32_t any = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      any |= x[i];
+      r[i] = 0;
+    }
+    return any != 0;
+  }
+  const int ws = sh >> 5, bs = sh & 31;
+  uint32_t lost = 0;
+  // bits lost: words below ws and the low bs bits of word ws
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const uint32_t m = (i < ws) ? 0xffffffffu : ((i == ws && bs) ? (0xffffffffu >> (32 - bs)) : 0u);
+    lost |= x[i] & m;
+  }
+  uint32_t t[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) t[i] = x[i];
+#pragma unroll
+  for (int step = 32; step >= 1; step >>= 1) {
+    if (step >= N) continue;
+    const bool on = (ws & step) != 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) t[i] = on ? ((i + step < N) ? t[i + step] : 0u) : t[i];
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const uint32_t hi = (i + 1 < N) ? t[i + 1] : 0u;
+    r[i] = bs ? ((t[i] >> bs) | (hi << (32 - bs))) : t[i];
+  }
+  return lost != 0;
+}
+
+template <int N>
+QB_HD int rmsb(const uint32_t (&x)[N]) {
+  int m = -1;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    if (x[i]) m = i * 32 + 31 - (clz64((uint64_t)x[i]) - 32);
+  return m;
+}
+
+template <int N>
+QB_HD bool rzero(const uint32_t (&x)[N]) {
+  uint32_t o = 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) o |= x[i];
+  return o == 0;
+}
+
+// Round (-1)^s * M * 2^lsb to binary128 (nearest even, subnormals, overflow).  M is exact, or
+// rounded to odd at bit 0 with bit 0 at least two positions below the rounding position.
+template <int N>
+QB_HD q128 rround(const uint32_t (&M)[N], int lsb, uint32_t s) {
+  const int msb = rmsb<N>(M);
+  if (msb < 0) return q_zero(s);
+  const int e = lsb + msb;
+  int keep = msb - 112;
+  if (e < EMIN) keep += EMIN - e;
+  // bring bit keep-2 down to bit 0: bits 0,1 are round bits, 2..114 the significand
+  uint32_t T[N];
+  uint32_t sticky = 0;
+  if (keep - 2 >= 0) {
+    sticky = rshr<N>(T, M, keep - 2);
+  } else {
+    rshl<N>(T, M, 2 - keep);
+  }
+  uint64_t lo = ((uint64_t)T[1] << 32) | T[0];
+  uint64_t hi = (N > 3) ? (((uint64_t)T[3] << 32) | T[2]) : ((N > 2) ? T[2] : 0);
+  // significand = (hi:lo) >> 2 (113 bits), guard = bit 1, sticky = bit 0 | shifted-out bits
+  const uint32_t guard = (uint32_t)(lo >> 1) & 1u;
+  const uint32_t st = ((uint32_t)lo & 1u) | sticky;
+  uint64_t sl = (lo >> 2) | (hi << 62);
+  uint64_t sh = (hi >> 2) & ((1ull << 49) - 1);
+  const uint64_t inc = guard & (st | (uint32_t)(sl & 1));
+  sl += inc;
+  sh += (sl == 0) & inc;
+  int ulp = lsb + keep;
+  if (sh >> 49) {
+    sh >>= 1;
+    ulp += 1;
+  }
+  if (sh >> 48) {
+    const int be = ulp + 112 + QBIAS;
+    if (be >= 0x7fff) return q_inf(s);
+    return q_make(sl, ((uint64_t)s << 63) | ((uint64_t)be << 48) | (sh & FRAC_HI_MASK));
+  }
+  return q_make(sl, ((uint64_t)s << 63) | sh);
+}
+
+// r[0..N+3] = x[0..N-1] * a.m (113-bit significand)
+template <int N>
+QB_HD void rmul_q(uint32_t (&r)[N + 4], const uint32_t (&x)[N], const uq &a) {
+  const uint32_t y[4] = {(uint32_t)a.ml, (uint32_t)(a.ml >> 32), (uint32_t)a.mh, (uint32_t)(a.mh >> 32)};
+#pragma unroll
+  for (int i = 0; i < N + 4; ++i) r[i] = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    uint64_t carry = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const uint64_t t = (uint64_t)x[i] * y[j] + ((uint64_t)r[i + j] + carry);
+      r[i + j] = (uint32_t)t;
+      carry = t >> 32;
+    }
+    r[N + j] = (uint32_t)carry;
+  }
+}
+
+constexpr int SW = 24;         // words of |S| handled by the register epilogue
+constexpr int FW = SW + 4 + 12;  // combine frame for alpha*S + beta*y
+
+// Correctly rounded alpha*S + beta*y, with S = (-1)^sS * M * 2^lsb exact (sticky must be 0 when
+// alpha/beta are used; for the plain sum, sticky marks a nonzero tail below M, i.e. M is S
+// rounded to odd).  flags carry Inf/NaN products.
+QB_HD q128 finish_reg(const uint32_t (&M)[SW], int lsb, uint32_t sS, uint32_t sticky, uint32_t flags, bool has_ab,
+                      q128 alpha, q128 beta, q128 yv) {
+  const bool sflag = (flags & (FL_NAN | FL_PINF | FL_NINF)) != 0;
+  if (!has_ab) {
+    if (sflag) return special_result(flags);
+    uint32_t Mx[SW];
+#pragma unroll
+    for (int i = 0; i < SW; ++i) Mx[i] = M[i];
+    Mx[0] |= sticky;
+    return rround<SW>(Mx, lsb, sS);
+  }
+  const bool abspecial = (q_bexp(alpha) == 0x7fffu) || (q_bexp(beta) == 0x7fffu) || (q_bexp(yv) == 0x7fffu);
+  if (sflag || abspecial) {
+    const q128 sq = sflag ? special_result(flags) : rround<SW>(M, lsb, sS);
+    return fma_packed(alpha, sq, mul_packed(beta, yv));
+  }
+  const uq ua = unpack_finite(alpha), ub = unpack_finite(beta), uy = unpack_finite(yv);
+  // T1 = alpha * S  (SW + 4 words), T2 = beta * y (8 words)
+  uint32_t T1[SW + 4];
+  const bool z1 = rzero<SW>(M) || ((ua.mh | ua.ml) == 0);
+  if (!z1) {
+    rmul_q<SW>(T1, M, ua);
+  } else {
+#pragma unroll
+    for (int i = 0; i < SW + 4; ++i) T1[i] = 0;
+  }
+  const int e1 = lsb + ua.e - 112;
+  const uint32_t s1 = sS ^ ua.s;
+  uint32_t P2[8];
+  const bool z2 = ((ub.mh | ub.ml) == 0) || ((uy.mh | uy.ml) == 0);
+  if (!z2) mul113_w(P2, ub, uy);
+  const int e2 = ub.e + uy.e - 224;
+  const uint32_t s2 = ub.s ^ uy.s;
+  if (z1 && z2) return q_zero(((uint32_t)(alpha.hi >> 63) ^ sS) & s2);
+  uint32_t F1[FW], F2[FW];
+#pragma unroll
+  for (int i = 0; i < FW; ++i) {
+    F1[i] = (i < SW + 4) ? T1[i] : 0u;
+    F2[i] = (i < 8 && !z2) ? P2[i] : 0u;
+  }
+  if (z2) return rround<FW>(F1, e1, s1);
+  if (z1) return rround<FW>(F2, e2, s2);
+  // frame: the operand with the higher leading bit is placed with its leading bit at FW*32-2
+  const int m1 = rmsb<FW>(F1), m2 = rmsb<FW>(F2);
+  const bool one_big = (e1 + m1) >= (e2 + m2);
+  const int topL = one_big ? (e1 + m1) : (e2 + m2);
+  const int flsb = topL - (FW * 32 - 2);
+  uint32_t L[FW], S[FW];
+  rshl<FW>(L, one_big ? F1 : F2, (FW * 32 - 2) - (one_big ? m1 : m2));
+  const int es = one_big ? e2 : e1;
+  uint32_t lost = 0;
+  if (es >= flsb) {
+    rshl<FW>(S, one_big ? F2 : F1, es - flsb);
+  } else {
+    lost = rshr<FW>(S, one_big ? F2 : F1, flsb - es);
+  }
+  const uint32_t sL = one_big ? s1 : s2, sSm = one_big ? s2 : s1;
+  uint32_t R[FW];
+  uint32_t sR = sL;
+  if (sL == sSm) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < FW; ++i) R[i] = addc32(L[i], S[i], c, c);
+  } else {
+    // R = L - S - lost (two's complement); a borrow out means |S| > |L| (only without loss)
+    uint32_t c = lost ? 0u : 1u;
+#pragma unroll
+    for (int i = 0; i < FW; ++i) R[i] = addc32(L[i], ~S[i], c, c);
+    if (!c) {
+      uint32_t c2 = 1;
+#pragma unroll
+      for (int i = 0; i < FW; ++i) R[i] = addc32(~R[i], 0u, c2, c2);
+      sR = sSm;
+    }
+    if (!lost && rzero<FW>(R)) return q_zero(0);
+  }
+  R[0] |= lost;
+  return rround<FW>(R, flsb, sR);
+}
+
+
+// S of one output from its lo/hi window sums (same anchor A: hi is A + WIN_BAND): the magnitude
+// in SW words with lsb exponent A, and its sign.
+QB_HD void s_from_windows(uint32_t (&M)[SW], uint32_t &sign, const uint32_t (&lo)[WW], const uint32_t (&hi)[WW]) {
+  // T = sext(lo) + (sext(hi) << WIN_BAND) over SW words, two's complement
+  uint32_t L[SW], H[SW], Hs[SW];
+  const uint32_t el = (lo[WW - 1] >> 31) ? 0xffffffffu : 0u, eh = (hi[WW - 1] >> 31) ? 0xffffffffu : 0u;
+#pragma unroll
+  for (int i = 0; i < SW; ++i) {
+    L[i] = (i < WW) ? lo[i] : el;
+    H[i] = (i < WW) ? hi[i] : eh;
+  }
+  rshl<SW>(Hs, H, WIN_BAND);  // sign bits above stay ones: H was sign-extended to SW words
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < SW; ++i) M[i] = addc32(L[i], Hs[i], c, c);
+  sign = M[SW - 1] >> 31;
+  if (sign) {
+    uint32_t c2 = 1;
+#pragma unroll
+    for (int i = 0; i < SW; ++i) M[i] = addc32(~M[i], 0u, c2, c2);
+  }
+}
+
+// Compact a wide accumulator (two's complement, ACCW words at acc.base) to its top SW words of
+// magnitude: lsb exponent, sign, and a sticky bit for nonzero words below (round-to-odd).
+QB_HD void compact_wide(const wide_acc &acc, uint32_t (&M)[SW], int &lsb, uint32_t &sign, uint32_t &sticky) {
+  uint32_t A[ACCW];
+  sign = acc.w[ACCW - 1] >> 31;
+  uint32_t c = sign;
+  for (int i = 0; i < ACCW; ++i) A[i] = sign ? addc32(~acc.w[i], 0u, c, c) : acc.w[i];
+  int top = -1;
+  for (int i = 0; i < ACCW; ++i)
+    if (A[i]) top = i;
+  const int lo_w = (top >= SW) ? top - SW + 1 : 0;
+  sticky = 0;
+  for (int i = 0; i < lo_w; ++i) sticky |= (A[i] != 0);
+  for (int i = 0; i < SW; ++i) M[i] = A[lo_w + i < ACCW ? lo_w + i : ACCW - 1] & (lo_w + i < ACCW ? 0xffffffffu : 0u);
+  lsb = acc.base + 32 * lo_w;
+}
 }  // namespace qb

@@ -3,13 +3,15 @@
 // Computes the same routines as the reference (include/quadblas/algorithms/level1.hpp:80-137,
 // level2.hpp:15-99) but returns RN(alpha * sum_j a_j b_j + beta * y), the exact value rounded once
 // (exact.hpp).  Kernels:
-//   k_dot_exact_partial    grid-stride lanes, wave-uniform window anchor, wave reduction,
-//                          one partial window per wave.
+//   k_dot_exact_partial    grid-stride lanes, block-uniform window anchor, wave and block
+//                          reductions, one lo/hi window pair per block.
 //   k_gemv_row_exact       one wave per (row, column chunk), wave-uniform anchor, wave reduction.
 //   k_gemv_col_exact       one lane per row, blockIdx.y = column chunk, per-lane anchor.
-//   k_exact_finish_dot     combines all wave partials (common base) and rounds.
-//   k_exact_finish_rows    one thread per row: combines the row's chunk partials, forms
-//                          alpha*S + beta*y exactly and rounds into y.
+//   k_exact_combine        sums the block partials on a common base (the minimum anchor) in
+//                          registers, column sums per block.
+//   k_exact_finish_dot     sums the column records, ripples carries once, rounds once.
+//   k_exact_finish_rows    one thread per row: sums the row's chunk windows (shared anchor),
+//                          forms alpha*S + beta*y exactly in registers and rounds into y.
 // Loads of the streamed operands are coalesced 128-bit loads.  If any product falls outside its
 // window the result is not exact: the output is set to NaN and bit 0 of the status word is set
 // (the host entry points report it as an error).
@@ -119,12 +121,27 @@ constexpr int XD_BS = 256;
 __global__ void __launch_bounds__(XD_BS) k_dot_exact_partial(const q128 *__restrict__ x, int64_t incx,
                                                              const q128 *__restrict__ y, int64_t incy, int64_t n,
                                                              part_t *__restrict__ parts) {
+  __shared__ int s_mn[XD_BS / 64], s_mx[XD_BS / 64];
+  __shared__ uint32_t s_w[XD_BS / 64][2 * WW + 1];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t stride = (int64_t)gridDim.x * XD_BS;
   const int64_t i0 = (int64_t)blockIdx.x * XD_BS + threadIdx.x;
-  // anchor from the first element of every lane of the wave
+  // block-uniform anchor, centred between the smallest and largest first products of the block
   int e0 = EXP_NONE;
   if (i0 < n) e0 = pair_exp(ldq_c(x + i0 * incx), ldq_c(y + i0 * incy));
-  const int A = wave_anchor(e0);
+  const int mn = wave_min(e0), mx = wave_max(e0 == EXP_NONE ? -EXP_NONE : e0);
+  if (lane == 0) {
+    s_mn[wv] = mn;
+    s_mx[wv] = mx;
+  }
+  __syncthreads();
+  int bmn = s_mn[0], bmx = s_mx[0];
+#pragma unroll
+  for (int w = 1; w < XD_BS / 64; ++w) {
+    bmn = min(bmn, s_mn[w]);
+    bmx = max(bmx, s_mx[w]);
+  }
+  const int A = (bmn == EXP_NONE) ? 0 : ((bmn + bmx) >> 1) - WIN2_CENTER;
   uint32_t Wl[WW], Wh[WW];
 #pragma unroll
   for (int k = 0; k < WW; ++k) {
@@ -147,7 +164,30 @@ __global__ void __launch_bounds__(XD_BS) k_dot_exact_partial(const q128 *__restr
   wave_sum_win(Wl);
   wave_sum_win(Wh);
   flags = wave_or(flags);
-  if ((threadIdx.x & 63) == 0) put_parts(parts + 2 * ((int64_t)blockIdx.x * (XD_BS / 64) + (threadIdx.x >> 6)), A, flags, Wl, Wh);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < WW; ++k) {
+      s_w[wv][k] = Wl[k];
+      s_w[wv][WW + k] = Wh[k];
+    }
+    s_w[wv][2 * WW] = flags;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 1; w < XD_BS / 64; ++w) {
+      uint32_t X[WW], Y[WW];
+#pragma unroll
+      for (int k = 0; k < WW; ++k) {
+        X[k] = s_w[w][k];
+        Y[k] = s_w[w][WW + k];
+      }
+      win_add_win(Wl, X);
+      win_add_win(Wh, Y);
+      flags |= s_w[w][2 * WW];
+    }
+    put_parts(parts + 2 * (int64_t)blockIdx.x, A, flags, Wl, Wh);
+  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -168,8 +208,9 @@ __global__ void __launch_bounds__(XR_BS) k_gemv_row_exact(int64_t m, int64_t n, 
   const int64_t j0 = (int64_t)c * chunk;
   const int64_t j1 = (j0 + chunk < n) ? j0 + chunk : n;
   const q128 *r = A + row * lda;
+  // anchor from the row's first elements, so every column chunk of the row shares it
   int e0 = EXP_NONE;
-  if (j0 + lane < j1) e0 = pair_exp(ldq_c(r + j0 + lane), ldq_c(x + (j0 + lane) * incx));
+  if (lane < n) e0 = pair_exp(ldq_c(r + lane), ldq_c(x + (int64_t)lane * incx));
   const int An = wave_anchor(e0);
   uint32_t Wl[WW], Wh[WW];
 #pragma unroll
@@ -206,9 +247,10 @@ __global__ void __launch_bounds__(XC_BS) k_gemv_col_exact(int64_t m, int64_t n, 
   const int64_t j0 = (int64_t)blockIdx.y * chunk;
   const int64_t j1 = (j0 + chunk < n) ? j0 + chunk : n;
   if (i >= m) return;
-  // lane anchor: centred on the first regular product of the chunk (scan a few columns)
+  // lane anchor: centred on the row's first regular product (first columns of the matrix), so
+  // every column chunk of the row shares it
   int e0 = EXP_NONE;
-  for (int64_t j = j0; j < j1 && j < j0 + 4 && e0 == EXP_NONE; ++j) e0 = pair_exp(ldq_c(A + j * lda + i), ldq_c(x + j * incx));
+  for (int64_t j = 0; j < n && j < 4 && e0 == EXP_NONE; ++j) e0 = pair_exp(ldq_c(A + j * lda + i), ldq_c(x + j * incx));
   const int An = (e0 == EXP_NONE) ? 0 : e0 - WIN2_CENTER;
   uint32_t Wl[WW], Wh[WW];
 #pragma unroll
@@ -228,15 +270,29 @@ __global__ void __launch_bounds__(XC_BS) k_gemv_col_exact(int64_t m, int64_t n, 
   put_parts(parts + 2 * ((int64_t)blockIdx.y * m + i), An, flags, Wl, Wh);
 }
 
-// One block: combines all wave partials (two windows per wave) on a common base and rounds.
-__global__ void __launch_bounds__(256) k_exact_finish_dot(const part_t *__restrict__ parts, int nparts,
-                                                          q128 *__restrict__ out, uint32_t *__restrict__ status) {
-  // common base: the minimum anchor over nonzero partials
-  __shared__ int s_min[4];
-  __shared__ uint32_t s_fl[4];
-  int amin = EXP_NONE;
-  uint32_t fl = 0;
-  for (int p = threadIdx.x; p < nparts; p += 256) {
+// Combining the block partials, in two stages on a common base (the minimum anchor over nonzero
+// windows, minus 32):
+//   k_exact_combine: XC blocks; every thread adds its windows into a register-resident
+//     ACCW-word accumulator (select-based shifts), then lanes 0..ACCW-1 sum one word column each
+//     over the block's threads (64-bit column sums, no carry chains) into one record per block.
+//   k_exact_finish_dot: one block sums the column records, ripples the carries once and rounds.
+constexpr int XF_BS = 256;
+constexpr int XF_BLOCKS = 32;
+struct col_rec {
+  uint64_t c[ACCW];
+  uint32_t flags;
+  uint32_t lost;
+  int amin;
+  uint32_t pad;
+};
+
+__device__ __forceinline__ void parts_min_flags(const part_t *__restrict__ parts, int nparts, int &amin,
+                                                uint32_t &fl) {
+  __shared__ int s_min[XF_BS / 64];
+  __shared__ uint32_t s_fl[XF_BS / 64];
+  amin = EXP_NONE;
+  fl = 0;
+  for (int p = threadIdx.x; p < nparts; p += XF_BS) {
     fl |= parts[p].flags;
     if (!win_is_zero(parts[p].W)) amin = min(amin, parts[p].A);
   }
@@ -247,69 +303,137 @@ __global__ void __launch_bounds__(256) k_exact_finish_dot(const part_t *__restri
     s_fl[threadIdx.x >> 6] = fl;
   }
   __syncthreads();
-  amin = min(min(s_min[0], s_min[1]), min(s_min[2], s_min[3]));
-  fl = s_fl[0] | s_fl[1] | s_fl[2] | s_fl[3];
-  wide_acc acc;
-  wide_init(acc);
-  if (amin != EXP_NONE) {
-    acc.any = true;
-    acc.base = amin - 32;
-  }
-  for (int p = threadIdx.x; p < nparts; p += 256) wide_add_window(acc, parts[p].W, parts[p].A);
-  // tree reduction of the 256 accumulators through LDS (same base everywhere)
-  __shared__ uint32_t red[256][ACCW + 1];
-  for (int k = 0; k < ACCW; ++k) red[threadIdx.x][k] = acc.w[k];
-  red[threadIdx.x][ACCW] = acc.lost ? 1u : 0u;
-  __syncthreads();
-  for (int s = 128; s >= 1; s >>= 1) {
-    if (threadIdx.x < s) {
-      uint32_t c = 0;
-      for (int k = 0; k < ACCW; ++k) {
-        const uint64_t v = (uint64_t)red[threadIdx.x][k] + red[threadIdx.x + s][k] + c;
-        red[threadIdx.x][k] = (uint32_t)v;
-        c = (uint32_t)(v >> 32);
-      }
-      red[threadIdx.x][ACCW] |= red[threadIdx.x + s][ACCW];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    for (int k = 0; k < ACCW; ++k) acc.w[k] = red[0][k];
-    acc.lost = red[0][ACCW] != 0;
-    if (acc.lost) fl |= FL_LOST;
-    q128 r;
-    if (fl & FL_LOST) {
-      r = q_nan();
-      atomicOr(status, 1u);
-    } else {
-      r = finish_value(acc, fl, false, q_one(), q_zero(0), q_zero(0));
-    }
-    stq_c(out, r);
+  amin = s_min[0];
+  fl = s_fl[0];
+#pragma unroll
+  for (int w = 1; w < XF_BS / 64; ++w) {
+    amin = min(amin, s_min[w]);
+    fl |= s_fl[w];
   }
 }
 
-// one thread per row: combine nch chunk partials, then RN(alpha*S + beta*y)
-__global__ void k_exact_finish_rows(int64_t m, int nch, const part_t *__restrict__ parts, q128 alpha, q128 beta,
-                                    q128 *__restrict__ y, int64_t incy, uint32_t *__restrict__ status) {
+__global__ void __launch_bounds__(XF_BS) k_exact_combine(const part_t *__restrict__ parts, int nparts,
+                                                         col_rec *__restrict__ recs) {
+  __shared__ uint32_t tile[XF_BS][ACCW + 1];
+  int amin;
+  uint32_t fl;
+  parts_min_flags(parts, nparts, amin, fl);
+  const int base = amin - 32;
+  uint32_t acc[ACCW];
+#pragma unroll
+  for (int k = 0; k < ACCW; ++k) acc[k] = 0;
+  uint32_t lost = 0;
+  if (amin != EXP_NONE) {
+    for (int p = blockIdx.x * XF_BS + threadIdx.x; p < nparts; p += gridDim.x * XF_BS) {
+      if (win_is_zero(parts[p].W)) continue;
+      const int off = parts[p].A - base;
+      if (off < 0 || off + WW * 32 + 32 > ACCW * 32) {
+        lost = 1;
+        continue;
+      }
+      uint32_t W[ACCW], S[ACCW];
+      const uint32_t sext = (parts[p].W[WW - 1] >> 31) ? 0xffffffffu : 0u;
+#pragma unroll
+      for (int k = 0; k < ACCW; ++k) W[k] = (k < WW) ? parts[p].W[k] : sext;
+      rshl<ACCW>(S, W, off);  // zeros enter from below, the sign extension stays above
+      uint32_t c = 0;
+#pragma unroll
+      for (int k = 0; k < ACCW; ++k) acc[k] = addc32(acc[k], S[k], c, c);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < ACCW; ++k) tile[threadIdx.x][k] = acc[k];
+  tile[threadIdx.x][ACCW] = lost;
+  __syncthreads();
+  if (threadIdx.x < ACCW) {
+    uint64_t s = 0;
+    for (int t = 0; t < XF_BS; ++t) s += tile[t][threadIdx.x];
+    recs[blockIdx.x].c[threadIdx.x] = s;
+  } else if (threadIdx.x == ACCW) {
+    uint32_t l = 0;
+    for (int t = 0; t < XF_BS; ++t) l |= tile[t][ACCW];
+    recs[blockIdx.x].lost = l;
+    recs[blockIdx.x].flags = fl;
+    recs[blockIdx.x].amin = amin;
+  }
+}
+
+__global__ void __launch_bounds__(64) k_exact_finish_dot(const col_rec *__restrict__ recs, int nrec,
+                                                         q128 *__restrict__ out, uint32_t *__restrict__ status) {
+  __shared__ uint64_t col[ACCW];
+  if (threadIdx.x < ACCW) {
+    uint64_t s = 0;
+    for (int b = 0; b < nrec; ++b) s += recs[b].c[threadIdx.x];  // < 2^45 for nrec <= 32
+    col[threadIdx.x] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  uint32_t fl = recs[0].flags, lost = 0;
+  for (int b = 0; b < nrec; ++b) lost |= recs[b].lost;
+  const int amin = recs[0].amin;
+  if (lost) fl |= FL_LOST;
+  q128 r;
+  if (fl & FL_LOST) {
+    r = q_nan();
+    atomicOr(status, 1u);
+  } else if (amin == EXP_NONE) {
+    r = (fl & (FL_NAN | FL_PINF | FL_NINF)) ? special_result(fl) : q_zero(0);
+  } else {
+    wide_acc wa;
+    wide_init(wa);
+    wa.any = true;
+    wa.base = amin - 32;
+    uint64_t carry = 0;
+    for (int k = 0; k < ACCW; ++k) {
+      const uint64_t v = col[k] + carry;  // two's complement words: the top carry is dropped
+      wa.w[k] = (uint32_t)v;
+      carry = v >> 32;
+    }
+    uint32_t M[SW], sign, sticky;
+    int lsb;
+    compact_wide(wa, M, lsb, sign, sticky);
+    r = finish_reg(M, lsb, sign, sticky, fl, false, q_one(), q_zero(0), q_zero(0));
+  }
+  stq_c(out, r);
+}
+
+// one thread per row: sums the row's chunk windows (they share the row's anchor), then
+// RN(alpha*S + beta*y) in registers
+__global__ void __launch_bounds__(64) k_exact_finish_rows(int64_t m, int nch, const part_t *__restrict__ parts,
+                                                         q128 alpha, q128 beta, q128 *__restrict__ y, int64_t incy,
+                                                         uint32_t *__restrict__ status) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
-  wide_acc acc;
-  wide_init(acc);
+  uint32_t lo[WW], hi[WW];
+#pragma unroll
+  for (int k = 0; k < WW; ++k) {
+    lo[k] = 0;
+    hi[k] = 0;
+  }
   uint32_t fl = 0;
+  const int A0 = parts[2 * i].A;
   for (int c = 0; c < nch; ++c) {
     const part_t *p = parts + 2 * ((int64_t)c * m + i);
     fl |= p[0].flags | p[1].flags;
-    wide_add_window(acc, p[0].W, p[0].A);
-    wide_add_window(acc, p[1].W, p[1].A);
+    if (p[0].A != A0) fl |= FL_LOST;  // chunks of a row share its anchor by construction
+    uint32_t X[WW], Y[WW];
+#pragma unroll
+    for (int k = 0; k < WW; ++k) {
+      X[k] = p[0].W[k];
+      Y[k] = p[1].W[k];
+    }
+    win_add_win(lo, X);
+    win_add_win(hi, Y);
   }
-  if (acc.lost) fl |= FL_LOST;
   q128 *yi = y + i * incy;
   q128 r;
   if (fl & FL_LOST) {
     r = q_nan();
     atomicOr(status, 1u);
   } else {
-    r = finish_value(acc, fl, true, alpha, beta, ldq_c(yi));
+    uint32_t M[SW], sign;
+    s_from_windows(M, sign, lo, hi);
+    r = finish_reg(M, A0, sign, 0u, fl, true, alpha, beta, ldq_c(yi));
   }
   stq_c(yi, r);
 }
@@ -341,7 +465,9 @@ static void gemv_exact_split(int colmajor, int64_t m, int64_t n, int &nch, int64
   chunk = (n + want - 1) / want;
   nch = (int)((n + chunk - 1) / chunk);
 }
-size_t dot_exact_ws(int64_t n) { return 256 + 2 * (size_t)dot_exact_blocks(n) * (XD_BS / 64) * sizeof(part_t); }
+size_t dot_exact_ws(int64_t n) {
+  return 256 + 2 * (size_t)dot_exact_blocks(n) * sizeof(part_t) + XF_BLOCKS * sizeof(col_rec);
+}
 size_t gemv_exact_ws(int colmajor, int64_t m, int64_t n) {
   int nch;
   int64_t chunk;
@@ -353,12 +479,16 @@ size_t gemv_exact_ws(int colmajor, int64_t m, int64_t n) {
 hipError_t launch_dot_exact(const q128 *x, int64_t incx, const q128 *y, int64_t incy, int64_t n, void *ws,
                             q128 *out, hipStream_t st) {
   uint32_t *status = (uint32_t *)ws;
-  part_t *parts = (part_t *)((char *)ws + 256);
   const int blocks = dot_exact_blocks(n);
+  part_t *parts = (part_t *)((char *)ws + 256);
+  col_rec *recs = (col_rec *)(parts + 2 * blocks);
   hipError_t e = launch_zero((q128 *)ws, 1, st);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_dot_exact_partial, dim3(blocks), dim3(XD_BS), 0, st, x, incx, y, incy, n, parts);
-  hipLaunchKernelGGL(k_exact_finish_dot, dim3(1), dim3(256), 0, st, parts, 2 * blocks * (XD_BS / 64), out, status);
+  const int nparts = 2 * blocks;
+  const int cblocks = nparts < XF_BLOCKS * XF_BS ? (nparts + XF_BS - 1) / XF_BS : XF_BLOCKS;
+  hipLaunchKernelGGL(k_exact_combine, dim3(cblocks), dim3(XF_BS), 0, st, parts, nparts, recs);
+  hipLaunchKernelGGL(k_exact_finish_dot, dim3(1), dim3(64), 0, st, recs, cblocks, out, status);
   return hipGetLastError();
 }
 

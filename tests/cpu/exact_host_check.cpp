@@ -77,25 +77,29 @@ static q128 dot_emul(const std::vector<q128> &x, const std::vector<q128> &y, int
     wide_add_window(acc, Whs[w].data(), As[w] + WIN_BAND);
   }
   lost = acc.lost || (fl_all & FL_LOST);
-  return finish_value(acc, fl_all, false, q_one(), q_zero(0), q_zero(0));
+  uint32_t M[SW], sign, sticky;
+  int lsb;
+  compact_wide(acc, M, lsb, sign, sticky);
+  if (!acc.any) return (fl_all & (FL_NAN | FL_PINF | FL_NINF)) ? special_result(fl_all) : q_zero(0);
+  return finish_reg(M, lsb, sign, sticky, fl_all, false, q_one(), q_zero(0), q_zero(0));
 }
 
-// emulate one gemv row: one wave per row over nch column chunks, then the exact epilogue
+// emulate one gemv row: nch column chunks sharing the row anchor (from the row's first 64
+// products), window sums, register epilogue
 static q128 row_emul(const q128 *r, const q128 *x, int64_t n, int nch, q128 alpha, q128 beta, q128 yv, bool &lost) {
   const int64_t chunk = (n + nch - 1) / nch;
-  wide_acc acc;
-  wide_init(acc);
   uint32_t fl_all = 0;
+  int mn = EXP_NONE, mx = -EXP_NONE;
+  for (int l = 0; l < 64; ++l) {
+    int e0 = (l < n) ? pair_exp(r[l], x[l]) : EXP_NONE;
+    mn = e0 < mn ? e0 : mn;
+    int ev = e0 == EXP_NONE ? -EXP_NONE : e0;
+    mx = ev > mx ? ev : mx;
+  }
+  const int A = (mn == EXP_NONE) ? 0 : ((mn + mx) >> 1) - WIN2_CENTER;
+  uint32_t lo[WW] = {0}, hi[WW] = {0};
   for (int c = 0; c < nch; ++c) {
     const int64_t j0 = c * chunk, j1 = (j0 + chunk < n) ? j0 + chunk : n;
-    int mn = EXP_NONE, mx = -EXP_NONE;
-    for (int l = 0; l < 64; ++l) {
-      int e0 = (j0 + l < j1) ? pair_exp(r[j0 + l], x[j0 + l]) : EXP_NONE;
-      mn = e0 < mn ? e0 : mn;
-      int ev = e0 == EXP_NONE ? -EXP_NONE : e0;
-      mx = ev > mx ? ev : mx;
-    }
-    const int A = (mn == EXP_NONE) ? 0 : ((mn + mx) >> 1) - WIN2_CENTER;
     uint32_t W[WW] = {0}, Wh[WW] = {0};
     for (int64_t j = j0; j < j1; ++j) {
       uq ua, ub;
@@ -103,11 +107,13 @@ static q128 row_emul(const q128 *r, const q128 *x, int64_t n, int nch, q128 alph
       if (k == 0 || k == 3)
         if (!win2_add_product(W, Wh, A, ua, ub)) fl_all |= FL_LOST;
     }
-    wide_add_window(acc, W, A);
-    wide_add_window(acc, Wh, A + WIN_BAND);
+    win_add_win(lo, W);
+    win_add_win(hi, Wh);
   }
-  lost = acc.lost || (fl_all & FL_LOST);
-  return finish_value(acc, fl_all, true, alpha, beta, yv);
+  lost = (fl_all & FL_LOST) != 0;
+  uint32_t M[SW], sign;
+  s_from_windows(M, sign, lo, hi);
+  return finish_reg(M, A, sign, 0u, fl_all, true, alpha, beta, yv);
 }
 
 static bool same(q128 a, q128 b) { return (q_isnan(a) && q_isnan(b)) || (a.lo == b.lo && a.hi == b.hi); }
@@ -167,10 +173,10 @@ int main(int argc, char **argv) {
     const q128 one = q_one(), m113 = q_make(0, (uint64_t)(QBIAS - 113) << 48);  // 2^-113
     for (int sgn_y = 0; sgn_y < 2; ++sgn_y)
       for (int sgn_a = 0; sgn_a < 2; ++sgn_a)
-        for (int far = 0; far < 3; ++far) {
+        for (int far = 0; far < 5; ++far) {
           std::vector<q128> x = {one, m113, one}, y = {one, one, q_zero(0)};
           q128 alpha = sgn_a ? q_make(0, one.hi | (1ull << 63)) : one;
-          const int ey = far == 0 ? -300 : (far == 1 ? -200 : -130);
+          const int ey = far == 0 ? -300 : (far == 1 ? -200 : (far == 2 ? -130 : (far == 3 ? -1500 : -3000)));
           q128 yc = q_make(0, ((uint64_t)sgn_y << 63) | ((uint64_t)(QBIAS + ey) << 48));
           for (int nch = 1; nch <= 2; ++nch) {
             bool lost = false;
@@ -197,6 +203,54 @@ int main(int argc, char **argv) {
       checked++;
       if (lost || !same(gr, wr)) {
         if (bad < 5) printf("zero-sign MISMATCH case %d got %016llx want %016llx\n", c, (unsigned long long)gr.hi, (unsigned long long)wr.hi);
+        bad++;
+      }
+    }
+  }
+  // wide accumulators with sparse words far apart (ties broken only by low words): compaction
+  // to SW words with a sticky, then rounding; the exact value is rebuilt as a sum of exact
+  // binary128 terms w_i * 2^(base + 32 i) for the oracle
+  {
+    uint64_t st = seed * 0x9e3779b97f4a7c15ull + 17;
+    auto rnd = [&]() { st ^= st << 13; st ^= st >> 7; st ^= st << 17; return st; };
+    for (int t = 0; t < 3000; ++t) {
+      uint32_t mag[ACCW] = {0};
+      const int top = 20 + (int)(rnd() % (ACCW - 21));
+      mag[top] = (uint32_t)rnd() | 1u;
+      if (t % 3 == 0) {  // tie pattern right below the top 113 bits, resolved by a far-away word
+        for (int i = top - 4; i < top; ++i) mag[i] = 0;
+        const int gbit = 32 * top + 31 - __builtin_clz(mag[top]) - 113;  // bit position of guard
+        if (gbit >= 0) mag[gbit >> 5] |= 1u << (gbit & 31);
+        if (rnd() & 1) mag[rnd() % (uint64_t)(top > 26 ? top - 26 : 1)] |= 1u << (rnd() % 32);
+      } else {
+        const int nlow = (int)(rnd() % 4);
+        for (int k = 0; k < nlow; ++k) mag[rnd() % (uint64_t)top] = (uint32_t)rnd();
+        for (int i = top - 3; i < top; ++i) mag[i] = (uint32_t)rnd();
+      }
+      const uint32_t sg = (uint32_t)(rnd() & 1);
+      wide_acc acc;
+      wide_init(acc);
+      acc.any = true;
+      acc.base = -600 - (int)(rnd() % 200);
+      uint32_t c = sg;
+      for (int i = 0; i < ACCW; ++i) acc.w[i] = sg ? addc32(~mag[i], 0u, c, c) : mag[i];
+      uint32_t M[SW], sign, sticky;
+      int lsb;
+      compact_wide(acc, M, lsb, sign, sticky);
+      q128 got = finish_reg(M, lsb, sign, sticky, 0, false, q_one(), q_zero(0), q_zero(0));
+      std::vector<q128> xs, ys;
+      for (int i = 0; i < ACCW; ++i) {
+        if (!mag[i]) continue;
+        q128 w = from_double((double)mag[i]);
+        if (sg) w.hi |= 1ull << 63;
+        xs.push_back(w);
+        ys.push_back(q_make(0, (uint64_t)(QBIAS + acc.base + 32 * i) << 48));
+      }
+      q128 want;
+      or_exact_dot(xs.size(), xs.data(), 1, ys.data(), 1, q_one(), q_zero(0), q_zero(0), nullptr, 0, &want, nullptr);
+      checked++;
+      if (!same(got, want)) {
+        if (bad < 5) printf("wide-acc MISMATCH t %d top %d\n", t, top);
         bad++;
       }
     }
