@@ -75,7 +75,11 @@ void quadblas_gemm_q(int layout, size_t m, size_t n, size_t k, const void *alpha
                          rounded result, independent of order and T).
    QBLAS_MODE_DD:        double-double arithmetic on FP64 FMA (inputs rounded to 106 bits); faster,
                          ~2^-104 relative accuracy on well-conditioned data.
-   QBLAS_MODE_OZAKI:     gemm only: int8 slices on MFMA with exact int32 accumulation. */
+   QBLAS_MODE_OZAKI:     gemm only: each operand split into S int8 slices (per-row / per-column
+                         power-of-two scaling), slice products on the int8 matrix cores with exact
+                         int32 accumulation, recombined exactly and rounded once.  Every entry is
+                         certified to <= 2^-104 relative error by an a-priori bound, or recomputed
+                         exactly (correctly rounded) when the bound cannot certify it. */
 enum {
   QBLAS_MODE_REFERENCE = 0,
   QBLAS_MODE_EXACT = 1,
@@ -89,7 +93,8 @@ enum {
   QBLAS_ERR_ARG = 1,      /* invalid argument (negative size/stride, misaligned pointer, ...) */
   QBLAS_ERR_HIP = 2,      /* a HIP runtime call failed */
   QBLAS_ERR_MODE = 3,     /* mode not available for this routine */
-  QBLAS_ERR_RANGE = 4     /* EXACT form: product exponents too far apart for the window */
+  QBLAS_ERR_RANGE = 4     /* EXACT form (and the exact repair of the OZAKI form): product exponents
+                             too far apart for the accumulation window */
 };
 
 /* Mode used by Parts 1 and 2 (default QBLAS_MODE_REFERENCE; environment QBLAS_MODE overrides at
@@ -101,6 +106,16 @@ int qblas_get_mode(void);
    126 on x86-64 Linux (the default), 256 on Apple silicon. */
 void qblas_set_kc(int kc);
 int qblas_get_kc(void);
+
+/* Slice count S of QBLAS_MODE_OZAKI (8..18); 0 (the default) lets the library choose.  Returns
+   QBLAS_ERR_ARG for other values. */
+int qblas_set_ozaki_slices(int slices);
+int qblas_get_ozaki_slices(void);
+
+/* After a QBLAS_MODE_OZAKI gemm on this thread: synchronizes the stream and returns how many
+   entries were recomputed exactly because the bound could not certify them (-1 if there was no
+   such call or on a HIP error). */
+long long qblas_hip_ozaki_repairs(void *stream);
 
 /* Last error message of the calling thread ("" if none); cleared by qblas_clear_error(). */
 const char *quadblas_hip_last_error(void);

@@ -82,6 +82,9 @@ def lib():
         "qblas_hip_gemm": (I, [I, I, I64, I64, I64, P, P, I64, P, I64, P, P, I64, P]),
         "qblas_hip_fill": (I, [I, ctypes.c_uint64, I64, I, I, P, P]),
         "qblas_hip_exact_status": (I, [P]),
+        "qblas_set_ozaki_slices": (I, [I]),
+        "qblas_get_ozaki_slices": (I, []),
+        "qblas_hip_ozaki_repairs": (ctypes.c_longlong, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -238,6 +241,15 @@ def get_kc():
     return lib().qblas_get_kc()
 
 
+def set_ozaki_slices(s):
+    """Slice count of the OZAKI gemm form (8..18), 0 = chosen by the library."""
+    _check(lib().qblas_set_ozaki_slices(int(s)), "qblas_set_ozaki_slices")
+
+
+def get_ozaki_slices():
+    return lib().qblas_get_ozaki_slices()
+
+
 # ------------------------------------------------------------------------------------------------
 # device-pointer API
 # ------------------------------------------------------------------------------------------------
@@ -261,6 +273,11 @@ def dev_gemm(mode, layout, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, stream=
 def dev_exact_status(stream=0):
     """1 if the last EXACT-form device call of this thread lost a product (output NaN), else 0."""
     return lib().qblas_hip_exact_status(stream)
+
+
+def dev_ozaki_repairs(stream=0):
+    """Entries of the last OZAKI gemm of this thread that were recomputed exactly (-1: none)."""
+    return lib().qblas_hip_ozaki_repairs(stream)
 
 
 FILL_UNIFORM_FULL = 0

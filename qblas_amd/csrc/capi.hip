@@ -45,6 +45,7 @@ void set_err_msg(const std::string &m) { t_err = m; }
 std::atomic<int> g_threads{0};  // 0: host default
 std::atomic<int> g_kc{126};
 std::atomic<int> g_mode{-1};    // -1: not yet read from the environment
+std::atomic<int> g_oz_slices{0};  // OZAKI slice count, 0: chosen per call
 
 int default_threads() {
   // omp_get_max_threads() without an OpenMP runtime: OMP_NUM_THREADS's first entry, else the
@@ -137,6 +138,7 @@ int ws_release(Workspace &w, hipStream_t s) {
 
 thread_local Workspace t_ws;           // partials / scaled x for the device routines
 thread_local const uint32_t *t_exact_status = nullptr;  // status word of the last EXACT-form call
+thread_local const uint32_t *t_oz_hdr = nullptr;        // header of the last OZAKI-form call
 
 // After a synchronous EXACT-form call: report a product that fell outside its window.
 int check_exact_status(hipStream_t s) {
@@ -255,6 +257,17 @@ int dev_gemv(int mode, int layout, int64_t m, int64_t n, q128 alpha, const void 
   return QBLAS_OK;
 }
 
+// Slice count of the OZAKI form: the value set by qblas_set_ozaki_slices, else 17 (which, for
+// operands spanning up to ~2^60 in magnitude per row and k <= 2^14, certifies all but a small
+// fraction of entries at 2^-104; the rest are recomputed exactly).
+int oz_slices(int64_t m, int64_t n, int64_t k) {
+  (void)m;
+  (void)n;
+  (void)k;
+  const int s = g_oz_slices.load();
+  return s ? s : 17;
+}
+
 int dev_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, q128 alpha, const void *A, int64_t lda,
              const void *B, int64_t ldb, q128 beta, void *C, int64_t ldc, hipStream_t s) {
   if (m < 0 || n < 0 || k < 0 || lda < 0 || ldb < 0 || ldc < 0) {
@@ -266,11 +279,22 @@ int dev_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, q128 alpha, 
     return QBLAS_ERR_ARG;
   }
   if (m == 0 || n == 0 || k == 0) return QBLAS_OK;  // level3.hpp:221-222
+  const int colmajor = layout != 0;
+  if (mode == QBLAS_MODE_OZAKI) {
+    const int S = oz_slices(m, n, k);
+    void *ws;
+    int rc = ws_acquire(t_ws, qbk::gemm_ozaki_ws(m, n, k, S), s, &ws);
+    if (rc) return rc;
+    QB_CHECK(qbk::launch_gemm_ozaki(colmajor, m, n, k, S, alpha, (const q128 *)A, lda, (const q128 *)B, ldb, beta,
+                                    (q128 *)C, ldc, ws, s));
+    t_exact_status = (const uint32_t *)ws;
+    t_oz_hdr = (const uint32_t *)ws;
+    return ws_release(t_ws, s);
+  }
   if (mode != QBLAS_MODE_REFERENCE) {
     set_err_msg("qblas_hip_gemm: mode not available");
     return QBLAS_ERR_MODE;
   }
-  const int colmajor = layout != 0;
   // gemm_simple for m,n,k <= 64 is one chain over all of k; otherwise blocks of kc
   // (level3.hpp:224-229, :244-246).  For k <= kc both are one block.
   const int64_t kc = (m <= 64 && n <= 64 && k <= 64) ? k : (int64_t)g_kc.load();
@@ -368,7 +392,7 @@ int host_gemm(int layout, int64_t m, int64_t n, int64_t k, q128 alpha, const voi
   QB_CHECK(hipMemcpyAsync(C, dC, cbytes, hipMemcpyDeviceToHost, s));
   QB_CHECK(hipStreamSynchronize(s));
   for (int i = 0; i < 3; ++i) ws_release(t_stage[i], s);
-  return QBLAS_OK;
+  return check_exact_status(s);
 }
 
 bool is_colmajor_char(char c) { return c == 'C' || c == 'c'; }
@@ -434,6 +458,18 @@ int qblas_set_mode(int mode) {
   return QBLAS_OK;
 }
 int qblas_get_mode(void) { return cur_mode(); }
+int qblas_set_ozaki_slices(int slices) {
+  if (slices != 0 && (slices < qbk::OZ_SMIN || slices > qbk::OZ_SMAX)) return QBLAS_ERR_ARG;
+  g_oz_slices.store(slices);
+  return QBLAS_OK;
+}
+int qblas_get_ozaki_slices(void) { return g_oz_slices.load(); }
+long long qblas_hip_ozaki_repairs(void *stream) {
+  if (!t_oz_hdr) return -1;
+  uint32_t v = 0;
+  if (qbk::ozaki_repair_count(t_oz_hdr, &v, (hipStream_t)stream) != hipSuccess) return -1;
+  return (long long)v;
+}
 void qblas_set_kc(int kc) { g_kc.store(kc > 0 ? kc : 126); }
 int qblas_get_kc(void) { return g_kc.load(); }
 

@@ -632,32 +632,35 @@ constexpr int FW = SW + 4 + 12;  // combine frame for alpha*S + beta*y
 
 // Correctly rounded alpha*S + beta*y, with S = (-1)^sS * M * 2^lsb exact (sticky must be 0 when
 // alpha/beta are used; for the plain sum, sticky marks a nonzero tail below M, i.e. M is S
-// rounded to odd).  flags carry Inf/NaN products.
-QB_HD q128 finish_reg(const uint32_t (&M)[SW], int lsb, uint32_t sS, uint32_t sticky, uint32_t flags, bool has_ab,
-                      q128 alpha, q128 beta, q128 yv) {
+// rounded to odd).  flags carry Inf/NaN products.  N is the word count of M; the combine frame
+// has N + 16 words (room for alpha*S, N + 4 words, plus 12 words below it).
+template <int N>
+QB_HD q128 finish_reg_t(const uint32_t (&M)[N], int lsb, uint32_t sS, uint32_t sticky, uint32_t flags, bool has_ab,
+                        q128 alpha, q128 beta, q128 yv) {
+  constexpr int FWN = N + 16;
   const bool sflag = (flags & (FL_NAN | FL_PINF | FL_NINF)) != 0;
   if (!has_ab) {
     if (sflag) return special_result(flags);
-    uint32_t Mx[SW];
+    uint32_t Mx[N];
 #pragma unroll
-    for (int i = 0; i < SW; ++i) Mx[i] = M[i];
+    for (int i = 0; i < N; ++i) Mx[i] = M[i];
     Mx[0] |= sticky;
-    return rround<SW>(Mx, lsb, sS);
+    return rround<N>(Mx, lsb, sS);
   }
   const bool abspecial = (q_bexp(alpha) == 0x7fffu) || (q_bexp(beta) == 0x7fffu) || (q_bexp(yv) == 0x7fffu);
   if (sflag || abspecial) {
-    const q128 sq = sflag ? special_result(flags) : rround<SW>(M, lsb, sS);
+    const q128 sq = sflag ? special_result(flags) : rround<N>(M, lsb, sS);
     return fma_packed(alpha, sq, mul_packed(beta, yv));
   }
   const uq ua = unpack_finite(alpha), ub = unpack_finite(beta), uy = unpack_finite(yv);
-  // T1 = alpha * S  (SW + 4 words), T2 = beta * y (8 words)
-  uint32_t T1[SW + 4];
-  const bool z1 = rzero<SW>(M) || ((ua.mh | ua.ml) == 0);
+  // T1 = alpha * S  (N + 4 words), T2 = beta * y (8 words)
+  uint32_t T1[N + 4];
+  const bool z1 = rzero<N>(M) || ((ua.mh | ua.ml) == 0);
   if (!z1) {
-    rmul_q<SW>(T1, M, ua);
+    rmul_q<N>(T1, M, ua);
   } else {
 #pragma unroll
-    for (int i = 0; i < SW + 4; ++i) T1[i] = 0;
+    for (int i = 0; i < N + 4; ++i) T1[i] = 0;
   }
   const int e1 = lsb + ua.e - 112;
   const uint32_t s1 = sS ^ ua.s;
@@ -667,52 +670,56 @@ QB_HD q128 finish_reg(const uint32_t (&M)[SW], int lsb, uint32_t sS, uint32_t st
   const int e2 = ub.e + uy.e - 224;
   const uint32_t s2 = ub.s ^ uy.s;
   if (z1 && z2) return q_zero(((uint32_t)(alpha.hi >> 63) ^ sS) & s2);
-  uint32_t F1[FW], F2[FW];
+  uint32_t F1[FWN], F2[FWN];
 #pragma unroll
-  for (int i = 0; i < FW; ++i) {
-    F1[i] = (i < SW + 4) ? T1[i] : 0u;
+  for (int i = 0; i < FWN; ++i) {
+    F1[i] = (i < N + 4) ? T1[i] : 0u;
     F2[i] = (i < 8 && !z2) ? P2[i] : 0u;
   }
-  if (z2) return rround<FW>(F1, e1, s1);
-  if (z1) return rround<FW>(F2, e2, s2);
-  // frame: the operand with the higher leading bit is placed with its leading bit at FW*32-2
-  const int m1 = rmsb<FW>(F1), m2 = rmsb<FW>(F2);
+  if (z2) return rround<FWN>(F1, e1, s1);
+  if (z1) return rround<FWN>(F2, e2, s2);
+  // frame: the operand with the higher leading bit is placed with its leading bit at FWN*32-2
+  const int m1 = rmsb<FWN>(F1), m2 = rmsb<FWN>(F2);
   const bool one_big = (e1 + m1) >= (e2 + m2);
   const int topL = one_big ? (e1 + m1) : (e2 + m2);
-  const int flsb = topL - (FW * 32 - 2);
-  uint32_t L[FW], S[FW];
-  rshl<FW>(L, one_big ? F1 : F2, (FW * 32 - 2) - (one_big ? m1 : m2));
+  const int flsb = topL - (FWN * 32 - 2);
+  uint32_t L[FWN], S[FWN];
+  rshl<FWN>(L, one_big ? F1 : F2, (FWN * 32 - 2) - (one_big ? m1 : m2));
   const int es = one_big ? e2 : e1;
   uint32_t lost = 0;
   if (es >= flsb) {
-    rshl<FW>(S, one_big ? F2 : F1, es - flsb);
+    rshl<FWN>(S, one_big ? F2 : F1, es - flsb);
   } else {
-    lost = rshr<FW>(S, one_big ? F2 : F1, flsb - es);
+    lost = rshr<FWN>(S, one_big ? F2 : F1, flsb - es);
   }
   const uint32_t sL = one_big ? s1 : s2, sSm = one_big ? s2 : s1;
-  uint32_t R[FW];
+  uint32_t R[FWN];
   uint32_t sR = sL;
   if (sL == sSm) {
     uint32_t c = 0;
 #pragma unroll
-    for (int i = 0; i < FW; ++i) R[i] = addc32(L[i], S[i], c, c);
+    for (int i = 0; i < FWN; ++i) R[i] = addc32(L[i], S[i], c, c);
   } else {
     // R = L - S - lost (two's complement); a borrow out means |S| > |L| (only without loss)
     uint32_t c = lost ? 0u : 1u;
 #pragma unroll
-    for (int i = 0; i < FW; ++i) R[i] = addc32(L[i], ~S[i], c, c);
+    for (int i = 0; i < FWN; ++i) R[i] = addc32(L[i], ~S[i], c, c);
     if (!c) {
       uint32_t c2 = 1;
 #pragma unroll
-      for (int i = 0; i < FW; ++i) R[i] = addc32(~R[i], 0u, c2, c2);
+      for (int i = 0; i < FWN; ++i) R[i] = addc32(~R[i], 0u, c2, c2);
       sR = sSm;
     }
-    if (!lost && rzero<FW>(R)) return q_zero(0);
+    if (!lost && rzero<FWN>(R)) return q_zero(0);
   }
   R[0] |= lost;
-  return rround<FW>(R, flsb, sR);
+  return rround<FWN>(R, flsb, sR);
 }
 
+QB_HD q128 finish_reg(const uint32_t (&M)[SW], int lsb, uint32_t sS, uint32_t sticky, uint32_t flags, bool has_ab,
+                      q128 alpha, q128 beta, q128 yv) {
+  return finish_reg_t<SW>(M, lsb, sS, sticky, flags, has_ab, alpha, beta, yv);
+}
 
 // S of one output from its lo/hi window sums (same anchor A: hi is A + WIN_BAND): the magnitude
 // in SW words with lsb exponent A, and its sign.

@@ -39,4 +39,13 @@ size_t gemv_exact_ws(int colmajor, int64_t m, int64_t n);
 hipError_t launch_gemv_exact(int colmajor, int64_t m, int64_t n, q128 alpha, const q128 *A, int64_t lda,
                              const q128 *x, int64_t incx, q128 beta, q128 *y, int64_t incy, void *ws,
                              hipStream_t st);
+
+// OZAKI form of gemm (kernels_ozaki.hip): S int8 slices per operand.  ws[0] (u32) is the status
+// word as for the exact form; ws[1] counts the entries the call recomputed exactly.
+constexpr int OZ_SMIN = 8, OZ_SMAX = 19;
+size_t gemm_ozaki_ws(int64_t m, int64_t n, int64_t k, int S);
+hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int S, q128 alpha, const q128 *A,
+                             int64_t lda, const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc, void *ws,
+                             hipStream_t st);
+hipError_t ozaki_repair_count(const void *ws, uint32_t *out, hipStream_t st);
 }  // namespace qbk
