@@ -107,7 +107,7 @@ int qblas_get_mode(void);
 void qblas_set_kc(int kc);
 int qblas_get_kc(void);
 
-/* Slice count S of QBLAS_MODE_OZAKI (8..18); 0 (the default) lets the library choose.  Returns
+/* Slice count S of QBLAS_MODE_OZAKI (8..19); 0 (the default) lets the library choose.  Returns
    QBLAS_ERR_ARG for other values. */
 int qblas_set_ozaki_slices(int slices);
 int qblas_get_ozaki_slices(void);

@@ -242,7 +242,7 @@ def get_kc():
 
 
 def set_ozaki_slices(s):
-    """Slice count of the OZAKI gemm form (8..18), 0 = chosen by the library."""
+    """Slice count of the OZAKI gemm form (8..19), 0 = chosen by the library."""
     _check(lib().qblas_set_ozaki_slices(int(s)), "qblas_set_ozaki_slices")
 
 

@@ -228,30 +228,46 @@ __global__ void __launch_bounds__(256) k_oz_slice(const q128 *__restrict__ X, in
 // ------------------------------------------------------------------------------------------
 // 3. slice products on the matrix cores
 // ------------------------------------------------------------------------------------------
-// Workgroup = 4 waves, 64x64 outputs; wave (wr, wc) owns the 32x32 block (wr, wc).  Per k block
-// the workgroup stages 4 S fragments in LDS (A row blocks 0,1 and B column blocks 0,1; each wave
-// copies one group of S with direct global->LDS loads), double buffered: the copy of block
-// kb+1 is issued after the fragments of block kb are in registers, so it runs under the
-// S(S+1)/2 MFMAs of block kb.
+// Workgroup = 4 waves, 64x64 outputs; wave (wr, wc) owns the 32x32 block (wr, wc).  The k loop
+// runs over half-stages h = (k block h/2, slice group h%2): group 0 holds slices [0, G0), group 1
+// slices [G0, S).  A half-stage is 4 fragment groups (A row blocks 0,1 and B column blocks 0,1;
+// each wave copies one with direct global->LDS loads) in one of 4 LDS slots, so the copies run
+// 3 half-stages ahead of the MFMAs.  The MFMAs of group 0 are the slice pairs with t, u < G0;
+// those of group 1 the pairs with max(t, u) >= G0 (t + u <= S-1), whose group-0 fragments are
+// still in registers.  Within a group, pairs are ordered by max(t, u) so each MFMA needs only
+// fragments read before it.
 template <int S>
 __global__ void __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(1, 1)))
 k_oz_gemm(const int8_t *__restrict__ As, const int8_t *__restrict__ Bs, int64_t Kb, int64_t Mp, int64_t Np,
           uint32_t *__restrict__ part) {
-  __shared__ __attribute__((aligned(1024))) int8_t lds[2][4 * S * OZ_FRAG];
+  constexpr int G0 = (S + 1) / 2, G1 = S - G0;
+  constexpr int SLOT = 4 * G0 * OZ_FRAG;
+  // s_waitcnt vmcnt(S) lgkmcnt(0): this wave's copies of the two newer half-stages (G0 + G1 = S
+  // loads) may still be in flight
+  constexpr int WAIT_S = (S & 15) | ((S >> 4) << 14) | (7 << 4);
+  __shared__ __attribute__((aligned(1024))) int8_t lds[4 * SLOT];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t tn = blockIdx.x, tm = blockIdx.y, z = blockIdx.z;
   const int64_t kb0 = z * OZ_KPASS_BLOCKS;
-  const int64_t kb1 = (kb0 + OZ_KPASS_BLOCKS < Kb) ? kb0 + OZ_KPASS_BLOCKS : Kb;
+  const int64_t nkb = ((kb0 + OZ_KPASS_BLOCKS < Kb) ? kb0 + OZ_KPASS_BLOCKS : Kb) - kb0;
+  const int64_t nst = 2 * nkb;
   // this wave's copy source: A row block 2tm + wave (waves 0, 1) or B column block 2tn + wave - 2
-  const int8_t *src = (wave < 2) ? As + ((2 * tm + wave) * Kb) * (int64_t)(S * OZ_FRAG)
-                                 : Bs + ((2 * tn + wave - 2) * Kb) * (int64_t)(S * OZ_FRAG);
+  const int8_t *src = (wave < 2) ? As + ((2 * tm + wave) * Kb + kb0) * (int64_t)(S * OZ_FRAG)
+                                 : Bs + ((2 * tn + wave - 2) * Kb + kb0) * (int64_t)(S * OZ_FRAG);
   src += lane * 16;
-  auto copy = [&](int64_t kb, int buf) {
-    const int8_t *s = src + kb * (int64_t)(S * OZ_FRAG);
-    int8_t *d = &lds[buf][wave * S * OZ_FRAG];
+  auto copy = [&](int64_t h) {
+    if (h >= nst) return;
+    const int8_t *s = src + (h >> 1) * (int64_t)(S * OZ_FRAG) + ((h & 1) ? G0 * OZ_FRAG : 0);
+    int8_t *d = &lds[(int)(h & 3) * SLOT + wave * G0 * OZ_FRAG];
+    if ((h & 1) == 0) {
 #pragma unroll
-    for (int t = 0; t < S; ++t)
-      __builtin_amdgcn_global_load_lds((glb_void *)(s + t * OZ_FRAG), (lds_void *)(d + t * OZ_FRAG), 16, 0, 0);
+      for (int t = 0; t < G0; ++t)
+        __builtin_amdgcn_global_load_lds((glb_void *)(s + t * OZ_FRAG), (lds_void *)(d + t * OZ_FRAG), 16, 0, 0);
+    } else {
+#pragma unroll
+      for (int t = 0; t < G1; ++t)
+        __builtin_amdgcn_global_load_lds((glb_void *)(s + t * OZ_FRAG), (lds_void *)(d + t * OZ_FRAG), 16, 0, 0);
+    }
   };
   const int wr = wave >> 1, wc = wave & 1;
   i32x16v acc[S];
@@ -259,24 +275,57 @@ k_oz_gemm(const int8_t *__restrict__ As, const int8_t *__restrict__ Bs, int64_t 
   for (int d = 0; d < S; ++d)
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[d][r] = 0;
-  if (kb0 < kb1) copy(kb0, 0);
-  for (int64_t kb = kb0; kb < kb1; ++kb) {
-    const int buf = (int)((kb - kb0) & 1);
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    i32x4v fa[S], fb[S];
-    const int8_t *la = &lds[buf][wr * S * OZ_FRAG + lane * 16];
-    const int8_t *lb = &lds[buf][(2 + wc) * S * OZ_FRAG + lane * 16];
+  copy(0);
+  copy(1);
+  copy(2);
+  i32x4v fa[S], fb[S];
+  for (int64_t h = 0; h < nst; h += 2) {
+    // slice group 0 of k block h/2
+    if (h + 2 < nst)
+      __builtin_amdgcn_s_waitcnt(WAIT_S);
+    else
+      __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_s_barrier();
+    {
+      const int8_t *base = &lds[(int)(h & 3) * SLOT + lane * 16];
 #pragma unroll
-    for (int t = 0; t < S; ++t) {
-      fa[t] = *reinterpret_cast<const i32x4v *>(la + t * OZ_FRAG);
-      fb[t] = *reinterpret_cast<const i32x4v *>(lb + t * OZ_FRAG);
+      for (int t = 0; t < G0; ++t) {
+        fa[t] = *reinterpret_cast<const i32x4v *>(base + (wr * G0 + t) * OZ_FRAG);
+        fb[t] = *reinterpret_cast<const i32x4v *>(base + ((2 + wc) * G0 + t) * OZ_FRAG);
+      }
     }
-    if (kb + 1 < kb1) copy(kb + 1, buf ^ 1);
+    copy(h + 3);
 #pragma unroll
-    for (int t = 0; t < S; ++t)
+    for (int mm = 0; mm < G0; ++mm) {
 #pragma unroll
-      for (int u = 0; u + t < S; ++u) acc[t + u] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[t], fb[u], acc[t + u], 0, 0, 0);
+      for (int u = 0; u <= mm; ++u) acc[mm + u] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[mm], fb[u], acc[mm + u], 0, 0, 0);
+#pragma unroll
+      for (int t = 0; t < mm; ++t) acc[t + mm] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[t], fb[mm], acc[t + mm], 0, 0, 0);
+    }
+    // slice group 1
+    if (h + 3 < nst)
+      __builtin_amdgcn_s_waitcnt(WAIT_S);
+    else
+      __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_s_barrier();
+    {
+      const int8_t *base = &lds[(int)((h + 1) & 3) * SLOT + lane * 16];
+#pragma unroll
+      for (int t = 0; t < G1; ++t) {
+        fa[G0 + t] = *reinterpret_cast<const i32x4v *>(base + (wr * G0 + t) * OZ_FRAG);
+        fb[G0 + t] = *reinterpret_cast<const i32x4v *>(base + ((2 + wc) * G0 + t) * OZ_FRAG);
+      }
+    }
+    copy(h + 4);
+#pragma unroll
+    for (int mm = G0; mm < S; ++mm) {
+#pragma unroll
+      for (int u = 0; u <= mm && mm + u < S; ++u)
+        acc[mm + u] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[mm], fb[u], acc[mm + u], 0, 0, 0);
+#pragma unroll
+      for (int t = 0; t < mm && t + mm < S; ++t)
+        acc[t + mm] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[t], fb[mm], acc[t + mm], 0, 0, 0);
+    }
   }
   // V = sum_d acc_d 256^(S-1-d), Horner in 6 signed words; result element r of this lane is
   // row 8(r/4) + 4(lane/32) + r%4, column lane%32 of the wave's block
