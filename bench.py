@@ -1,17 +1,28 @@
 """Benchmark of the QuadBLAS binary128 hot path on one MI355X.
 
-Workload (the reference's memory-bound Level 1/2 configuration, synthetic seeded inputs generated
-on the device: every entry is a + b*2^-60 with a, b uniform [-1, 1) doubles, summed in binary128):
-  qdot  n = 2^24                          (32 B per element read)
-  qgemv 8192 x 8192 RowMajor, alpha = 1.5, beta = -0.5
-  qgemv 8192 x 8192 ColMajor, alpha = 1.5, beta = -0.5
-One step = one call of each.  value = algorithmic HBM bytes of the step / step time (GB/s), with
-all operands resident in HBM before the timed region.  Algorithmic bytes: dot 2*16*n, gemv
-16*(m*n + n + 2*m) (A and x read, y read and written).
+Headline (`value`): qgemm M = N = K = 4096, RowMajor, alpha = 1, beta = 0, entries with random sign,
+magnitude log-uniform over [2^-30, 2^30) and a full 113-bit mantissa (seeded device generator),
+in the fastest form that meets 2^-104 relative error: OZAKI (int8 slices on the matrix cores with
+exact int32 accumulation and exact recombination; every entry is certified by an a-priori bound to
+<= 2^-105.9 relative error or recomputed exactly).  value = 2 N^3 / (ms per call), quad-equivalent
+GFLOP/s (the reference's own GFLOPS formula, benchmarks/benchmark.cpp:198-202).  Inputs are resident
+in HBM before the timed region; one step = one qgemm call.
 
-Also reported: per-kernel times, the roofline object for the dominant kernel, and the CPU
-baseline (the oracle's restatement of the reference's OpenMP loops on this host's cores).
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--form reference|exact|dd|auto]
+Also reported:
+  qgemm_forms     the same problem in the plain soft-binary128 form (the reference's summation
+                  order and blocking, one correctly rounded FMA at a time) and in the OZAKI form.
+  memory_bound    qdot n = 2^24 and qgemv 8192 x 8192 RowMajor / ColMajor (alpha = 1.5,
+                  beta = -0.5, entries a + b 2^-60) in the DD and EXACT forms: GB/s against the
+                  8 TB/s HBM peak.  Algorithmic bytes: dot 32 n, gemv 16 (m n + n + 2 m).
+  roofline        the slice-product kernel of the headline call: int8 ops 2 M N K P per launch
+                  (P = S(S+1)/2 slice pairs) over its launch duration, timed with HIP events
+                  recorded by the library on the call's stream, against 5000 TOP/s dense int8.
+                  traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE runs
+                  (profiles/ozaki_traffic.json) when present.
+  cpu_baseline    the oracle's restatement of the reference qgemm (level3.hpp blocking, soft
+                  binary128 FMA, OpenMP over the host cores) on a 256-row slab of the same
+                  4096^3 problem, scaled to GFLOP/s (per-element cost is uniform).
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-plain] [--size N]
 """
 import argparse
 import json
@@ -24,6 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+INT8_PEAK_TOPS = 5000.0
 METRIC = "quad-equiv GFLOP/s, qgemm 4096³ at ≤2^-104 rel err; qdot/qgemv HBM GB/s"
 
 
@@ -32,42 +44,81 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--form", default="auto", choices=["auto", "reference", "exact", "dd"])
+    p.add_argument("--size", type=int, default=4096, help="qgemm M = N = K")
+    p.add_argument("--slices", type=int, default=0, help="OZAKI slice count (0 = library choice)")
+    p.add_argument("--no-plain", action="store_true", help="skip the plain soft-binary128 qgemm")
+    p.add_argument("--no-memory", action="store_true", help="skip qdot / qgemv")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    p.add_argument("--cpu-rows", type=int, default=256, help="rows of C in the CPU qgemm slab")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="first run one device copy of 1 GiB (a known-size reference for HBM counters)")
     p.add_argument("--n-dot", type=int, default=1 << 24)
     p.add_argument("--m", type=int, default=8192)
-    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    p.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0 = OMP default)")
-    p.add_argument("--gemm", default="", help="also time qgemm 4096^3 in these forms (comma list)")
-    p.add_argument("--gemm-size", type=int, default=4096)
     return p.parse_args()
 
 
-def pick_form(Q, form):
-    if form != "auto":
-        return Q.MODES[form]
-    return Q.MODE_REFERENCE
+def events(torch):
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
-    import numpy as np
-    import torch
-    import qblas_amd as Q
-
-    Q.lib()
-    dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
-    stream = torch.cuda.current_stream()
+def bench_ozaki(Q, torch, dev, stream, N, steps, warmup, slices):
     sh = stream.cuda_stream
-    mode = pick_form(Q, args.form)
-    form_name = [k for k, v in Q.MODES.items() if v == mode][0]
+    A = torch.empty((N * N, 2), dtype=torch.int64, device=dev)
+    B = torch.empty((N * N, 2), dtype=torch.int64, device=dev)
+    C = torch.zeros((N * N, 2), dtype=torch.int64, device=dev)
+    Q.dev_fill(Q.FILL_LOG_UNIFORM, 11, N * N, A.data_ptr(), -30, 30, stream=sh)
+    Q.dev_fill(Q.FILL_LOG_UNIFORM, 12, N * N, B.data_ptr(), -30, 30, stream=sh)
+    Q.set_ozaki_slices(slices)
 
-    n, m = args.n_dot, args.m
-    T = Q.get_num_threads()
+    def call():
+        Q.dev_gemm(Q.MODE_OZAKI, 0, N, N, N, 1.0, A.data_ptr(), N, B.data_ptr(), N, 0.0, C.data_ptr(), N, sh)
+
+    Q.set_phase_timing(True)
+    for _ in range(warmup):
+        call()
+    torch.cuda.synchronize()
+    step_ms, phases = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s_ev, e_ev = events(torch)
+        s_ev.record(stream)
+        call()
+        e_ev.record(stream)
+        phases.append(Q.phase_times())       # synchronizes with the call's last phase event
+        e_ev.synchronize()
+        step_ms.append(s_ev.elapsed_time(e_ev))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    Q.set_phase_timing(False)
+    S = Q.ozaki_last_slices()
+    repairs = Q.dev_ozaki_repairs(sh)
+    status = Q.dev_exact_status(sh)
+    ms = statistics.mean(step_ms)
+    ph = {k: round(statistics.mean(p[k] for p in phases), 4) for k in phases[0]}
+    return {"ms": ms, "wall_s": wall, "slices": S, "repairs": int(repairs), "status": int(status), "phases_ms": ph,
+            "quad_GFLOPs": 2.0 * N ** 3 / (ms * 1e-3) / 1e9}
+
+
+def bench_plain(Q, torch, dev, stream, N):
+    sh = stream.cuda_stream
+    A = torch.empty((N * N, 2), dtype=torch.int64, device=dev)
+    B = torch.empty((N * N, 2), dtype=torch.int64, device=dev)
+    C = torch.zeros((N * N, 2), dtype=torch.int64, device=dev)
+    Q.dev_fill(Q.FILL_LOG_UNIFORM, 11, N * N, A.data_ptr(), -30, 30, stream=sh)
+    Q.dev_fill(Q.FILL_LOG_UNIFORM, 12, N * N, B.data_ptr(), -30, 30, stream=sh)
+    torch.cuda.synchronize()
+    s_ev, e_ev = events(torch)
+    s_ev.record(stream)
+    Q.dev_gemm(Q.MODE_REFERENCE, 0, N, N, N, 1.0, A.data_ptr(), N, B.data_ptr(), N, 0.0, C.data_ptr(), N, sh)
+    e_ev.record(stream)
+    torch.cuda.synchronize()
+    ms = s_ev.elapsed_time(e_ev)
+    return {"ms": round(ms, 3), "quad_GFLOPs": round(2.0 * N ** 3 / (ms * 1e-3) / 1e9, 2)}
+
+
+def bench_memory(Q, torch, dev, stream, n, m, steps):
+    sh = stream.cuda_stream
     x = torch.empty((n, 2), dtype=torch.int64, device=dev)
     y = torch.empty((n, 2), dtype=torch.int64, device=dev)
     A = torch.empty((m * m, 2), dtype=torch.int64, device=dev)
@@ -77,132 +128,146 @@ def main():
     res = torch.empty((1, 2), dtype=torch.int64, device=dev)
     for t, seed in ((x, 1), (y, 2), (A, 3), (xv, 4), (yv0, 5)):
         Q.dev_fill(Q.FILL_UNIFORM_FULL, seed, t.shape[0], t.data_ptr(), stream=sh)
-    torch.cuda.synchronize()
-
-    def k_dot():
-        Q.dev_dot(mode, n, x.data_ptr(), 1, y.data_ptr(), 1, T, res.data_ptr(), sh)
-
-    def k_gemv(layout):
-        def f():
-            Q.dev_gemv(mode, layout, m, m, 1.5, A.data_ptr(), m, xv.data_ptr(), 1, -0.5, yv.data_ptr(), 1, sh)
-        return f
-
-    kernels = [("qdot", k_dot, 32.0 * n), ("qgemv_rowmajor", k_gemv(0), 16.0 * (m * m + m + 2 * m)),
-               ("qgemv_colmajor", k_gemv(1), 16.0 * (m * m + m + 2 * m))]
-
-    def reset_y():
-        yv.copy_(yv0)
-
-    for _ in range(args.warmup):
-        for name, f, _b in kernels:
-            reset_y()
+    gemv_bytes = 16.0 * (m * m + m + 2 * m)
+    out = {}
+    for form in ("dd", "exact"):
+        mode = Q.MODES[form]
+        kernels = [("qdot", lambda: Q.dev_dot(mode, n, x.data_ptr(), 1, y.data_ptr(), 1, 1, res.data_ptr(), sh),
+                    32.0 * n)]
+        for lay, name in ((0, "qgemv_rowmajor"), (1, "qgemv_colmajor")):
+            kernels.append((name, (lambda L: lambda: Q.dev_gemv(mode, L, m, m, 1.5, A.data_ptr(), m, xv.data_ptr(), 1,
+                                                                 -0.5, yv.data_ptr(), 1, sh))(lay), gemv_bytes))
+        for _name, f, _b in kernels:   # warm-up
+            yv.copy_(yv0)
             f()
-    torch.cuda.synchronize()
-
-    # timed region: K steps, per-kernel events on the launch stream
-    ev = {name: [] for name, _, _ in kernels}
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        for name, f, _b in kernels:
-            reset_y()
-            s_ev, e_ev = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            s_ev.record(stream)
-            f()
-            e_ev.record(stream)
-            ev[name].append((s_ev, e_ev))
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    per_kernel_ms = {name: statistics.mean(s.elapsed_time(e) for s, e in ev[name]) for name in ev}
-    step_ms = sum(per_kernel_ms.values())
-    step_bytes = sum(b for _, _, b in kernels)
-    value = step_bytes / (step_ms * 1e-3) / 1e9
-
-    dom = max(kernels, key=lambda k: per_kernel_ms[k[0]])
-    achieved = dom[2] / (per_kernel_ms[dom[0]] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom[0],
-                "algorithmic_bytes": dom[2]}
-    per_kernel = {name: {"ms": round(per_kernel_ms[name], 4),
-                         "GB/s": round(b / (per_kernel_ms[name] * 1e-3) / 1e9, 1),
-                         "hbm_frac": round(b / (per_kernel_ms[name] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-                  for name, _, b in kernels}
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(n, m, T, args.cpu_threads)
-
-    gemm = {}
-    for gform in [f for f in args.gemm.split(",") if f]:
-        gemm[gform] = bench_gemm(Q, torch, dev, stream, Q.MODES[gform], args.gemm_size)
-
-    line = {
-        "metric": METRIC, "value": round(value, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(step_ms, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "binary128 (soft, u64/u32 integer ops)" if mode in (0, 1) else "binary128 via f64 double-double",
-        "data": "synthetic (seeded device generator, full 113-bit mantissa)",
-        "config": {"workload": "qdot n=2^24 + qgemv 8192x8192 RowMajor + ColMajor, alpha=1.5 beta=-0.5",
-                   "form": form_name, "dot_T": T},
-        "per_kernel": per_kernel, "roofline": roofline, "cpu_baseline": cpu,
-        "wall_s": round(wall, 3),
-    }
-    if gemm:
-        line["qgemm"] = gemm
-    print(json.dumps(line))
-
-
-def bench_gemm(Q, torch, dev, stream, mode, N, reps=2):
-    """qgemm N^3 RowMajor, alpha = 1, beta = 0, entries with random sign, exponent uniform in
-    [-30, 30) and a full 113-bit mantissa.  Returns time and quad-equiv GFLOP/s (2 N^3 / t)."""
-    sh = stream.cuda_stream
-    A = torch.empty((N * N, 2), dtype=torch.int64, device=dev)
-    B = torch.empty((N * N, 2), dtype=torch.int64, device=dev)
-    C = torch.empty((N * N, 2), dtype=torch.int64, device=dev)
-    Q.dev_fill(Q.FILL_LOG_UNIFORM, 11, N * N, A.data_ptr(), -30, 30, stream=sh)
-    Q.dev_fill(Q.FILL_LOG_UNIFORM, 12, N * N, B.data_ptr(), -30, 30, stream=sh)
-    C.zero_()
-    Q.dev_gemm(mode, 0, N, N, N, 1.0, A.data_ptr(), N, B.data_ptr(), N, 0.0, C.data_ptr(), N, sh)  # warm-up
-    torch.cuda.synchronize()
-    times = []
-    for _ in range(reps):
-        s_ev, e_ev = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        s_ev.record(stream)
-        Q.dev_gemm(mode, 0, N, N, N, 1.0, A.data_ptr(), N, B.data_ptr(), N, 0.0, C.data_ptr(), N, sh)
-        e_ev.record(stream)
         torch.cuda.synchronize()
-        times.append(s_ev.elapsed_time(e_ev))
-    ms = min(times)
-    return {"N": N, "ms": round(ms, 3), "quad_GFLOPs": round(2.0 * N ** 3 / (ms * 1e-3) / 1e9, 2)}
+        res_f = {}
+        for name, f, b in kernels:
+            times = []
+            for _ in range(steps):
+                yv.copy_(yv0)
+                s_ev, e_ev = events(torch)
+                s_ev.record(stream)
+                f()
+                e_ev.record(stream)
+                times.append((s_ev, e_ev))
+            torch.cuda.synchronize()
+            ms = statistics.mean(s.elapsed_time(e) for s, e in times)
+            gbs = b / (ms * 1e-3) / 1e9
+            res_f[name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                           "algorithmic_bytes": b}
+        out[form] = res_f
+    return out
 
 
-def cpu_baseline(n, m, T, threads):
-    """The oracle's restatement of the reference order (level1.hpp dot with T chunks, level2.hpp
-    gemv) on this host's cores: one full step of the same workload, median of 3 runs."""
+def load_traffic():
+    path = os.path.join(ROOT, "profiles", "ozaki_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline_gemm(N, rows):
+    """The oracle's reference-order qgemm (level3.hpp blocking, kc = 126, soft binary128 FMA, OpenMP
+    over rows x columns) on rows x N of C for the same N^3 problem's distribution."""
     import oracle as O
-    import numpy as np
-    if threads:
-        os.environ["OMP_NUM_THREADS"] = str(threads)
+    A = O.fill(1, 11, rows * N, -30, 30)
+    B = O.fill(1, 12, N * N, -30, 30)
+    C = O.fill(0, 13, rows * N)
+    C[:] = 0
+    t0 = time.perf_counter()
+    O.gemm(0, rows, N, N, 1.0, A, N, B, N, 0.0, C, N)
+    t = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    return {"value": round(2.0 * rows * N * N / t / 1e9, 4), "unit": "quad-equiv GFLOP/s", "cores": cores,
+            "kind": "port",
+            "sample": f"reference-order qgemm on a {rows} x {N} slab of C ({rows}x{N}x{N} FMAs, same entry "
+                      f"distribution), oracle soft-binary128 FMA, OpenMP; {t:.1f} s",
+            "seconds": round(t, 2), "extrapolated_full_problem_s": round(t * N / rows, 1)}
+
+
+def cpu_baseline_memory(n, m):
+    """The oracle's reference-order qdot (T chunks) + qgemv RowMajor + ColMajor, one step."""
+    import oracle as O
     x = O.fill(0, 1, n)
     y = O.fill(0, 2, n)
     A = O.fill(0, 3, m * m)
     xv = O.fill(0, 4, m)
     yv0 = O.fill(0, 5, m)
-    times = []
-    for _ in range(3):
-        yv = yv0.copy()
-        t0 = time.perf_counter()
-        O.dot(n, x, 1, y, 1, T)
-        O.gemv(0, m, m, 1.5, A, m, xv, 1, -0.5, yv, 1)
-        yv = yv0.copy()
-        O.gemv(1, m, m, 1.5, A, m, xv, 1, -0.5, yv, 1)
-        times.append(time.perf_counter() - t0)
-    t = statistics.median(times)
+    T = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    t0 = time.perf_counter()
+    O.dot(n, x, 1, y, 1, T)
+    yv = yv0.copy()
+    O.gemv(0, m, m, 1.5, A, m, xv, 1, -0.5, yv, 1)
+    yv = yv0.copy()
+    O.gemv(1, m, m, 1.5, A, m, xv, 1, -0.5, yv, 1)
+    t = time.perf_counter() - t0
     step_bytes = 32.0 * n + 2 * 16.0 * (m * m + m + 2 * m)
-    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
-    return {"value": round(step_bytes / t / 1e9, 3), "unit": "GB/s", "cores": cores, "kind": "port",
-            "sample": "one full step (qdot 2^24 + qgemv 8192^2 RowMajor + ColMajor), reference order, "
-                      "oracle soft-binary128 FMA, OpenMP, median of 3",
-            "seconds_per_step": round(t, 3)}
+    return {"value": round(step_bytes / t / 1e9, 3), "unit": "GB/s", "cores": T, "kind": "port",
+            "sample": "one step: qdot 2^24 + qgemv 8192^2 RowMajor + ColMajor, reference order, oracle soft "
+                      "binary128 FMA, OpenMP", "seconds": round(t, 3)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+    import qblas_amd as Q
+
+    Q.lib()
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    N = args.size
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.int32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    oz = bench_ozaki(Q, torch, dev, stream, N, args.steps, args.warmup, args.slices)
+    S = oz["slices"]
+    pairs = S * (S + 1) // 2
+    ops = 2.0 * N * N * N * pairs
+    prod_ms = oz["phases_ms"]["products"]
+    achieved = ops / (prod_ms * 1e-3) / 1e12
+    traffic = load_traffic()
+    roofline = {"bound": "mfma", "achieved": round(achieved, 1), "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
+                "frac": round(achieved / INT8_PEAK_TOPS, 4),
+                "traffic": (traffic or {}).get("k_oz_gemm", {}).get("hbm_bytes_per_launch"),
+                "kernel": f"k_oz_gemm<{S}> (slice products, v_mfma_i32_32x32x32_i8)",
+                "algorithmic_ops_per_launch": ops, "launch_ms": prod_ms}
+    forms = {"ozaki": {"ms": round(oz["ms"], 3), "quad_GFLOPs": round(oz["quad_GFLOPs"], 1), "slices": S,
+                       "repaired_entries": oz["repairs"], "phases_ms": oz["phases_ms"]}}
+    if not args.no_plain:
+        forms["plain"] = bench_plain(Q, torch, dev, stream, N)
+        forms["ozaki"]["speedup_vs_plain"] = round(forms["plain"]["ms"] / oz["ms"], 1)
+    torch.cuda.empty_cache()
+    memory = None
+    if not args.no_memory:
+        memory = bench_memory(Q, torch, dev, stream, args.n_dot, args.m, max(args.steps, 3))
+    cpu = cpu_mem = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_gemm(N, args.cpu_rows)
+        cpu_mem = cpu_baseline_memory(args.n_dot, args.m)
+
+    line = {
+        "metric": METRIC, "value": round(oz["quad_GFLOPs"], 2), "unit": "quad-equiv GFLOP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(oz["ms"], 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "i8 (int8 x int8 -> int32 MFMA; binary128 in/out)",
+        "data": "synthetic (seeded device generator: random sign, log-uniform magnitude [2^-30, 2^30), "
+                "full 113-bit mantissa)",
+        "config": {"workload": f"qgemm {N}x{N}x{N} RowMajor alpha=1 beta=0", "form": "ozaki", "slices": S,
+                   "error_bound": "each entry <= 2^-105.9 rel (certified) or correctly rounded (repaired)"},
+        "roofline": roofline, "cpu_baseline": cpu, "qgemm_forms": forms, "memory_bound": memory,
+        "cpu_baseline_memory_bound": cpu_mem, "ozaki_status": oz["status"], "wall_s": round(oz["wall_s"], 3),
+    }
+    print(json.dumps(line))
 
 
 if __name__ == "__main__":

@@ -116,6 +116,15 @@ int qblas_get_ozaki_slices(void);
    entries were recomputed exactly because the bound could not certify them (-1 if there was no
    such call or on a HIP error). */
 long long qblas_hip_ozaki_repairs(void *stream);
+/* Slice count used by the last QBLAS_MODE_OZAKI gemm on this thread (0 if none). */
+int qblas_hip_ozaki_last_slices(void);
+
+/* Phase timing of QBLAS_MODE_OZAKI gemm calls (off by default; costs four event records per
+   call).  qblas_hip_phase_times writes the durations in ms of the last such call on this thread
+   -- [0] scaling and slicing, [1] slice products (matrix cores), [2] finish and certification,
+   [3] exact repair -- after synchronizing with it; returns 4, or -1 if no timed call exists. */
+void qblas_hip_set_phase_timing(int on);
+int qblas_hip_phase_times(float *ms);
 
 /* Last error message of the calling thread ("" if none); cleared by qblas_clear_error(). */
 const char *quadblas_hip_last_error(void);

@@ -85,6 +85,9 @@ def lib():
         "qblas_set_ozaki_slices": (I, [I]),
         "qblas_get_ozaki_slices": (I, []),
         "qblas_hip_ozaki_repairs": (ctypes.c_longlong, [P]),
+        "qblas_hip_ozaki_last_slices": (I, []),
+        "qblas_hip_set_phase_timing": (None, [I]),
+        "qblas_hip_phase_times": (I, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -278,6 +281,27 @@ def dev_exact_status(stream=0):
 def dev_ozaki_repairs(stream=0):
     """Entries of the last OZAKI gemm of this thread that were recomputed exactly (-1: none)."""
     return lib().qblas_hip_ozaki_repairs(stream)
+
+
+def ozaki_last_slices():
+    """Slice count used by the last OZAKI gemm of this thread (0: none)."""
+    return lib().qblas_hip_ozaki_last_slices()
+
+
+OZAKI_PHASES = ("slicing", "products", "finish", "repair")
+
+
+def set_phase_timing(on):
+    """Record HIP events between the phases of OZAKI gemm calls (on the call's stream)."""
+    lib().qblas_hip_set_phase_timing(1 if on else 0)
+
+
+def phase_times():
+    """{phase: ms} of the last OZAKI gemm of this thread (synchronizes with it), or None."""
+    buf = (ctypes.c_float * 4)()
+    if lib().qblas_hip_phase_times(ctypes.cast(buf, ctypes.c_void_p)) != 4:
+        return None
+    return dict(zip(OZAKI_PHASES, [float(v) for v in buf]))
 
 
 FILL_UNIFORM_FULL = 0

@@ -139,6 +139,10 @@ int ws_release(Workspace &w, hipStream_t s) {
 thread_local Workspace t_ws;           // partials / scaled x for the device routines
 thread_local const uint32_t *t_exact_status = nullptr;  // status word of the last EXACT-form call
 thread_local const uint32_t *t_oz_hdr = nullptr;        // header of the last OZAKI-form call
+std::atomic<int> g_phase_timing{0};                     // record phase events in OZAKI calls
+thread_local hipEvent_t t_oz_ev[5] = {};
+thread_local bool t_oz_ev_valid = false;
+thread_local int t_oz_slices = 0;                       // slice count of the last OZAKI call
 
 // After a synchronous EXACT-form call: report a product that fell outside its window.
 int check_exact_status(hipStream_t s) {
@@ -282,11 +286,19 @@ int dev_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, q128 alpha, 
   const int colmajor = layout != 0;
   if (mode == QBLAS_MODE_OZAKI) {
     const int S = oz_slices(m, n, k);
+    t_oz_slices = S;
     void *ws;
     int rc = ws_acquire(t_ws, qbk::gemm_ozaki_ws(m, n, k, S), s, &ws);
     if (rc) return rc;
+    hipEvent_t *ev = nullptr;
+    if (g_phase_timing.load()) {
+      if (!t_oz_ev[0])
+        for (auto &e : t_oz_ev) QB_CHECK(hipEventCreate(&e));
+      ev = t_oz_ev;
+    }
+    t_oz_ev_valid = ev != nullptr;
     QB_CHECK(qbk::launch_gemm_ozaki(colmajor, m, n, k, S, alpha, (const q128 *)A, lda, (const q128 *)B, ldb, beta,
-                                    (q128 *)C, ldc, ws, s));
+                                    (q128 *)C, ldc, ws, s, ev));
     t_exact_status = (const uint32_t *)ws;
     t_oz_hdr = (const uint32_t *)ws;
     return ws_release(t_ws, s);
@@ -469,6 +481,15 @@ long long qblas_hip_ozaki_repairs(void *stream) {
   uint32_t v = 0;
   if (qbk::ozaki_repair_count(t_oz_hdr, &v, (hipStream_t)stream) != hipSuccess) return -1;
   return (long long)v;
+}
+int qblas_hip_ozaki_last_slices(void) { return t_oz_slices; }
+void qblas_hip_set_phase_timing(int on) { g_phase_timing.store(on ? 1 : 0); }
+int qblas_hip_phase_times(float *ms) {
+  if (!t_oz_ev_valid) return -1;
+  if (hipEventSynchronize(t_oz_ev[4]) != hipSuccess) return -1;
+  for (int i = 0; i < 4; ++i)
+    if (hipEventElapsedTime(&ms[i], t_oz_ev[i], t_oz_ev[i + 1]) != hipSuccess) return -1;
+  return 4;
 }
 void qblas_set_kc(int kc) { g_kc.store(kc > 0 ? kc : 126); }
 int qblas_get_kc(void) { return g_kc.load(); }

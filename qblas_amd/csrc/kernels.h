@@ -46,6 +46,7 @@ constexpr int OZ_SMIN = 8, OZ_SMAX = 19;
 size_t gemm_ozaki_ws(int64_t m, int64_t n, int64_t k, int S);
 hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int S, q128 alpha, const q128 *A,
                              int64_t lda, const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc, void *ws,
-                             hipStream_t st);
+                             hipStream_t st, hipEvent_t *ev /* null, or 5 events: start, after slicing, after
+                             the slice products, after the finish, after the repair */);
 hipError_t ozaki_repair_count(const void *ws, uint32_t *out, hipStream_t st);
 }  // namespace qbk

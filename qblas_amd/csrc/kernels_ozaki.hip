@@ -509,18 +509,21 @@ __global__ void __launch_bounds__(OZR_BS) k_oz_repair(int64_t K, const q128 *__r
 size_t gemm_ozaki_ws(int64_t m, int64_t n, int64_t k, int S) { return oz_plan(m, n, k, S).total; }
 
 template <int S>
-static hipError_t oz_launch_slices(const oz_layout &L, const q128 *A, int64_t m, int64_t k, int64_t asr, int64_t asc,
-                                   const q128 *B, int64_t n, int64_t bsr, int64_t bsc, char *ws, hipStream_t st) {
+static void oz_launch_slices(const oz_layout &L, const q128 *A, int64_t m, int64_t k, int64_t asr, int64_t asc,
+                             const q128 *B, int64_t n, int64_t bsr, int64_t bsc, char *ws, hipStream_t st) {
   const int64_t ta = (L.Mp / 32) * L.Kb * 64, tb = (L.Np / 32) * L.Kb * 64;
   k_oz_slice<S><<<(unsigned)((ta + 255) / 256), 256, 0, st>>>(A, m, k, asr, asc, (const int32_t *)(ws + L.eA),
                                                                (int8_t *)(ws + L.sA), L.Kb, ta);
   // B as rows j, columns l
   k_oz_slice<S><<<(unsigned)((tb + 255) / 256), 256, 0, st>>>(B, n, k, bsc, bsr, (const int32_t *)(ws + L.eB),
                                                                (int8_t *)(ws + L.sB), L.Kb, tb);
+}
+
+template <int S>
+static void oz_launch_products(const oz_layout &L, char *ws, hipStream_t st) {
   dim3 grid((unsigned)(L.Np / OZ_TILE), (unsigned)(L.Mp / OZ_TILE), (unsigned)L.npass);
   k_oz_gemm<S><<<grid, 256, 0, st>>>((const int8_t *)(ws + L.sA), (const int8_t *)(ws + L.sB), L.Kb, L.Mp, L.Np,
                                      (uint32_t *)(ws + L.part));
-  return hipGetLastError();
 }
 
 static void oz_scale(const q128 *X, int64_t R, int64_t Cn, int64_t sr, int64_t sc, int32_t *E, uint32_t *F,
@@ -539,7 +542,7 @@ static void oz_scale(const q128 *X, int64_t R, int64_t Cn, int64_t sr, int64_t s
 
 hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int S, q128 alpha, const q128 *A,
                              int64_t lda, const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc, void *wsv,
-                             hipStream_t st) {
+                             hipStream_t st, hipEvent_t *ev) {
   if (S < OZ_SMIN || S > OZ_SMAX) return hipErrorInvalidValue;
   char *ws = (char *)wsv;
   const oz_layout L = oz_plan(m, n, k, S);
@@ -550,19 +553,30 @@ hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int 
   int32_t *eA = (int32_t *)(ws + L.eA), *eB = (int32_t *)(ws + L.eB);
   uint32_t *fA = (uint32_t *)(ws + L.fA), *fB = (uint32_t *)(ws + L.fB);
   uint32_t *hdr = (uint32_t *)ws;
+  if (ev) hipEventRecord(ev[0], st);
   const int64_t ninit = (L.Mp > L.Np ? L.Mp : L.Np) > OZ_HDR / 4 ? (L.Mp > L.Np ? L.Mp : L.Np) : OZ_HDR / 4;
   k_oz_init<<<(unsigned)((ninit + 255) / 256), 256, 0, st>>>(eA, fA, L.Mp, eB, fB, L.Np, hdr);
   oz_scale(A, m, k, asr, asc, eA, fA, st);
   oz_scale(B, n, k, bsc, bsr, eB, fB, st);  // B columns: r = j, c = l
-  hipError_t e = hipErrorInvalidValue;
   switch (S) {
 #define QB_OZ_CASE(s) \
-  case s: e = oz_launch_slices<s>(L, A, m, k, asr, asc, B, n, bsr, bsc, ws, st); break;
+  case s: oz_launch_slices<s>(L, A, m, k, asr, asc, B, n, bsr, bsc, ws, st); break;
     QB_OZ_CASE(8) QB_OZ_CASE(9) QB_OZ_CASE(10) QB_OZ_CASE(11) QB_OZ_CASE(12) QB_OZ_CASE(13) QB_OZ_CASE(14)
     QB_OZ_CASE(15) QB_OZ_CASE(16) QB_OZ_CASE(17) QB_OZ_CASE(18) QB_OZ_CASE(19)
 #undef QB_OZ_CASE
   default: break;
   }
+  if (ev) hipEventRecord(ev[1], st);
+  switch (S) {
+#define QB_OZ_CASE(s) \
+  case s: oz_launch_products<s>(L, ws, st); break;
+    QB_OZ_CASE(8) QB_OZ_CASE(9) QB_OZ_CASE(10) QB_OZ_CASE(11) QB_OZ_CASE(12) QB_OZ_CASE(13) QB_OZ_CASE(14)
+    QB_OZ_CASE(15) QB_OZ_CASE(16) QB_OZ_CASE(17) QB_OZ_CASE(18) QB_OZ_CASE(19)
+#undef QB_OZ_CASE
+  default: break;
+  }
+  if (ev) hipEventRecord(ev[2], st);
+  hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   // kfac = ceil(log2(K (S+1) / 4))
   const double kb = (double)k * (S + 1) / 4.0;
@@ -572,8 +586,10 @@ hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int 
   k_oz_finish<<<(unsigned)((mn + 255) / 256), 256, 0, st>>>(m, n, S, kfac, (int)L.npass, (const uint32_t *)(ws + L.part),
                                                             L.Mp, L.Np, eA, fA, eB, fB, alpha, beta, C, csr, csc, hdr,
                                                             (uint32_t *)(ws + L.list));
+  if (ev) hipEventRecord(ev[3], st);
   k_oz_repair<<<OZ_REPAIR_BLOCKS, OZR_BS, 0, st>>>(k, A, asr, asc, B, bsr, bsc, eA, eB, alpha, beta, C, csr, csc, hdr,
                                                    (const uint32_t *)(ws + L.list));
+  if (ev) hipEventRecord(ev[4], st);
   return hipGetLastError();
 }
 
