@@ -216,18 +216,15 @@ def test_dot_thread_chunks(O):
     y = qa(O, [1] * n)
     got = {T: val(O.to_pair(O.dot(n, x, 1, y, 1, T))) for T in (1, 2, 3, 1000)}
     assert got == {1: 2, 2: 1, 3: 1, 1000: 1}
-    # last chunk takes the remainder (level1.hpp:58-59): n = 1001, T = 2 -> chunks [0,500),
-    # [500,1001).  Put -B at 1000: in the 501-long last chunk it is the odd tail, applied after the
-    # lane sum: lanes give 1 (x501? no: offsets 0,1 = x500, x501) ... use a direct construction:
+    # The last chunk takes the remainder (level1.hpp:58-59): n = 1001, T = 2 gives chunks [0, 500)
+    # and [500, 1001).  With 1 at offsets 500, 501 and B at 1000, chunk 1's lanes add to 2 and its
+    # odd tail fma(B, 1, 2) = B + 2 is exact; chunk 0 is 0.
     v = [0] * 1001
     v[500], v[501], v[1000] = 1, 1, B113
     x = qa(O, v)
     y = qa(O, [1] * 1001)
-    # chunk 1: lanes give 1 and 1 -> add = 2; tail fma(B, 1, 2) = B + 2 (exact).  Chunk 0 = 0.
     assert val(O.to_pair(O.dot(1001, x, 1, y, 1, 2))) == B113 + 2
-    # T = 1: one chunk of 1001: lane 0 gets x500 (even) = 1, lane 1 gets x501 = 1 -> 2; tail B+2.
-    # With the tail added *before* the lane sum the result would differ only in rounding; here the
-    # discriminating case is below.
+    # The odd tail is applied after the lane sum (level1.hpp:27-32), which rounding distinguishes:
     v = [0] * 1001
     v[0], v[1], v[1000] = B113, 1, -B113
     x = qa(O, v)
