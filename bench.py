@@ -10,7 +10,8 @@ in HBM before the timed region; one step = one qgemm call.
 
 Also reported:
   qgemm_forms     the same problem in the plain soft-binary128 form (the reference's summation
-                  order and blocking, one correctly rounded FMA at a time) and in the OZAKI form.
+                  order and blocking, one correctly rounded FMA at a time), the DD form (double-
+                  double on FP64 FMA) and the OZAKI form.
   memory_bound    qdot n = 2^24 and qgemv 8192 x 8192 RowMajor / ColMajor (alpha = 1.5,
                   beta = -0.5, entries a + b 2^-60) in the DD and EXACT forms: GB/s against the
                   8 TB/s HBM peak.  Algorithmic bytes: dot 32 n, gemv 16 (m n + n + 2 m).
@@ -98,6 +99,29 @@ def bench_ozaki(Q, torch, dev, stream, N, steps, warmup, slices):
     ph = {k: round(statistics.mean(p[k] for p in phases), 4) for k in phases[0]}
     return {"ms": ms, "wall_s": wall, "slices": S, "repairs": int(repairs), "status": int(status), "phases_ms": ph,
             "quad_GFLOPs": 2.0 * N ** 3 / (ms * 1e-3) / 1e9}
+
+
+def bench_form_once(Q, torch, dev, stream, N, mode):
+    """One warm-up and one timed qgemm N^3 call of the given form (same data as the headline)."""
+    sh = stream.cuda_stream
+    A = torch.empty((N * N, 2), dtype=torch.int64, device=dev)
+    B = torch.empty((N * N, 2), dtype=torch.int64, device=dev)
+    C = torch.zeros((N * N, 2), dtype=torch.int64, device=dev)
+    Q.dev_fill(Q.FILL_LOG_UNIFORM, 11, N * N, A.data_ptr(), -30, 30, stream=sh)
+    Q.dev_fill(Q.FILL_LOG_UNIFORM, 12, N * N, B.data_ptr(), -30, 30, stream=sh)
+
+    def call():
+        Q.dev_gemm(mode, 0, N, N, N, 1.0, A.data_ptr(), N, B.data_ptr(), N, 0.0, C.data_ptr(), N, sh)
+
+    call()
+    torch.cuda.synchronize()
+    s_ev, e_ev = events(torch)
+    s_ev.record(stream)
+    call()
+    e_ev.record(stream)
+    torch.cuda.synchronize()
+    ms = s_ev.elapsed_time(e_ev)
+    return {"ms": round(ms, 3), "quad_GFLOPs": round(2.0 * N ** 3 / (ms * 1e-3) / 1e9, 2)}
 
 
 def bench_plain(Q, torch, dev, stream, N):
@@ -244,8 +268,11 @@ def main():
                 "algorithmic_ops_per_launch": ops, "launch_ms": prod_ms}
     forms = {"ozaki": {"ms": round(oz["ms"], 3), "quad_GFLOPs": round(oz["quad_GFLOPs"], 1), "slices": S,
                        "repaired_entries": oz["repairs"], "phases_ms": oz["phases_ms"]}}
+    forms["dd"] = bench_form_once(Q, torch, dev, stream, N, Q.MODE_DD)
+    forms["dd"]["speedup_vs_plain"] = None
     if not args.no_plain:
         forms["plain"] = bench_plain(Q, torch, dev, stream, N)
+        forms["dd"]["speedup_vs_plain"] = round(forms["plain"]["ms"] / forms["dd"]["ms"], 1)
         forms["ozaki"]["speedup_vs_plain"] = round(forms["plain"]["ms"] / oz["ms"], 1)
     torch.cuda.empty_cache()
     memory = None

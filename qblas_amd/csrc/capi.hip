@@ -303,6 +303,14 @@ int dev_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, q128 alpha, 
     t_oz_hdr = (const uint32_t *)ws;
     return ws_release(t_ws, s);
   }
+  if (mode == QBLAS_MODE_DD) {
+    void *ws;
+    int rc = ws_acquire(t_ws, qbk::gemm_dd_ws(m, n, k), s, &ws);
+    if (rc) return rc;
+    QB_CHECK(qbk::launch_gemm_dd(colmajor, m, n, k, alpha, (const q128 *)A, lda, (const q128 *)B, ldb, beta,
+                                 (q128 *)C, ldc, ws, s));
+    return ws_release(t_ws, s);
+  }
   if (mode != QBLAS_MODE_REFERENCE) {
     set_err_msg("qblas_hip_gemm: mode not available");
     return QBLAS_ERR_MODE;

@@ -30,6 +30,10 @@ hipError_t launch_gemv_dd(int colmajor, int64_t m, int64_t n, q128 alpha, const 
                           const q128 *x, int64_t incx, q128 beta, q128 *y, int64_t incy, void *ws,
                           hipStream_t st);
 
+size_t gemm_dd_ws(int64_t m, int64_t n, int64_t k);
+hipError_t launch_gemm_dd(int colmajor, int64_t m, int64_t n, int64_t k, q128 alpha, const q128 *A, int64_t lda,
+                          const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc, void *ws, hipStream_t st);
+
 // exact (correctly rounded) form (kernels_exact.hip); ws[0] (u32) is the status word:
 // bit 0 set when a product fell outside its window (the output is then NaN)
 size_t dot_exact_ws(int64_t n);

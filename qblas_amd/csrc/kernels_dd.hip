@@ -182,6 +182,97 @@ __global__ void k_gemv_col_dd_final(int64_t m, int nchunks, const dd *__restrict
 }
 
 // ------------------------------------------------------------------------------------------
+// gemm (level3.hpp:215-336 semantics, C = alpha * A B + beta * C, RowMajor; a ColMajor call is
+// the RowMajor product C^T = B^T A^T on the same storage)
+// ------------------------------------------------------------------------------------------
+constexpr int GD_T = 64;   // output tile
+constexpr int GD_K = 16;   // k step staged in LDS
+
+// out[r * Cp + c] = dd(X[r * ld + c]) for r < R, c < C; zeros in the padding up to Rp x Cp
+__global__ void k_rows_to_dd(const q128 *__restrict__ X, int64_t R, int64_t C, int64_t ld, dd *__restrict__ out,
+                             int64_t Rp, int64_t Cp) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= Rp * Cp) return;
+  const int64_t r = g / Cp, c = g - r * Cp;
+  st_dd(out + g, (r < R && c < C) ? q_to_dd(ldq(X + r * ld + c)) : dd_zero());
+}
+
+// C entry = RN(alpha * RN(acc) + RN(beta * c)), the reference's epilogue (level3.hpp:297-300);
+// kept out of line so the 16-entry epilogue loop stays unrolled with the accumulators in registers
+__device__ __attribute__((noinline)) q128 dd_gemm_out(double hi, double lo, q128 alpha, q128 beta, q128 c) {
+  dd v;
+  v.hi = hi;
+  v.lo = lo;
+  return fma_packed(alpha, dd_to_q(v), mul_packed(beta, c));
+}
+
+// 256 threads = 16 x 16; thread (tx, ty) owns outputs (ty + 16 r, tx + 16 c), r, c < 4.  Per k
+// step the tile A[64 x 16] (stored k-major in LDS) and B[16 x 64] are staged through registers,
+// so the next step's global loads overlap this step's 16 x 16 double-double FMAs.
+__global__ void __launch_bounds__(256) k_gemm_dd(int64_t M, int64_t N, int64_t Kp, const dd *__restrict__ Ad,
+                                                 const dd *__restrict__ Bd, int64_t Np, q128 alpha, q128 beta,
+                                                 q128 *__restrict__ C, int64_t ldc) {
+  __shared__ dd sA[GD_K][GD_T];
+  __shared__ dd sB[GD_K][GD_T];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4, t = threadIdx.x;
+  const int64_t i0 = (int64_t)blockIdx.y * GD_T, j0 = (int64_t)blockIdx.x * GD_T;
+  // loader roles: A row t/4, k (t%4)*4..+3;  B row t/16, columns (t%16)*4..+3
+  const int ar = t >> 2, ak = (t & 3) * 4, br = t >> 4, bc = (t & 15) * 4;
+  const dd *ap = Ad + (i0 + ar) * Kp + ak;
+  const dd *bp = Bd + (int64_t)br * Np + j0 + bc;
+  dd ra[4], rb[4];
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      ra[q] = ld_dd(ap + k0 + q);
+      rb[q] = ld_dd(bp + k0 * Np + q);
+    }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      sA[ak + q][ar] = ra[q];
+      sB[br][bc + q] = rb[q];
+    }
+  };
+  dd acc[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[r][c] = dd_zero();
+  gload(0);
+  for (int64_t k0 = 0; k0 < Kp; k0 += GD_K) {
+    __syncthreads();
+    sstore();
+    __syncthreads();
+    if (k0 + GD_K < Kp) gload(k0 + GD_K);
+#pragma unroll
+    for (int k = 0; k < GD_K; ++k) {
+      dd a[4], b[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        a[q] = sA[k][ty + 16 * q];
+        b[q] = sB[k][tx + 16 * q];
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = dd_fma(a[r], b[c], acc[r][c]);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int64_t i = i0 + ty + 16 * r, j = j0 + tx + 16 * c;
+      if (i < M && j < N) {
+        q128 *cp = C + i * ldc + j;
+        stq(cp, dd_gemm_out(acc[r][c].hi, acc[r][c].lo, alpha, beta, ldq_cached(cp)));
+      }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------------------------------
 int dot_dd_blocks(int64_t n) {
@@ -237,4 +328,33 @@ hipError_t launch_gemv_dd(int colmajor, int64_t m, int64_t n, q128 alpha, const 
   return hipGetLastError();
 }
 
+
+static int64_t round_up(int64_t v, int64_t q) { return (v + q - 1) / q * q; }
+
+size_t gemm_dd_ws(int64_t m, int64_t n, int64_t k) {
+  const int64_t Mp = round_up(m, GD_T), Np = round_up(n, GD_T), Kp = round_up(k, GD_K);
+  return (size_t)(Mp * Kp + Kp * Np) * sizeof(dd);
+}
+
+hipError_t launch_gemm_dd(int colmajor, int64_t m, int64_t n, int64_t k, q128 alpha, const q128 *A, int64_t lda,
+                          const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc, void *ws, hipStream_t st) {
+  if (colmajor) {  // C^T (n x m, row-major, ld = ldc) = B^T (n x k, ld = ldb) A^T (k x m, ld = lda)
+    const q128 *t = A;
+    A = B;
+    B = t;
+    const int64_t tl = lda;
+    lda = ldb;
+    ldb = tl;
+    const int64_t tm = m;
+    m = n;
+    n = tm;
+  }
+  const int64_t Mp = round_up(m, GD_T), Np = round_up(n, GD_T), Kp = round_up(k, GD_K);
+  dd *Ad = (dd *)ws, *Bd = Ad + Mp * Kp;
+  k_rows_to_dd<<<(unsigned)((Mp * Kp + 255) / 256), 256, 0, st>>>(A, m, k, lda, Ad, Mp, Kp);
+  k_rows_to_dd<<<(unsigned)((Kp * Np + 255) / 256), 256, 0, st>>>(B, k, n, ldb, Bd, Kp, Np);
+  dim3 grid((unsigned)(Np / GD_T), (unsigned)(Mp / GD_T));
+  k_gemm_dd<<<grid, 256, 0, st>>>(m, n, Kp, Ad, Bd, Np, alpha, beta, C, ldc);
+  return hipGetLastError();
+}
 }  // namespace qbk

@@ -553,7 +553,7 @@ hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int 
   int32_t *eA = (int32_t *)(ws + L.eA), *eB = (int32_t *)(ws + L.eB);
   uint32_t *fA = (uint32_t *)(ws + L.fA), *fB = (uint32_t *)(ws + L.fB);
   uint32_t *hdr = (uint32_t *)ws;
-  if (ev) hipEventRecord(ev[0], st);
+  if (ev) (void)hipEventRecord(ev[0], st);
   const int64_t ninit = (L.Mp > L.Np ? L.Mp : L.Np) > OZ_HDR / 4 ? (L.Mp > L.Np ? L.Mp : L.Np) : OZ_HDR / 4;
   k_oz_init<<<(unsigned)((ninit + 255) / 256), 256, 0, st>>>(eA, fA, L.Mp, eB, fB, L.Np, hdr);
   oz_scale(A, m, k, asr, asc, eA, fA, st);
@@ -566,7 +566,7 @@ hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int 
 #undef QB_OZ_CASE
   default: break;
   }
-  if (ev) hipEventRecord(ev[1], st);
+  if (ev) (void)hipEventRecord(ev[1], st);
   switch (S) {
 #define QB_OZ_CASE(s) \
   case s: oz_launch_products<s>(L, ws, st); break;
@@ -575,7 +575,7 @@ hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int 
 #undef QB_OZ_CASE
   default: break;
   }
-  if (ev) hipEventRecord(ev[2], st);
+  if (ev) (void)hipEventRecord(ev[2], st);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   // kfac = ceil(log2(K (S+1) / 4))
@@ -586,10 +586,10 @@ hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int 
   k_oz_finish<<<(unsigned)((mn + 255) / 256), 256, 0, st>>>(m, n, S, kfac, (int)L.npass, (const uint32_t *)(ws + L.part),
                                                             L.Mp, L.Np, eA, fA, eB, fB, alpha, beta, C, csr, csc, hdr,
                                                             (uint32_t *)(ws + L.list));
-  if (ev) hipEventRecord(ev[3], st);
+  if (ev) (void)hipEventRecord(ev[3], st);
   k_oz_repair<<<OZ_REPAIR_BLOCKS, OZR_BS, 0, st>>>(k, A, asr, asc, B, bsr, bsc, eA, eB, alpha, beta, C, csr, csc, hdr,
                                                    (const uint32_t *)(ws + L.list));
-  if (ev) hipEventRecord(ev[4], st);
+  if (ev) (void)hipEventRecord(ev[4], st);
   return hipGetLastError();
 }
 

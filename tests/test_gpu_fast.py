@@ -226,3 +226,31 @@ def test_gemv_dd_within_bound(O, Q, cuda, layout, m, n):
     rows = range(m) if m <= 1000 else range(0, m, 97)
     for i in rows:
         assert _err_ok(pair(got[i]), pair(want[i]), to_fraction(*pair(scale[i]))), i
+
+
+GEMM_DD_CASES = [(0, 64, 64, 64, "full"), (0, 100, 70, 300, "full"), (1, 100, 70, 300, "full"), (0, 65, 129, 33, "log"),
+                 (1, 7, 300, 5, "log"), (0, 1, 1, 1, "full"), (0, 256, 192, 1000, "log")]
+
+
+@pytest.mark.parametrize("layout,m,n,k,kind", GEMM_DD_CASES)
+def test_gemm_dd_within_bound(O, Q, cuda, layout, m, n, k, kind):
+    _say(f"gemm dd layout={layout} m={m} n={n} k={k} kind={kind}")
+    lda, ldb, ldc = (k, n, n) if layout == 0 else (m, k, m)
+    A, _ = _vecs(O, kind, m * k, 61 + m)
+    B, _ = _vecs(O, kind, k * n, 62 + n)
+    C0 = O.fill(0, 63, m * n)
+    alpha, beta = 1.5, -0.5
+    dA, dB, dC = to_dev(A), to_dev(B), to_dev(C0)
+    Q.dev_gemm(Q.MODE_DD, layout, m, n, k, alpha, dA.data_ptr(), lda, dB.data_ptr(), ldb, beta, dC.data_ptr(), ldc)
+    cuda.cuda.synchronize()
+    got = from_dev(dC)
+    st = (lambda ld: (ld, 1)) if layout == 0 else (lambda ld: (1, ld))
+    ii, jj = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
+    ii, jj = ii.ravel(), jj.ravel()
+    want, _ = O.exact_gemm_samples(ii, jj, k, alpha, A, st(lda), B, st(ldb), beta, C0, st(ldc))
+    scale, _ = O.exact_gemm_samples(ii, jj, k, alpha, O.abs_array(A), st(lda), O.abs_array(B), st(ldb), 0.5,
+                                    O.abs_array(C0), st(ldc))
+    cs = st(ldc)
+    for s in range(ii.shape[0]):
+        g = got[ii[s] * cs[0] + jj[s] * cs[1]]
+        assert _err_ok(pair(g), pair(want[s]), to_fraction(*pair(scale[s]))), (ii[s], jj[s])
