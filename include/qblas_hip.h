@@ -149,6 +149,52 @@ int qblas_hip_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, const 
                    int64_t lda, const void *d_B, int64_t ldb, const void *beta, void *d_C, int64_t ldc,
                    void *stream);
 
+/* ---- Batched problems (new; no reference counterpart) --------------------------------------
+   Many independent problems of different sizes in one call: a device array of descriptors, each
+   problem evaluated with the semantics of the single-problem routine in the given mode (REFERENCE:
+   level3.hpp:215-336 / level2.hpp:15-99 per problem, bit-identical to a loop of single calls).
+   All problems share the layout.  The work is one grouped persistent kernel over the tiles (gemm)
+   or rows (gemv) of all problems; the launch sequence allocates nothing when the workspace of an
+   earlier call of the same count suffices, synchronizes nothing and may be captured in a hipGraph.
+   alpha/beta are binary128 {lo, hi} words. */
+typedef struct qblas_gemm_problem {
+  int64_t m, n, k;
+  int64_t lda, ldb, ldc;
+  const void *A;
+  const void *B;
+  void *C;
+  uint64_t alpha[2];
+  uint64_t beta[2];
+  int64_t reserved;
+} qblas_gemm_problem;
+
+typedef struct qblas_gemv_problem {
+  int64_t m, n;
+  int64_t lda, incx, incy;
+  const void *A;
+  const void *x;
+  void *y;
+  uint64_t alpha[2];
+  uint64_t beta[2];
+} qblas_gemv_problem;
+
+/* d_problems: device array of count descriptors; modes REFERENCE and DD. */
+int qblas_hip_gemm_batched(int mode, int layout, int64_t count, const qblas_gemm_problem *d_problems, void *stream);
+int qblas_hip_gemv_batched(int mode, int layout, int64_t count, const qblas_gemv_problem *d_problems, void *stream);
+
+/* Autotuner for the batched gemm: times every tile configuration of the mode on a replica of the
+   batch's first problems (same shapes, synthetic data in a scratch workspace; the caller's C is not
+   touched), keeps the fastest for calls with this (mode, layout, count), and returns its index
+   (>= 0) or an error code (< 0).  Synchronizes; call it outside graph capture.  Without a tuned
+   entry the batched call uses configuration 0. */
+int qblas_hip_gemm_batched_tune(int mode, int layout, int64_t count, const qblas_gemm_problem *d_problems,
+                                void *stream);
+/* Description of tile configuration idx of a mode ("TMxTN ..."), or NULL. */
+const char *qblas_hip_gemm_batched_config(int mode, int idx);
+/* Use configuration idx for batched gemm calls with this (mode, layout, count), as the autotuner
+   does with its choice.  QBLAS_ERR_ARG for an unknown mode or index. */
+int qblas_hip_gemm_batched_select(int mode, int layout, int64_t count, int idx);
+
 /* Seeded synthetic inputs written to device memory (definitions shared with the test oracle):
    kind 0: a + b*2^-60 summed in binary128, a, b uniform [-1, 1) doubles (full 113-bit mantissa);
    kind 1: random sign, exponent uniform in [lo_exp, hi_exp), 112 random fraction bits;

@@ -87,6 +87,11 @@ def lib():
         "qblas_hip_ozaki_repairs": (ctypes.c_longlong, [P]),
         "qblas_hip_ozaki_last_slices": (I, []),
         "qblas_hip_set_phase_timing": (None, [I]),
+        "qblas_hip_gemm_batched": (I, [I, I, I64, P, P]),
+        "qblas_hip_gemv_batched": (I, [I, I, I64, P, P]),
+        "qblas_hip_gemm_batched_tune": (I, [I, I, I64, P, P]),
+        "qblas_hip_gemm_batched_config": (ctypes.c_char_p, [I, I]),
+        "qblas_hip_gemm_batched_select": (I, [I, I, I64, I]),
         "qblas_hip_phase_times": (I, [P]),
     }
     for name, (res, args) in sig.items():
@@ -302,6 +307,43 @@ def phase_times():
     if lib().qblas_hip_phase_times(ctypes.cast(buf, ctypes.c_void_p)) != 4:
         return None
     return dict(zip(OZAKI_PHASES, [float(v) for v in buf]))
+
+
+# ------------------------------------------------------------------------------------------------
+# batched problems: descriptor arrays with the layout of qblas_gemm_problem / qblas_gemv_problem
+# ------------------------------------------------------------------------------------------------
+GEMM_PROBLEM = np.dtype([("m", "<i8"), ("n", "<i8"), ("k", "<i8"), ("lda", "<i8"), ("ldb", "<i8"), ("ldc", "<i8"),
+                         ("A", "<u8"), ("B", "<u8"), ("C", "<u8"), ("alpha", "<u8", (2,)), ("beta", "<u8", (2,)),
+                         ("reserved", "<i8")])
+GEMV_PROBLEM = np.dtype([("m", "<i8"), ("n", "<i8"), ("lda", "<i8"), ("incx", "<i8"), ("incy", "<i8"),
+                         ("A", "<u8"), ("x", "<u8"), ("y", "<u8"), ("alpha", "<u8", (2,)), ("beta", "<u8", (2,))])
+assert GEMM_PROBLEM.itemsize == 112 and GEMV_PROBLEM.itemsize == 96
+
+
+def dev_gemm_batched(mode, layout, count, problems, stream=0):
+    """problems: device pointer to count qblas_gemm_problem descriptors."""
+    _check(lib().qblas_hip_gemm_batched(int(mode), int(layout), count, problems, stream), "qblas_hip_gemm_batched")
+
+
+def dev_gemv_batched(mode, layout, count, problems, stream=0):
+    _check(lib().qblas_hip_gemv_batched(int(mode), int(layout), count, problems, stream), "qblas_hip_gemv_batched")
+
+
+def dev_gemm_batched_tune(mode, layout, count, problems, stream=0):
+    """Time every tile configuration on a replica of the batch; returns the chosen index."""
+    r = lib().qblas_hip_gemm_batched_tune(int(mode), int(layout), count, problems, stream)
+    if r < 0:
+        raise RuntimeError(f"qblas_hip_gemm_batched_tune: status {-r}: {last_error()}")
+    return r
+
+
+def gemm_batched_config(mode, idx):
+    s = lib().qblas_hip_gemm_batched_config(int(mode), int(idx))
+    return s.decode() if s else None
+
+
+def gemm_batched_select(mode, layout, count, idx):
+    _check(lib().qblas_hip_gemm_batched_select(int(mode), int(layout), count, int(idx)), "qblas_hip_gemm_batched_select")
 
 
 FILL_UNIFORM_FULL = 0

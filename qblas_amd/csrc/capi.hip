@@ -17,7 +17,11 @@
 
 #include <atomic>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
+#include <vector>
 
 #include "../../include/qblas_hip.h"
 #include "kernels.h"
@@ -415,6 +419,140 @@ int host_gemm(int layout, int64_t m, int64_t n, int64_t k, q128 alpha, const voi
   return check_exact_status(s);
 }
 
+// ---------------------------------------------------------------------------------------------
+// batched problems: a per-thread prefix-sum buffer that only grows (so repeated calls of the same
+// count launch kernels and nothing else, and can be captured in a graph), and the tuned tile
+// configuration per (mode, layout, count)
+// ---------------------------------------------------------------------------------------------
+thread_local void *t_bws = nullptr;
+thread_local size_t t_bws_size = 0;
+
+int batch_ws(size_t bytes, void **out) {
+  if (bytes > t_bws_size) {
+    if (t_bws) {
+      QB_CHECK(hipDeviceSynchronize());  // an earlier batched call may still read it
+      QB_CHECK(hipFree(t_bws));
+      t_bws = nullptr;
+      t_bws_size = 0;
+    }
+    const size_t want = bytes < 4096 ? 4096 : bytes;
+    QB_CHECK(hipMalloc(&t_bws, want));
+    t_bws_size = want;
+  }
+  *out = t_bws;
+  return QBLAS_OK;
+}
+
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int64_t>, int> g_tuned;
+
+int tuned_config(int mode, int layout, int64_t count) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tuned.find(std::make_tuple(mode, layout, count));
+  return it == g_tuned.end() ? 0 : it->second;
+}
+
+int check_batched(const char *what, int mode, int64_t count, const void *P) {
+  if (count < 0 || (count > 0 && (!P || ((uintptr_t)P % 8) != 0))) {
+    set_err_msg(std::string(what) + ": bad count or descriptor pointer");
+    return QBLAS_ERR_ARG;
+  }
+  if (mode != QBLAS_MODE_REFERENCE && mode != QBLAS_MODE_DD) {
+    set_err_msg(std::string(what) + ": batched problems support QBLAS_MODE_REFERENCE and QBLAS_MODE_DD");
+    return QBLAS_ERR_MODE;
+  }
+  return QBLAS_OK;
+}
+
+int gemm_batched(int mode, int layout, int64_t count, const qblas_gemm_problem *P, int cfg, hipStream_t s) {
+  int rc = check_batched("qblas_hip_gemm_batched", mode, count, P);
+  if (rc || count == 0) return rc;
+  void *ws;
+  if ((rc = batch_ws(qbk::batched_ws(count), &ws))) return rc;
+  QB_CHECK(qbk::launch_gemm_batched(mode == QBLAS_MODE_DD, layout != 0, count, P, cfg, (int64_t)g_kc.load(), ws, s));
+  return QBLAS_OK;
+}
+
+// Replica of the first min(count, 512) problems: same m, n, k, packed leading dimensions,
+// synthetic full-mantissa data; every configuration runs once untimed and three times timed.
+int gemm_batched_tune(int mode, int layout, int64_t count, const qblas_gemm_problem *dP, hipStream_t s) {
+  int rc = check_batched("qblas_hip_gemm_batched_tune", mode, count, dP);
+  if (rc) return -rc;
+  if (count == 0) return 0;
+  const int64_t R = count < 512 ? count : 512;
+  std::vector<qblas_gemm_problem> hp(R);
+  QB_CHECK(hipMemcpyAsync(hp.data(), dP, R * sizeof(qblas_gemm_problem), hipMemcpyDeviceToHost, s));
+  QB_CHECK(hipStreamSynchronize(s));
+  int64_t elems = 0;
+  for (auto &p : hp) {
+    const int64_t m = p.m > 0 ? p.m : 0, n = p.n > 0 ? p.n : 0, k = p.k > 0 ? p.k : 0;
+    elems += m * k + k * n + m * n;
+  }
+  char *scratch = nullptr;
+  const size_t dbytes = R * sizeof(qblas_gemm_problem);
+  const size_t bytes = (size_t)elems * 16 + dbytes + qbk::batched_ws(R) + 256;
+  QB_CHECK(hipMalloc(&scratch, bytes));
+  std::vector<qblas_gemm_problem> rp(hp);
+  q128 *data = (q128 *)scratch;
+  int64_t off = 0;
+  for (auto &p : rp) {
+    const int64_t m = p.m > 0 ? p.m : 0, n = p.n > 0 ? p.n : 0, k = p.k > 0 ? p.k : 0;
+    p.A = data + off;
+    p.lda = layout ? (m > 0 ? m : 1) : (k > 0 ? k : 1);
+    off += m * k;
+    p.B = data + off;
+    p.ldb = layout ? (k > 0 ? k : 1) : (n > 0 ? n : 1);
+    off += k * n;
+    p.C = data + off;
+    p.ldc = layout ? (m > 0 ? m : 1) : (n > 0 ? n : 1);
+    off += m * n;
+  }
+  char *dR = scratch + (size_t)elems * 16;
+  char *ws = dR + ((dbytes + 255) / 256) * 256;
+  int best = 0;
+  float best_ms = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto fail = [&](hipError_t e, const char *where) {
+    set_err(where, e);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(scratch);
+    return -QBLAS_ERR_HIP;
+  };
+  hipError_t e;
+  if ((e = qbk::launch_fill(0, 1234, elems, 0, 0, data, s)) != hipSuccess) return fail(e, "tune fill");
+  if ((e = hipMemcpyAsync(dR, rp.data(), dbytes, hipMemcpyHostToDevice, s)) != hipSuccess) return fail(e, "tune copy");
+  if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) return fail(e, "tune events");
+  const int ncfg = qbk::gemm_batched_configs(mode == QBLAS_MODE_DD);
+  for (int c = 0; c < ncfg; ++c) {
+    const qblas_gemm_problem *P = (const qblas_gemm_problem *)dR;
+    if ((e = qbk::launch_gemm_batched(mode == QBLAS_MODE_DD, layout != 0, R, P, c, (int64_t)g_kc.load(), ws, s)) !=
+        hipSuccess)
+      return fail(e, "tune launch");
+    if ((e = hipEventRecord(e0, s)) != hipSuccess) return fail(e, "tune record");
+    for (int rep = 0; rep < 3; ++rep)
+      if ((e = qbk::launch_gemm_batched(mode == QBLAS_MODE_DD, layout != 0, R, P, c, (int64_t)g_kc.load(), ws, s)) !=
+          hipSuccess)
+        return fail(e, "tune launch");
+    if ((e = hipEventRecord(e1, s)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess)
+      return fail(e, "tune sync");
+    float ms = 0;
+    if ((e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) return fail(e, "tune time");
+    if (c == 0 || ms < best_ms) {
+      best = c;
+      best_ms = ms;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  QB_CHECK(hipFree(scratch));
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[std::make_tuple(mode, layout, count)] = best;
+  }
+  return best;
+}
+
 bool is_colmajor_char(char c) { return c == 'C' || c == 'c'; }
 bool is_trans_char(char c) { return c == 'T' || c == 't' || c == 'C' || c == 'c'; }
 
@@ -524,6 +662,33 @@ int qblas_hip_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, const 
                    void *stream) {
   return dev_gemm(mode, layout, m, n, k, load_q(alpha), d_A, lda, d_B, ldb, load_q(beta), d_C, ldc,
                   (hipStream_t)stream);
+}
+int qblas_hip_gemm_batched(int mode, int layout, int64_t count, const qblas_gemm_problem *d_problems, void *stream) {
+  return gemm_batched(mode, layout, count, d_problems, tuned_config(mode, layout, count), (hipStream_t)stream);
+}
+int qblas_hip_gemv_batched(int mode, int layout, int64_t count, const qblas_gemv_problem *d_problems, void *stream) {
+  int rc = check_batched("qblas_hip_gemv_batched", mode, count, d_problems);
+  if (rc || count == 0) return rc;
+  void *ws;
+  if ((rc = batch_ws(qbk::batched_ws(count), &ws))) return rc;
+  QB_CHECK(qbk::launch_gemv_batched(mode == QBLAS_MODE_DD, layout != 0, count, d_problems, ws, (hipStream_t)stream));
+  return QBLAS_OK;
+}
+int qblas_hip_gemm_batched_tune(int mode, int layout, int64_t count, const qblas_gemm_problem *d_problems,
+                                void *stream) {
+  return gemm_batched_tune(mode, layout, count, d_problems, (hipStream_t)stream);
+}
+const char *qblas_hip_gemm_batched_config(int mode, int idx) {
+  if (mode != QBLAS_MODE_REFERENCE && mode != QBLAS_MODE_DD) return nullptr;
+  return qbk::gemm_batched_config_name(mode == QBLAS_MODE_DD, idx);
+}
+int qblas_hip_gemm_batched_select(int mode, int layout, int64_t count, int idx) {
+  if ((mode != QBLAS_MODE_REFERENCE && mode != QBLAS_MODE_DD) || idx < 0 ||
+      idx >= qbk::gemm_batched_configs(mode == QBLAS_MODE_DD))
+    return QBLAS_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tuned[std::make_tuple(mode, layout, count)] = idx;
+  return QBLAS_OK;
 }
 int qblas_hip_fill(int kind, uint64_t seed, int64_t n, int lo_exp, int hi_exp, void *d_out, void *stream) {
   if (n < 0 || kind < 0 || kind > 3 || (kind == 1 && hi_exp <= lo_exp) || !aligned16(d_out)) {

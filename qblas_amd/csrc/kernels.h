@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "q128.hpp"
+#include "../../include/qblas_hip.h"
 
 namespace qbk {
 using qb::q128;
@@ -53,4 +54,13 @@ hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int 
                              hipStream_t st, hipEvent_t *ev /* null, or 5 events: start, after slicing, after
                              the slice products, after the finish, after the repair */);
 hipError_t ozaki_repair_count(const void *ws, uint32_t *out, hipStream_t st);
+
+// batched problems (kernels_batched.hip); ws holds count + 1 prefix sums
+size_t batched_ws(int64_t count);
+int gemm_batched_configs(int dd_mode);
+const char *gemm_batched_config_name(int dd_mode, int idx);
+hipError_t launch_gemm_batched(int dd_mode, int colmajor, int64_t count, const qblas_gemm_problem *P, int cfg,
+                               int64_t kc, void *ws, hipStream_t st);
+hipError_t launch_gemv_batched(int dd_mode, int colmajor, int64_t count, const qblas_gemv_problem *P, void *ws,
+                               hipStream_t st);
 }  // namespace qbk

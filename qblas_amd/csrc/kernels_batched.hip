@@ -1,0 +1,405 @@
+// kernels_batched.hip -- many independent, unevenly sized qgemm / qgemv problems in one launch.
+//
+// A device array of descriptors (qblas_gemm_problem / qblas_gemv_problem) is turned into a flat
+// work list by k_batch_scan (work items per problem, exclusive prefix sums, all on the device),
+// then one grouped persistent kernel walks the list: every workgroup (or wave) takes items
+// blockIdx.x, blockIdx.x + gridDim.x, ..., finds its problem by binary search in the prefix sums
+// and computes one tile (gemm) or one group of rows (gemv).  Nothing is synchronized or copied
+// between the two launches, so the sequence can be captured in a hipGraph.
+//
+// Per problem the semantics are those of the single-problem routines:
+//   REFERENCE gemm  level3.hpp:215-336: per entry, k in blocks of kc (kc = k when m, n, k <= 64,
+//                   level3.hpp:224-229), each block a fresh correctly rounded FMA chain t, then
+//                   C = fma(alpha, t, mul(beta on the first block / 1 after, C))  -> bit-identical
+//                   to a loop of single calls.
+//   REFERENCE gemv  level2.hpp:15-99: RowMajor two lane chains per row (one when incx != 1),
+//                   y = fma(alpha, sum, mul(beta, y)); ColMajor y = mul(beta, y) then
+//                   y = fma(A_ij, mul(alpha, x_j), y) over j.
+//   DD gemm / gemv  double-double accumulation (dd.hpp), C = fma(alpha, RN(acc), mul(beta, C)).
+// Tile shapes are template parameters; the host picks one configuration per call (the
+// autotuner in capi.hip measures them).
+#include "dd.hpp"
+#include "kernels.h"
+
+using namespace qb;
+
+namespace qbk {
+
+__device__ __forceinline__ q128 bld(const q128 *p) {
+  const uint4 v = *reinterpret_cast<const uint4 *>(p);
+  return q_make(((uint64_t)v.y << 32) | v.x, ((uint64_t)v.w << 32) | v.z);
+}
+__device__ __forceinline__ void bst(q128 *p, q128 v) {
+  uint4 w;
+  w.x = (uint32_t)v.lo;
+  w.y = (uint32_t)(v.lo >> 32);
+  w.z = (uint32_t)v.hi;
+  w.w = (uint32_t)(v.hi >> 32);
+  *reinterpret_cast<uint4 *>(p) = w;
+}
+__device__ __forceinline__ q128 qv(const uint64_t (&w)[2]) { return q_make(w[0], w[1]); }
+
+// largest p in [0, count) with start[p] <= t (start is nondecreasing, start[0] = 0 <= t)
+__device__ __forceinline__ int64_t find_problem(const int64_t *__restrict__ start, int64_t count, int64_t t) {
+  int64_t lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (start[mid] <= t)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// work items of problem p:
+//   kind 0 (gemm): tiles of tm x tn outputs (none when m, n or k is 0: level3.hpp:221-222)
+//   kind 1 (gemv, REFERENCE): groups of 64 lanes: 32 rows (RowMajor, incx == 1) or 64 rows
+//   kind 2 (gemv, DD): RowMajor one row per wave; ColMajor 64 rows per wave
+// (gemv has no work when m or n is 0: level2.hpp:21-22, :59-60)
+__device__ __forceinline__ int64_t batch_items(const void *P, int kind, int64_t p, int tm, int tn, int colmajor) {
+  if (kind == 0) {
+    const qblas_gemm_problem &g = reinterpret_cast<const qblas_gemm_problem *>(P)[p];
+    if (g.m <= 0 || g.n <= 0 || g.k <= 0) return 0;
+    return cdiv(g.m, tm) * cdiv(g.n, tn);
+  }
+  const qblas_gemv_problem &v = reinterpret_cast<const qblas_gemv_problem *>(P)[p];
+  if (v.m <= 0 || v.n <= 0) return 0;
+  if (kind == 1) return cdiv(v.m, (!colmajor && v.incx == 1) ? 32 : 64);
+  return colmajor ? cdiv(v.m, 64) : v.m;
+}
+
+// start[0] = 0, start[p + 1] = items of problems 0..p; one block, chunks of 1024 problems
+__global__ void __launch_bounds__(1024) k_batch_scan(const void *__restrict__ P, int kind, int64_t count, int tm,
+                                                     int tn, int colmajor, int64_t *__restrict__ start) {
+  __shared__ int64_t s[1024];
+  int64_t base = 0;
+  if (threadIdx.x == 0) start[0] = 0;
+  for (int64_t b = 0; b < count; b += 1024) {
+    const int64_t p = b + threadIdx.x;
+    s[threadIdx.x] = (p < count) ? batch_items(P, kind, p, tm, tn, colmajor) : 0;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const int64_t t = (threadIdx.x >= off) ? s[threadIdx.x - off] : 0;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (p < count) start[p + 1] = base + s[threadIdx.x];
+    base += s[1023];
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// gemm, REFERENCE order: one lane per C entry, tile TM x (256 / TM), rows fastest
+// ------------------------------------------------------------------------------------------
+template <int TM>
+__global__ void __launch_bounds__(256) k_gemm_batched_ref(const qblas_gemm_problem *__restrict__ P, int64_t count,
+                                                          const int64_t *__restrict__ start, int colmajor,
+                                                          int64_t kc_default) {
+  constexpr int TN = 256 / TM;
+  const int64_t total = start[count];
+  for (int64_t t = blockIdx.x; t < total; t += gridDim.x) {
+    const int64_t p = find_problem(start, count, t);
+    const qblas_gemm_problem &g = P[p];
+    const int64_t m = g.m, n = g.n, k = g.k;
+    const int64_t tl = t - start[p];
+    const int64_t tiles_n = cdiv(n, TN);
+    const int64_t ti = tl / tiles_n, tj = tl - ti * tiles_n;
+    const int64_t i = ti * TM + (threadIdx.x % TM), j = tj * TN + (threadIdx.x / TM);
+    if (i >= m || j >= n) continue;
+    const int64_t kc = (m <= 64 && n <= 64 && k <= 64) ? k : kc_default;
+    const q128 *A = (const q128 *)g.A;
+    const q128 *B = (const q128 *)g.B;
+    q128 *C = (q128 *)g.C;
+    const int64_t as_i = colmajor ? 1 : g.lda, as_l = colmajor ? g.lda : 1;
+    const int64_t bs_l = colmajor ? 1 : g.ldb, bs_j = colmajor ? g.ldb : 1;
+    const q128 alpha = qv(g.alpha), beta = qv(g.beta);
+    q128 *ci = C + (colmajor ? j * g.ldc + i : i * g.ldc + j);
+    q128 c = bld(ci);
+    const q128 *ap = A + i * as_i;
+    const q128 *bp = B + j * bs_j;
+    for (int64_t kk = 0; kk < k; kk += kc) {
+      const int64_t ke = (kk + kc < k) ? kk + kc : k;
+      chain ch;
+      chain_init(ch);
+      for (int64_t l = kk; l < ke; ++l) chain_fma(ch, bld(ap + l * as_l), bld(bp + l * bs_l));
+      c = fma_packed(alpha, chain_get(ch), mul_packed(kk == 0 ? beta : q_one(), c));
+    }
+    bst(ci, c);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// gemm, DD: tile TM x TN, thread block of (TM/RM) x (TN/RN) = 256 threads, RM x RN entries per
+// thread, k steps of 16 staged in LDS as double-doubles (converted on load)
+// ------------------------------------------------------------------------------------------
+__device__ __attribute__((noinline)) q128 bdd_out(double hi, double lo, q128 alpha, q128 beta, q128 c) {
+  dd v;
+  v.hi = hi;
+  v.lo = lo;
+  return fma_packed(alpha, dd_to_q(v), mul_packed(beta, c));
+}
+
+template <int TM, int TN, int RM, int RN>
+__global__ void __launch_bounds__(256) k_gemm_batched_dd(const qblas_gemm_problem *__restrict__ P, int64_t count,
+                                                         const int64_t *__restrict__ start, int colmajor) {
+  constexpr int KS = 16, CT = TN / RN;
+  static_assert((TM / RM) * (TN / RN) == 256, "256 threads");
+  __shared__ double sAh[KS][TM], sAl[KS][TM], sBh[KS][TN], sBl[KS][TN];
+  const int tr = threadIdx.x / CT, tc = threadIdx.x % CT;
+  const int64_t total = start[count];
+  for (int64_t t = blockIdx.x; t < total; t += gridDim.x) {
+    const int64_t p = find_problem(start, count, t);
+    const qblas_gemm_problem &g = P[p];
+    const int64_t m = g.m, n = g.n, k = g.k;
+    const int64_t tl = t - start[p];
+    const int64_t tiles_n = cdiv(n, TN);
+    const int64_t i0 = (tl / tiles_n) * TM, j0 = (tl - (tl / tiles_n) * tiles_n) * TN;
+    const q128 *A = (const q128 *)g.A;
+    const q128 *B = (const q128 *)g.B;
+    const int64_t as_i = colmajor ? 1 : g.lda, as_l = colmajor ? g.lda : 1;
+    const int64_t bs_l = colmajor ? 1 : g.ldb, bs_j = colmajor ? g.ldb : 1;
+    dd acc[RM][RN];
+#pragma unroll
+    for (int r = 0; r < RM; ++r)
+#pragma unroll
+      for (int c = 0; c < RN; ++c) acc[r][c] = dd_zero();
+    for (int64_t k0 = 0; k0 < k; k0 += KS) {
+      __syncthreads();
+      // stage A[i0.., k0..] and B[k0.., j0..]; the contiguous index runs fastest across threads
+      for (int e = threadIdx.x; e < TM * KS; e += 256) {
+        const int r = colmajor ? e % TM : e / KS, kk = colmajor ? e / TM : e % KS;
+        const int64_t i = i0 + r, l = k0 + kk;
+        const dd v = (i < m && l < k) ? q_to_dd(bld(A + i * as_i + l * as_l)) : dd_zero();
+        sAh[kk][r] = v.hi;
+        sAl[kk][r] = v.lo;
+      }
+      for (int e = threadIdx.x; e < KS * TN; e += 256) {
+        const int c = colmajor ? e / KS : e % TN, kk = colmajor ? e % KS : e / TN;
+        const int64_t j = j0 + c, l = k0 + kk;
+        const dd v = (j < n && l < k) ? q_to_dd(bld(B + l * bs_l + j * bs_j)) : dd_zero();
+        sBh[kk][c] = v.hi;
+        sBl[kk][c] = v.lo;
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int kk = 0; kk < KS; ++kk) {
+        dd a[RM], b[RN];
+#pragma unroll
+        for (int r = 0; r < RM; ++r) {
+          a[r].hi = sAh[kk][tr + (TM / RM) * r];
+          a[r].lo = sAl[kk][tr + (TM / RM) * r];
+        }
+#pragma unroll
+        for (int c = 0; c < RN; ++c) {
+          b[c].hi = sBh[kk][tc + CT * c];
+          b[c].lo = sBl[kk][tc + CT * c];
+        }
+#pragma unroll
+        for (int r = 0; r < RM; ++r)
+#pragma unroll
+          for (int c = 0; c < RN; ++c) acc[r][c] = dd_fma(a[r], b[c], acc[r][c]);
+      }
+    }
+    const q128 alpha = qv(g.alpha), beta = qv(g.beta);
+    q128 *C = (q128 *)g.C;
+#pragma unroll
+    for (int r = 0; r < RM; ++r)
+#pragma unroll
+      for (int c = 0; c < RN; ++c) {
+        const int64_t i = i0 + tr + (TM / RM) * r, j = j0 + tc + CT * c;
+        if (i < m && j < n) {
+          q128 *ci = C + (colmajor ? j * g.ldc + i : i * g.ldc + j);
+          bst(ci, bdd_out(acc[r][c].hi, acc[r][c].lo, alpha, beta, bld(ci)));
+        }
+      }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// gemv, REFERENCE order: units of 64 lanes (one wave) of one problem
+// ------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_gemv_batched_ref(const qblas_gemv_problem *__restrict__ P, int64_t count,
+                                                          const int64_t *__restrict__ start, int colmajor) {
+  const int lane = threadIdx.x & 63;
+  const int64_t total = start[count];
+  const int64_t waves = (int64_t)gridDim.x * 4;
+  for (int64_t u = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); u < total; u += waves) {
+    const int64_t p = find_problem(start, count, u);
+    const qblas_gemv_problem &v = P[p];
+    const int64_t m = v.m, n = v.n, lda = v.lda, incx = v.incx, incy = v.incy;
+    const int64_t ul = u - start[p];
+    const q128 *A = (const q128 *)v.A;
+    const q128 *x = (const q128 *)v.x;
+    q128 *y = (q128 *)v.y;
+    const q128 alpha = qv(v.alpha), beta = qv(v.beta);
+    chain ch;
+    if (!colmajor && incx == 1) {
+      // two lane chains per row (level2.hpp:35-38 with dot_kernel_vectorized), rows ul*32 + lane/2
+      const int64_t i = ul * 32 + (lane >> 1);
+      const int par = lane & 1;
+      const bool live = i < m;
+      chain_init(ch);
+      if (live) {
+        const q128 *row = A + i * lda;
+        const int64_t pairs = n / 2;
+        for (int64_t jj = 0; jj < pairs; ++jj) chain_fma(ch, bld(row + 2 * jj + par), bld(x + 2 * jj + par));
+      }
+      const q128 mine = chain_get(ch);
+      const uint64_t olo = (uint64_t)__shfl_xor((unsigned long long)mine.lo, 1);
+      const uint64_t ohi = (uint64_t)__shfl_xor((unsigned long long)mine.hi, 1);
+      if (live && par == 0) {
+        q128 sum = add_packed(mine, q_make(olo, ohi));
+        if (n & 1) sum = fma_packed(bld(A + i * lda + n - 1), bld(x + n - 1), sum);
+        q128 *yi = y + i * incy;
+        bst(yi, fma_packed(alpha, sum, mul_packed(beta, bld(yi))));
+      }
+    } else if (!colmajor) {
+      const int64_t i = ul * 64 + lane;
+      if (i < m) {
+        chain_init(ch);
+        const q128 *row = A + i * lda;
+        for (int64_t j = 0; j < n; ++j) chain_fma(ch, bld(row + j), bld(x + j * incx));
+        q128 *yi = y + i * incy;
+        bst(yi, fma_packed(alpha, chain_get(ch), mul_packed(beta, bld(yi))));
+      }
+    } else {
+      // level2.hpp:63-80: y_i = mul(beta, y_i), then fma(A_ij, mul(alpha, x_j), y_i) over j
+      const int64_t i = ul * 64 + lane;
+      if (i < m) {
+        q128 *yi = y + i * incy;
+        chain_set(ch, mul_packed(beta, bld(yi)));
+        for (int64_t j = 0; j < n; ++j) chain_fma(ch, bld(A + j * lda + i), mul_packed(alpha, bld(x + j * incx)));
+        bst(yi, chain_get(ch));
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// gemv, DD: RowMajor one wave per row (lanes along the row, butterfly sum); ColMajor one lane
+// per row over the columns (coalesced column reads).  y = fma(alpha, RN(sum), mul(beta, y)).
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ dd bshfl_xor_dd(dd v, int msk) {
+  dd r;
+  r.hi = __shfl_xor(v.hi, msk);
+  r.lo = __shfl_xor(v.lo, msk);
+  return r;
+}
+
+__global__ void __launch_bounds__(256) k_gemv_batched_dd(const qblas_gemv_problem *__restrict__ P, int64_t count,
+                                                         const int64_t *__restrict__ start, int colmajor) {
+  const int lane = threadIdx.x & 63;
+  const int64_t total = start[count];
+  const int64_t waves = (int64_t)gridDim.x * 4;
+  for (int64_t u = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); u < total; u += waves) {
+    const int64_t p = find_problem(start, count, u);
+    const qblas_gemv_problem &v = P[p];
+    const int64_t m = v.m, n = v.n, lda = v.lda, incx = v.incx, incy = v.incy;
+    const int64_t ul = u - start[p];
+    const q128 *A = (const q128 *)v.A;
+    const q128 *x = (const q128 *)v.x;
+    q128 *y = (q128 *)v.y;
+    const q128 alpha = qv(v.alpha), beta = qv(v.beta);
+    if (!colmajor) {
+      const int64_t i = ul;
+      const q128 *row = A + i * lda;
+      dd acc0 = dd_zero(), acc1 = dd_zero();
+      int64_t j = lane;
+      for (; j + 64 < n; j += 128) {
+        acc0 = dd_fma(q_to_dd(bld(row + j)), q_to_dd(bld(x + j * incx)), acc0);
+        acc1 = dd_fma(q_to_dd(bld(row + j + 64)), q_to_dd(bld(x + (j + 64) * incx)), acc1);
+      }
+      if (j < n) acc0 = dd_fma(q_to_dd(bld(row + j)), q_to_dd(bld(x + j * incx)), acc0);
+      dd s = dd_add(acc0, acc1);
+#pragma unroll
+      for (int msk = 32; msk >= 1; msk >>= 1) s = dd_add(s, bshfl_xor_dd(s, msk));
+      if (lane == 0) {
+        q128 *yi = y + i * incy;
+        bst(yi, fma_packed(alpha, dd_to_q(s), mul_packed(beta, bld(yi))));
+      }
+    } else {
+      const int64_t i = ul * 64 + lane;
+      if (i < m) {
+        dd acc0 = dd_zero(), acc1 = dd_zero();
+        int64_t j = 0;
+        for (; j + 1 < n; j += 2) {
+          acc0 = dd_fma(q_to_dd(bld(A + j * lda + i)), q_to_dd(bld(x + j * incx)), acc0);
+          acc1 = dd_fma(q_to_dd(bld(A + (j + 1) * lda + i)), q_to_dd(bld(x + (j + 1) * incx)), acc1);
+        }
+        if (j < n) acc0 = dd_fma(q_to_dd(bld(A + j * lda + i)), q_to_dd(bld(x + j * incx)), acc0);
+        q128 *yi = y + i * incy;
+        bst(yi, fma_packed(alpha, dd_to_q(dd_add(acc0, acc1)), mul_packed(beta, bld(yi))));
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------------------------
+constexpr int BATCH_GRID = 2048;  // persistent workgroups (8 per CU)
+
+struct gemm_cfg {
+  int tm, tn;
+  const char *name;
+};
+static const gemm_cfg REF_CFG[] = {{16, 16, "16x16 tile, 1 entry/lane"},
+                                   {8, 32, "8x32 tile, 1 entry/lane"},
+                                   {32, 8, "32x8 tile, 1 entry/lane"},
+                                   {4, 64, "4x64 tile, 1 entry/lane"},
+                                   {64, 4, "64x4 tile, 1 entry/lane"}};
+static const gemm_cfg DD_CFG[] = {{32, 32, "32x32 tile, 2x2 entries/thread"},
+                                  {64, 64, "64x64 tile, 4x4 entries/thread"},
+                                  {16, 16, "16x16 tile, 1 entry/thread"},
+                                  {64, 32, "64x32 tile, 4x2 entries/thread"}};
+
+int gemm_batched_configs(int dd_mode) { return dd_mode ? 4 : 5; }
+const char *gemm_batched_config_name(int dd_mode, int idx) {
+  if (idx < 0 || idx >= gemm_batched_configs(dd_mode)) return nullptr;
+  return dd_mode ? DD_CFG[idx].name : REF_CFG[idx].name;
+}
+
+size_t batched_ws(int64_t count) { return (size_t)(count + 1) * sizeof(int64_t); }
+
+hipError_t launch_gemm_batched(int dd_mode, int colmajor, int64_t count, const qblas_gemm_problem *P, int cfg,
+                               int64_t kc, void *ws, hipStream_t st) {
+  if (cfg < 0 || cfg >= gemm_batched_configs(dd_mode)) return hipErrorInvalidValue;
+  int64_t *start = (int64_t *)ws;
+  const gemm_cfg c = dd_mode ? DD_CFG[cfg] : REF_CFG[cfg];
+  k_batch_scan<<<1, 1024, 0, st>>>(P, 0, count, c.tm, c.tn, colmajor, start);
+  if (!dd_mode) {
+    switch (cfg) {
+      case 0: k_gemm_batched_ref<16><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, kc); break;
+      case 1: k_gemm_batched_ref<8><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, kc); break;
+      case 2: k_gemm_batched_ref<32><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, kc); break;
+      case 3: k_gemm_batched_ref<4><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, kc); break;
+      default: k_gemm_batched_ref<64><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, kc); break;
+    }
+  } else {
+    switch (cfg) {
+      case 0: k_gemm_batched_dd<32, 32, 2, 2><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor); break;
+      case 1: k_gemm_batched_dd<64, 64, 4, 4><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor); break;
+      case 2: k_gemm_batched_dd<16, 16, 1, 1><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor); break;
+      default: k_gemm_batched_dd<64, 32, 4, 2><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor); break;
+    }
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_gemv_batched(int dd_mode, int colmajor, int64_t count, const qblas_gemv_problem *P, void *ws,
+                               hipStream_t st) {
+  int64_t *start = (int64_t *)ws;
+  k_batch_scan<<<1, 1024, 0, st>>>(P, dd_mode ? 2 : 1, count, 0, 0, colmajor, start);
+  if (!dd_mode)
+    k_gemv_batched_ref<<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor);
+  else
+    k_gemv_batched_dd<<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor);
+  return hipGetLastError();
+}
+
+}  // namespace qbk
