@@ -20,6 +20,13 @@ Also reported:
                   recorded by the library on the call's stream, against 5000 TOP/s dense int8.
                   traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE runs
                   (profiles/ozaki_traffic.json) when present.
+  batched         4096 qgemm problems (m, n, k uniform in [8, 256]) and 16384 qgemv problems
+                  (m, n uniform in [32, 1024]) with per-problem alpha, beta, each batch one grouped
+                  persistent launch (tile configuration from the autotuner) replayed from a
+                  captured graph: problems/s, quad GFLOP/s or GB/s, REFERENCE and DD forms.
+  ill_conditioned qgemm 8192 x 8192 x 16384 ColMajor whose entries cancel to ~1e-18 of the
+                  largest products: OZAKI (26 slices: the bound certifies all but entries within
+                  ~2^-12 of zero) and DD, with HBM in use.
   cpu_baseline    the oracle's restatement of the reference qgemm (level3.hpp blocking, soft
                   binary128 FMA, OpenMP over the host cores) on a 256-row slab of the same
                   4096^3 problem, scaled to GFLOP/s (per-element cost is uniform).
@@ -50,6 +57,10 @@ def parse():
     p.add_argument("--no-plain", action="store_true", help="skip the plain soft-binary128 qgemm")
     p.add_argument("--no-memory", action="store_true", help="skip qdot / qgemv")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    p.add_argument("--no-batched", action="store_true", help="skip the batched problems")
+    p.add_argument("--no-ill", action="store_true", help="skip the ill-conditioned 8192x8192x16384 qgemm")
+    p.add_argument("--batch-gemm", type=int, default=4096, help="problems in the batched qgemm")
+    p.add_argument("--batch-gemv", type=int, default=16384, help="problems in the batched qgemv")
     p.add_argument("--cpu-rows", type=int, default=256, help="rows of C in the CPU qgemm slab")
     p.add_argument("--copy-calibration", action="store_true",
                    help="first run one device copy of 1 GiB (a known-size reference for HBM counters)")
@@ -184,6 +195,134 @@ def bench_memory(Q, torch, dev, stream, n, m, steps):
     return out
 
 
+def bench_batched(Q, torch, dev, stream, steps, gemm_count, gemv_count, seed=77):
+    """Config: 4096 independent qgemm problems with m, n, k uniform in [8, 256] and 16384 qgemv
+    problems with m, n uniform in [32, 1024], RowMajor, entries and per-problem alpha, beta
+    full-mantissa uniform [-1, 1) (device generator).  One grouped persistent launch per batch,
+    tile configuration chosen by the autotuner, captured once in a graph and replayed."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = {}
+    sh = stream.cuda_stream
+
+    def timed_graph(call):
+        call()                                  # allocates the work-list buffer outside capture
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                call(s.cuda_stream)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        s_ev, e_ev = events(torch)
+        s_ev.record(stream)
+        for _ in range(steps):
+            g.replay()
+        e_ev.record(stream)
+        torch.cuda.synchronize()
+        return s_ev.elapsed_time(e_ev) / steps
+
+    # ---- gemm batch
+    dims = rng.integers(8, 257, size=(gemm_count, 3))
+    sizes = dims[:, 0] * dims[:, 2] + dims[:, 2] * dims[:, 1] + dims[:, 0] * dims[:, 1]
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    pool = torch.empty((int(offs[-1]), 2), dtype=torch.int64, device=dev)
+    Q.dev_fill(Q.FILL_UNIFORM_FULL, seed, pool.shape[0], pool.data_ptr(), stream=sh)
+    ab = torch.empty((2 * gemm_count, 2), dtype=torch.int64, device=dev)
+    Q.dev_fill(Q.FILL_UNIFORM_FULL, seed + 1, ab.shape[0], ab.data_ptr(), stream=sh)
+    abh = ab.cpu().numpy().view(np.uint64)
+    desc = np.zeros(gemm_count, Q.GEMM_PROBLEM)
+    base = pool.data_ptr()
+    for p in range(gemm_count):
+        m, n, k = (int(v) for v in dims[p])
+        o = int(offs[p])
+        desc[p] = (m, n, k, k, n, n, base + 16 * o, base + 16 * (o + m * k), base + 16 * (o + m * k + k * n),
+                   abh[2 * p], abh[2 * p + 1], 0)
+    ddesc = torch.from_numpy(desc.view(np.uint8)).to(dev)
+    flops = float(np.sum(2.0 * dims[:, 0] * dims[:, 1] * dims[:, 2]))
+    g = {"problems": gemm_count, "dims": "m, n, k uniform in [8, 256]", "quad_flops": flops}
+    for form in ("dd", "reference"):
+        mode = Q.MODES[form]
+        cfg = Q.dev_gemm_batched_tune(mode, 0, gemm_count, ddesc.data_ptr(), sh)
+        ms = timed_graph(lambda s=sh: Q.dev_gemm_batched(mode, 0, gemm_count, ddesc.data_ptr(), s))
+        g[form] = {"ms": round(ms, 3), "problems_per_s": round(gemm_count / (ms * 1e-3), 1),
+                   "quad_GFLOPs": round(flops / (ms * 1e-3) / 1e9, 1), "tuned_config": Q.gemm_batched_config(mode, cfg)}
+    out["qgemm"] = g
+    del pool, ab, ddesc
+    torch.cuda.empty_cache()
+
+    # ---- gemv batch
+    mn = rng.integers(32, 1025, size=(gemv_count, 2))
+    asz = mn[:, 0] * mn[:, 1]
+    offs = np.concatenate([[0], np.cumsum(asz + mn[:, 1] + mn[:, 0])])
+    pool = torch.empty((int(offs[-1]), 2), dtype=torch.int64, device=dev)
+    Q.dev_fill(Q.FILL_UNIFORM_FULL, seed + 2, pool.shape[0], pool.data_ptr(), stream=sh)
+    ab = torch.empty((2 * gemv_count, 2), dtype=torch.int64, device=dev)
+    Q.dev_fill(Q.FILL_UNIFORM_FULL, seed + 3, ab.shape[0], ab.data_ptr(), stream=sh)
+    abh = ab.cpu().numpy().view(np.uint64)
+    desc = np.zeros(gemv_count, Q.GEMV_PROBLEM)
+    base = pool.data_ptr()
+    for p in range(gemv_count):
+        m, n = int(mn[p, 0]), int(mn[p, 1])
+        o = int(offs[p])
+        desc[p] = (m, n, n, 1, 1, base + 16 * o, base + 16 * (o + m * n), base + 16 * (o + m * n + n),
+                   abh[2 * p], abh[2 * p + 1])
+    ddesc = torch.from_numpy(desc.view(np.uint8)).to(dev)
+    nbytes = float(np.sum(16.0 * (asz + mn[:, 1] + 2 * mn[:, 0])))
+    v = {"problems": gemv_count, "dims": "m, n uniform in [32, 1024]", "algorithmic_bytes": nbytes}
+    for form in ("dd", "reference"):
+        mode = Q.MODES[form]
+        ms = timed_graph(lambda s=sh: Q.dev_gemv_batched(mode, 0, gemv_count, ddesc.data_ptr(), s))
+        v[form] = {"ms": round(ms, 3), "problems_per_s": round(gemv_count / (ms * 1e-3), 1),
+                   "GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+    out["qgemv"] = v
+    del pool, ab, ddesc
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_ill_conditioned(Q, torch, dev, stream, M, N, K, slices):
+    """qgemm M x N x K ColMajor, alpha = 1, beta = 0: rows of A carry +-1e20 u pairs along k and
+    the columns of B equal values on each pair (qblas_hip_fill_2d kinds 2 and 3), so the exact
+    entries are ~1e-18 of the largest products.  OZAKI form with the given slice count (enough
+    to certify every entry) and the DD form, one timed call each after a warm-up."""
+    sh = stream.cuda_stream
+    A = torch.empty((M * K, 2), dtype=torch.int64, device=dev)
+    B = torch.empty((K * N, 2), dtype=torch.int64, device=dev)
+    C = torch.zeros((M * N, 2), dtype=torch.int64, device=dev)
+    Q.dev_fill_2d(2, 5, M, K, M, 1, A.data_ptr(), stream=sh)
+    Q.dev_fill_2d(3, 5, N, K, K, 0, B.data_ptr(), stream=sh)
+    flops = 2.0 * M * N * K
+    out = {"workload": f"qgemm {M}x{N}x{K} ColMajor alpha=1 beta=0, entries ~1e-18 of the largest products",
+           "quad_flops": flops}
+    for form in ("ozaki", "dd"):
+        mode = Q.MODES[form]
+        if form == "ozaki":
+            Q.set_ozaki_slices(slices)
+        call = lambda: Q.dev_gemm(mode, 1, M, N, K, 1.0, A.data_ptr(), M, B.data_ptr(), K, 0.0, C.data_ptr(), M, sh)
+        call()
+        torch.cuda.synchronize()
+        s_ev, e_ev = events(torch)
+        s_ev.record(stream)
+        call()
+        e_ev.record(stream)
+        torch.cuda.synchronize()
+        ms = s_ev.elapsed_time(e_ev)
+        free, total = torch.cuda.mem_get_info()
+        r = {"ms": round(ms, 2), "quad_GFLOPs": round(flops / (ms * 1e-3) / 1e9, 1),
+             "hbm_in_use_GB": round((total - free) / 1e9, 1)}
+        if form == "ozaki":
+            r["slices"] = Q.ozaki_last_slices()
+            r["repaired_entries"] = int(Q.dev_ozaki_repairs(sh))
+            Q.set_ozaki_slices(0)
+        out[form] = r
+    del A, B, C
+    torch.cuda.empty_cache()
+    return out
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "ozaki_traffic.json")
     try:
@@ -278,6 +417,12 @@ def main():
     memory = None
     if not args.no_memory:
         memory = bench_memory(Q, torch, dev, stream, args.n_dot, args.m, max(args.steps, 3))
+    ill = None
+    if not args.no_ill:
+        ill = bench_ill_conditioned(Q, torch, dev, stream, 8192, 8192, 16384, 26)
+    batched = None
+    if not args.no_batched:
+        batched = bench_batched(Q, torch, dev, stream, max(args.steps, 3), args.batch_gemm, args.batch_gemv)
     cpu = cpu_mem = None
     if not args.no_cpu:
         cpu = cpu_baseline_gemm(N, args.cpu_rows)
@@ -292,7 +437,7 @@ def main():
         "config": {"workload": f"qgemm {N}x{N}x{N} RowMajor alpha=1 beta=0", "form": "ozaki", "slices": S,
                    "error_bound": "each entry <= 2^-105.9 rel (certified) or correctly rounded (repaired)"},
         "roofline": roofline, "cpu_baseline": cpu, "qgemm_forms": forms, "memory_bound": memory,
-        "cpu_baseline_memory_bound": cpu_mem, "ozaki_status": oz["status"], "wall_s": round(oz["wall_s"], 3),
+        "cpu_baseline_memory_bound": cpu_mem, "batched": batched, "ill_conditioned": ill, "ozaki_status": oz["status"], "wall_s": round(oz["wall_s"], 3),
     }
     print(json.dumps(line))
 

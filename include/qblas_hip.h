@@ -107,7 +107,7 @@ int qblas_get_mode(void);
 void qblas_set_kc(int kc);
 int qblas_get_kc(void);
 
-/* Slice count S of QBLAS_MODE_OZAKI (8..19); 0 (the default) lets the library choose.  Returns
+/* Slice count S of QBLAS_MODE_OZAKI (8..28); 0 (the default) lets the library choose.  Returns
    QBLAS_ERR_ARG for other values. */
 int qblas_set_ozaki_slices(int slices);
 int qblas_get_ozaki_slices(void);
@@ -201,6 +201,12 @@ int qblas_hip_gemm_batched_select(int mode, int layout, int64_t count, int idx);
    kind 2: the qdot cancellation variant's x: +-(1e20*u) pairs on slots 0,1 of every 8, kind 0
            elsewhere; kind 3: the matching y (equal values on each pair). */
 int qblas_hip_fill(int kind, uint64_t seed, int64_t n, int lo_exp, int hi_exp, void *d_out, void *stream);
+/* The same generators over a rows x cols matrix: value number r*cols + c is stored at element
+   (r, c) (layout 0: r*ld + c, 1: r + c*ld).  With kind 2 for A (rows = m, cols = k) and kind 3 for
+   B^T (rows = n, cols = k) every product pair A_il B_lj, l % 8 in {0, 1}, cancels exactly: the
+   ill-conditioned qgemm workload (exact sums ~1e-18 of the largest products). */
+int qblas_hip_fill_2d(int kind, uint64_t seed, int64_t rows, int64_t cols, int64_t ld, int layout, int lo_exp,
+                      int hi_exp, void *d_out, void *stream);
 
 #ifdef __cplusplus
 }

@@ -81,6 +81,7 @@ def lib():
         "qblas_hip_gemv": (I, [I, I, I64, I64, P, P, I64, P, I64, P, P, I64, P]),
         "qblas_hip_gemm": (I, [I, I, I64, I64, I64, P, P, I64, P, I64, P, P, I64, P]),
         "qblas_hip_fill": (I, [I, ctypes.c_uint64, I64, I, I, P, P]),
+        "qblas_hip_fill_2d": (I, [I, ctypes.c_uint64, I64, I64, I64, I, I, I, P, P]),
         "qblas_hip_exact_status": (I, [P]),
         "qblas_set_ozaki_slices": (I, [I]),
         "qblas_get_ozaki_slices": (I, []),
@@ -250,7 +251,7 @@ def get_kc():
 
 
 def set_ozaki_slices(s):
-    """Slice count of the OZAKI gemm form (8..19), 0 = chosen by the library."""
+    """Slice count of the OZAKI gemm form (8..28), 0 = chosen by the library."""
     _check(lib().qblas_set_ozaki_slices(int(s)), "qblas_set_ozaki_slices")
 
 
@@ -354,3 +355,9 @@ FILL_CANCEL_Y = 3
 
 def dev_fill(kind, seed, n, out, lo_exp=-30, hi_exp=30, stream=0):
     _check(lib().qblas_hip_fill(int(kind), int(seed), n, int(lo_exp), int(hi_exp), out, stream), "qblas_hip_fill")
+
+
+def dev_fill_2d(kind, seed, rows, cols, ld, layout, out, lo_exp=-30, hi_exp=30, stream=0):
+    """Generator value r*cols + c stored at (r, c) of a rows x cols matrix in the given layout."""
+    _check(lib().qblas_hip_fill_2d(int(kind), int(seed), rows, cols, ld, int(layout), int(lo_exp), int(hi_exp), out,
+                                   stream), "qblas_hip_fill_2d")

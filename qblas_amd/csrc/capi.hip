@@ -703,4 +703,20 @@ int qblas_hip_fill(int kind, uint64_t seed, int64_t n, int lo_exp, int hi_exp, v
   return QBLAS_OK;
 }
 
+int qblas_hip_fill_2d(int kind, uint64_t seed, int64_t rows, int64_t cols, int64_t ld, int layout, int lo_exp,
+                      int hi_exp, void *d_out, void *stream) {
+  if (rows < 0 || cols < 0 || kind < 0 || kind > 3 || (kind == 1 && hi_exp <= lo_exp) || !aligned16(d_out) ||
+      ld < (layout ? rows : cols)) {
+    set_err_msg("qblas_hip_fill_2d: bad arguments");
+    return QBLAS_ERR_ARG;
+  }
+  if (kind == 1 && (lo_exp < -16382 || hi_exp > 16384)) {
+    set_err_msg("qblas_hip_fill_2d: exponent range outside binary128 normals");
+    return QBLAS_ERR_ARG;
+  }
+  QB_CHECK(qbk::launch_fill_2d(kind, seed, rows, cols, ld, layout != 0, lo_exp, hi_exp, (q128 *)d_out,
+                               (hipStream_t)stream));
+  return QBLAS_OK;
+}
+
 }  // extern "C"

@@ -21,6 +21,8 @@ hipError_t launch_gemm_ref(int colmajor, int64_t m, int64_t n, int64_t k, int64_
                            hipStream_t st);
 hipError_t launch_zero(q128 *p, int64_t n, hipStream_t st);
 hipError_t launch_fill(int kind, uint64_t seed, int64_t n, int lo_exp, int hi_exp, q128 *out, hipStream_t st);
+hipError_t launch_fill_2d(int kind, uint64_t seed, int64_t rows, int64_t cols, int64_t ld, int colmajor, int lo_exp,
+                          int hi_exp, q128 *out, hipStream_t st);
 
 // double-double form (kernels_dd.hip); ws sizes from the *_ws functions
 size_t dot_dd_ws(int64_t n);
@@ -47,7 +49,7 @@ hipError_t launch_gemv_exact(int colmajor, int64_t m, int64_t n, q128 alpha, con
 
 // OZAKI form of gemm (kernels_ozaki.hip): S int8 slices per operand.  ws[0] (u32) is the status
 // word as for the exact form; ws[1] counts the entries the call recomputed exactly.
-constexpr int OZ_SMIN = 8, OZ_SMAX = 19;
+constexpr int OZ_SMIN = 8, OZ_SMAX = 28;
 size_t gemm_ozaki_ws(int64_t m, int64_t n, int64_t k, int S);
 hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int S, q128 alpha, const q128 *A,
                              int64_t lda, const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc, void *ws,

@@ -57,7 +57,7 @@ __device__ __forceinline__ int64_t cdiv(int64_t a, int64_t b) { return (a + b - 
 // work items of problem p:
 //   kind 0 (gemm): tiles of tm x tn outputs (none when m, n or k is 0: level3.hpp:221-222)
 //   kind 1 (gemv, REFERENCE): groups of 64 lanes: 32 rows (RowMajor, incx == 1) or 64 rows
-//   kind 2 (gemv, DD): RowMajor one row per wave; ColMajor 64 rows per wave
+//   kind 2 (gemv, DD): RowMajor 4 rows per wave (16 lanes per row); ColMajor 64 rows per wave
 // (gemv has no work when m or n is 0: level2.hpp:21-22, :59-60)
 __device__ __forceinline__ int64_t batch_items(const void *P, int kind, int64_t p, int tm, int tn, int colmajor) {
   if (kind == 0) {
@@ -68,7 +68,7 @@ __device__ __forceinline__ int64_t batch_items(const void *P, int kind, int64_t 
   const qblas_gemv_problem &v = reinterpret_cast<const qblas_gemv_problem *>(P)[p];
   if (v.m <= 0 || v.n <= 0) return 0;
   if (kind == 1) return cdiv(v.m, (!colmajor && v.incx == 1) ? 32 : 64);
-  return colmajor ? cdiv(v.m, 64) : v.m;
+  return colmajor ? cdiv(v.m, 64) : cdiv(v.m, 4);
 }
 
 // start[0] = 0, start[p + 1] = items of problems 0..p; one block, chunks of 1024 problems
@@ -281,8 +281,9 @@ __global__ void __launch_bounds__(256) k_gemv_batched_ref(const qblas_gemv_probl
 }
 
 // ------------------------------------------------------------------------------------------
-// gemv, DD: RowMajor one wave per row (lanes along the row, butterfly sum); ColMajor one lane
-// per row over the columns (coalesced column reads).  y = fma(alpha, RN(sum), mul(beta, y)).
+// gemv, DD: RowMajor 16 lanes per row, 4 rows per wave (lanes along the row, butterfly sum over
+// the 16); ColMajor one lane per row over the columns (coalesced column reads).
+// y = fma(alpha, RN(sum), mul(beta, y)).
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ dd bshfl_xor_dd(dd v, int msk) {
   dd r;
@@ -306,19 +307,22 @@ __global__ void __launch_bounds__(256) k_gemv_batched_dd(const qblas_gemv_proble
     q128 *y = (q128 *)v.y;
     const q128 alpha = qv(v.alpha), beta = qv(v.beta);
     if (!colmajor) {
-      const int64_t i = ul;
-      const q128 *row = A + i * lda;
+      const int64_t i = ul * 4 + (lane >> 4);
+      const int sub = lane & 15;
       dd acc0 = dd_zero(), acc1 = dd_zero();
-      int64_t j = lane;
-      for (; j + 64 < n; j += 128) {
-        acc0 = dd_fma(q_to_dd(bld(row + j)), q_to_dd(bld(x + j * incx)), acc0);
-        acc1 = dd_fma(q_to_dd(bld(row + j + 64)), q_to_dd(bld(x + (j + 64) * incx)), acc1);
+      if (i < m) {
+        const q128 *row = A + i * lda;
+        int64_t j = sub;
+        for (; j + 16 < n; j += 32) {
+          acc0 = dd_fma(q_to_dd(bld(row + j)), q_to_dd(bld(x + j * incx)), acc0);
+          acc1 = dd_fma(q_to_dd(bld(row + j + 16)), q_to_dd(bld(x + (j + 16) * incx)), acc1);
+        }
+        if (j < n) acc0 = dd_fma(q_to_dd(bld(row + j)), q_to_dd(bld(x + j * incx)), acc0);
       }
-      if (j < n) acc0 = dd_fma(q_to_dd(bld(row + j)), q_to_dd(bld(x + j * incx)), acc0);
       dd s = dd_add(acc0, acc1);
 #pragma unroll
-      for (int msk = 32; msk >= 1; msk >>= 1) s = dd_add(s, bshfl_xor_dd(s, msk));
-      if (lane == 0) {
+      for (int msk = 8; msk >= 1; msk >>= 1) s = dd_add(s, bshfl_xor_dd(s, msk));
+      if (i < m && sub == 0) {
         q128 *yi = y + i * incy;
         bst(yi, fma_packed(alpha, dd_to_q(s), mul_packed(beta, bld(yi))));
       }

@@ -14,9 +14,12 @@
 //    lane l's 16 bytes are row l%32, k 16*(l/32)..+15 -- the operand layout of
 //    v_mfma_i32_32x32x32_i8.  Layout [row block][k block][slice][1 KB].
 // 3. Products.  P = sum_{t+u<=S-1} 2^(-8(t+u+2)) (D_t^A (D_u^B)^T) exactly: one kernel keeps the
-//    S diagonal sums acc_d = sum_{t+u=d} D_t D_u^T of its 32x32 tile in int32 registers (no
-//    overflow: (d+1) * 4096 * 2^14 < 2^31 for S <= 31 and k passes of 4096), and writes
-//    V = sum_d acc_d 256^(S-1-d) (exact, <= 192 bits) per k pass.
+//    diagonal sums acc_d = sum_{t+u=d} D_t D_u^T, d in [D0, D1], of its 32x32 tile in int32
+//    registers (no overflow: (d+1) * 4096 * 2^14 < 2^31 for S <= 31 and k passes of 4096), and
+//    writes V = sum_{d=D0..D1} acc_d 256^(S-1-d) (exact: |V| < 2^(8S+17), 8 signed words for
+//    S <= 28) per k pass.  Up to 19
+//    slices one launch covers all diagonals; for 20..28 slices two launches split the diagonals
+//    so the accumulators of each fit in the register file.
 // 4. Finish.  The k passes are summed exactly, alpha*V*2^(E_i+F_j-8(S+1)) + beta*C is rounded
 //    once to binary128 (the same register epilogue as the EXACT form), and the entry is
 //    certified by the a-priori bound
@@ -39,16 +42,23 @@ typedef __attribute__((address_space(1))) void glb_void;
 constexpr int OZ_TILE = 64;             // output rows and columns of one workgroup
 constexpr int OZ_FRAG = 1024;           // bytes of one operand fragment (32 rows x 32 k)
 constexpr int OZ_KPASS_BLOCKS = 128;    // k blocks (of 32) per int32 accumulation pass
-constexpr int OZ_PW = 6;                // signed 32-bit words of a pass sum V
+constexpr int OZ_PW = 8;                // signed 32-bit words of a pass sum V (and of N + bias)
 constexpr int OZ_PREC = 8;              // words per pass-sum record (32 bytes)
-constexpr int OZ_FINW = 8;              // words of |sum of passes| in the finish
+constexpr int OZ_FINW = 9;              // words of |sum of passes| in the finish
 constexpr int32_t OZ_EZERO = INT32_MIN; // row/column exponent of an all-zero row/column
 constexpr uint32_t OZF_NONFINITE = 1u;
 constexpr int OZ_REPAIR_BLOCKS = 1024;
 constexpr int OZ_HDR = 256;             // workspace header bytes: status, repair count
 
+// diagonal ranges: up to 19 slices one range [0, S-1] with 2 slice groups per k block; 20..28
+// slices two ranges [0, D-1] and [D, S-1], D = (3S+4)/5 (balancing accumulators against
+// fragments in the register file), with 3 slice groups so 4 LDS slots fit in 160 KB
+constexpr int OZ_ONE_RANGE_MAX = 19;
+__host__ __device__ constexpr int oz_ranges(int S) { return S <= OZ_ONE_RANGE_MAX ? 1 : 2; }
+__host__ __device__ constexpr int oz_split(int S) { return (3 * S + 4) / 5; }
+
 struct oz_layout {
-  int64_t Mp, Np, Kb, npass;
+  int64_t Mp, Np, Kb, npass, npass_k;
   size_t eA, fA, eB, fB, sA, sB, part, list, total;
 };
 
@@ -57,7 +67,8 @@ static oz_layout oz_plan(int64_t m, int64_t n, int64_t k, int S) {
   L.Mp = (m + OZ_TILE - 1) / OZ_TILE * OZ_TILE;
   L.Np = (n + OZ_TILE - 1) / OZ_TILE * OZ_TILE;
   L.Kb = (k + 31) / 32;
-  L.npass = (L.Kb + OZ_KPASS_BLOCKS - 1) / OZ_KPASS_BLOCKS;
+  L.npass_k = (L.Kb + OZ_KPASS_BLOCKS - 1) / OZ_KPASS_BLOCKS;
+  L.npass = L.npass_k * oz_ranges(S);
   auto al = [](size_t v) { return (v + 255) / 256 * 256; };
   size_t o = OZ_HDR;
   L.eA = o;
@@ -140,7 +151,7 @@ __global__ void __launch_bounds__(256) k_oz_scale(const q128 *__restrict__ X, in
 // ------------------------------------------------------------------------------------------
 // 2. slicing
 // ------------------------------------------------------------------------------------------
-// W (6 words) = N + bias where N = round(x 2^(8S - Es)) (magnitudes rounded half up, then signed)
+// W (8 words) = N + bias where N = round(x 2^(8S - Es)) (magnitudes rounded half up, then signed)
 template <int S>
 __device__ __forceinline__ void oz_digits(q128 x, int Es, uint32_t (&W)[OZ_PW]) {
   uint32_t Nw[OZ_PW];
@@ -150,7 +161,7 @@ __device__ __forceinline__ void oz_digits(q128 x, int Es, uint32_t (&W)[OZ_PW]) 
   if (be != 0x7fffu && !q_iszero(x)) {
     const uq u = unpack_finite(x);
     const int sh = u.e - 112 - Es + 8 * S;  // N = m * 2^sh;  u.e <= Es - 3 so m 2^sh < 2^(8S-2)
-    uint32_t m[OZ_PW] = {(uint32_t)u.ml, (uint32_t)(u.ml >> 32), (uint32_t)u.mh, (uint32_t)(u.mh >> 32), 0u, 0u};
+    uint32_t m[OZ_PW] = {(uint32_t)u.ml, (uint32_t)(u.ml >> 32), (uint32_t)u.mh, (uint32_t)(u.mh >> 32), 0u, 0u, 0u, 0u};
     if (sh >= 0) {
       rshl<OZ_PW>(Nw, m, sh);
     } else if (sh > -115) {
@@ -229,116 +240,128 @@ __global__ void __launch_bounds__(256) k_oz_slice(const q128 *__restrict__ X, in
 // 3. slice products on the matrix cores
 // ------------------------------------------------------------------------------------------
 // Workgroup = 4 waves, 64x64 outputs; wave (wr, wc) owns the 32x32 block (wr, wc).  The k loop
-// runs over half-stages h = (k block h/2, slice group h%2): group 0 holds slices [0, G0), group 1
-// slices [G0, S).  A half-stage is 4 fragment groups (A row blocks 0,1 and B column blocks 0,1;
-// each wave copies one with direct global->LDS loads) in one of 4 LDS slots, so the copies run
-// 3 half-stages ahead of the MFMAs.  The MFMAs of group 0 are the slice pairs with t, u < G0;
-// those of group 1 the pairs with max(t, u) >= G0 (t + u <= S-1), whose group-0 fragments are
-// still in registers.  Within a group, pairs are ordered by max(t, u) so each MFMA needs only
-// fragments read before it.
-template <int S>
+// runs over stages h = (k block h/NG, slice group h%NG): group g holds slices [gG, min(S, gG+G)),
+// G = ceil(S/NG).  A stage is 4 fragment groups (A row blocks 0,1 and B column blocks 0,1; each
+// wave copies one with direct global->LDS loads) in one of 4 LDS slots, so the copies run 3
+// stages ahead of the MFMAs.  The MFMAs of stage g are the slice pairs (t, u) with max(t, u) in
+// group g and D0 <= t + u <= D1; fragments of earlier groups of the same k block are still in
+// registers.  Within a stage pairs are ordered by max(t, u), so each MFMA needs only fragments
+// read before it.  Pass sums go to record z = zbase + k pass.
+template <int S, int D0, int D1, int NG>
 __global__ void __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(1, 1)))
 k_oz_gemm(const int8_t *__restrict__ As, const int8_t *__restrict__ Bs, int64_t Kb, int64_t Mp, int64_t Np,
-          uint32_t *__restrict__ part) {
-  constexpr int G0 = (S + 1) / 2, G1 = S - G0;
-  constexpr int SLOT = 4 * G0 * OZ_FRAG;
-  // s_waitcnt vmcnt(S) lgkmcnt(0): this wave's copies of the two newer half-stages (G0 + G1 = S
-  // loads) may still be in flight
-  constexpr int WAIT_S = (S & 15) | ((S >> 4) << 14) | (7 << 4);
+          int zbase, uint32_t *__restrict__ part) {
+  constexpr int G = (S + NG - 1) / NG;
+  constexpr int SLOT = 4 * G * OZ_FRAG;
+  constexpr int ND = D1 - D0 + 1;
+  constexpr int TMAX = (D1 < S - 1) ? D1 : S - 1;  // highest slice any pair of the range uses
   __shared__ __attribute__((aligned(1024))) int8_t lds[4 * SLOT];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t tn = blockIdx.x, tm = blockIdx.y, z = blockIdx.z;
-  const int64_t kb0 = z * OZ_KPASS_BLOCKS;
+  const int64_t tn = blockIdx.x, tm = blockIdx.y, kp = blockIdx.z;
+  const int64_t kb0 = kp * OZ_KPASS_BLOCKS;
   const int64_t nkb = ((kb0 + OZ_KPASS_BLOCKS < Kb) ? kb0 + OZ_KPASS_BLOCKS : Kb) - kb0;
-  const int64_t nst = 2 * nkb;
+  const int64_t nst = NG * nkb;
   // this wave's copy source: A row block 2tm + wave (waves 0, 1) or B column block 2tn + wave - 2
   const int8_t *src = (wave < 2) ? As + ((2 * tm + wave) * Kb + kb0) * (int64_t)(S * OZ_FRAG)
                                  : Bs + ((2 * tn + wave - 2) * Kb + kb0) * (int64_t)(S * OZ_FRAG);
   src += lane * 16;
   auto copy = [&](int64_t h) {
     if (h >= nst) return;
-    const int8_t *s = src + (h >> 1) * (int64_t)(S * OZ_FRAG) + ((h & 1) ? G0 * OZ_FRAG : 0);
-    int8_t *d = &lds[(int)(h & 3) * SLOT + wave * G0 * OZ_FRAG];
-    if ((h & 1) == 0) {
+    const int64_t kb = h / NG;
+    const int g = (int)(h - kb * NG);
+    const int8_t *sg = src + kb * (int64_t)(S * OZ_FRAG) + g * G * OZ_FRAG;
+    int8_t *d = &lds[(int)(h & 3) * SLOT + wave * G * OZ_FRAG];
 #pragma unroll
-      for (int t = 0; t < G0; ++t)
-        __builtin_amdgcn_global_load_lds((glb_void *)(s + t * OZ_FRAG), (lds_void *)(d + t * OZ_FRAG), 16, 0, 0);
-    } else {
+    for (int gg = 0; gg < NG; ++gg) {
+      if (g == gg) {
+        const int cnt = (S - gg * G < G) ? S - gg * G : G;
 #pragma unroll
-      for (int t = 0; t < G1; ++t)
-        __builtin_amdgcn_global_load_lds((glb_void *)(s + t * OZ_FRAG), (lds_void *)(d + t * OZ_FRAG), 16, 0, 0);
+        for (int t = 0; t < G; ++t)
+          if (t < cnt)
+            __builtin_amdgcn_global_load_lds((glb_void *)(sg + t * OZ_FRAG), (lds_void *)(d + t * OZ_FRAG), 16, 0,
+                                             0);
+      }
     }
   };
   const int wr = wave >> 1, wc = wave & 1;
-  i32x16v acc[S];
+  i32x16v acc[ND];
 #pragma unroll
-  for (int d = 0; d < S; ++d)
+  for (int d = 0; d < ND; ++d)
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[d][r] = 0;
   copy(0);
   copy(1);
   copy(2);
   i32x4v fa[S], fb[S];
-  for (int64_t h = 0; h < nst; h += 2) {
-    // slice group 0 of k block h/2
-    if (h + 2 < nst)
-      __builtin_amdgcn_s_waitcnt(WAIT_S);
-    else
-      __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_s_barrier();
-    {
-      const int8_t *base = &lds[(int)(h & 3) * SLOT + lane * 16];
+  for (int64_t kb = 0; kb < nkb; ++kb) {
 #pragma unroll
-      for (int t = 0; t < G0; ++t) {
-        fa[t] = *reinterpret_cast<const i32x4v *>(base + (wr * G0 + t) * OZ_FRAG);
-        fb[t] = *reinterpret_cast<const i32x4v *>(base + ((2 + wc) * G0 + t) * OZ_FRAG);
+    for (int g = 0; g < NG; ++g) {
+      const int64_t h = kb * NG + g;
+      // this wave's copies of the two newer stages may stay in flight
+      const int c1 = (S - ((g + 1) % NG) * G < G) ? S - ((g + 1) % NG) * G : G;
+      const int c2 = (S - ((g + 2) % NG) * G < G) ? S - ((g + 2) % NG) * G : G;
+      const int pend = c1 + c2;
+      if (h + 2 < nst) {
+        // s_waitcnt vmcnt(pend) lgkmcnt(0); pend is a compile-time constant after unrolling
+        switch (pend) {
+#define QB_WAIT_CASE(n) \
+  case n: __builtin_amdgcn_s_waitcnt(((n) & 15) | (((n) >> 4) << 14) | (7 << 4)); break;
+          QB_WAIT_CASE(2) QB_WAIT_CASE(3) QB_WAIT_CASE(4) QB_WAIT_CASE(5) QB_WAIT_CASE(6) QB_WAIT_CASE(7)
+          QB_WAIT_CASE(8) QB_WAIT_CASE(9) QB_WAIT_CASE(10) QB_WAIT_CASE(11) QB_WAIT_CASE(12) QB_WAIT_CASE(13)
+          QB_WAIT_CASE(14) QB_WAIT_CASE(15) QB_WAIT_CASE(16) QB_WAIT_CASE(17) QB_WAIT_CASE(18) QB_WAIT_CASE(19)
+          QB_WAIT_CASE(20) QB_WAIT_CASE(21) QB_WAIT_CASE(22) QB_WAIT_CASE(23) QB_WAIT_CASE(24) QB_WAIT_CASE(25)
+          QB_WAIT_CASE(26) QB_WAIT_CASE(27) QB_WAIT_CASE(28)
+#undef QB_WAIT_CASE
+          default: __builtin_amdgcn_s_waitcnt(0); break;
+        }
+      } else {
+        __builtin_amdgcn_s_waitcnt(0);
       }
-    }
-    copy(h + 3);
+      __builtin_amdgcn_s_barrier();
+      {
+        const int8_t *base = &lds[(int)(h & 3) * SLOT + lane * 16];
 #pragma unroll
-    for (int mm = 0; mm < G0; ++mm) {
-#pragma unroll
-      for (int u = 0; u <= mm; ++u) acc[mm + u] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[mm], fb[u], acc[mm + u], 0, 0, 0);
-#pragma unroll
-      for (int t = 0; t < mm; ++t) acc[t + mm] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[t], fb[mm], acc[t + mm], 0, 0, 0);
-    }
-    // slice group 1
-    if (h + 3 < nst)
-      __builtin_amdgcn_s_waitcnt(WAIT_S);
-    else
-      __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_s_barrier();
-    {
-      const int8_t *base = &lds[(int)((h + 1) & 3) * SLOT + lane * 16];
-#pragma unroll
-      for (int t = 0; t < G1; ++t) {
-        fa[G0 + t] = *reinterpret_cast<const i32x4v *>(base + (wr * G0 + t) * OZ_FRAG);
-        fb[G0 + t] = *reinterpret_cast<const i32x4v *>(base + ((2 + wc) * G0 + t) * OZ_FRAG);
+        for (int t = 0; t < G; ++t) {
+          const int sl = g * G + t;
+          if (sl < S && sl <= TMAX) {
+            fa[sl] = *reinterpret_cast<const i32x4v *>(base + (wr * G + t) * OZ_FRAG);
+            fb[sl] = *reinterpret_cast<const i32x4v *>(base + ((2 + wc) * G + t) * OZ_FRAG);
+          }
+        }
       }
-    }
-    copy(h + 4);
+      copy(h + 3);
 #pragma unroll
-    for (int mm = G0; mm < S; ++mm) {
+      for (int t2 = 0; t2 < G; ++t2) {
+        const int mm = g * G + t2;
+        if (mm < S) {
 #pragma unroll
-      for (int u = 0; u <= mm && mm + u < S; ++u)
-        acc[mm + u] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[mm], fb[u], acc[mm + u], 0, 0, 0);
+          for (int u = 0; u <= mm; ++u) {
+            const int d = mm + u;
+            if (d >= D0 && d <= D1) acc[d - D0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[mm], fb[u], acc[d - D0], 0, 0, 0);
+          }
 #pragma unroll
-      for (int t = 0; t < mm && t + mm < S; ++t)
-        acc[t + mm] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[t], fb[mm], acc[t + mm], 0, 0, 0);
+          for (int t = 0; t < mm; ++t) {
+            const int d = t + mm;
+            if (d >= D0 && d <= D1) acc[d - D0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[t], fb[mm], acc[d - D0], 0, 0, 0);
+          }
+        }
+      }
     }
   }
-  // V = sum_d acc_d 256^(S-1-d), Horner in 6 signed words; result element r of this lane is
-  // row 8(r/4) + 4(lane/32) + r%4, column lane%32 of the wave's block
+  // V = sum_{d=D0..D1} acc_d 256^(S-1-d): Horner over the range in OZ_PW signed words, then a
+  // shift by 8 (S-1-D1); result element r of this lane is row 8(r/4) + 4(lane/32) + r%4,
+  // column lane%32 of the wave's block
   const int64_t row0 = tm * OZ_TILE + wr * 32 + 4 * (lane >> 5);
   const int64_t col = tn * OZ_TILE + wc * 32 + (lane & 31);
-  uint32_t *pz = part + z * Mp * Np * OZ_PREC;
-#pragma unroll
+  uint32_t *pz = part + (zbase + kp) * Mp * Np * OZ_PREC;
+  constexpr int SH = 8 * (S - 1 - D1);
+#pragma clang loop unroll(full)
   for (int r = 0; r < 16; ++r) {
     uint32_t V[OZ_PW];
 #pragma unroll
     for (int i = 0; i < OZ_PW; ++i) V[i] = 0;
 #pragma unroll
-    for (int d = 0; d < S; ++d) {
+    for (int d = 0; d < ND; ++d) {
 #pragma unroll
       for (int i = OZ_PW - 1; i > 0; --i) V[i] = (V[i] << 8) | (V[i - 1] >> 24);
       V[0] <<= 8;
@@ -346,6 +369,19 @@ k_oz_gemm(const int8_t *__restrict__ As, const int8_t *__restrict__ Bs, int64_t 
       uint32_t c = 0;
 #pragma unroll
       for (int i = 0; i < OZ_PW; ++i) V[i] = addc32(V[i], i == 0 ? a : ext, c, c);
+    }
+    if (SH > 0) {
+      // left shift of the two's complement value by SH bits (SH < 32 * OZ_PW)
+      constexpr int WS = SH / 32, BS = SH % 32;
+      uint32_t T[OZ_PW];
+#pragma unroll
+      for (int i = OZ_PW - 1; i >= 0; --i) {
+        const uint32_t hi = (i - WS >= 0) ? V[i - WS] : 0u;
+        const uint32_t lo = (i - WS - 1 >= 0) ? V[i - WS - 1] : 0u;
+        T[i] = BS ? ((hi << BS) | (lo >> (32 - BS))) : hi;
+      }
+#pragma unroll
+      for (int i = 0; i < OZ_PW; ++i) V[i] = T[i];
     }
     const int64_t row = row0 + 8 * (r >> 2) + (r & 3);
     uint32_t *p = pz + (row * Np + col) * OZ_PREC;
@@ -356,8 +392,8 @@ k_oz_gemm(const int8_t *__restrict__ As, const int8_t *__restrict__ Bs, int64_t 
     v0.w = (int)V[3];
     v1.x = (int)V[4];
     v1.y = (int)V[5];
-    v1.z = 0;
-    v1.w = 0;
+    v1.z = (int)V[6];
+    v1.w = (int)V[7];
     *reinterpret_cast<i32x4v *>(p) = v0;
     *reinterpret_cast<i32x4v *>(p + 4) = v1;
   }
@@ -397,7 +433,7 @@ __global__ void __launch_bounds__(256) k_oz_finish(int64_t M, int64_t N, int S, 
       const i32x4v v0 = *reinterpret_cast<const i32x4v *>(p);
       const i32x4v v1 = *reinterpret_cast<const i32x4v *>(p + 4);
       const uint32_t x[OZ_PW] = {(uint32_t)v0.x, (uint32_t)v0.y, (uint32_t)v0.z, (uint32_t)v0.w, (uint32_t)v1.x,
-                                 (uint32_t)v1.y};
+                                 (uint32_t)v1.y, (uint32_t)v1.z, (uint32_t)v1.w};
       const uint32_t ext = (x[OZ_PW - 1] >> 31) ? 0xffffffffu : 0u;
       uint32_t c = 0;
 #pragma unroll
@@ -521,9 +557,16 @@ static void oz_launch_slices(const oz_layout &L, const q128 *A, int64_t m, int64
 
 template <int S>
 static void oz_launch_products(const oz_layout &L, char *ws, hipStream_t st) {
-  dim3 grid((unsigned)(L.Np / OZ_TILE), (unsigned)(L.Mp / OZ_TILE), (unsigned)L.npass);
-  k_oz_gemm<S><<<grid, 256, 0, st>>>((const int8_t *)(ws + L.sA), (const int8_t *)(ws + L.sB), L.Kb, L.Mp, L.Np,
-                                     (uint32_t *)(ws + L.part));
+  dim3 grid((unsigned)(L.Np / OZ_TILE), (unsigned)(L.Mp / OZ_TILE), (unsigned)L.npass_k);
+  const int8_t *As = (const int8_t *)(ws + L.sA), *Bs = (const int8_t *)(ws + L.sB);
+  uint32_t *part = (uint32_t *)(ws + L.part);
+  if constexpr (S <= OZ_ONE_RANGE_MAX) {
+    k_oz_gemm<S, 0, S - 1, 2><<<grid, 256, 0, st>>>(As, Bs, L.Kb, L.Mp, L.Np, 0, part);
+  } else {
+    constexpr int D = oz_split(S);
+    k_oz_gemm<S, 0, D - 1, 3><<<grid, 256, 0, st>>>(As, Bs, L.Kb, L.Mp, L.Np, 0, part);
+    k_oz_gemm<S, D, S - 1, 3><<<grid, 256, 0, st>>>(As, Bs, L.Kb, L.Mp, L.Np, (int)L.npass_k, part);
+  }
 }
 
 static void oz_scale(const q128 *X, int64_t R, int64_t Cn, int64_t sr, int64_t sc, int32_t *E, uint32_t *F,
@@ -562,7 +605,8 @@ hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int 
 #define QB_OZ_CASE(s) \
   case s: oz_launch_slices<s>(L, A, m, k, asr, asc, B, n, bsr, bsc, ws, st); break;
     QB_OZ_CASE(8) QB_OZ_CASE(9) QB_OZ_CASE(10) QB_OZ_CASE(11) QB_OZ_CASE(12) QB_OZ_CASE(13) QB_OZ_CASE(14)
-    QB_OZ_CASE(15) QB_OZ_CASE(16) QB_OZ_CASE(17) QB_OZ_CASE(18) QB_OZ_CASE(19)
+    QB_OZ_CASE(15) QB_OZ_CASE(16) QB_OZ_CASE(17) QB_OZ_CASE(18) QB_OZ_CASE(19) QB_OZ_CASE(20) QB_OZ_CASE(21)
+    QB_OZ_CASE(22) QB_OZ_CASE(23) QB_OZ_CASE(24) QB_OZ_CASE(25) QB_OZ_CASE(26) QB_OZ_CASE(27) QB_OZ_CASE(28)
 #undef QB_OZ_CASE
   default: break;
   }
@@ -571,7 +615,8 @@ hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int 
 #define QB_OZ_CASE(s) \
   case s: oz_launch_products<s>(L, ws, st); break;
     QB_OZ_CASE(8) QB_OZ_CASE(9) QB_OZ_CASE(10) QB_OZ_CASE(11) QB_OZ_CASE(12) QB_OZ_CASE(13) QB_OZ_CASE(14)
-    QB_OZ_CASE(15) QB_OZ_CASE(16) QB_OZ_CASE(17) QB_OZ_CASE(18) QB_OZ_CASE(19)
+    QB_OZ_CASE(15) QB_OZ_CASE(16) QB_OZ_CASE(17) QB_OZ_CASE(18) QB_OZ_CASE(19) QB_OZ_CASE(20) QB_OZ_CASE(21)
+    QB_OZ_CASE(22) QB_OZ_CASE(23) QB_OZ_CASE(24) QB_OZ_CASE(25) QB_OZ_CASE(26) QB_OZ_CASE(27) QB_OZ_CASE(28)
 #undef QB_OZ_CASE
   default: break;
   }

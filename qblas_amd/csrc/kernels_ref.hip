@@ -208,9 +208,8 @@ __device__ __forceinline__ q128 full_uniform(uint64_t seed, uint64_t i) {
   return add_packed(from_double(a), mul_packed(from_double(b), from_double(0x1p-60)));
 }
 
-__global__ void k_fill(int kind, uint64_t seed, int64_t n, int lo_exp, int hi_exp, q128 *__restrict__ out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// value number i of generator kind (0..3)
+__device__ q128 gen_value(int kind, uint64_t seed, int64_t i, int lo_exp, int hi_exp) {
   q128 v;
   if (kind == 0) {
     v = full_uniform(seed, (uint64_t)i);
@@ -234,7 +233,24 @@ __global__ void k_fill(int kind, uint64_t seed, int64_t n, int lo_exp, int hi_ex
     const int64_t p = ((p0 % 8) < 2) ? p0 : i;
     v = full_uniform(seed + 1, (uint64_t)p);
   }
-  st(out + i, v);
+  return v;
+}
+
+__global__ void k_fill(int kind, uint64_t seed, int64_t n, int lo_exp, int hi_exp, q128 *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  st(out + i, gen_value(kind, seed, i, lo_exp, hi_exp));
+}
+
+// value number r*cols + c of the generator stored at element (r, c) of a rows x cols matrix
+// (RowMajor r*ld + c or ColMajor r + c*ld)
+__global__ void k_fill_2d(int kind, uint64_t seed, int64_t rows, int64_t cols, int64_t ld, int colmajor, int lo_exp,
+                          int hi_exp, q128 *__restrict__ out) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= rows * cols) return;
+  // walk the storage order so that writes coalesce
+  const int64_t r = colmajor ? g % rows : g / cols, c = colmajor ? g / rows : g % cols;
+  st(out + (colmajor ? r + c * ld : r * ld + c), gen_value(kind, seed, r * cols + c, lo_exp, hi_exp));
 }
 
 // ------------------------------------------------------------------------------------------
@@ -301,4 +317,13 @@ hipError_t launch_fill(int kind, uint64_t seed, int64_t n, int lo_exp, int hi_ex
   return hipGetLastError();
 }
 
+
+hipError_t launch_fill_2d(int kind, uint64_t seed, int64_t rows, int64_t cols, int64_t ld, int colmajor, int lo_exp,
+                          int hi_exp, q128 *out, hipStream_t st) {
+  const int64_t n = rows * cols;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_2d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, kind, seed, rows, cols, ld,
+                     colmajor, lo_exp, hi_exp, out);
+  return hipGetLastError();
+}
 }  // namespace qbk

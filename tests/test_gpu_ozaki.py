@@ -80,6 +80,9 @@ CASES = [
     (0, 96, 64, 4100, "log30", 1.0, 0.0, 18),     # two int32 accumulation passes
     (1, 64, 96, 9000, "log10", 2.0, 0.25, 16),    # three passes, ColMajor
     (0, 128, 192, 512, "log30", 1.0, 0.0, 12),    # few slices: mostly repaired
+    (0, 96, 64, 300, "log30", 1.0, 0.0, 20),      # 20..28 slices: two diagonal ranges
+    (1, 100, 70, 4200, "full", 1.5, -0.5, 24),    # two ranges x two k passes, ColMajor
+    (0, 65, 129, 33, "log30", "q", "q", 28),
 ]
 
 
@@ -188,3 +191,42 @@ def test_ozaki_4096_cube_sampled(O, Q, cuda):
     _say(f"  repairs={repairs} of {N * N}; max rel {rel.max():.3e} (2^{np.log2(max(rel.max(), 1e-300)):.1f}), "
          f"median 2^{np.log2(max(np.median(rel), 1e-300)):.1f}")
     assert rel.max() <= TOL
+
+
+def test_fill_2d_matches_host_layout(O, Q, cuda):
+    m, k = 24, 40
+    d = cuda.empty((m * k, 2), dtype=cuda.int64, device="cuda:0")
+    Q.dev_fill_2d(2, 9, m, k, m, 1, d.data_ptr())          # ColMajor m x k, value r*k + c
+    cuda.cuda.synchronize()
+    want = O.fill(2, 9, m * k).reshape(m, k, 2).transpose(1, 0, 2).reshape(-1, 2)
+    assert np.array_equal(from_dev(d), want)
+
+
+def test_ozaki_ill_conditioned_colmajor(O, Q, cuda):
+    # ColMajor 64 x 64 x 16384: A rows carry +-1e20 u pairs along k, B columns equal values on
+    # each pair, so the products of each pair cancel exactly and the entries are ~1e-18 of the
+    # largest products (|C| ~ 2^-68 of 2^(E_i+F_j)).  26 slices certify all but entries within
+    # ~2^-12 of zero; 17 slices certify nothing and every entry is recomputed exactly.
+    m, n, k = 64, 64, 16384
+    dA = cuda.empty((m * k, 2), dtype=cuda.int64, device="cuda:0")
+    dB = cuda.empty((k * n, 2), dtype=cuda.int64, device="cuda:0")
+    Q.dev_fill_2d(2, 5, m, k, m, 1, dA.data_ptr())         # A(i, l) = value i*k + l, at i + l*m
+    Q.dev_fill_2d(3, 5, n, k, k, 0, dB.data_ptr())         # B(l, j) = value j*k + l, at l + j*k
+    cuda.cuda.synchronize()
+    A, B = from_dev(dA), from_dev(dB)
+    C0 = np.zeros((m * n, 2), dtype=np.uint64)
+    ii, jj = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
+    ii, jj = ii.ravel(), jj.ravel()
+    for S, max_repairs in ((26, 4), (17, None)):
+        _say(f"ozaki ill-conditioned ColMajor 64x64x16384 S={S}")
+        got, status, repairs = _run(Q, cuda, 1, m, n, k, 1.0, 0.0, A, m, B, k, C0, m, S)
+        assert status == 0
+        rounded, rel, cand = _exact(O, 1, k, 1.0, A, m, B, k, 0.0, C0, m, got, ii, jj)
+        big = abs(O.to_double(O.q(tuple(int(v) for v in A[0]))))
+        _say(f"  repairs={repairs}; max rel 2^{np.log2(max(rel.max(), 1e-300)):.1f}; |C| ~ "
+             f"{np.median(np.abs(O.to_double(rounded))):.3g} vs |A| up to {big:.3g}")
+        if max_repairs is not None:
+            assert repairs <= max_repairs
+        else:
+            assert repairs == m * n
+        assert rel.max() <= TOL
