@@ -107,8 +107,9 @@ int qblas_get_mode(void);
 void qblas_set_kc(int kc);
 int qblas_get_kc(void);
 
-/* Slice count S of QBLAS_MODE_OZAKI (8..28); 0 (the default) lets the library choose.  Returns
-   QBLAS_ERR_ARG for other values. */
+/* Slice count S of QBLAS_MODE_OZAKI (8..28); 0 (the default) lets the library choose per call:
+   it computes 256 sampled entries exactly and takes the smallest S whose error bound certifies
+   all of them (this probe synchronizes with the stream).  Returns QBLAS_ERR_ARG for other values. */
 int qblas_set_ozaki_slices(int slices);
 int qblas_get_ozaki_slices(void);
 

@@ -265,15 +265,28 @@ int dev_gemv(int mode, int layout, int64_t m, int64_t n, q128 alpha, const void 
   return QBLAS_OK;
 }
 
-// Slice count of the OZAKI form: the value set by qblas_set_ozaki_slices, else 17 (which, for
-// operands spanning up to ~2^60 in magnitude per row and k <= 2^14, certifies all but a small
-// fraction of entries at 2^-104; the rest are recomputed exactly).
-int oz_slices(int64_t m, int64_t n, int64_t k) {
-  (void)m;
-  (void)n;
-  (void)k;
-  const int s = g_oz_slices.load();
-  return s ? s : 17;
+// Slice count of the OZAKI form: the value set by qblas_set_ozaki_slices, else chosen from the
+// data: a probe computes 256 sampled entries exactly and measures how far below 2^(E_i + F_j)
+// they lie; the slice count is the smallest whose certification bound passes all of them (so
+// that few entries need the exact repair).  The probe synchronizes with the stream.
+thread_local Workspace t_pws;
+int oz_slices(int colmajor, int64_t m, int64_t n, int64_t k, const void *A, int64_t lda, const void *B, int64_t ldb,
+              hipStream_t s, int *out) {
+  const int fixed = g_oz_slices.load();
+  if (fixed) {
+    *out = fixed;
+    return QBLAS_OK;
+  }
+  void *ws;
+  int rc = ws_acquire(t_pws, qbk::ozaki_probe_ws(m, n), s, &ws);
+  if (rc) return rc;
+  QB_CHECK(qbk::launch_ozaki_probe(colmajor, m, n, k, (const q128 *)A, lda, (const q128 *)B, ldb, 256, ws, s));
+  int r = 0;
+  QB_CHECK(hipMemcpyAsync(&r, (const char *)ws + 8, 4, hipMemcpyDeviceToHost, s));
+  QB_CHECK(hipStreamSynchronize(s));
+  if ((rc = ws_release(t_pws, s))) return rc;
+  *out = qbk::ozaki_slices_for(k, r);
+  return QBLAS_OK;
 }
 
 int dev_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, q128 alpha, const void *A, int64_t lda,
@@ -289,7 +302,9 @@ int dev_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, q128 alpha, 
   if (m == 0 || n == 0 || k == 0) return QBLAS_OK;  // level3.hpp:221-222
   const int colmajor = layout != 0;
   if (mode == QBLAS_MODE_OZAKI) {
-    const int S = oz_slices(m, n, k);
+    int S = 0;
+    int rc0 = oz_slices(colmajor, m, n, k, A, lda, B, ldb, s, &S);
+    if (rc0) return rc0;
     t_oz_slices = S;
     void *ws;
     int rc = ws_acquire(t_ws, qbk::gemm_ozaki_ws(m, n, k, S), s, &ws);

@@ -56,6 +56,12 @@ hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int 
                              hipStream_t st, hipEvent_t *ev /* null, or 5 events: start, after slicing, after
                              the slice products, after the finish, after the repair */);
 hipError_t ozaki_repair_count(const void *ws, uint32_t *out, hipStream_t st);
+// probe: scales of A and B, then the minimum over `samples` exact entries of ilogb|AB_ij| -
+// (E_i + F_j) into ws word 2 (int); ozaki_slices_for turns it into a slice count
+size_t ozaki_probe_ws(int64_t m, int64_t n);
+hipError_t launch_ozaki_probe(int colmajor, int64_t m, int64_t n, int64_t k, const q128 *A, int64_t lda,
+                              const q128 *B, int64_t ldb, int samples, void *ws, hipStream_t st);
+int ozaki_slices_for(int64_t k, int r);
 
 // batched problems (kernels_batched.hip); ws holds count + 1 prefix sums
 size_t batched_ws(int64_t count);

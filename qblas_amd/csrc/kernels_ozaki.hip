@@ -105,7 +105,7 @@ __global__ void k_oz_init(int32_t *eA, uint32_t *fA, int64_t Mp, int32_t *eB, ui
     eB[g] = OZ_EZERO;
     fB[g] = 0;
   }
-  if (g < OZ_HDR / 4) hdr[g] = 0;
+  if (g < OZ_HDR / 4) hdr[g] = (g == 2) ? 0x7fffffffu : 0u;  // word 2: probe minimum
 }
 
 __device__ __forceinline__ int oz_ilogb(q128 x, uint32_t &fl) {
@@ -474,6 +474,54 @@ __global__ void __launch_bounds__(256) k_oz_finish(int64_t M, int64_t N, int S, 
 // exact alpha*S + beta*C rounded once.  A product below the low window sets bit 0 of the
 // status word and the entry becomes NaN (as in the EXACT form).
 constexpr int OZR_BS = 256;
+
+// Exact sum_l A_il B_lj by one workgroup: two windows per lane (anchor An), wave and block
+// reductions; thread 0 returns with the block's windows and flags.
+__device__ __forceinline__ void oz_block_dot(int64_t K, const q128 *__restrict__ ar, int64_t asc,
+                                             const q128 *__restrict__ bc, int64_t bsr, int An, uint32_t (&Wl)[WW],
+                                             uint32_t (&Wh)[WW], uint32_t &flags, uint32_t (*s_w)[2 * WW + 1]) {
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < WW; ++k) {
+    Wl[k] = 0;
+    Wh[k] = 0;
+  }
+  flags = 0;
+  for (int64_t l = threadIdx.x; l < K; l += OZR_BS) accumulate(Wl, Wh, An, ldq_c(ar + l * asc), ldq_c(bc + l * bsr), flags);
+  wave_sum_win(Wl);
+  wave_sum_win(Wh);
+  flags = wave_or(flags);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < WW; ++k) {
+      s_w[wv][k] = Wl[k];
+      s_w[wv][WW + k] = Wh[k];
+    }
+    s_w[wv][2 * WW] = flags;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 1; w < OZR_BS / 64; ++w) {
+      uint32_t X[WW], Y[WW];
+#pragma unroll
+      for (int k = 0; k < WW; ++k) {
+        X[k] = s_w[w][k];
+        Y[k] = s_w[w][WW + k];
+      }
+      win_add_win(Wl, X);
+      win_add_win(Wh, Y);
+      flags |= s_w[w][2 * WW];
+    }
+  }
+}
+
+// window anchor placing the largest possible product (from the row and column maxima) at the top
+// of the high window
+__device__ __forceinline__ int oz_anchor(int ei, int ej) {
+  return (ei == OZ_EZERO || ej == OZ_EZERO) ? 0 : ei + ej - 224 - (WIN_BAND + WIN_OMAX);
+}
+
 __global__ void __launch_bounds__(OZR_BS) k_oz_repair(int64_t K, const q128 *__restrict__ A, int64_t asr, int64_t asc,
                                                       const q128 *__restrict__ B, int64_t bsr, int64_t bsc,
                                                       const int32_t *__restrict__ eA, const int32_t *__restrict__ eB,
@@ -482,46 +530,12 @@ __global__ void __launch_bounds__(OZR_BS) k_oz_repair(int64_t K, const q128 *__r
                                                       const uint32_t *__restrict__ list) {
   __shared__ uint32_t s_w[OZR_BS / 64][2 * WW + 1];
   const uint32_t count = hdr[1];
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) {
     const int64_t i = list[2 * (int64_t)e], j = list[2 * (int64_t)e + 1];
-    const int ei = eA[i], ej = eB[j];
-    const int An = (ei == OZ_EZERO || ej == OZ_EZERO) ? 0 : ei + ej - 224 - (WIN_BAND + WIN_OMAX);
-    uint32_t Wl[WW], Wh[WW];
-#pragma unroll
-    for (int k = 0; k < WW; ++k) {
-      Wl[k] = 0;
-      Wh[k] = 0;
-    }
-    uint32_t flags = 0;
-    const q128 *ar = A + i * asr;
-    const q128 *bc = B + j * bsc;
-    for (int64_t l = threadIdx.x; l < K; l += OZR_BS) accumulate(Wl, Wh, An, ldq_c(ar + l * asc), ldq_c(bc + l * bsr), flags);
-    wave_sum_win(Wl);
-    wave_sum_win(Wh);
-    flags = wave_or(flags);
-    if (lane == 0) {
-#pragma unroll
-      for (int k = 0; k < WW; ++k) {
-        s_w[wv][k] = Wl[k];
-        s_w[wv][WW + k] = Wh[k];
-      }
-      s_w[wv][2 * WW] = flags;
-    }
-    __syncthreads();
+    const int An = oz_anchor(eA[i], eB[j]);
+    uint32_t Wl[WW], Wh[WW], flags;
+    oz_block_dot(K, A + i * asr, asc, B + j * bsc, bsr, An, Wl, Wh, flags, s_w);
     if (threadIdx.x == 0) {
-#pragma unroll
-      for (int w = 1; w < OZR_BS / 64; ++w) {
-        uint32_t X[WW], Y[WW];
-#pragma unroll
-        for (int k = 0; k < WW; ++k) {
-          X[k] = s_w[w][k];
-          Y[k] = s_w[w][WW + k];
-        }
-        win_add_win(Wl, X);
-        win_add_win(Wh, Y);
-        flags |= s_w[w][2 * WW];
-      }
       q128 *cp = C + i * csr + j * csc;
       q128 out;
       if (flags & FL_LOST) {
@@ -534,6 +548,41 @@ __global__ void __launch_bounds__(OZR_BS) k_oz_repair(int64_t K, const q128 *__r
         out = finish_reg(Mw, An, sign, 0u, flags, true, alpha, beta, ldq_c(cp));
       }
       stq_c(cp, out);
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// 6. slice-count probe
+// ------------------------------------------------------------------------------------------
+// For P pseudo-random entries (i, j): the exact sum_l A_il B_lj, and r = ilogb(|sum|) -
+// (E_i + F_j) (how far the entry lies below the scale of its row and column).  The minimum over
+// the sampled entries goes to out[0] (atomic min; entries that are zero, non-finite or outside
+// the windows are skipped).  This drives the choice of the slice count.
+__global__ void __launch_bounds__(OZR_BS) k_oz_probe(int64_t M, int64_t N, int64_t K, const q128 *__restrict__ A,
+                                                     int64_t asr, int64_t asc, const q128 *__restrict__ B,
+                                                     int64_t bsr, int64_t bsc, const int32_t *__restrict__ eA,
+                                                     const int32_t *__restrict__ eB, int P, int *__restrict__ out) {
+  __shared__ uint32_t s_w[OZR_BS / 64][2 * WW + 1];
+  for (int e = blockIdx.x; e < P; e += gridDim.x) {
+    // splitmix64 of the sample number picks (i, j)
+    uint64_t z = 0x5eed0000ull + (uint64_t)(e + 1) * 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    const int64_t i = (int64_t)((z & 0xffffffffull) % (uint64_t)M), j = (int64_t)((z >> 32) % (uint64_t)N);
+    const int ei = eA[i], ej = eB[j];
+    if (ei == OZ_EZERO || ej == OZ_EZERO) continue;  // uniform per block
+    const int An = oz_anchor(ei, ej);
+    uint32_t Wl[WW], Wh[WW], flags;
+    oz_block_dot(K, A + i * asr, asc, B + j * bsc, bsr, An, Wl, Wh, flags, s_w);
+    if (threadIdx.x == 0 && !(flags & (FL_LOST | FL_NAN | FL_PINF | FL_NINF))) {
+      uint32_t Mw[SW];
+      uint32_t sign;
+      s_from_windows(Mw, sign, Wl, Wh);
+      const int msb = rmsb<SW>(Mw);
+      if (msb >= 0) atomicMin(out, An + msb - (ei + 3) - (ej + 3));
     }
     __syncthreads();
   }
@@ -636,6 +685,45 @@ hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int 
                                                    (const uint32_t *)(ws + L.list));
   if (ev) (void)hipEventRecord(ev[4], st);
   return hipGetLastError();
+}
+
+size_t ozaki_probe_ws(int64_t m, int64_t n) {
+  return (size_t)OZ_HDR + 256 + (size_t)(m + 64) * 8 + (size_t)(n + 64) * 8 + 1024;
+}
+
+hipError_t launch_ozaki_probe(int colmajor, int64_t m, int64_t n, int64_t k, const q128 *A, int64_t lda,
+                              const q128 *B, int64_t ldb, int samples, void *wsv, hipStream_t st) {
+  char *ws = (char *)wsv;
+  auto al = [](size_t v) { return (v + 255) / 256 * 256; };
+  uint32_t *hdr = (uint32_t *)ws;
+  int32_t *eA = (int32_t *)(ws + OZ_HDR);
+  uint32_t *fA = (uint32_t *)(ws + al(OZ_HDR + (size_t)m * 4));
+  int32_t *eB = (int32_t *)(ws + al(OZ_HDR + (size_t)m * 4) + al((size_t)m * 4));
+  uint32_t *fB = (uint32_t *)((char *)eB + al((size_t)n * 4));
+  const int64_t asr = colmajor ? 1 : lda, asc = colmajor ? lda : 1;
+  const int64_t bsr = colmajor ? 1 : ldb, bsc = colmajor ? ldb : 1;
+  const int64_t ninit = (m > n ? m : n) > OZ_HDR / 4 ? (m > n ? m : n) : OZ_HDR / 4;
+  k_oz_init<<<(unsigned)((ninit + 255) / 256), 256, 0, st>>>(eA, fA, m, eB, fB, n, hdr);
+  oz_scale(A, m, k, asr, asc, eA, fA, st);
+  oz_scale(B, n, k, bsc, bsr, eB, fB, st);
+  k_oz_probe<<<(unsigned)(samples < 256 ? samples : 256), OZR_BS, 0, st>>>(m, n, k, A, asr, asc, B, bsr, bsc, eA, eB,
+                                                                         samples, (int *)(hdr + 2));
+  return hipGetLastError();
+}
+
+// Slice count for a probe minimum r (the sampled entries lie >= 2^r below 2^(E_i+F_j)): the
+// smallest S with 1 + kfac(S) - 8S + 2 <= r - 106, i.e. the certification bound (alpha = 1)
+// passes every sampled entry with two bits to spare; clamped to [12, OZ_SMAX].  r = INT_MAX (no
+// usable sample) gives 17.
+int ozaki_slices_for(int64_t k, int r) {
+  if (r == 0x7fffffff) return 17;
+  for (int S = 12; S <= OZ_SMAX; ++S) {
+    const double kb = (double)k * (S + 1) / 4.0;
+    int kfac = 0;
+    while ((double)(1ull << (kfac < 62 ? kfac : 62)) < kb && kfac < 80) ++kfac;
+    if (1 + kfac - 8 * S + 2 <= r - 106) return S;
+  }
+  return OZ_SMAX;
 }
 
 // number of entries the last call queued for exact repair (header word 1)

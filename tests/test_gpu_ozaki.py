@@ -230,3 +230,38 @@ def test_ozaki_ill_conditioned_colmajor(O, Q, cuda):
         else:
             assert repairs == m * n
         assert rel.max() <= TOL
+
+
+def test_ozaki_automatic_slice_count(O, Q, cuda):
+    # the library's choice (slice count 0): a probe of 256 exact entries sets S so the bound
+    # certifies them: 113-bit mantissas at k = 700 need 17 slices whether the entries span one
+    # or sixty binades; the ill-conditioned workload (entries ~2^-68 below their scale) needs
+    # about 26 -- and then almost nothing is left to the exact repair
+    chosen = {}
+    m = n = 192
+    k = 700
+    for kind in ("full", "log30"):
+        A, lda, B, ldb, C0, ldc = _operands(O, 0, m, n, k, kind, 81)
+        got, status, repairs = _run(Q, cuda, 0, m, n, k, 1.0, 0.0, A, lda, B, ldb, C0, ldc, 0)
+        S = Q.ozaki_last_slices()
+        ii, jj = _all_idx(m, n)
+        _, rel, _ = _exact(O, 0, k, 1.0, A, lda, B, ldb, 0.0, C0, ldc, got, ii, jj)
+        _say(f"auto slices {kind}: S={S} repairs={repairs} max rel 2^{np.log2(max(rel.max(), 1e-300)):.1f}")
+        assert status == 0 and rel.max() <= TOL
+        assert repairs <= m * n // 100
+        chosen[kind] = S
+    m, n, k = 64, 64, 16384
+    dA = cuda.empty((m * k, 2), dtype=cuda.int64, device="cuda:0")
+    dB = cuda.empty((k * n, 2), dtype=cuda.int64, device="cuda:0")
+    Q.dev_fill_2d(2, 5, m, k, m, 1, dA.data_ptr())
+    Q.dev_fill_2d(3, 5, n, k, k, 0, dB.data_ptr())
+    cuda.cuda.synchronize()
+    A, B = from_dev(dA), from_dev(dB)
+    C0 = np.zeros((m * n, 2), dtype=np.uint64)
+    got, status, repairs = _run(Q, cuda, 1, m, n, k, 1.0, 0.0, A, m, B, k, C0, m, 0)
+    chosen["ill"] = Q.ozaki_last_slices()
+    ii, jj = _all_idx(m, n)
+    _, rel, _ = _exact(O, 1, k, 1.0, A, m, B, k, 0.0, C0, m, got, ii, jj)
+    _say(f"auto slices ill-conditioned: S={chosen['ill']} repairs={repairs}")
+    assert status == 0 and rel.max() <= TOL and repairs <= 40
+    assert 15 <= chosen["full"] <= chosen["log30"] <= 19 and chosen["ill"] >= 24
