@@ -316,8 +316,9 @@ int dev_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, q128 alpha, 
       ev = t_oz_ev;
     }
     t_oz_ev_valid = ev != nullptr;
+    // the automatic slice count already synchronizes, so the repair may too
     QB_CHECK(qbk::launch_gemm_ozaki(colmajor, m, n, k, S, alpha, (const q128 *)A, lda, (const q128 *)B, ldb, beta,
-                                    (q128 *)C, ldc, ws, s, ev));
+                                    (q128 *)C, ldc, ws, s, ev, g_oz_slices.load() == 0));
     t_exact_status = (const uint32_t *)ws;
     t_oz_hdr = (const uint32_t *)ws;
     return ws_release(t_ws, s);

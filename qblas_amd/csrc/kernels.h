@@ -54,7 +54,8 @@ size_t gemm_ozaki_ws(int64_t m, int64_t n, int64_t k, int S);
 hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int S, q128 alpha, const q128 *A,
                              int64_t lda, const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc, void *ws,
                              hipStream_t st, hipEvent_t *ev /* null, or 5 events: start, after slicing, after
-                             the slice products, after the finish, after the repair */);
+                             the slice products, after the finish, after the repair */,
+                             int sync_repair /* 1: may synchronize to choose how the repair reads B */);
 hipError_t ozaki_repair_count(const void *ws, uint32_t *out, hipStream_t st);
 // probe: scales of A and B, then the minimum over `samples` exact entries of ilogb|AB_ij| -
 // (E_i + F_j) into ws word 2 (int); ozaki_slices_for turns it into a slice count

@@ -59,7 +59,7 @@ __host__ __device__ constexpr int oz_split(int S) { return (3 * S + 4) / 5; }
 
 struct oz_layout {
   int64_t Mp, Np, Kb, npass, npass_k;
-  size_t eA, fA, eB, fB, sA, sB, part, list, total;
+  size_t eA, fA, eB, fB, sA, sB, part, list, bt, total;
 };
 
 static oz_layout oz_plan(int64_t m, int64_t n, int64_t k, int S) {
@@ -87,6 +87,8 @@ static oz_layout oz_plan(int64_t m, int64_t n, int64_t k, int S) {
   o = al(o + (size_t)L.npass * L.Mp * L.Np * OZ_PREC * 4);
   L.list = o;
   o = al(o + (size_t)m * n * 8);
+  L.bt = o;  // B^T (n x k) for the repair of many entries when B's columns are strided
+  o = al(o + (size_t)n * k * 16);
   L.total = o;
   return L;
 }
@@ -233,6 +235,66 @@ __global__ void __launch_bounds__(256) k_oz_slice(const q128 *__restrict__ X, in
     v.z = (int)D[t][2];
     v.w = (int)D[t][3];
     *reinterpret_cast<i32x4v *>(o + t * OZ_FRAG) = v;
+  }
+}
+
+// Row-contiguous operands (sc == 1): one workgroup per (32-row block, 4 k blocks).  Threads
+// read 4 consecutive entries (64 bytes) each, so a load instruction covers 4 KB of two rows;
+// digits are packed 4 per word into an LDS tile [slice][32 rows][128 k], then written out as
+// whole 1 KB fragments.  Same output as k_oz_slice.
+constexpr int OZS_KB = 4;  // k blocks per workgroup
+template <int S>
+__global__ void __launch_bounds__(256) k_oz_slice_rows(const q128 *__restrict__ X, int64_t R, int64_t C, int64_t sr,
+                                                       const int32_t *__restrict__ E, int8_t *__restrict__ out,
+                                                       int64_t Kb) {
+  constexpr int KW = OZS_KB * 32;  // k per tile
+  __shared__ __attribute__((aligned(16))) uint32_t tile[S][32][KW / 4];
+  const int64_t rb = blockIdx.y, kb0 = (int64_t)blockIdx.x * OZS_KB;
+  const int t = threadIdx.x;
+  // phase 1: 32 x 128 entries, 4 consecutive per thread and pass, 4 passes
+#pragma unroll 1
+  for (int q = 0; q < (32 * KW) / (4 * 256); ++q) {
+    const int e4 = t + 256 * q;        // group of 4 entries
+    const int row = e4 / (KW / 4), c4 = e4 % (KW / 4);
+    const int64_t r = rb * 32 + row;
+    const int64_t c = kb0 * 32 + 4 * c4;
+    uint32_t D[S];
+#pragma unroll
+    for (int sl = 0; sl < S; ++sl) D[sl] = 0;
+    const int Er = (r < R) ? E[r] : OZ_EZERO;
+    if (Er != OZ_EZERO) {
+      const q128 *rowp = X + r * sr;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (c + u < C) {
+          uint32_t W[OZ_PW];
+          oz_digits<S>(ldq_c(rowp + c + u), Er + 3, W);
+#pragma unroll
+          for (int sl = 0; sl < S; ++sl) {
+            const int byte = S - 1 - sl;
+            D[sl] |= (((W[byte >> 2] >> (8 * (byte & 3))) & 0xffu) ^ 0x80u) << (8 * u);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int sl = 0; sl < S; ++sl) tile[sl][row][c4] = D[sl];
+  }
+  __syncthreads();
+  // phase 2: fragments (k block kbl, slice sl): lane L holds row L%32, k 16(L/32)..+15
+  const int64_t nkb = (kb0 + OZS_KB <= Kb) ? OZS_KB : Kb - kb0;
+  const int L = t & 63;
+  for (int f = t >> 6; f < OZS_KB * S; f += 4) {
+    const int kbl = f / S, sl = f % S;
+    if (kbl < nkb) {
+      const uint32_t *src = &tile[sl][L & 31][(kbl * 32 + (L >> 5) * 16) / 4];
+      i32x4v v;
+      v.x = (int)src[0];
+      v.y = (int)src[1];
+      v.z = (int)src[2];
+      v.w = (int)src[3];
+      *reinterpret_cast<i32x4v *>(out + ((rb * Kb + kb0 + kbl) * S + sl) * (int64_t)OZ_FRAG + L * 16) = v;
+    }
   }
 }
 
@@ -588,20 +650,49 @@ __global__ void __launch_bounds__(OZR_BS) k_oz_probe(int64_t M, int64_t N, int64
   }
 }
 
+// out[j * K + l] = B[l * ldb + j] (K x N row-major -> N x K row-major), 32 x 32 tiles via LDS
+__global__ void __launch_bounds__(256) k_oz_transpose(const q128 *__restrict__ B, int64_t K, int64_t N, int64_t ldb,
+                                                      q128 *__restrict__ out) {
+  __shared__ uint4 t[32][33];
+  const int64_t l0 = (int64_t)blockIdx.y * 32, j0 = (int64_t)blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+#pragma unroll
+  for (int r = ty; r < 32; r += 8) {
+    const int64_t l = l0 + r, j = j0 + tx;
+    if (l < K && j < N) t[r][tx] = *reinterpret_cast<const uint4 *>(B + l * ldb + j);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = ty; r < 32; r += 8) {
+    const int64_t j = j0 + r, l = l0 + tx;
+    if (l < K && j < N) *reinterpret_cast<uint4 *>(out + j * K + l) = t[tx][r];
+  }
+}
+
 // ------------------------------------------------------------------------------------------
 // host side
 // ------------------------------------------------------------------------------------------
 size_t gemm_ozaki_ws(int64_t m, int64_t n, int64_t k, int S) { return oz_plan(m, n, k, S).total; }
 
+// one operand X(r, c) = X[r*sr + c*sc], R x C, into slices (rows padded to Rp)
+template <int S>
+static void oz_slice_operand(const q128 *X, int64_t R, int64_t C, int64_t sr, int64_t sc, int64_t Rp,
+                             const int32_t *E, int8_t *out, int64_t Kb, hipStream_t st) {
+  if (sc == 1) {
+    dim3 grid((unsigned)((Kb + OZS_KB - 1) / OZS_KB), (unsigned)(Rp / 32));
+    k_oz_slice_rows<S><<<grid, 256, 0, st>>>(X, R, C, sr, E, out, Kb);
+  } else {
+    const int64_t nt = (Rp / 32) * Kb * 64;
+    k_oz_slice<S><<<(unsigned)((nt + 255) / 256), 256, 0, st>>>(X, R, C, sr, sc, E, out, Kb, nt);
+  }
+}
+
 template <int S>
 static void oz_launch_slices(const oz_layout &L, const q128 *A, int64_t m, int64_t k, int64_t asr, int64_t asc,
                              const q128 *B, int64_t n, int64_t bsr, int64_t bsc, char *ws, hipStream_t st) {
-  const int64_t ta = (L.Mp / 32) * L.Kb * 64, tb = (L.Np / 32) * L.Kb * 64;
-  k_oz_slice<S><<<(unsigned)((ta + 255) / 256), 256, 0, st>>>(A, m, k, asr, asc, (const int32_t *)(ws + L.eA),
-                                                               (int8_t *)(ws + L.sA), L.Kb, ta);
+  oz_slice_operand<S>(A, m, k, asr, asc, L.Mp, (const int32_t *)(ws + L.eA), (int8_t *)(ws + L.sA), L.Kb, st);
   // B as rows j, columns l
-  k_oz_slice<S><<<(unsigned)((tb + 255) / 256), 256, 0, st>>>(B, n, k, bsc, bsr, (const int32_t *)(ws + L.eB),
-                                                               (int8_t *)(ws + L.sB), L.Kb, tb);
+  oz_slice_operand<S>(B, n, k, bsc, bsr, L.Np, (const int32_t *)(ws + L.eB), (int8_t *)(ws + L.sB), L.Kb, st);
 }
 
 template <int S>
@@ -634,7 +725,7 @@ static void oz_scale(const q128 *X, int64_t R, int64_t Cn, int64_t sr, int64_t s
 
 hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int S, q128 alpha, const q128 *A,
                              int64_t lda, const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc, void *wsv,
-                             hipStream_t st, hipEvent_t *ev) {
+                             hipStream_t st, hipEvent_t *ev, int sync_repair) {
   if (S < OZ_SMIN || S > OZ_SMAX) return hipErrorInvalidValue;
   char *ws = (char *)wsv;
   const oz_layout L = oz_plan(m, n, k, S);
@@ -681,8 +772,26 @@ hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int 
                                                             L.Mp, L.Np, eA, fA, eB, fB, alpha, beta, C, csr, csc, hdr,
                                                             (uint32_t *)(ws + L.list));
   if (ev) (void)hipEventRecord(ev[3], st);
-  k_oz_repair<<<OZ_REPAIR_BLOCKS, OZR_BS, 0, st>>>(k, A, asr, asc, B, bsr, bsc, eA, eB, alpha, beta, C, csr, csc, hdr,
-                                                   (const uint32_t *)(ws + L.list));
+  const q128 *Br = B;
+  int64_t rbsr = bsr, rbsc = bsc;
+  if (sync_repair && bsr != 1 && n > 1) {
+    // the caller accepts a synchronization: read the queue length, and when it is long, read B's
+    // columns from a transposed copy (contiguous) instead of gathering them with stride ldb
+    uint32_t count = 0;
+    hipError_t e2 = hipMemcpyAsync(&count, hdr + 1, 4, hipMemcpyDeviceToHost, st);
+    if (e2 == hipSuccess) e2 = hipStreamSynchronize(st);
+    if (e2 != hipSuccess) return e2;
+    if (count >= 512) {
+      q128 *Bt = (q128 *)(ws + L.bt);
+      dim3 tg((unsigned)((n + 31) / 32), (unsigned)((k + 31) / 32));
+      k_oz_transpose<<<tg, 256, 0, st>>>(B, k, n, bsr, Bt);
+      Br = Bt;
+      rbsr = 1;
+      rbsc = k;
+    }
+  }
+  k_oz_repair<<<OZ_REPAIR_BLOCKS, OZR_BS, 0, st>>>(k, A, asr, asc, Br, rbsr, rbsc, eA, eB, alpha, beta, C, csr, csc,
+                                                   hdr, (const uint32_t *)(ws + L.list));
   if (ev) (void)hipEventRecord(ev[4], st);
   return hipGetLastError();
 }
