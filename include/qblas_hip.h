@@ -43,6 +43,14 @@ void quadblas_qgemm(char layout, char transa, char transb, int m, int n, int k, 
 
 /* c_interface.hpp:122-131.  The thread count fixes the reference's dot chunking (T); the
    default is the host's OpenMP default (OMP_NUM_THREADS, else the available cores). */
+/* quadblas_qnrm2 c_interface.hpp:34-46: (double) RN(sqrt(dot(x, x))), the dot in the current
+   mode (reference order by default), the square root correctly rounded (Sleef_sqrtq1_u05). */
+double quadblas_qnrm2(int n, void *x, int incx);
+
+/* quadblas_qaxpy c_interface.hpp:49-58 (QuadBLAS::axpy, level1.hpp:139-223):
+   y_i = fma(alpha, x_i, y_i), one correctly rounded FMA per element; n <= 0 is a no-op. */
+void quadblas_qaxpy(int n, double alpha, void *x, int incx, void *y, int incy);
+
 void quadblas_set_num_threads(int num_threads);
 int quadblas_get_num_threads(void);
 
@@ -55,6 +63,12 @@ int quadblas_is_aligned(const void *ptr);
 /* QuadBLAS::dot (level1.hpp:80-137; Vector<L>::dot, cpp_classes.hpp:66-69): writes the
    binary128 result to *result. */
 void quadblas_dot_q(size_t n, const void *x, size_t incx, const void *y, size_t incy, void *result);
+
+/* QuadBLAS::axpy (level1.hpp:190-223; Vector<L>::axpy, cpp_classes.hpp:72-76). */
+void quadblas_axpy_q(size_t n, const void *alpha, const void *x, size_t incx, void *y, size_t incy);
+
+/* Vector<L>::norm (cpp_classes.hpp:78-81): *result = RN(sqrt(QuadBLAS::dot(x, x))). */
+void quadblas_nrm2_q(size_t n, const void *x, size_t incx, void *result);
 
 /* QuadBLAS::gemv (level2.hpp:85-99; Matrix<L>::gemv, cpp_classes.hpp:143-147).
    layout: 0 = RowMajor, 1 = ColMajor.  alpha, beta point to binary128 values. */
@@ -140,6 +154,13 @@ int qblas_hip_exact_status(void *stream);
    QBLAS_MODE_REFERENCE (ignored otherwise).  stream: hipStream_t or NULL. */
 int qblas_hip_dot(int mode, int64_t n, const void *d_x, int64_t incx, const void *d_y, int64_t incy, int T,
                   void *d_result, void *stream);
+
+/* axpy on device pointers; alpha is a HOST pointer to a binary128 value. */
+int qblas_hip_axpy(int64_t n, const void *alpha, const void *d_x, int64_t incx, void *d_y, int64_t incy,
+                   void *stream);
+
+/* nrm2: *d_result = RN(sqrt(dot(x, x))) with the dot of `mode` (T as for qblas_hip_dot). */
+int qblas_hip_nrm2(int mode, int64_t n, const void *d_x, int64_t incx, int T, void *d_result, void *stream);
 
 /* gemv on device pointers; alpha, beta are HOST pointers to binary128 values. */
 int qblas_hip_gemv(int mode, int layout, int64_t m, int64_t n, const void *alpha, const void *d_A, int64_t lda,

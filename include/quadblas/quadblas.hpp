@@ -3,6 +3,7 @@
 // Same names, signatures and semantics as the reference's C++ API for dot / gemv / gemm:
 //   enum class Layout                          include/quadblas/core/types.hpp:8-12
 //   QuadBLAS::dot                              include/quadblas/algorithms/level1.hpp:80-137
+//   QuadBLAS::axpy                             include/quadblas/algorithms/level1.hpp:190-223
 //   QuadBLAS::gemv                             include/quadblas/algorithms/level2.hpp:85-99
 //   QuadBLAS::gemm                             include/quadblas/algorithms/level3.hpp:215-336
 //   QuadBLAS::get_num_threads / set_num_threads  include/quadblas/threading/openmp_utils.hpp:10-24
@@ -71,6 +72,11 @@ inline Sleef_quad dot(size_t n, const Sleef_quad *x, size_t incx, const Sleef_qu
   return r;
 }
 
+inline void axpy(size_t n, Sleef_quad alpha, const Sleef_quad *x, size_t incx, Sleef_quad *y, size_t incy) {
+  if (n == 0) return;
+  quadblas_axpy_q(n, &alpha, x, incx, y, incy);
+}
+
 inline void gemv(Layout layout, size_t m, size_t n, Sleef_quad alpha, const Sleef_quad *A, size_t lda,
                  const Sleef_quad *x, size_t incx, Sleef_quad beta, Sleef_quad *y, size_t incy) {
   quadblas_gemv_q(layout == Layout::ColMajor ? 1 : 0, m, n, &alpha, A, lda, x, incx, &beta, y, incy);
@@ -105,6 +111,16 @@ class Vector {
   const Sleef_quad *data() const { return data_; }
 
   Sleef_quad dot(const Vector &other) const { return QuadBLAS::dot(size_, data_, stride_, other.data_, other.stride_); }
+  // this = alpha * other + this
+  void axpy(Sleef_quad alpha, const Vector &other) {
+    QuadBLAS::axpy(size_, alpha, other.data_, other.stride_, data_, stride_);
+  }
+  // correctly rounded square root of dot(*this)
+  Sleef_quad norm() const {
+    Sleef_quad r = 0;
+    if (size_) quadblas_nrm2_q(size_, data_, stride_, &r);
+    return r;
+  }
 
  private:
   Sleef_quad *data_;

@@ -72,6 +72,7 @@ def lib():
                                             ctypes.c_int64, ctypes.c_int64, P, P, P]
         L.or_exact_gemv.restype = ctypes.c_int
         L.or_exact_gemv.argtypes = [ctypes.c_int, sz, sz, Q128, P, sz, P, sz, Q128, P, sz]
+        L.or_axpy.argtypes = [sz, Q128, P, sz, P, sz]
         L.or_rand64.restype = ctypes.c_uint64
         L.or_rand64.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
         L.or_fill.argtypes = [ctypes.c_int, ctypes.c_uint64, sz, ctypes.c_int, ctypes.c_int, P]
@@ -178,6 +179,12 @@ def dot(n, x, incx, y, incy, T):
     """level1.hpp:80-137 with T = omp_get_max_threads()."""
     _chk(x), _chk(y)
     return lib().or_dot(n, _ptr(x), incx, _ptr(y), incy, int(T))
+
+
+def axpy(n, alpha, x, incx, y, incy):
+    """level1.hpp:139-223; updates y in place."""
+    _chk(x), _chk(y)
+    lib().or_axpy(n, q(alpha), _ptr(x), incx, _ptr(y), incy)
 
 
 def gemv(colmajor, m, n, alpha, A, lda, x, incx, beta, y, incy):

@@ -348,6 +348,12 @@ q128 or_dot_kernel(const q128 *x, const q128 *y, size_t n) {
     return r;
 }
 
+/* level1.hpp:139-223: y_i = fma(alpha, x_i, y_i), one rounding per element in every path
+   (axpy_kernel_vectorized's lanes, its tail and the strided loop compute the same FMA). */
+void or_axpy(size_t n, q128 alpha, const q128 *x, size_t incx, q128 *y, size_t incy) {
+    for (size_t i = 0; i < n; ++i) y[i * incy] = or_fma(alpha, x[i * incx], y[i * incy]);
+}
+
 /* level1.hpp:38-137.  T is omp_get_max_threads() at the call (level1.hpp:46,96). */
 q128 or_dot(size_t n, const q128 *x, size_t incx, const q128 *y, size_t incy, int T) {
     if (n == 0) return QZERO;                                  /* :83-84 */

@@ -64,6 +64,12 @@ def lib():
         "quadblas_qdot": (D, [I, P, I, P, I]),
         "quadblas_qgemv": (None, [C, C, I, I, D, P, I, P, I, D, P, I]),
         "quadblas_qgemm": (None, [C, C, C, I, I, I, D, P, I, P, I, D, P, I]),
+        "quadblas_qnrm2": (D, [I, P, I]),
+        "quadblas_qaxpy": (None, [I, D, P, I, P, I]),
+        "quadblas_axpy_q": (None, [sz, P, P, sz, P, sz]),
+        "quadblas_nrm2_q": (None, [sz, P, sz, P]),
+        "qblas_hip_axpy": (I, [I64, P, P, I64, P, I64, P]),
+        "qblas_hip_nrm2": (I, [I, I64, P, I64, I, P, P]),
         "quadblas_set_num_threads": (None, [I]),
         "quadblas_get_num_threads": (I, []),
         "quadblas_get_version": (ctypes.c_char_p, []),
@@ -361,3 +367,12 @@ def dev_fill_2d(kind, seed, rows, cols, ld, layout, out, lo_exp=-30, hi_exp=30, 
     """Generator value r*cols + c stored at (r, c) of a rows x cols matrix in the given layout."""
     _check(lib().qblas_hip_fill_2d(int(kind), int(seed), rows, cols, ld, int(layout), int(lo_exp), int(hi_exp), out,
                                    stream), "qblas_hip_fill_2d")
+
+
+def dev_axpy(n, alpha, x, incx, y, incy, stream=0):
+    a = _q(alpha)
+    _check(lib().qblas_hip_axpy(n, _ptr(a), x, incx, y, incy, stream), "qblas_hip_axpy")
+
+
+def dev_nrm2(mode, n, x, incx, T, result, stream=0):
+    _check(lib().qblas_hip_nrm2(int(mode), n, x, incx, int(T), result, stream), "qblas_hip_nrm2")

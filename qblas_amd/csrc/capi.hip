@@ -343,6 +343,28 @@ int dev_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, q128 alpha, 
   return QBLAS_OK;
 }
 
+int dev_axpy(int64_t n, q128 alpha, const void *x, int64_t incx, void *y, int64_t incy, hipStream_t s) {
+  if (n <= 0) return QBLAS_OK;  // c_interface.hpp:49-50, level1.hpp:192-193
+  if (incx < 0 || incy < 0) {
+    set_err_msg("qblas_hip_axpy: negative stride");
+    return QBLAS_ERR_ARG;
+  }
+  if (!aligned16(x) || !aligned16(y)) {
+    set_err_msg("qblas_hip_axpy: binary128 pointers must be 16-byte aligned");
+    return QBLAS_ERR_ARG;
+  }
+  QB_CHECK(qbk::launch_axpy(n, alpha, (const q128 *)x, incx, (q128 *)y, incy, s));
+  return QBLAS_OK;
+}
+
+// RN(sqrt(dot(x, x))) with the dot of the given mode (c_interface.hpp:34-46; cpp_classes.hpp:78-81)
+int dev_nrm2(int mode, int64_t n, const void *x, int64_t incx, int T, void *res, hipStream_t s) {
+  int rc = dev_dot(mode, n < 0 ? 0 : n, x, incx, x, incx, T, res, s);
+  if (rc) return rc;
+  QB_CHECK(qbk::launch_sqrt1((q128 *)res, s));
+  return QBLAS_OK;
+}
+
 // ---------------------------------------------------------------------------------------------
 // host-pointer wrappers
 // ---------------------------------------------------------------------------------------------
@@ -569,6 +591,51 @@ int gemm_batched_tune(int mode, int layout, int64_t count, const qblas_gemm_prob
   return best;
 }
 
+int host_axpy(int64_t n, q128 alpha, const void *x, int64_t incx, void *y, int64_t incy) {
+  if (n <= 0) return QBLAS_OK;
+  if (incx < 0 || incy < 0) {
+    set_err_msg("quadblas axpy: negative stride");
+    return QBLAS_ERR_ARG;
+  }
+  hipStream_t s;
+  int rc = host_stream(&s);
+  if (rc) return rc;
+  const size_t xb = span_bytes(n, incx), yb = span_bytes(n, incy);
+  void *dx, *dy;
+  if ((rc = stage_in(0, x, xb, s, &dx))) return rc;
+  if ((rc = stage_in(1, y, yb, s, &dy))) return rc;
+  if ((rc = dev_axpy(n, alpha, dx, incx, dy, incy, s))) return rc;
+  QB_CHECK(hipMemcpyAsync(y, dy, yb, hipMemcpyDeviceToHost, s));
+  QB_CHECK(hipStreamSynchronize(s));
+  for (int i = 0; i < 2; ++i) ws_release(t_stage[i], s);
+  return QBLAS_OK;
+}
+
+int host_nrm2(int64_t n, const void *x, int64_t incx, q128 *out) {
+  if (n <= 0) {
+    *out = qb::q_zero(0);
+    return QBLAS_OK;
+  }
+  if (incx < 0) {
+    set_err_msg("quadblas nrm2: negative stride");
+    return QBLAS_ERR_ARG;
+  }
+  hipStream_t s;
+  int rc = host_stream(&s);
+  if (rc) return rc;
+  void *dx, *dr;
+  if ((rc = stage_in(0, x, span_bytes(n, incx), s, &dx))) return rc;
+  if ((rc = stage_in(2, nullptr, 0, s, &dr))) return rc;
+  if ((rc = dev_nrm2(cur_mode(), n, dx, incx, cur_threads(), dr, s))) return rc;
+  q128 r;
+  QB_CHECK(hipMemcpyAsync(&r, dr, 16, hipMemcpyDeviceToHost, s));
+  QB_CHECK(hipStreamSynchronize(s));
+  ws_release(t_stage[0], s);
+  ws_release(t_stage[2], s);
+  *out = r;
+  return check_exact_status(s);
+}
+
 bool is_colmajor_char(char c) { return c == 'C' || c == 'c'; }
 bool is_trans_char(char c) { return c == 'T' || c == 't' || c == 'C' || c == 'c'; }
 
@@ -605,6 +672,16 @@ void quadblas_qgemm(char layout, char transa, char transb, int m, int n, int k, 
             qb::from_double(beta), C, ldc);
 }
 
+double quadblas_qnrm2(int n, void *x, int incx) {
+  q128 r;
+  if (host_nrm2(n, x, incx, &r) != QBLAS_OK) return std::nan("");
+  return qb::to_double(r);
+}
+
+void quadblas_qaxpy(int n, double alpha, void *x, int incx, void *y, int incy) {
+  host_axpy(n, qb::from_double(alpha), x, incx, y, incy);  // c_interface.hpp:49-58
+}
+
 void quadblas_set_num_threads(int num_threads) { g_threads.store(num_threads > 0 ? num_threads : 0); }
 int quadblas_get_num_threads(void) { return cur_threads(); }
 
@@ -614,6 +691,13 @@ int quadblas_is_aligned(const void *ptr) { return ((uintptr_t)ptr % 32u) == 0; }
 void quadblas_dot_q(size_t n, const void *x, size_t incx, const void *y, size_t incy, void *result) {
   q128 r;
   if (host_dot((int64_t)n, x, (int64_t)incx, y, (int64_t)incy, &r) == QBLAS_OK) memcpy(result, &r, 16);
+}
+void quadblas_axpy_q(size_t n, const void *alpha, const void *x, size_t incx, void *y, size_t incy) {
+  host_axpy((int64_t)n, load_q(alpha), x, (int64_t)incx, y, (int64_t)incy);
+}
+void quadblas_nrm2_q(size_t n, const void *x, size_t incx, void *result) {
+  q128 r;
+  if (host_nrm2((int64_t)n, x, (int64_t)incx, &r) == QBLAS_OK) memcpy(result, &r, 16);
 }
 void quadblas_gemv_q(int layout, size_t m, size_t n, const void *alpha, const void *A, size_t lda, const void *x,
                      size_t incx, const void *beta, void *y, size_t incy) {
@@ -667,6 +751,13 @@ void qblas_clear_error(void) { t_err.clear(); }
 int qblas_hip_dot(int mode, int64_t n, const void *d_x, int64_t incx, const void *d_y, int64_t incy, int T,
                   void *d_result, void *stream) {
   return dev_dot(mode, n, d_x, incx, d_y, incy, T, d_result, (hipStream_t)stream);
+}
+int qblas_hip_axpy(int64_t n, const void *alpha, const void *d_x, int64_t incx, void *d_y, int64_t incy,
+                   void *stream) {
+  return dev_axpy(n, load_q(alpha), d_x, incx, d_y, incy, (hipStream_t)stream);
+}
+int qblas_hip_nrm2(int mode, int64_t n, const void *d_x, int64_t incx, int T, void *d_result, void *stream) {
+  return dev_nrm2(mode, n, d_x, incx, T, d_result, (hipStream_t)stream);
 }
 int qblas_hip_gemv(int mode, int layout, int64_t m, int64_t n, const void *alpha, const void *d_A, int64_t lda,
                    const void *d_x, int64_t incx, const void *beta, void *d_y, int64_t incy, void *stream) {

@@ -20,6 +20,8 @@ hipError_t launch_gemm_ref(int colmajor, int64_t m, int64_t n, int64_t k, int64_
                            int64_t lda, const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc,
                            hipStream_t st);
 hipError_t launch_zero(q128 *p, int64_t n, hipStream_t st);
+hipError_t launch_axpy(int64_t n, q128 alpha, const q128 *x, int64_t incx, q128 *y, int64_t incy, hipStream_t st);
+hipError_t launch_sqrt1(q128 *p, hipStream_t st);  // *p = RN(sqrt(*p))
 hipError_t launch_fill(int kind, uint64_t seed, int64_t n, int lo_exp, int hi_exp, q128 *out, hipStream_t st);
 hipError_t launch_fill_2d(int kind, uint64_t seed, int64_t rows, int64_t cols, int64_t ld, int colmajor, int lo_exp,
                           int hi_exp, q128 *out, hipStream_t st);

@@ -326,4 +326,31 @@ hipError_t launch_fill_2d(int kind, uint64_t seed, int64_t rows, int64_t cols, i
                      colmajor, lo_exp, hi_exp, out);
   return hipGetLastError();
 }
+
+// ------------------------------------------------------------------------------------------
+// axpy and the square root of nrm2 (level1.hpp:139-223, c_interface.hpp:34-59)
+// ------------------------------------------------------------------------------------------
+// y_i = fma(alpha, x_i, y_i): one correctly rounded FMA per element, independent of order and
+// chunking (axpy_kernel_vectorized's lanes and tail compute the same per-element FMA)
+__global__ void k_axpy(int64_t n, q128 alpha, const q128 *__restrict__ x, int64_t incx, q128 *__restrict__ y,
+                       int64_t incy) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    st(y + i * incy, fma_packed(alpha, ld(x + i * incx), ld(y + i * incy)));
+}
+__global__ void k_sqrt1(q128 *p) {
+  if (threadIdx.x == 0) st(p, sqrt_packed(ld(p)));
+}
+
+hipError_t launch_axpy(int64_t n, q128 alpha, const q128 *x, int64_t incx, q128 *y, int64_t incy, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_axpy, dim3((unsigned)blocks), dim3(256), 0, st, n, alpha, x, incx, y, incy);
+  return hipGetLastError();
+}
+hipError_t launch_sqrt1(q128 *p, hipStream_t st) {
+  hipLaunchKernelGGL(k_sqrt1, dim3(1), dim3(64), 0, st, p);
+  return hipGetLastError();
+}
 }  // namespace qbk

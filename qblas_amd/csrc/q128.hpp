@@ -589,4 +589,60 @@ QB_HD double to_double(q128 a) {
   return d;
 }
 
+// Correctly rounded square root (round to nearest even; Sleef_sqrtq1_u05's contract):
+// sqrt(+-0) = +-0, sqrt(+inf) = +inf, negative or NaN -> NaN.  With value = m 2^(e-112),
+// X = m 2^t (t = 116 or 117, so e - 112 - t is even and X has 229..230 bits) and the 115-bit
+// integer root q = floor(sqrt(X)) from the digit-by-digit method; the significand is q >> 2,
+// rounded with guard bit 1 of q and a sticky bit from bit 0 and the remainder X - q^2.
+QB_HD q128 sqrt_packed(q128 a) {
+  const uint32_t be = q_bexp(a);
+  const uint32_t s = (uint32_t)(a.hi >> 63);
+  if (be == 0x7fffu) {
+    if (q_isnan(a) || s) return q_nan();
+    return a;
+  }
+  if (q_iszero(a)) return a;
+  if (s) return q_nan();
+  const uq u = unpack_finite(a);
+  const int t = 116 + (u.e & 1);
+  const int bs = t - 64;  // 52 or 53
+  uint64_t X[4];
+  X[0] = 0;
+  X[1] = u.ml << bs;
+  X[2] = (u.mh << bs) | (u.ml >> (64 - bs));
+  X[3] = u.mh >> (64 - bs);
+  uint64_t r0 = 0, r1 = 0;          // root
+  uint64_t m0 = 0, m1 = 0, m2 = 0;  // remainder
+  for (int i = 114; i >= 0; --i) {
+    const int b = 2 * i;
+    const uint64_t two = (X[b >> 6] >> (b & 63)) & 3u;
+    m2 = (m2 << 2) | (m1 >> 62);
+    m1 = (m1 << 2) | (m0 >> 62);
+    m0 = (m0 << 2) | two;
+    const uint64_t t2 = r1 >> 62, t1 = (r1 << 2) | (r0 >> 62), t0 = (r0 << 2) | 1u;
+    const bool ge = (m2 != t2) ? (m2 > t2) : ((m1 != t1) ? (m1 > t1) : (m0 >= t0));
+    r1 = (r1 << 1) | (r0 >> 63);
+    r0 = (r0 << 1) | (ge ? 1u : 0u);
+    if (ge) {
+      const uint64_t d0 = m0 - t0, b0 = m0 < t0;
+      const uint64_t d1 = m1 - t1 - b0, b1 = (m1 < t1) || (m1 - t1 < b0);
+      m2 = m2 - t2 - b1;
+      m1 = d1;
+      m0 = d0;
+    }
+  }
+  uint64_t ml = (r0 >> 2) | (r1 << 62), mh = r1 >> 2;
+  const uint64_t guard = (r0 >> 1) & 1u;
+  const uint64_t sticky = (r0 & 1u) | ((m0 | m1 | m2) != 0);
+  const uint64_t inc = guard & (sticky | (ml & 1u));
+  ml += inc;
+  mh += (ml == 0) & inc;
+  int er = 114 + (u.e - 112 - t) / 2;
+  if (mh >> 49) {
+    mh >>= 1;
+    er += 1;
+  }
+  return q_make(ml, ((uint64_t)(er + QBIAS) << 48) | (mh & FRAC_HI_MASK));
+}
+
 }  // namespace qb

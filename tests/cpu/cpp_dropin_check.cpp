@@ -93,6 +93,22 @@ int main() {
     for (size_t i = 0; i < 10; ++i) y[i] = 1;
     check(same(x.dot(y), 1), "cancellation dot = 1");
   }
+  // Vector::axpy and Vector::norm (cpp_classes.hpp:72-81), quadblas_qaxpy / quadblas_qnrm2
+  {
+    QuadBLAS::VectorRowMajor x(3), y(3), v(2);
+    for (size_t i = 0; i < 3; ++i) {
+      x[i] = i + 1;
+      y[i] = 1;
+    }
+    y.axpy(2, x);
+    check(same(y[0], 3) && same(y[1], 5) && same(y[2], 7), "Vector::axpy 2*[1,2,3] + 1 = [3,5,7]");
+    v[0] = 3;
+    v[1] = 4;
+    check(same(v.norm(), 5), "Vector::norm [3,4] = 5");
+    check(quadblas_qnrm2(2, v.data(), 1) == 5.0, "quadblas_qnrm2 [3,4] = 5");
+    quadblas_qaxpy(3, -1.0, x.data(), 1, y.data(), 1);
+    check(same(y[0], 2) && same(y[1], 3) && same(y[2], 4), "quadblas_qaxpy");
+  }
   // the OZAKI form through the unchanged C++ API: within 2^-104 of the reference-order result
   {
     const size_t n = 96;

@@ -64,7 +64,32 @@ static bool same(q128 x, q128 y) {
   if (qb::q_isnan(x) && qb::q_isnan(y)) return true;
   return x.lo == y.lo && x.hi == y.hi;
 }
+// "sqrt <seed> <count>": print operand and qb::sqrt_packed result as hex words, one pair per line
+// (random positive values over the whole exponent range, subnormals included, and exact
+// squares); the caller checks correct rounding in exact integer arithmetic
+static int sqrt_dump(long count) {
+  for (long i = 0; i < count; ++i) {
+    q128 a = qb::q_make(rnd(), rnd());
+    const uint64_t k = rnd() % 4;
+    const int be = k == 0 ? (int)(rnd() % 0x7fff) : (k == 1 ? 0 : 16383 + (int)(rnd() % 200) - 100);
+    a.hi = (a.hi & qb::FRAC_HI_MASK) | ((uint64_t)be << 48);
+    if (k == 3) {
+      const __float128 v = (__float128)(double)(rnd() >> 11) * (__float128)0x1p-20;
+      a = g(v * v);
+    }
+    if (qb::q_iszero(a)) continue;
+    const q128 r = qb::sqrt_packed(a);
+    printf("%016llx %016llx %016llx %016llx\n", (unsigned long long)a.hi, (unsigned long long)a.lo,
+           (unsigned long long)r.hi, (unsigned long long)r.lo);
+  }
+  return 0;
+}
+
 int main(int argc, char** argv) {
+  if (argc > 1 && !strcmp(argv[1], "sqrt")) {
+    st = argc > 2 ? strtoull(argv[2], 0, 10) : 1;
+    return sqrt_dump(argc > 3 ? atol(argv[3]) : 10000);
+  }
   st = argc > 1 ? strtoull(argv[1], 0, 10) : 1;
   long count = argc > 2 ? atol(argv[2]) : 100000;
   long bad = 0;

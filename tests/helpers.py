@@ -54,6 +54,24 @@ def round_fraction(v):
     return (frac & ((1 << 64) - 1), (s << 63) | (be << 48) | (frac >> 64))
 
 
+def sqrt_rn(lo, hi):
+    """Correctly rounded (nearest even) binary128 square root of a finite value >= 0, computed
+    exactly with integers: returns (lo, hi)."""
+    import math
+    v = to_fraction(lo, hi)
+    assert v >= 0
+    if v == 0:
+        return (int(lo), int(hi))
+    # v = num / den; sqrt(v) scaled by 2^k with k large enough for > 120 significant bits
+    k = 256 - (v.numerator.bit_length() - v.denominator.bit_length()) // 2
+    X = (v.numerator << (2 * k)) // v.denominator if k >= 0 else v.numerator // (v.denominator << (-2 * k))
+    exact_div = ((v.numerator << (2 * k)) % v.denominator == 0) if k >= 0 else True
+    q = math.isqrt(X)
+    exact = exact_div and q * q == X
+    # sqrt(v) = (q + f) 2^-k, 0 <= f < 1 (f = 0 iff exact); value to round: (q + f) 2^-k
+    return round_fraction(Fraction(q, 1) / (Fraction(2) ** k) + (0 if exact else Fraction(1, 2 ** (k + 400))))
+
+
 def pair(a):
     a = np.asarray(a, dtype=np.uint64).reshape(2)
     return (int(a[0]), int(a[1]))

@@ -23,6 +23,39 @@ def test_q128_core_matches_libquadmath(tmp_path):
         assert "mismatches 0 " in r.stdout
 
 
+def _decode(hi, lo):
+    """(sign, m, e) with value = (-1)^sign m 2^e for a finite binary128."""
+    be = (hi >> 48) & 0x7FFF
+    frac = ((hi & ((1 << 48) - 1)) << 64) | lo
+    if be == 0:
+        return hi >> 63, frac, 1 - 16383 - 112
+    return hi >> 63, frac | (1 << 112), be - 16383 - 112
+
+
+def test_sqrt_is_correctly_rounded(tmp_path):
+    # qb::sqrt_packed (the square root of Vector::norm / quadblas_qnrm2) against the definition of
+    # round-to-nearest: r = m 2^e with (2m-1)^2 2^(2e) <= 4a <= (2m+1)^2 2^(2e), in exact integers
+    exe = tmp_path / "q128_host_check"
+    src = os.path.join(ROOT, "tests", "cpu", "q128_host_check.cpp")
+    subprocess.run(["g++", "-O2", "-std=gnu++17", "-o", str(exe), src, "-lquadmath"], check=True)
+    out = subprocess.run([str(exe), "sqrt", "7", "20000"], capture_output=True, text=True, check=True).stdout
+    n = 0
+    for line in out.splitlines():
+        ahi, alo, rhi, rlo = (int(v, 16) for v in line.split())
+        sa, ma, ea = _decode(ahi, alo)
+        sr, mr, er = _decode(rhi, rlo)
+        assert sa == 0 and sr == 0 and mr >> 112 == 1, line        # positive, normal result
+        # 4a vs (2m -+ 1)^2 2^(2e): scale both sides to a common power of two
+        lo_side, hi_side = (2 * mr - 1) ** 2, (2 * mr + 1) ** 2
+        sh = ea + 2 - 2 * er                                          # 4a = ma 2^(ea+2)
+        lhs = ma << sh if sh >= 0 else ma
+        lo_s, hi_s = (lo_side, hi_side) if sh >= 0 else (lo_side << -sh, hi_side << -sh)
+        assert lo_s <= lhs <= hi_s, line
+        assert lhs != lo_s and lhs != hi_s, line                        # never a tie
+        n += 1
+    assert n > 15000
+
+
 def _declared_functions():
     hdr = open(os.path.join(ROOT, "include", "qblas_hip.h")).read()
     hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)

@@ -114,6 +114,20 @@ def test_known_answer_cancellation(O):
     assert O.to_pair(r) == O.to_pair(O.q(1.0))
 
 
+def test_axpy_known_answer_and_elementwise_fma(O):
+    x = qa(O, [1, 2, 3])
+    y = qa(O, [1, 1, 1])
+    O.axpy(3, 2.0, x, 1, y, 1)
+    assert [val(O.to_pair(O.q(p))) for p in y] == [3, 5, 7]
+    # every element is one correctly rounded fma (level1.hpp:156, :207-220): compare with fmaq
+    x = O.fill(0, 1, 50)
+    y = O.fill(0, 2, 50)
+    a = O.fill(0, 3, 1)[0]
+    want = [O.to_pair(O.fmaq(a, x[i], y[i])) for i in range(50)]
+    O.axpy(50, a, x, 1, y, 1)
+    assert [tuple(int(v) for v in y[i]) for i in range(50)] == want
+
+
 def test_known_answer_gemv(O):
     # test_quadblas.cpp:327-350: [[1,2,3],[4,5,6],[7,8,9]] . 1 = [6, 15, 24]
     A = qa(O, range(1, 10))
