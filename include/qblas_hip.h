@@ -200,7 +200,8 @@ typedef struct qblas_gemv_problem {
   uint64_t beta[2];
 } qblas_gemv_problem;
 
-/* d_problems: device array of count descriptors; modes REFERENCE and DD. */
+/* d_problems: device array of count descriptors; gemm: modes REFERENCE and DD; gemv: REFERENCE,
+   DD and EXACT (qblas_hip_exact_status then reports products outside the accumulation window). */
 int qblas_hip_gemm_batched(int mode, int layout, int64_t count, const qblas_gemm_problem *d_problems, void *stream);
 int qblas_hip_gemv_batched(int mode, int layout, int64_t count, const qblas_gemv_problem *d_problems, void *stream);
 

@@ -774,11 +774,14 @@ int qblas_hip_gemm_batched(int mode, int layout, int64_t count, const qblas_gemm
   return gemm_batched(mode, layout, count, d_problems, tuned_config(mode, layout, count), (hipStream_t)stream);
 }
 int qblas_hip_gemv_batched(int mode, int layout, int64_t count, const qblas_gemv_problem *d_problems, void *stream) {
-  int rc = check_batched("qblas_hip_gemv_batched", mode, count, d_problems);
+  const int exact = mode == QBLAS_MODE_EXACT;
+  int rc = check_batched("qblas_hip_gemv_batched", exact ? QBLAS_MODE_REFERENCE : mode, count, d_problems);
   if (rc || count == 0) return rc;
   void *ws;
   if ((rc = batch_ws(qbk::batched_ws(count), &ws))) return rc;
-  QB_CHECK(qbk::launch_gemv_batched(mode == QBLAS_MODE_DD, layout != 0, count, d_problems, ws, (hipStream_t)stream));
+  const int form = exact ? 2 : (mode == QBLAS_MODE_DD ? 1 : 0);
+  QB_CHECK(qbk::launch_gemv_batched(form, layout != 0, count, d_problems, ws, (hipStream_t)stream));
+  if (exact) t_exact_status = (const uint32_t *)((const char *)ws + qbk::batched_status_offset(count));
   return QBLAS_OK;
 }
 int qblas_hip_gemm_batched_tune(int mode, int layout, int64_t count, const qblas_gemm_problem *d_problems,

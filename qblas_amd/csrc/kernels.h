@@ -72,6 +72,8 @@ int gemm_batched_configs(int dd_mode);
 const char *gemm_batched_config_name(int dd_mode, int idx);
 hipError_t launch_gemm_batched(int dd_mode, int colmajor, int64_t count, const qblas_gemm_problem *P, int cfg,
                                int64_t kc, void *ws, hipStream_t st);
-hipError_t launch_gemv_batched(int dd_mode, int colmajor, int64_t count, const qblas_gemv_problem *P, void *ws,
+// form 0 REFERENCE, 1 DD, 2 EXACT (status word at ws + batched_status_offset(count))
+size_t batched_status_offset(int64_t count);
+hipError_t launch_gemv_batched(int form, int colmajor, int64_t count, const qblas_gemv_problem *P, void *ws,
                                hipStream_t st);
 }  // namespace qbk

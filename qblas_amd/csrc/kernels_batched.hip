@@ -16,9 +16,13 @@
 //                   y = fma(alpha, sum, mul(beta, y)); ColMajor y = mul(beta, y) then
 //                   y = fma(A_ij, mul(alpha, x_j), y) over j.
 //   DD gemm / gemv  double-double accumulation (dd.hpp), C = fma(alpha, RN(acc), mul(beta, C)).
+//   EXACT gemv      RN(alpha * sum_j A_ij x_j + beta * y_i), the exact value rounded once (exact.hpp);
+//                   a product outside the accumulation windows makes y_i NaN and sets bit 0 of
+//                   the status word.
 // Tile shapes are template parameters; the host picks one configuration per call (the
 // autotuner in capi.hip measures them).
 #include "dd.hpp"
+#include "exact_dev.hpp"
 #include "kernels.h"
 
 using namespace qb;
@@ -58,6 +62,7 @@ __device__ __forceinline__ int64_t cdiv(int64_t a, int64_t b) { return (a + b - 
 //   kind 0 (gemm): tiles of tm x tn outputs (none when m, n or k is 0: level3.hpp:221-222)
 //   kind 1 (gemv, REFERENCE): groups of 64 lanes: 32 rows (RowMajor, incx == 1) or 64 rows
 //   kind 2 (gemv, DD): RowMajor 4 rows per wave (16 lanes per row); ColMajor 64 rows per wave
+//   kind 3 (gemv, EXACT): RowMajor one row per wave; ColMajor 64 rows per wave
 // (gemv has no work when m or n is 0: level2.hpp:21-22, :59-60)
 __device__ __forceinline__ int64_t batch_items(const void *P, int kind, int64_t p, int tm, int tn, int colmajor) {
   if (kind == 0) {
@@ -68,6 +73,7 @@ __device__ __forceinline__ int64_t batch_items(const void *P, int kind, int64_t 
   const qblas_gemv_problem &v = reinterpret_cast<const qblas_gemv_problem *>(P)[p];
   if (v.m <= 0 || v.n <= 0) return 0;
   if (kind == 1) return cdiv(v.m, (!colmajor && v.incx == 1) ? 32 : 64);
+  if (kind == 3) return colmajor ? cdiv(v.m, 64) : v.m;
   return colmajor ? cdiv(v.m, 64) : cdiv(v.m, 4);
 }
 
@@ -344,6 +350,72 @@ __global__ void __launch_bounds__(256) k_gemv_batched_dd(const qblas_gemv_proble
 }
 
 // ------------------------------------------------------------------------------------------
+// gemv, EXACT: RowMajor one wave per row (wave-uniform anchor from the row's first products);
+// ColMajor one lane per row over the columns (anchor from the row's first regular product)
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ q128 exact_row_out(const uint32_t (&Wl)[WW], const uint32_t (&Wh)[WW], int An,
+                                              uint32_t flags, q128 alpha, q128 beta, q128 yv, uint32_t *status) {
+  if (flags & FL_LOST) {
+    atomicOr(status, 1u);
+    return q_nan();
+  }
+  uint32_t Mw[SW];
+  uint32_t sign;
+  s_from_windows(Mw, sign, Wl, Wh);
+  return finish_reg(Mw, An, sign, 0u, flags, true, alpha, beta, yv);
+}
+
+__global__ void __launch_bounds__(256) k_gemv_batched_exact(const qblas_gemv_problem *__restrict__ P, int64_t count,
+                                                            const int64_t *__restrict__ start, int colmajor,
+                                                            uint32_t *__restrict__ status) {
+  const int lane = threadIdx.x & 63;
+  const int64_t total = start[count];
+  const int64_t waves = (int64_t)gridDim.x * 4;
+  for (int64_t u = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); u < total; u += waves) {
+    const int64_t p = find_problem(start, count, u);
+    const qblas_gemv_problem &v = P[p];
+    const int64_t m = v.m, n = v.n, lda = v.lda, incx = v.incx, incy = v.incy;
+    const int64_t ul = u - start[p];
+    const q128 *A = (const q128 *)v.A;
+    const q128 *x = (const q128 *)v.x;
+    q128 *y = (q128 *)v.y;
+    const q128 alpha = qv(v.alpha), beta = qv(v.beta);
+    uint32_t Wl[WW], Wh[WW];
+#pragma unroll
+    for (int k = 0; k < WW; ++k) {
+      Wl[k] = 0;
+      Wh[k] = 0;
+    }
+    uint32_t flags = 0;
+    if (!colmajor) {
+      const int64_t i = ul;
+      const q128 *row = A + i * lda;
+      int e0 = EXP_NONE;
+      if (lane < n) e0 = pair_exp(ldq_c(row + lane), ldq_c(x + (int64_t)lane * incx));
+      const int An = wave_anchor(e0);
+      for (int64_t j = lane; j < n; j += 64) accumulate(Wl, Wh, An, ldq_c(row + j), ldq_c(x + j * incx), flags);
+      wave_sum_win(Wl);
+      wave_sum_win(Wh);
+      flags = wave_or(flags);
+      if (lane == 0) {
+        q128 *yi = y + i * incy;
+        stq_c(yi, exact_row_out(Wl, Wh, An, flags, alpha, beta, ldq_c(yi), status));
+      }
+    } else {
+      const int64_t i = ul * 64 + lane;
+      if (i < m) {
+        int e0 = EXP_NONE;
+        for (int64_t j = 0; j < n && j < 4 && e0 == EXP_NONE; ++j) e0 = pair_exp(ldq_c(A + j * lda + i), ldq_c(x + j * incx));
+        const int An = (e0 == EXP_NONE) ? 0 : e0 - WIN2_CENTER;
+        for (int64_t j = 0; j < n; ++j) accumulate(Wl, Wh, An, ldq_c(A + j * lda + i), ldq_c(x + j * incx), flags);
+        q128 *yi = y + i * incy;
+        stq_c(yi, exact_row_out(Wl, Wh, An, flags, alpha, beta, ldq_c(yi), status));
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
 // host side
 // ------------------------------------------------------------------------------------------
 constexpr int BATCH_GRID = 2048;  // persistent workgroups (8 per CU)
@@ -368,7 +440,13 @@ const char *gemm_batched_config_name(int dd_mode, int idx) {
   return dd_mode ? DD_CFG[idx].name : REF_CFG[idx].name;
 }
 
-size_t batched_ws(int64_t count) { return (size_t)(count + 1) * sizeof(int64_t); }
+// prefix sums (count + 1) and, 256 bytes further on, the status word of the EXACT form
+size_t batched_ws(int64_t count) { return ((size_t)(count + 1) * sizeof(int64_t) + 255) / 256 * 256 + 256; }
+size_t batched_status_offset(int64_t count) { return ((size_t)(count + 1) * sizeof(int64_t) + 255) / 256 * 256; }
+
+__global__ void k_batch_status_zero(uint32_t *s) {
+  if (threadIdx.x == 0) s[0] = 0;
+}
 
 hipError_t launch_gemm_batched(int dd_mode, int colmajor, int64_t count, const qblas_gemm_problem *P, int cfg,
                                int64_t kc, void *ws, hipStream_t st) {
@@ -395,14 +473,20 @@ hipError_t launch_gemm_batched(int dd_mode, int colmajor, int64_t count, const q
   return hipGetLastError();
 }
 
-hipError_t launch_gemv_batched(int dd_mode, int colmajor, int64_t count, const qblas_gemv_problem *P, void *ws,
+hipError_t launch_gemv_batched(int form, int colmajor, int64_t count, const qblas_gemv_problem *P, void *ws,
                                hipStream_t st) {
   int64_t *start = (int64_t *)ws;
-  k_batch_scan<<<1, 1024, 0, st>>>(P, dd_mode ? 2 : 1, count, 0, 0, colmajor, start);
-  if (!dd_mode)
+  const int kind = form == 1 ? 2 : (form == 2 ? 3 : 1);
+  k_batch_scan<<<1, 1024, 0, st>>>(P, kind, count, 0, 0, colmajor, start);
+  if (form == 0) {
     k_gemv_batched_ref<<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor);
-  else
+  } else if (form == 1) {
     k_gemv_batched_dd<<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor);
+  } else {
+    uint32_t *status = (uint32_t *)((char *)ws + batched_status_offset(count));
+    k_batch_status_zero<<<1, 64, 0, st>>>(status);
+    k_gemv_batched_exact<<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, status);
+  }
   return hipGetLastError();
 }
 

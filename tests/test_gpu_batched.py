@@ -5,6 +5,7 @@ routine on that problem bit for bit (level3.hpp:215-336 / level2.hpp:15-99 per p
 tile configuration, both layouts, padded leading dimensions and strides, problems that take the
 simple path (m, n, k <= 64) and the kc = 126 blocked path, and empty problems.
 DD form: every entry within |got - exact| <= 2^-100 (|alpha| sum|a b| + |beta c|) + 1 ulp.
+EXACT gemv: every problem equals the oracle's exact accumulation rounded once, bit for bit.
 The launch sequence is captured in a graph and replayed against the eager result, and the
 autotuner's choice is a valid configuration.
 """
@@ -226,3 +227,18 @@ def test_gemm_batched_graph_replays_and_tuner(O, Q, cuda):
         assert torch.equal(pool.dev, eager), rep
         _ = (pool.dev.float().sum() if rep == 1 else None)   # unrelated eager work between replays
     Q.gemm_batched_select(Q.MODE_REFERENCE, layout, len(dims), 0)
+
+
+def test_gemv_batched_exact(O, Q, cuda):
+    # EXACT form: every problem's y equals the oracle's exact accumulation rounded once, bit for bit
+    for layout in (0, 1):
+        _say(f"gemv batched exact layout={layout}")
+        dims, arrays, pool, ddesc = make_gemv_batch(O, cuda, layout, 40, 1, 500, 61 + layout)
+        Q.dev_gemv_batched(Q.MODE_EXACT, layout, len(dims), ddesc.data_ptr())
+        cuda.cuda.synchronize()
+        assert Q.dev_exact_status() == 0
+        for p, (m, n, lda, incx, incy, a, b) in enumerate(dims):
+            y = arrays[3 * p + 2].copy()
+            if m and n:
+                O.exact_gemv(layout, m, n, a, arrays[3 * p], lda, arrays[3 * p + 1], incx, b, y, incy)
+            assert np.array_equal(pool.get(3 * p + 2, y.shape[0]), y), (layout, p, m, n, incx)
