@@ -112,7 +112,9 @@ enum {
 };
 
 /* Mode used by Parts 1 and 2 (default QBLAS_MODE_REFERENCE; environment QBLAS_MODE overrides at
-   load: "reference", "exact", "dd", "ozaki"). */
+   load: "reference", "exact", "dd", "ozaki").  Per routine: with QBLAS_MODE_OZAKI, dot / gemv /
+   nrm2 use QBLAS_MODE_EXACT; with QBLAS_MODE_EXACT, gemm uses QBLAS_MODE_OZAKI.  Part 3 takes the
+   mode as given and reports QBLAS_ERR_MODE where a routine lacks it. */
 int qblas_set_mode(int mode);
 int qblas_get_mode(void);
 

@@ -81,6 +81,18 @@ int env_mode() {
   if (!strcmp(e, "ozaki")) return QBLAS_MODE_OZAKI;
   return QBLAS_MODE_REFERENCE;
 }
+// The global mode for the host-pointer entry points, per routine: OZAKI exists for gemm only,
+// so dot / gemv / nrm2 then use EXACT (correctly rounded, a stronger guarantee); EXACT has no gemm,
+// which then uses OZAKI (certified to 2^-104).
+int cur_mode();
+int vec_mode() {
+  const int m = cur_mode();
+  return m == QBLAS_MODE_OZAKI ? QBLAS_MODE_EXACT : m;
+}
+int mat_mode() {
+  const int m = cur_mode();
+  return m == QBLAS_MODE_EXACT ? QBLAS_MODE_OZAKI : m;
+}
 int cur_mode() {
   int m = g_mode.load();
   if (m < 0) {
@@ -394,7 +406,7 @@ int host_dot(int64_t n, const void *x, int64_t incx, const void *y, int64_t incy
   if ((rc = stage_in(0, x, span_bytes(n, incx), s, &dx))) return rc;
   if ((rc = stage_in(1, y, span_bytes(n, incy), s, &dy))) return rc;
   if ((rc = stage_in(2, nullptr, 0, s, &dr))) return rc;
-  if ((rc = dev_dot(cur_mode(), n, dx, incx, dy, incy, cur_threads(), dr, s))) return rc;
+  if ((rc = dev_dot(vec_mode(), n, dx, incx, dy, incy, cur_threads(), dr, s))) return rc;
   q128 r;
   QB_CHECK(hipMemcpyAsync(&r, dr, 16, hipMemcpyDeviceToHost, s));
   QB_CHECK(hipStreamSynchronize(s));
@@ -419,7 +431,7 @@ int host_gemv(int layout, int64_t m, int64_t n, q128 alpha, const void *A, int64
   if ((rc = stage_in(0, A, abytes, s, &dA))) return rc;
   if ((rc = stage_in(1, x, xbytes, s, &dx))) return rc;
   if ((rc = stage_in(2, y, ybytes, s, &dy))) return rc;
-  if ((rc = dev_gemv(cur_mode(), layout, m, n, alpha, dA, lda, dx, incx, beta, dy, incy, s))) return rc;
+  if ((rc = dev_gemv(vec_mode(), layout, m, n, alpha, dA, lda, dx, incx, beta, dy, incy, s))) return rc;
   QB_CHECK(hipMemcpyAsync(y, dy, ybytes, hipMemcpyDeviceToHost, s));
   QB_CHECK(hipStreamSynchronize(s));
   for (int i = 0; i < 3; ++i) ws_release(t_stage[i], s);
@@ -450,7 +462,7 @@ int host_gemm(int layout, int64_t m, int64_t n, int64_t k, q128 alpha, const voi
   if ((rc = stage_in(0, A, abytes, s, &dA))) return rc;
   if ((rc = stage_in(1, B, bbytes, s, &dB))) return rc;
   if ((rc = stage_in(2, C, cbytes, s, &dC))) return rc;
-  if ((rc = dev_gemm(cur_mode(), layout, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc, s))) return rc;
+  if ((rc = dev_gemm(mat_mode(), layout, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc, s))) return rc;
   QB_CHECK(hipMemcpyAsync(C, dC, cbytes, hipMemcpyDeviceToHost, s));
   QB_CHECK(hipStreamSynchronize(s));
   for (int i = 0; i < 3; ++i) ws_release(t_stage[i], s);
@@ -626,7 +638,7 @@ int host_nrm2(int64_t n, const void *x, int64_t incx, q128 *out) {
   void *dx, *dr;
   if ((rc = stage_in(0, x, span_bytes(n, incx), s, &dx))) return rc;
   if ((rc = stage_in(2, nullptr, 0, s, &dr))) return rc;
-  if ((rc = dev_nrm2(cur_mode(), n, dx, incx, cur_threads(), dr, s))) return rc;
+  if ((rc = dev_nrm2(vec_mode(), n, dx, incx, cur_threads(), dr, s))) return rc;
   q128 r;
   QB_CHECK(hipMemcpyAsync(&r, dr, 16, hipMemcpyDeviceToHost, s));
   QB_CHECK(hipStreamSynchronize(s));

@@ -254,3 +254,25 @@ def test_gemm_dd_within_bound(O, Q, cuda, layout, m, n, k, kind):
     for s in range(ii.shape[0]):
         g = got[ii[s] * cs[0] + jj[s] * cs[1]]
         assert _err_ok(pair(g), pair(want[s]), to_fraction(*pair(scale[s]))), (ii[s], jj[s])
+
+
+def test_global_mode_maps_per_routine(O, Q, cuda):
+    # host-pointer entry points under one global mode: OZAKI has no dot, which then takes the EXACT
+    # form; EXACT has no gemm, which then takes the OZAKI form (<= 2^-104 per entry)
+    old = Q.get_mode()
+    try:
+        Q.set_mode("ozaki")
+        x, y = _vecs(O, "cancel", 3000, 91)
+        want, _ = O.exact_dot(x, y)
+        assert pair(Q.dot(3000, x, 1, y, 1)) == want
+        Q.set_mode("exact")
+        m = n = k = 80
+        A, B, C = O.fill(1, 92, m * k, -20, 20), O.fill(1, 93, k * n, -20, 20), O.fill(0, 94, m * n)
+        got = C.copy()
+        Q.gemm(0, m, n, k, 1.0, A, k, B, n, 0.5, got, n)
+        ii, jj = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
+        _, rel = O.exact_gemm_samples(ii.ravel(), jj.ravel(), k, 1.0, A, (k, 1), B, (n, 1), 0.5, C, (n, 1),
+                                      cand=got)
+        assert rel.max() <= 2.0 ** -104
+    finally:
+        Q.set_mode(old)
