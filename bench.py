@@ -172,22 +172,24 @@ def bench_memory(Q, torch, dev, stream, n, m, steps):
         for lay, name in ((0, "qgemv_rowmajor"), (1, "qgemv_colmajor")):
             kernels.append((name, (lambda L: lambda: Q.dev_gemv(mode, L, m, m, 1.5, A.data_ptr(), m, xv.data_ptr(), 1,
                                                                  -0.5, yv.data_ptr(), 1, sh))(lay), gemv_bytes))
-        for _name, f, _b in kernels:   # warm-up
+        for _name, f, _b in kernels:   # warm-up: ~100 calls (tens of ms) so clocks settle under load
             yv.copy_(yv0)
-            f()
+            for _ in range(100):
+                f()
         torch.cuda.synchronize()
         res_f = {}
         for name, f, b in kernels:
-            times = []
-            for _ in range(steps):
-                yv.copy_(yv0)
-                s_ev, e_ev = events(torch)
-                s_ev.record(stream)
-                f()
-                e_ev.record(stream)
-                times.append((s_ev, e_ev))
+            # back-to-back calls between two events (y keeps being updated: beta = -0.5 keeps it
+            # bounded), so per-call launch gaps are amortised as in a stream of calls
+            yv.copy_(yv0)
             torch.cuda.synchronize()
-            ms = statistics.mean(s.elapsed_time(e) for s, e in times)
+            s_ev, e_ev = events(torch)
+            s_ev.record(stream)
+            for _ in range(steps):
+                f()
+            e_ev.record(stream)
+            torch.cuda.synchronize()
+            ms = s_ev.elapsed_time(e_ev) / steps
             gbs = b / (ms * 1e-3) / 1e9
             res_f[name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
                            "algorithmic_bytes": b}
@@ -416,7 +418,7 @@ def main():
     torch.cuda.empty_cache()
     memory = None
     if not args.no_memory:
-        memory = bench_memory(Q, torch, dev, stream, args.n_dot, args.m, max(args.steps, 3))
+        memory = bench_memory(Q, torch, dev, stream, args.n_dot, args.m, max(args.steps, 50))
     ill = None
     if not args.no_ill:
         ill = bench_ill_conditioned(Q, torch, dev, stream, 8192, 8192, 16384, 26)

@@ -15,6 +15,7 @@
 #include <string.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <map>
@@ -279,8 +280,9 @@ int dev_gemv(int mode, int layout, int64_t m, int64_t n, q128 alpha, const void 
 
 // Slice count of the OZAKI form: the value set by qblas_set_ozaki_slices, else chosen from the
 // data: a probe computes 256 sampled entries exactly and measures how far below 2^(E_i + F_j)
-// they lie; the slice count is the smallest whose certification bound passes all of them (so
-// that few entries need the exact repair).  The probe synchronizes with the stream.
+// they lie; the slice count is the smallest whose certification bound passes all but the two
+// lowest of them (so that about 1% or fewer of the entries are left to the exact repair).  The
+// probe synchronizes with the stream.
 thread_local Workspace t_pws;
 int oz_slices(int colmajor, int64_t m, int64_t n, int64_t k, const void *A, int64_t lda, const void *B, int64_t ldb,
               hipStream_t s, int *out) {
@@ -292,12 +294,15 @@ int oz_slices(int colmajor, int64_t m, int64_t n, int64_t k, const void *A, int6
   void *ws;
   int rc = ws_acquire(t_pws, qbk::ozaki_probe_ws(m, n), s, &ws);
   if (rc) return rc;
-  QB_CHECK(qbk::launch_ozaki_probe(colmajor, m, n, k, (const q128 *)A, lda, (const q128 *)B, ldb, 256, ws, s));
-  int r = 0;
-  QB_CHECK(hipMemcpyAsync(&r, (const char *)ws + 8, 4, hipMemcpyDeviceToHost, s));
+  constexpr int P = 256;
+  QB_CHECK(qbk::launch_ozaki_probe(colmajor, m, n, k, (const q128 *)A, lda, (const q128 *)B, ldb, P, ws, s));
+  int r[P];
+  QB_CHECK(hipMemcpyAsync(r, (const char *)ws + qbk::ozaki_probe_out_offset(m, n), sizeof(r), hipMemcpyDeviceToHost,
+                          s));
   QB_CHECK(hipStreamSynchronize(s));
   if ((rc = ws_release(t_pws, s))) return rc;
-  *out = qbk::ozaki_slices_for(k, r);
+  std::sort(r, r + P);
+  *out = qbk::ozaki_slices_for(k, r[2]);  // third lowest: at most 2 of 256 samples below the bound
   return QBLAS_OK;
 }
 

@@ -59,9 +59,11 @@ hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int 
                              the slice products, after the finish, after the repair */,
                              int sync_repair /* 1: may synchronize to choose how the repair reads B */);
 hipError_t ozaki_repair_count(const void *ws, uint32_t *out, hipStream_t st);
-// probe: scales of A and B, then the minimum over `samples` exact entries of ilogb|AB_ij| -
-// (E_i + F_j) into ws word 2 (int); ozaki_slices_for turns it into a slice count
+// probe: scales of A and B, then for `samples` (<= 1024) exact entries ilogb|AB_ij| - (E_i + F_j)
+// as ints at ws + ozaki_probe_out_offset (INT_MAX where unusable); ozaki_slices_for turns a chosen
+// order statistic of them into a slice count
 size_t ozaki_probe_ws(int64_t m, int64_t n);
+size_t ozaki_probe_out_offset(int64_t m, int64_t n);
 hipError_t launch_ozaki_probe(int colmajor, int64_t m, int64_t n, int64_t k, const q128 *A, int64_t lda,
                               const q128 *B, int64_t ldb, int samples, void *ws, hipStream_t st);
 int ozaki_slices_for(int64_t k, int r);

@@ -126,34 +126,72 @@ __global__ void k_vec_to_dd(int64_t n, q128 scale, int use_scale, const q128 *__
   st_dd(xs + j, q_to_dd(v));
 }
 
-// RowMajor: one wave per row, lanes stride along the row (coalesced), dd partial per lane,
-// butterfly wave sum, then y_i = alpha*sum + beta*y_i in binary128.
+// RowMajor: R rows per wave (R = 2, and 1 for the last row of an odd m), lanes stride along the
+// rows (coalesced), GV_U column steps of 64 per iteration so that 2 * GV_U loads of A are in
+// flight per lane and each x element read serves both rows; dd partials per lane, butterfly wave
+// sum, then y_i = alpha*sum + beta*y_i in binary128.
 constexpr int GV_BS = 256;
+constexpr int GV_U = 4;
+template <int R>
+__device__ __forceinline__ void gemv_rows_dd(int64_t row0, int64_t n, q128 alpha, const q128 *__restrict__ A,
+                                             int64_t lda, const dd *__restrict__ xs, q128 beta, q128 *__restrict__ y,
+                                             int64_t incy, int lane) {
+  dd acc[R][GV_U];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int u = 0; u < GV_U; ++u) acc[r][u] = dd_zero();
+  const q128 *rp = A + row0 * lda;
+  int64_t j = lane;
+  for (; j + 64 * (GV_U - 1) < n; j += 64 * GV_U) {
+    dd xv[GV_U];
+    q128 a[R][GV_U];
+#pragma unroll
+    for (int u = 0; u < GV_U; ++u) xv[u] = ld_dd(xs + j + 64 * u);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int u = 0; u < GV_U; ++u) a[r][u] = ldq(rp + r * lda + j + 64 * u);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int u = 0; u < GV_U; ++u) acc[r][u] = dd_fma(q_to_dd(a[r][u]), xv[u], acc[r][u]);
+  }
+  for (; j < n; j += 64) {
+    const dd xv = ld_dd(xs + j);
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r][0] = dd_fma(q_to_dd(ldq(rp + r * lda + j)), xv, acc[r][0]);
+  }
+  // every lane holds every row's sum after the butterflies; lane r finishes row r, so the R
+  // binary128 epilogues run side by side
+  dd mine = dd_zero();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const dd s = wave_sum_dd(dd_add(dd_add(acc[r][0], acc[r][1]), dd_add(acc[r][2], acc[r][3])));
+    if (lane == r) mine = s;
+  }
+  if (lane < R) {
+    q128 *yi = y + (row0 + lane) * incy;
+    stq(yi, fma_packed(alpha, dd_to_q(mine), mul_packed(beta, ldq_cached(yi))));
+  }
+}
+static_assert(GV_U == 4, "the per-row combine above sums four accumulators");
 __global__ void __launch_bounds__(GV_BS) k_gemv_row_dd(int64_t m, int64_t n, q128 alpha, const q128 *__restrict__ A,
                                                        int64_t lda, const dd *__restrict__ xs, q128 beta,
                                                        q128 *__restrict__ y, int64_t incy) {
-  const int64_t row = (int64_t)blockIdx.x * (GV_BS / 64) + (threadIdx.x >> 6);
+  const int64_t row0 = ((int64_t)blockIdx.x * (GV_BS / 64) + (threadIdx.x >> 6)) * 2;
   const int lane = threadIdx.x & 63;
-  if (row >= m) return;  // whole waves exit together
-  const q128 *r = A + row * lda;
-  dd acc0 = dd_zero(), acc1 = dd_zero();
-  int64_t j = lane;
-  for (; j + 64 < n; j += 128) {
-    const q128 a0 = ldq(r + j), a1 = ldq(r + j + 64);
-    const dd x0 = ld_dd(xs + j), x1 = ld_dd(xs + j + 64);
-    acc0 = dd_fma(q_to_dd(a0), x0, acc0);
-    acc1 = dd_fma(q_to_dd(a1), x1, acc1);
-  }
-  if (j < n) acc0 = dd_fma(q_to_dd(ldq(r + j)), ld_dd(xs + j), acc0);
-  const dd s = wave_sum_dd(dd_add(acc0, acc1));
-  if (lane == 0) {
-    q128 *yi = y + row * incy;
-    stq(yi, fma_packed(alpha, dd_to_q(s), mul_packed(beta, ldq_cached(yi))));
-  }
+  if (row0 >= m) return;  // whole waves exit together
+  if (row0 + 1 < m)
+    gemv_rows_dd<2>(row0, n, alpha, A, lda, xs, beta, y, incy, lane);
+  else
+    gemv_rows_dd<1>(row0, n, alpha, A, lda, xs, beta, y, incy, lane);
 }
 
-// ColMajor: lane = row, blockIdx.y = column chunk; partial dd sums per (chunk, row).
+// ColMajor: lane = row, blockIdx.y = column chunk; GC_U columns per iteration (GC_U loads of A in
+// flight per lane); partial dd sums per (chunk, row).
 constexpr int GC_BS = 256;
+constexpr int GC_U = 8;
 __global__ void __launch_bounds__(GC_BS) k_gemv_col_dd_partial(int64_t m, int64_t n, int64_t chunk,
                                                                const q128 *__restrict__ A, int64_t lda,
                                                                const dd *__restrict__ axs, dd *__restrict__ part) {
@@ -161,22 +199,38 @@ __global__ void __launch_bounds__(GC_BS) k_gemv_col_dd_partial(int64_t m, int64_
   const int64_t j0 = (int64_t)blockIdx.y * chunk;
   const int64_t j1 = (j0 + chunk < n) ? j0 + chunk : n;
   if (i >= m) return;
-  dd acc0 = dd_zero(), acc1 = dd_zero();
+  dd acc[GC_U];
+#pragma unroll
+  for (int u = 0; u < GC_U; ++u) acc[u] = dd_zero();
   int64_t j = j0;
-  for (; j + 1 < j1; j += 2) {
-    const q128 a0 = ldq(A + j * lda + i), a1 = ldq(A + (j + 1) * lda + i);
-    acc0 = dd_fma(q_to_dd(a0), ld_dd(axs + j), acc0);
-    acc1 = dd_fma(q_to_dd(a1), ld_dd(axs + j + 1), acc1);
+  for (; j + GC_U - 1 < j1; j += GC_U) {
+    q128 a[GC_U];
+#pragma unroll
+    for (int u = 0; u < GC_U; ++u) a[u] = ldq(A + (j + u) * lda + i);
+#pragma unroll
+    for (int u = 0; u < GC_U; ++u) acc[u] = dd_fma(q_to_dd(a[u]), ld_dd(axs + j + u), acc[u]);
   }
-  if (j < j1) acc0 = dd_fma(q_to_dd(ldq(A + j * lda + i)), ld_dd(axs + j), acc0);
-  st_dd(part + blockIdx.y * m + i, dd_add(acc0, acc1));
+  for (; j < j1; ++j) acc[0] = dd_fma(q_to_dd(ldq(A + j * lda + i)), ld_dd(axs + j), acc[0]);
+  dd s = acc[0];
+#pragma unroll
+  for (int u = 1; u < GC_U; ++u) s = dd_add(s, acc[u]);
+  st_dd(part + blockIdx.y * m + i, s);
 }
+// y_i = RN(beta * y_i) + RN(sum of the chunk partials), the partials read 8 at a time
 __global__ void k_gemv_col_dd_final(int64_t m, int nchunks, const dd *__restrict__ part, q128 beta,
                                     q128 *__restrict__ y, int64_t incy) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
   dd s = dd_zero();
-  for (int c = 0; c < nchunks; ++c) s = dd_add(s, ld_dd(part + (int64_t)c * m + i));
+  int c = 0;
+  for (; c + 7 < nchunks; c += 8) {
+    dd t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t[u] = ld_dd(part + (int64_t)(c + u) * m + i);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s = dd_add(s, t[u]);
+  }
+  for (; c < nchunks; ++c) s = dd_add(s, ld_dd(part + (int64_t)c * m + i));
   q128 *yi = y + i * incy;
   stq(yi, add_packed(mul_packed(beta, ldq_cached(yi)), dd_to_q(s)));
 }
@@ -275,10 +329,11 @@ __global__ void __launch_bounds__(256) k_gemm_dd(int64_t M, int64_t N, int64_t K
 // ------------------------------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------------------------------
+// at most 3 blocks per CU: fewer, longer-running blocks stream faster than a deeper grid here
 int dot_dd_blocks(int64_t n) {
   int64_t b = (n + DOT_BS * 8 - 1) / (DOT_BS * 8);
   if (b < 1) b = 1;
-  if (b > 2048) b = 2048;
+  if (b > 768) b = 768;
   return (int)b;
 }
 size_t dot_dd_ws(int64_t n) { return (size_t)dot_dd_blocks(n) * sizeof(dd); }
@@ -294,7 +349,7 @@ hipError_t launch_dot_dd(const q128 *x, int64_t incx, const q128 *y, int64_t inc
 
 int gemv_col_chunks(int64_t m, int64_t n) {
   const int64_t rowblocks = (m + GC_BS - 1) / GC_BS;
-  int64_t c = (2048 + rowblocks - 1) / rowblocks;  // aim for >= 2048 blocks
+  int64_t c = (512 + rowblocks - 1) / rowblocks;  // aim for >= 512 blocks (2 per CU)
   if (c > n) c = n;
   if (c < 1) c = 1;
   if (c > 1024) c = 1024;
@@ -313,8 +368,9 @@ hipError_t launch_gemv_dd(int colmajor, int64_t m, int64_t n, q128 alpha, const 
   hipLaunchKernelGGL(k_vec_to_dd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, alpha, colmajor, x, incx,
                      xs);
   if (!colmajor) {
-    hipLaunchKernelGGL(k_gemv_row_dd, dim3((unsigned)((m + GV_BS / 64 - 1) / (GV_BS / 64))), dim3(GV_BS), 0, st, m,
-                       n, alpha, A, lda, xs, beta, y, incy);
+    const int64_t waves = (m + 1) / 2;  // two rows per wave
+    hipLaunchKernelGGL(k_gemv_row_dd, dim3((unsigned)((waves + GV_BS / 64 - 1) / (GV_BS / 64))), dim3(GV_BS), 0, st,
+                       m, n, alpha, A, lda, xs, beta, y, incy);
   } else {
     const int nch = gemv_col_chunks(m, n);
     const int64_t chunk = (n + nch - 1) / nch;

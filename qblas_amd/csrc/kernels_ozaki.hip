@@ -49,6 +49,7 @@ constexpr int32_t OZ_EZERO = INT32_MIN; // row/column exponent of an all-zero ro
 constexpr uint32_t OZF_NONFINITE = 1u;
 constexpr int OZ_REPAIR_BLOCKS = 1024;
 constexpr int OZ_HDR = 256;             // workspace header bytes: status, repair count
+constexpr int OZ_PROBE_MAX = 1024;      // sampled entries of the slice-count probe
 
 // diagonal ranges: up to 19 slices one range [0, S-1] with 2 slice groups per k block; 20..28
 // slices two ranges [0, D-1] and [D, S-1], D = (3S+4)/5 (balancing accumulators against
@@ -107,7 +108,7 @@ __global__ void k_oz_init(int32_t *eA, uint32_t *fA, int64_t Mp, int32_t *eB, ui
     eB[g] = OZ_EZERO;
     fB[g] = 0;
   }
-  if (g < OZ_HDR / 4) hdr[g] = (g == 2) ? 0x7fffffffu : 0u;  // word 2: probe minimum
+  if (g < OZ_HDR / 4) hdr[g] = 0u;
 }
 
 __device__ __forceinline__ int oz_ilogb(q128 x, uint32_t &fl) {
@@ -319,7 +320,18 @@ k_oz_gemm(const int8_t *__restrict__ As, const int8_t *__restrict__ Bs, int64_t 
   constexpr int TMAX = (D1 < S - 1) ? D1 : S - 1;  // highest slice any pair of the range uses
   __shared__ __attribute__((aligned(1024))) int8_t lds[4 * SLOT];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t tn = blockIdx.x, tm = blockIdx.y, kp = blockIdx.z;
+  int64_t tn = blockIdx.x, tm = blockIdx.y;
+  const int64_t kp = blockIdx.z;
+  if ((gridDim.x % 16) == 0 && (gridDim.y % 16) == 0) {
+    // Rounds of 256 consecutive workgroups cover a 16x16 super-tile (row-major over super-tiles),
+    // and the 32 of a round that share an XCD (workgroup id mod 8) a 4x8 block of it: a round's
+    // slice panels stay within the infinity cache and each XCD's L2 serves 4 + 8 of them.
+    const int64_t L = blockIdx.x + (int64_t)gridDim.x * blockIdx.y;
+    const int64_t r = L >> 8, w = L & 255, xcd = w & 7, idx = w >> 3;
+    const int64_t SC = gridDim.x >> 4;
+    tm = (r / SC) * 16 + (xcd >> 1) * 4 + (idx >> 3);
+    tn = (r % SC) * 16 + (xcd & 1) * 8 + (idx & 7);
+  }
   const int64_t kb0 = kp * OZ_KPASS_BLOCKS;
   const int64_t nkb = ((kb0 + OZ_KPASS_BLOCKS < Kb) ? kb0 + OZ_KPASS_BLOCKS : Kb) - kb0;
   const int64_t nst = NG * nkb;
@@ -619,9 +631,9 @@ __global__ void __launch_bounds__(OZR_BS) k_oz_repair(int64_t K, const q128 *__r
 // 6. slice-count probe
 // ------------------------------------------------------------------------------------------
 // For P pseudo-random entries (i, j): the exact sum_l A_il B_lj, and r = ilogb(|sum|) -
-// (E_i + F_j) (how far the entry lies below the scale of its row and column).  The minimum over
-// the sampled entries goes to out[0] (atomic min; entries that are zero, non-finite or outside
-// the windows are skipped).  This drives the choice of the slice count.
+// (E_i + F_j) (how far the entry lies below the scale of its row and column), into out[e]
+// (INT_MAX for entries that are zero, non-finite, outside the windows or in a zero row/column).
+// This drives the choice of the slice count.
 __global__ void __launch_bounds__(OZR_BS) k_oz_probe(int64_t M, int64_t N, int64_t K, const q128 *__restrict__ A,
                                                      int64_t asr, int64_t asc, const q128 *__restrict__ B,
                                                      int64_t bsr, int64_t bsc, const int32_t *__restrict__ eA,
@@ -635,16 +647,23 @@ __global__ void __launch_bounds__(OZR_BS) k_oz_probe(int64_t M, int64_t N, int64
     z ^= z >> 31;
     const int64_t i = (int64_t)((z & 0xffffffffull) % (uint64_t)M), j = (int64_t)((z >> 32) % (uint64_t)N);
     const int ei = eA[i], ej = eB[j];
-    if (ei == OZ_EZERO || ej == OZ_EZERO) continue;  // uniform per block
+    if (ei == OZ_EZERO || ej == OZ_EZERO) {  // uniform per block
+      if (threadIdx.x == 0) out[e] = 0x7fffffff;
+      continue;
+    }
     const int An = oz_anchor(ei, ej);
     uint32_t Wl[WW], Wh[WW], flags;
     oz_block_dot(K, A + i * asr, asc, B + j * bsc, bsr, An, Wl, Wh, flags, s_w);
-    if (threadIdx.x == 0 && !(flags & (FL_LOST | FL_NAN | FL_PINF | FL_NINF))) {
-      uint32_t Mw[SW];
-      uint32_t sign;
-      s_from_windows(Mw, sign, Wl, Wh);
-      const int msb = rmsb<SW>(Mw);
-      if (msb >= 0) atomicMin(out, An + msb - (ei + 3) - (ej + 3));
+    if (threadIdx.x == 0) {
+      int r = 0x7fffffff;
+      if (!(flags & (FL_LOST | FL_NAN | FL_PINF | FL_NINF))) {
+        uint32_t Mw[SW];
+        uint32_t sign;
+        s_from_windows(Mw, sign, Wl, Wh);
+        const int msb = rmsb<SW>(Mw);
+        if (msb >= 0) r = An + msb - (ei + 3) - (ej + 3);
+      }
+      out[e] = r;
     }
     __syncthreads();
   }
@@ -797,6 +816,9 @@ hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int 
 }
 
 size_t ozaki_probe_ws(int64_t m, int64_t n) {
+  return (size_t)OZ_HDR + 256 + (size_t)(m + 64) * 8 + (size_t)(n + 64) * 8 + 1024 + OZ_PROBE_MAX * 4;
+}
+size_t ozaki_probe_out_offset(int64_t m, int64_t n) {
   return (size_t)OZ_HDR + 256 + (size_t)(m + 64) * 8 + (size_t)(n + 64) * 8 + 1024;
 }
 
@@ -815,15 +837,16 @@ hipError_t launch_ozaki_probe(int colmajor, int64_t m, int64_t n, int64_t k, con
   k_oz_init<<<(unsigned)((ninit + 255) / 256), 256, 0, st>>>(eA, fA, m, eB, fB, n, hdr);
   oz_scale(A, m, k, asr, asc, eA, fA, st);
   oz_scale(B, n, k, bsc, bsr, eB, fB, st);
-  k_oz_probe<<<(unsigned)(samples < 256 ? samples : 256), OZR_BS, 0, st>>>(m, n, k, A, asr, asc, B, bsr, bsc, eA, eB,
-                                                                         samples, (int *)(hdr + 2));
+  if (samples > OZ_PROBE_MAX) samples = OZ_PROBE_MAX;
+  k_oz_probe<<<(unsigned)(samples < 256 ? samples : 256), OZR_BS, 0, st>>>(
+      m, n, k, A, asr, asc, B, bsr, bsc, eA, eB, samples, (int *)(ws + ozaki_probe_out_offset(m, n)));
   return hipGetLastError();
 }
 
-// Slice count for a probe minimum r (the sampled entries lie >= 2^r below 2^(E_i+F_j)): the
-// smallest S with 1 + kfac(S) - 8S + 2 <= r - 106, i.e. the certification bound (alpha = 1)
-// passes every sampled entry with two bits to spare; clamped to [12, OZ_SMAX].  r = INT_MAX (no
-// usable sample) gives 17.
+// Slice count for a probe value r (sampled entries at or above it lie >= 2^r below
+// 2^(E_i+F_j)): the smallest S with 1 + kfac(S) - 8S + 2 <= r - 106, i.e. the certification bound
+// (alpha = 1) passes those entries with two bits to spare; clamped to [12, OZ_SMAX].
+// r = INT_MAX (no usable sample) gives 17.
 int ozaki_slices_for(int64_t k, int r) {
   if (r == 0x7fffffff) return 17;
   for (int S = 12; S <= OZ_SMAX; ++S) {

@@ -207,7 +207,8 @@ def test_dot_dd_within_bound(O, Q, cuda, n, kind):
     assert _err_ok(got, want, to_fraction(*scale))
 
 
-@pytest.mark.parametrize("layout,m,n", [(0, 64, 1000), (0, 1000, 64), (1, 64, 1000), (1, 1000, 64), (0, 8192, 8192),
+@pytest.mark.parametrize("layout,m,n", [(0, 64, 1000), (0, 1000, 64), (1, 64, 1000), (1, 1000, 64), (0, 999, 130),
+                                        (1, 999, 130), (0, 1, 700), (0, 3, 257), (1, 3, 257), (0, 8192, 8192),
                                         (1, 8192, 8192)])
 def test_gemv_dd_within_bound(O, Q, cuda, layout, m, n):
     _say(f"gemv dd layout={layout} m={m} n={n}")
