@@ -64,6 +64,9 @@ def parse():
     p.add_argument("--cpu-rows", type=int, default=256, help="rows of C in the CPU qgemm slab")
     p.add_argument("--copy-calibration", action="store_true",
                    help="first run one device copy of 1 GiB (a known-size reference for HBM counters)")
+    p.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                   help="write the per-kernel HBM traffic of two rocprofv3 counter runs (FETCH_SIZE, "
+                        "WRITE_SIZE; --copy-calibration --steps 1 --warmup 0) as JSON to stdout, then exit")
     p.add_argument("--n-dot", type=int, default=1 << 24)
     p.add_argument("--m", type=int, default=8192)
     return p.parse_args()
@@ -325,6 +328,55 @@ def bench_ill_conditioned(Q, torch, dev, stream, M, N, K, slices):
     return out
 
 
+def summarize_pmc(fetch_csv, write_csv):
+    """HBM bytes per launch of the OZAKI kernels from two rocprofv3 counter-collection CSVs.
+    Each counter is scaled by what it reads for the 1 GiB calibration copy (`__amd_rocclr_copyBuffer`
+    reads and writes exactly 2^30 bytes).  Kernels are keyed by name in launch order; a name seen more
+    than once gets #2, #3 ... (the slice-count probe runs the scale kernels before the call's own)."""
+    import csv
+
+    def read(path, counter):
+        rows = []
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] == counter:
+                    rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])))
+        rows.sort()
+        return rows
+
+    def short(name):
+        base = name.split("(")[0].replace("void ", "").replace("qbk::", "").strip()
+        return base
+
+    fetch, write = read(fetch_csv, "FETCH_SIZE"), read(write_csv, "WRITE_SIZE")
+    copy_f = [v for _, n, v in fetch if "copyBuffer" in n]
+    copy_w = [v for _, n, v in write if "copyBuffer" in n]
+    if not copy_f or not copy_w:
+        raise SystemExit("no calibration copy in the counter files (run with --copy-calibration)")
+    gib = float(1 << 30)
+    fs, ws = gib / (copy_f[0] * 1024), gib / (copy_w[0] * 1024)
+    out = {"calibration": {"copy_bytes": 1 << 30, "copy_fetch_kib": copy_f[0], "copy_write_kib": copy_w[0],
+                           "fetch_scale": round(fs, 4), "write_scale": round(ws, 4)},
+           "note": "HBM bytes per launch = counter KiB * 1024 * scale"}
+    wmap = {d: v for d, _, v in write}
+    seen = {}
+    for d, name, fv in fetch:
+        if "k_oz_" not in name:
+            continue
+        key = short(name)
+        seen[key] = seen.get(key, 0) + 1
+        if seen[key] > 1:
+            key = f"{key}#{seen[key]}"
+        wv = wmap.get(d, 0.0)
+        rb, wb = int(fv * 1024 * fs), int(wv * 1024 * ws)
+        out[key] = {"fetch_kib": fv, "write_kib": wv, "read_bytes": rb, "write_bytes": wb,
+                    "hbm_bytes_per_launch": rb + wb}
+    gemm = [k for k in out if k.startswith("k_oz_gemm")]
+    if gemm:
+        out["k_oz_gemm"] = dict(out[gemm[-1]], kernel=gemm[-1])
+    return out
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "ozaki_traffic.json")
     try:
@@ -377,6 +429,9 @@ def cpu_baseline_memory(n, m):
 
 def main():
     args = parse()
+    if args.summarize_pmc:
+        print(json.dumps(summarize_pmc(*args.summarize_pmc), indent=1))
+        sys.exit(0)
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1", "n_gpus": args.gpus}))
         sys.exit(2)

@@ -106,7 +106,10 @@ int cur_mode() {
 }
 
 // ---------------------------------------------------------------------------------------------
-// device workspaces: grow-only per host thread; cross-stream reuse is ordered with an event
+// device workspaces: grow-only per host thread; cross-stream reuse is ordered with an event.  The
+// event only orders work on the workspace's own device (a stream wait, or a host wait before
+// hipFree), so it is recorded without the system-scope fence, which costs a cache writeback per
+// call.
 // ---------------------------------------------------------------------------------------------
 struct Workspace {
   void *p = nullptr;
@@ -127,7 +130,7 @@ int ws_acquire(Workspace &w, size_t bytes, hipStream_t s, void **out) {
     QB_CHECK(hipEventDestroy(w.ev));
     w = Workspace();
   }
-  if (!w.ev) QB_CHECK(hipEventCreateWithFlags(&w.ev, hipEventDisableTiming));
+  if (!w.ev) QB_CHECK(hipEventCreateWithFlags(&w.ev, hipEventDisableTiming | hipEventDisableSystemFence));
   if (bytes > w.cap) {
     if (w.p) {
       if (w.recorded) QB_CHECK(hipEventSynchronize(w.ev));

@@ -176,16 +176,20 @@ __device__ __forceinline__ void gemv_rows_dd(int64_t row0, int64_t n, q128 alpha
   }
 }
 static_assert(GV_U == 4, "the per-row combine above sums four accumulators");
+// Waves take row pairs grid-stride (a grid of 3 blocks per CU streams best: the row pairs of
+// one wave then follow each other, and the waves of a round end at different times).
+constexpr int GV_BLOCKS = 768;
 __global__ void __launch_bounds__(GV_BS) k_gemv_row_dd(int64_t m, int64_t n, q128 alpha, const q128 *__restrict__ A,
                                                        int64_t lda, const dd *__restrict__ xs, q128 beta,
                                                        q128 *__restrict__ y, int64_t incy) {
-  const int64_t row0 = ((int64_t)blockIdx.x * (GV_BS / 64) + (threadIdx.x >> 6)) * 2;
+  const int64_t W = (int64_t)gridDim.x * (GV_BS / 64);
   const int lane = threadIdx.x & 63;
-  if (row0 >= m) return;  // whole waves exit together
-  if (row0 + 1 < m)
-    gemv_rows_dd<2>(row0, n, alpha, A, lda, xs, beta, y, incy, lane);
-  else
-    gemv_rows_dd<1>(row0, n, alpha, A, lda, xs, beta, y, incy, lane);
+  for (int64_t row0 = ((int64_t)blockIdx.x * (GV_BS / 64) + (threadIdx.x >> 6)) * 2; row0 < m; row0 += 2 * W) {
+    if (row0 + 1 < m)  // uniform per wave
+      gemv_rows_dd<2>(row0, n, alpha, A, lda, xs, beta, y, incy, lane);
+    else
+      gemv_rows_dd<1>(row0, n, alpha, A, lda, xs, beta, y, incy, lane);
+  }
 }
 
 // ColMajor: lane = row, blockIdx.y = column chunk; GC_U columns per iteration (GC_U loads of A in
@@ -368,9 +372,11 @@ hipError_t launch_gemv_dd(int colmajor, int64_t m, int64_t n, q128 alpha, const 
   hipLaunchKernelGGL(k_vec_to_dd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, alpha, colmajor, x, incx,
                      xs);
   if (!colmajor) {
-    const int64_t waves = (m + 1) / 2;  // two rows per wave
-    hipLaunchKernelGGL(k_gemv_row_dd, dim3((unsigned)((waves + GV_BS / 64 - 1) / (GV_BS / 64))), dim3(GV_BS), 0, st,
-                       m, n, alpha, A, lda, xs, beta, y, incy);
+    const int64_t pairs = (m + 1) / 2;  // two rows per wave and step
+    int64_t blocks = (pairs + GV_BS / 64 - 1) / (GV_BS / 64);
+    if (blocks > GV_BLOCKS) blocks = GV_BLOCKS;
+    hipLaunchKernelGGL(k_gemv_row_dd, dim3((unsigned)blocks), dim3(GV_BS), 0, st, m, n, alpha, A, lda, xs, beta, y,
+                       incy);
   } else {
     const int nch = gemv_col_chunks(m, n);
     const int64_t chunk = (n + nch - 1) / nch;

@@ -104,6 +104,31 @@ def test_dot_exact_full_size_and_properties(O, Q, cuda):
                         np.ascontiguousarray(y[perm]), 1) == got
 
 
+def test_workspace_reuse_across_streams(O, Q, cuda):
+    # every device routine of a host thread shares one workspace: a DD dot on one stream and a DD
+    # ColMajor gemv issued right after on another both use it, and the library orders the second
+    # after the first, so each result equals its single-stream value bit for bit
+    n, m, k = 1 << 22, 2048, 2048
+    x, y = _vecs(O, "full", n, 61)
+    A, v, w0 = O.fill(0, 62, m * k), O.fill(0, 63, k), O.fill(0, 64, m)
+    dx, dy, dA, dv = to_dev(x), to_dev(y), to_dev(A), to_dev(v)
+
+    def run(s1, s2):
+        dr, dw = dev_empty(1), to_dev(w0)
+        cuda.cuda.synchronize()
+        Q.dev_dot(Q.MODE_DD, n, dx.data_ptr(), 1, dy.data_ptr(), 1, 1, dr.data_ptr(), s1)
+        Q.dev_gemv(Q.MODE_DD, 1, m, k, 1.5, dA.data_ptr(), m, dv.data_ptr(), 1, -0.5, dw.data_ptr(), 1, s2)
+        cuda.cuda.synchronize()
+        return from_dev(dr), from_dev(dw)
+
+    ref_r, ref_w = run(0, 0)              # also grows the workspace to its final size
+    sa, sb = cuda.cuda.Stream(), cuda.cuda.Stream()
+    for s1, s2 in ((sa, sb), (sb, sa), (sa, sb)):
+        _say("dd dot then dd gemv on two streams")
+        r, w = run(s1.cuda_stream, s2.cuda_stream)
+        assert np.array_equal(r, ref_r) and np.array_equal(w, ref_w)
+
+
 GEMV_CASES = [(0, 1, 1, 1.0, 0.0), (0, 7, 300, 1.5, -0.5), (0, 64, 1000, "q", "q"), (0, 300, 33, 1.0, 1.0),
               (1, 1, 1, 1.0, 0.0), (1, 7, 300, 1.5, -0.5), (1, 513, 1000, "q", "q"), (1, 300, 33, 2.0, 0.0)]
 
