@@ -2,9 +2,12 @@
 
 Headline (`value`): qgemm M = N = K = 4096, RowMajor, alpha = 1, beta = 0, entries with random sign,
 magnitude log-uniform over [2^-30, 2^30) and a full 113-bit mantissa (seeded device generator),
-in the fastest form that meets 2^-104 relative error: OZAKI (int8 slices on the matrix cores with
-exact int32 accumulation and exact recombination; every entry is certified by an a-priori bound to
-<= 2^-105.9 relative error or recomputed exactly).  value = 2 N^3 / (ms per call), quad-equivalent
+in the fastest form that meets 2^-104 relative error: OZAKI (operands scaled per row / column and
+rounded to integers, integer products on the int8 matrix cores with exact int32 accumulation,
+exact recombination; every entry is certified by an a-priori bound to <= 2^-105.9 relative error
+or recomputed exactly), with the library's automatic method: for this workload the CRT method
+(residues modulo ~40 moduli <= 256, one int8 GEMM per modulus) rather than the slice method
+(S(S+1)/2 slice-pair products; reported as qgemm_forms.ozaki_slices).  value = 2 N^3 / (ms per call), quad-equivalent
 GFLOP/s (the reference's own GFLOPS formula, benchmarks/benchmark.cpp:198-202).  Inputs are resident
 in HBM before the timed region; one step = one qgemm call.
 
@@ -15,9 +18,10 @@ Also reported:
   memory_bound    qdot n = 2^24 and qgemv 8192 x 8192 RowMajor / ColMajor (alpha = 1.5,
                   beta = -0.5, entries a + b 2^-60) in the DD and EXACT forms: GB/s against the
                   8 TB/s HBM peak.  Algorithmic bytes: dot 32 n, gemv 16 (m n + n + 2 m).
-  roofline        the slice-product kernel of the headline call: int8 ops 2 M N K P per launch
-                  (P = S(S+1)/2 slice pairs) over its launch duration, timed with HIP events
-                  recorded by the library on the call's stream, against 5000 TOP/s dense int8.
+  roofline        the product kernel of the headline call: int8 ops 2 M N K P per launch (P = the
+                  number of moduli for the CRT method, S(S+1)/2 slice pairs for the slice method)
+                  over its launch duration, timed with HIP events recorded by the library on the
+                  call's stream, against 5000 TOP/s dense int8.
                   traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE runs
                   (profiles/ozaki_traffic.json) when present.
   batched         4096 qgemm problems (m, n, k uniform in [8, 256]) and 16384 qgemv problems
@@ -54,6 +58,11 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--size", type=int, default=4096, help="qgemm M = N = K")
     p.add_argument("--slices", type=int, default=0, help="OZAKI slice count (0 = library choice)")
+    p.add_argument("--method", choices=("auto", "slices", "crt"), default="auto",
+                   help="OZAKI method of the headline call (auto: the library's choice)")
+    p.add_argument("--crt-bits", type=int, default=0, help="OZAKI CRT bits (0 = library choice)")
+    p.add_argument("--no-slices-form", action="store_true",
+                   help="skip the separate slice-method line when the headline uses the CRT method")
     p.add_argument("--no-plain", action="store_true", help="skip the plain soft-binary128 qgemm")
     p.add_argument("--no-memory", action="store_true", help="skip qdot / qgemv")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
@@ -76,7 +85,7 @@ def events(torch):
     return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
 
-def bench_ozaki(Q, torch, dev, stream, N, steps, warmup, slices):
+def bench_ozaki(Q, torch, dev, stream, N, steps, warmup, slices, method=0, crt_bits=0):
     sh = stream.cuda_stream
     A = torch.empty((N * N, 2), dtype=torch.int64, device=dev)
     B = torch.empty((N * N, 2), dtype=torch.int64, device=dev)
@@ -84,6 +93,8 @@ def bench_ozaki(Q, torch, dev, stream, N, steps, warmup, slices):
     Q.dev_fill(Q.FILL_LOG_UNIFORM, 11, N * N, A.data_ptr(), -30, 30, stream=sh)
     Q.dev_fill(Q.FILL_LOG_UNIFORM, 12, N * N, B.data_ptr(), -30, 30, stream=sh)
     Q.set_ozaki_slices(slices)
+    Q.set_ozaki_method(method)
+    Q.set_ozaki_crt_bits(crt_bits)
 
     def call():
         Q.dev_gemm(Q.MODE_OZAKI, 0, N, N, N, 1.0, A.data_ptr(), N, B.data_ptr(), N, 0.0, C.data_ptr(), N, sh)
@@ -107,12 +118,30 @@ def bench_ozaki(Q, torch, dev, stream, N, steps, warmup, slices):
     wall = time.perf_counter() - t0
     Q.set_phase_timing(False)
     S = Q.ozaki_last_slices()
+    used = Q.ozaki_last_method()
+    bits, moduli = Q.ozaki_last_crt()
     repairs = Q.dev_ozaki_repairs(sh)
     status = Q.dev_exact_status(sh)
+    Q.set_ozaki_slices(0)
+    Q.set_ozaki_method(Q.OZAKI_AUTO)
+    Q.set_ozaki_crt_bits(0)
     ms = statistics.mean(step_ms)
     ph = {k: round(statistics.mean(p[k] for p in phases), 4) for k in phases[0]}
-    return {"ms": ms, "wall_s": wall, "slices": S, "repairs": int(repairs), "status": int(status), "phases_ms": ph,
+    del A, B, C
+    torch.cuda.empty_cache()
+    return {"ms": ms, "wall_s": wall, "method": "crt" if used == Q.OZAKI_CRT else "slices", "slices": S,
+            "crt_bits": bits, "moduli": moduli, "repairs": int(repairs), "status": int(status), "phases_ms": ph,
             "quad_GFLOPs": 2.0 * N ** 3 / (ms * 1e-3) / 1e9}
+
+
+def ozaki_form(oz):
+    f = {"ms": round(oz["ms"], 3), "quad_GFLOPs": round(oz["quad_GFLOPs"], 1), "method": oz["method"]}
+    if oz["method"] == "crt":
+        f.update(crt_bits=oz["crt_bits"], moduli=oz["moduli"])
+    else:
+        f["slices"] = oz["slices"]
+    f.update(repaired_entries=oz["repairs"], phases_ms=oz["phases_ms"])
+    return f
 
 
 def bench_form_once(Q, torch, dev, stream, N, mode):
@@ -361,7 +390,7 @@ def summarize_pmc(fetch_csv, write_csv):
     wmap = {d: v for d, _, v in write}
     seen = {}
     for d, name, fv in fetch:
-        if "k_oz_" not in name:
+        if "k_oz_" not in name and "k_crt_" not in name:
             continue
         key = short(name)
         seen[key] = seen.get(key, 0) + 1
@@ -371,9 +400,10 @@ def summarize_pmc(fetch_csv, write_csv):
         rb, wb = int(fv * 1024 * fs), int(wv * 1024 * ws)
         out[key] = {"fetch_kib": fv, "write_kib": wv, "read_bytes": rb, "write_bytes": wb,
                     "hbm_bytes_per_launch": rb + wb}
-    gemm = [k for k in out if k.startswith("k_oz_gemm")]
-    if gemm:
-        out["k_oz_gemm"] = dict(out[gemm[-1]], kernel=gemm[-1])
+    for prefix in ("k_oz_gemm", "k_crt_gemm"):
+        gemm = [k for k in out if k.startswith(prefix) and k != prefix]
+        if gemm:
+            out[prefix] = dict(out[gemm[-1]], kernel=gemm[-1])
     return out
 
 
@@ -450,26 +480,36 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    oz = bench_ozaki(Q, torch, dev, stream, N, args.steps, args.warmup, args.slices)
+    methods = {"auto": Q.OZAKI_AUTO, "slices": Q.OZAKI_SLICES, "crt": Q.OZAKI_CRT}
+    oz = bench_ozaki(Q, torch, dev, stream, N, args.steps, args.warmup, args.slices, methods[args.method],
+                     args.crt_bits)
     S = oz["slices"]
-    pairs = S * (S + 1) // 2
-    ops = 2.0 * N * N * N * pairs
+    if oz["method"] == "crt":
+        ops = 2.0 * N * N * N * oz["moduli"]
+        kname, kdesc = "k_crt_gemm", f"k_crt_gemm ({oz['moduli']} int8 GEMMs, one per modulus, " \
+                                      f"v_mfma_i32_32x32x32_i8)"
+    else:
+        ops = 2.0 * N * N * N * (S * (S + 1) // 2)
+        kname, kdesc = "k_oz_gemm", f"k_oz_gemm<{S}> (slice products, v_mfma_i32_32x32x32_i8)"
     prod_ms = oz["phases_ms"]["products"]
     achieved = ops / (prod_ms * 1e-3) / 1e12
     traffic = load_traffic()
     roofline = {"bound": "mfma", "achieved": round(achieved, 1), "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
                 "frac": round(achieved / INT8_PEAK_TOPS, 4),
-                "traffic": (traffic or {}).get("k_oz_gemm", {}).get("hbm_bytes_per_launch"),
-                "kernel": f"k_oz_gemm<{S}> (slice products, v_mfma_i32_32x32x32_i8)",
-                "algorithmic_ops_per_launch": ops, "launch_ms": prod_ms}
-    forms = {"ozaki": {"ms": round(oz["ms"], 3), "quad_GFLOPs": round(oz["quad_GFLOPs"], 1), "slices": S,
-                       "repaired_entries": oz["repairs"], "phases_ms": oz["phases_ms"]}}
+                "traffic": (traffic or {}).get(kname, {}).get("hbm_bytes_per_launch"),
+                "kernel": kdesc, "algorithmic_ops_per_launch": ops, "launch_ms": prod_ms}
+    forms = {"ozaki": ozaki_form(oz)}
+    if oz["method"] == "crt" and not args.no_slices_form:
+        forms["ozaki_slices"] = ozaki_form(bench_ozaki(Q, torch, dev, stream, N, max(args.steps // 2, 1), 1, 0,
+                                                       Q.OZAKI_SLICES, 0))
     forms["dd"] = bench_form_once(Q, torch, dev, stream, N, Q.MODE_DD)
     forms["dd"]["speedup_vs_plain"] = None
     if not args.no_plain:
         forms["plain"] = bench_plain(Q, torch, dev, stream, N)
         forms["dd"]["speedup_vs_plain"] = round(forms["plain"]["ms"] / forms["dd"]["ms"], 1)
-        forms["ozaki"]["speedup_vs_plain"] = round(forms["plain"]["ms"] / oz["ms"], 1)
+        for key in ("ozaki", "ozaki_slices"):
+            if key in forms:
+                forms[key]["speedup_vs_plain"] = round(forms["plain"]["ms"] / forms[key]["ms"], 1)
     torch.cuda.empty_cache()
     memory = None
     if not args.no_memory:
@@ -491,7 +531,9 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "i8 (int8 x int8 -> int32 MFMA; binary128 in/out)",
         "data": "synthetic (seeded device generator: random sign, log-uniform magnitude [2^-30, 2^30), "
                 "full 113-bit mantissa)",
-        "config": {"workload": f"qgemm {N}x{N}x{N} RowMajor alpha=1 beta=0", "form": "ozaki", "slices": S,
+        "config": {"workload": f"qgemm {N}x{N}x{N} RowMajor alpha=1 beta=0",
+                   "form": f"ozaki, {oz['method']} method" + (f" ({oz['crt_bits']} bits, {oz['moduli']} moduli)"
+                                                              if oz["method"] == "crt" else f" ({S} slices)"),
                    "error_bound": "each entry <= 2^-105.9 rel (certified) or correctly rounded (repaired)"},
         "roofline": roofline, "cpu_baseline": cpu, "qgemm_forms": forms, "memory_bound": memory,
         "cpu_baseline_memory_bound": cpu_mem, "batched": batched, "ill_conditioned": ill, "ozaki_status": oz["status"], "wall_s": round(oz["wall_s"], 3),

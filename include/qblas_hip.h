@@ -89,11 +89,15 @@ void quadblas_gemm_q(int layout, size_t m, size_t n, size_t k, const void *alpha
                          rounded result, independent of order and T).
    QBLAS_MODE_DD:        double-double arithmetic on FP64 FMA (inputs rounded to 106 bits); faster,
                          ~2^-104 relative accuracy on well-conditioned data.
-   QBLAS_MODE_OZAKI:     gemm only: each operand split into S int8 slices (per-row / per-column
-                         power-of-two scaling), slice products on the int8 matrix cores with exact
-                         int32 accumulation, recombined exactly and rounded once.  Every entry is
-                         certified to <= 2^-104 relative error by an a-priori bound, or recomputed
-                         exactly (correctly rounded) when the bound cannot certify it. */
+   QBLAS_MODE_OZAKI:     gemm only: each operand scaled per row / per column by a power of two and
+                         rounded to an integer, the integer products formed on the int8 matrix
+                         cores with exact int32 accumulation, recombined exactly and rounded once.
+                         Two methods (qblas_set_ozaki_method): the integers split into S int8
+                         slices (slice-pair products), or carried as residues modulo up to 47
+                         pairwise coprime moduli <= 256 (one int8 GEMM per modulus, Chinese
+                         remainder reconstruction).  Every entry is certified to <= 2^-104
+                         relative error by an a-priori bound, or recomputed exactly (correctly
+                         rounded) when the bound cannot certify it. */
 enum {
   QBLAS_MODE_REFERENCE = 0,
   QBLAS_MODE_EXACT = 1,
@@ -129,17 +133,36 @@ int qblas_get_kc(void);
 int qblas_set_ozaki_slices(int slices);
 int qblas_get_ozaki_slices(void);
 
+/* Method of QBLAS_MODE_OZAKI.  QBLAS_OZAKI_SLICES: int8 slices (precision 8 S bits).
+   QBLAS_OZAKI_CRT: residues modulo the moduli (precision t bits, 48..158; t bits at depth k need
+   the fewest moduli whose product is >= k 2^(2t-2)); when the precision the data needs exceeds
+   158 bits the slice method is used.  QBLAS_OZAKI_AUTO (the default): a fixed slice count selects
+   the slice method and fixed CRT bits the CRT method; otherwise the probe sets both precisions
+   and the method with less matrix-core work is used.  Returns QBLAS_ERR_ARG for other values. */
+enum { QBLAS_OZAKI_AUTO = 0, QBLAS_OZAKI_SLICES = 1, QBLAS_OZAKI_CRT = 2 };
+int qblas_set_ozaki_method(int method);
+int qblas_get_ozaki_method(void);
+/* Bits t of the CRT method (48..158); 0 (the default): chosen per call by the probe. */
+int qblas_set_ozaki_crt_bits(int bits);
+int qblas_get_ozaki_crt_bits(void);
+/* Method (QBLAS_OZAKI_SLICES or QBLAS_OZAKI_CRT), CRT bits and CRT moduli used by the last
+   QBLAS_MODE_OZAKI gemm on this thread (0 where not applicable). */
+int qblas_hip_ozaki_last_method(void);
+int qblas_hip_ozaki_last_crt_bits(void);
+int qblas_hip_ozaki_last_moduli(void);
+
 /* After a QBLAS_MODE_OZAKI gemm on this thread: synchronizes the stream and returns how many
    entries were recomputed exactly because the bound could not certify them (-1 if there was no
    such call or on a HIP error). */
 long long qblas_hip_ozaki_repairs(void *stream);
-/* Slice count used by the last QBLAS_MODE_OZAKI gemm on this thread (0 if none). */
+/* Slice count used by the last QBLAS_MODE_OZAKI gemm on this thread (0 if none or CRT). */
 int qblas_hip_ozaki_last_slices(void);
 
 /* Phase timing of QBLAS_MODE_OZAKI gemm calls (off by default; costs four event records per
    call).  qblas_hip_phase_times writes the durations in ms of the last such call on this thread
-   -- [0] scaling and slicing, [1] slice products (matrix cores), [2] finish and certification,
-   [3] exact repair -- after synchronizing with it; returns 4, or -1 if no timed call exists. */
+   -- [0] scaling and slicing (or residues), [1] products (matrix cores), [2] finish and
+   certification (with the CRT reconstruction), [3] exact repair -- after synchronizing with it;
+   returns 4, or -1 if no timed call exists. */
 void qblas_hip_set_phase_timing(int on);
 int qblas_hip_phase_times(float *ms);
 

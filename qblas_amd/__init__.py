@@ -25,6 +25,9 @@ MODE_EXACT = 1
 MODE_DD = 2
 MODE_OZAKI = 3
 MODES = {"reference": MODE_REFERENCE, "exact": MODE_EXACT, "dd": MODE_DD, "ozaki": MODE_OZAKI}
+OZAKI_AUTO = 0
+OZAKI_SLICES = 1
+OZAKI_CRT = 2
 
 ROW_MAJOR = 0
 COL_MAJOR = 1
@@ -93,6 +96,13 @@ def lib():
         "qblas_get_ozaki_slices": (I, []),
         "qblas_hip_ozaki_repairs": (ctypes.c_longlong, [P]),
         "qblas_hip_ozaki_last_slices": (I, []),
+        "qblas_set_ozaki_method": (I, [I]),
+        "qblas_get_ozaki_method": (I, []),
+        "qblas_set_ozaki_crt_bits": (I, [I]),
+        "qblas_get_ozaki_crt_bits": (I, []),
+        "qblas_hip_ozaki_last_method": (I, []),
+        "qblas_hip_ozaki_last_crt_bits": (I, []),
+        "qblas_hip_ozaki_last_moduli": (I, []),
         "qblas_hip_set_phase_timing": (None, [I]),
         "qblas_hip_gemm_batched": (I, [I, I, I64, P, P]),
         "qblas_hip_gemv_batched": (I, [I, I, I64, P, P]),
@@ -265,6 +275,24 @@ def get_ozaki_slices():
     return lib().qblas_get_ozaki_slices()
 
 
+def set_ozaki_method(method):
+    """OZAKI_AUTO (default), OZAKI_SLICES or OZAKI_CRT."""
+    _check(lib().qblas_set_ozaki_method(int(method)), "qblas_set_ozaki_method")
+
+
+def get_ozaki_method():
+    return lib().qblas_get_ozaki_method()
+
+
+def set_ozaki_crt_bits(t):
+    """Bits per scaled operand of the CRT method (48..158), 0 = chosen by the library."""
+    _check(lib().qblas_set_ozaki_crt_bits(int(t)), "qblas_set_ozaki_crt_bits")
+
+
+def get_ozaki_crt_bits():
+    return lib().qblas_get_ozaki_crt_bits()
+
+
 # ------------------------------------------------------------------------------------------------
 # device-pointer API
 # ------------------------------------------------------------------------------------------------
@@ -296,8 +324,18 @@ def dev_ozaki_repairs(stream=0):
 
 
 def ozaki_last_slices():
-    """Slice count used by the last OZAKI gemm of this thread (0: none)."""
+    """Slice count used by the last OZAKI gemm of this thread (0: none, or the CRT method)."""
     return lib().qblas_hip_ozaki_last_slices()
+
+
+def ozaki_last_method():
+    """OZAKI_SLICES or OZAKI_CRT for the last OZAKI gemm of this thread (0: none)."""
+    return lib().qblas_hip_ozaki_last_method()
+
+
+def ozaki_last_crt():
+    """(bits, moduli) of the last OZAKI gemm of this thread if it used the CRT method, else (0, 0)."""
+    return lib().qblas_hip_ozaki_last_crt_bits(), lib().qblas_hip_ozaki_last_moduli()
 
 
 OZAKI_PHASES = ("slicing", "products", "finish", "repair")

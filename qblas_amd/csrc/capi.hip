@@ -51,6 +51,8 @@ std::atomic<int> g_threads{0};  // 0: host default
 std::atomic<int> g_kc{126};
 std::atomic<int> g_mode{-1};    // -1: not yet read from the environment
 std::atomic<int> g_oz_slices{0};  // OZAKI slice count, 0: chosen per call
+std::atomic<int> g_oz_method{QBLAS_OZAKI_AUTO};  // OZAKI method
+std::atomic<int> g_oz_bits{0};    // CRT method: bits per scaled operand, 0: chosen per call
 
 int default_threads() {
   // omp_get_max_threads() without an OpenMP runtime: OMP_NUM_THREADS's first entry, else the
@@ -163,6 +165,9 @@ std::atomic<int> g_phase_timing{0};                     // record phase events i
 thread_local hipEvent_t t_oz_ev[5] = {};
 thread_local bool t_oz_ev_valid = false;
 thread_local int t_oz_slices = 0;                       // slice count of the last OZAKI call
+thread_local int t_oz_method = 0;                       // method of the last OZAKI call
+thread_local int t_oz_bits = 0;                         // CRT bits of the last OZAKI call
+thread_local int t_oz_moduli = 0;                       // CRT moduli of the last OZAKI call
 
 // After a synchronous EXACT-form call: report a product that fell outside its window.
 int check_exact_status(hipStream_t s) {
@@ -281,19 +286,13 @@ int dev_gemv(int mode, int layout, int64_t m, int64_t n, q128 alpha, const void 
   return QBLAS_OK;
 }
 
-// Slice count of the OZAKI form: the value set by qblas_set_ozaki_slices, else chosen from the
-// data: a probe computes 256 sampled entries exactly and measures how far below 2^(E_i + F_j)
-// they lie; the slice count is the smallest whose certification bound passes all but the two
-// lowest of them (so that about 1% or fewer of the entries are left to the exact repair).  The
-// probe synchronizes with the stream.
+// Precision of the OZAKI form from the data: a probe computes 256 sampled entries exactly and
+// measures how far below 2^(E_i + F_j) they lie; the returned value is the third lowest (so that
+// the precision chosen from it certifies all but about 1% of the entries, the rest being left to
+// the exact repair), INT_MAX when no sample is usable.  The probe synchronizes with the stream.
 thread_local Workspace t_pws;
-int oz_slices(int colmajor, int64_t m, int64_t n, int64_t k, const void *A, int64_t lda, const void *B, int64_t ldb,
-              hipStream_t s, int *out) {
-  const int fixed = g_oz_slices.load();
-  if (fixed) {
-    *out = fixed;
-    return QBLAS_OK;
-  }
+int oz_probe(int colmajor, int64_t m, int64_t n, int64_t k, const void *A, int64_t lda, const void *B, int64_t ldb,
+             hipStream_t s, int *out) {
   void *ws;
   int rc = ws_acquire(t_pws, qbk::ozaki_probe_ws(m, n), s, &ws);
   if (rc) return rc;
@@ -305,7 +304,50 @@ int oz_slices(int colmajor, int64_t m, int64_t n, int64_t k, const void *A, int6
   QB_CHECK(hipStreamSynchronize(s));
   if ((rc = ws_release(t_pws, s))) return rc;
   std::sort(r, r + P);
-  *out = qbk::ozaki_slices_for(k, r[2]);  // third lowest: at most 2 of 256 samples below the bound
+  *out = r[2];
+  return QBLAS_OK;
+}
+
+// Method and precision of an OZAKI call.  Fixed settings win (a slice count selects the slice
+// method; CRT bits the CRT method); otherwise the probe's value sets both: the slice count S and
+// the CRT bits t whose certification bounds pass the probe's entries.  AUTO takes the CRT method
+// when t fits (<= 158 bits) and its padded GEMM work (moduli x 256-tiles) is below the slice
+// method's (slice pairs x 64-tiles).  sync is set when the probe ran (the call synchronizes anyway).
+struct oz_choice {
+  int method, S, t, sync;
+};
+int oz_choose(int colmajor, int64_t m, int64_t n, int64_t k, const void *A, int64_t lda, const void *B, int64_t ldb,
+              hipStream_t s, oz_choice *c) {
+  const int method = g_oz_method.load(), fixedS = g_oz_slices.load(), fixedT = g_oz_bits.load();
+  c->sync = 0;
+  c->S = fixedS;
+  c->t = fixedT;
+  if (method == QBLAS_OZAKI_SLICES || (method == QBLAS_OZAKI_AUTO && fixedS)) {
+    c->method = QBLAS_OZAKI_SLICES;
+    if (fixedS) return QBLAS_OK;
+  } else if (fixedT && (method == QBLAS_OZAKI_CRT || method == QBLAS_OZAKI_AUTO)) {
+    c->method = QBLAS_OZAKI_CRT;
+    return QBLAS_OK;
+  } else {
+    c->method = method;
+  }
+  int r = 0;
+  int rc = oz_probe(colmajor, m, n, k, A, lda, B, ldb, s, &r);
+  if (rc) return rc;
+  c->sync = 1;
+  c->S = qbk::ozaki_slices_for(k, r);
+  c->t = qbk::ozaki_crt_bits_for(k, r);
+  if (c->method == QBLAS_OZAKI_SLICES) return QBLAS_OK;
+  if (!c->t) {  // beyond the CRT method's range
+    c->method = QBLAS_OZAKI_SLICES;
+    return QBLAS_OK;
+  }
+  if (c->method == QBLAS_OZAKI_AUTO) {
+    auto up = [](int64_t v, int64_t q) { return (double)((v + q - 1) / q * q); };
+    const double crt = (double)qbk::crt_moduli_for(k, c->t) * up(m, 256) * up(n, 256);
+    const double sl = (double)c->S * (c->S + 1) / 2 * up(m, 64) * up(n, 64);
+    c->method = crt < sl ? QBLAS_OZAKI_CRT : QBLAS_OZAKI_SLICES;
+  }
   return QBLAS_OK;
 }
 
@@ -322,12 +364,21 @@ int dev_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, q128 alpha, 
   if (m == 0 || n == 0 || k == 0) return QBLAS_OK;  // level3.hpp:221-222
   const int colmajor = layout != 0;
   if (mode == QBLAS_MODE_OZAKI) {
-    int S = 0;
-    int rc0 = oz_slices(colmajor, m, n, k, A, lda, B, ldb, s, &S);
+    oz_choice ch;
+    int rc0 = oz_choose(colmajor, m, n, k, A, lda, B, ldb, s, &ch);
     if (rc0) return rc0;
-    t_oz_slices = S;
+    const bool crt = ch.method == QBLAS_OZAKI_CRT;
+    const size_t need = crt ? qbk::gemm_crt_ws(m, n, k, ch.t) : qbk::gemm_ozaki_ws(m, n, k, ch.S);
+    if (!need) {
+      set_err_msg("qblas_hip_gemm: the CRT bit count needs more moduli than the method has");
+      return QBLAS_ERR_ARG;
+    }
+    t_oz_method = ch.method;
+    t_oz_slices = crt ? 0 : ch.S;
+    t_oz_bits = crt ? ch.t : 0;
+    t_oz_moduli = crt ? qbk::crt_moduli_for(k, ch.t) : 0;
     void *ws;
-    int rc = ws_acquire(t_ws, qbk::gemm_ozaki_ws(m, n, k, S), s, &ws);
+    int rc = ws_acquire(t_ws, need, s, &ws);
     if (rc) return rc;
     hipEvent_t *ev = nullptr;
     if (g_phase_timing.load()) {
@@ -336,9 +387,13 @@ int dev_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, q128 alpha, 
       ev = t_oz_ev;
     }
     t_oz_ev_valid = ev != nullptr;
-    // the automatic slice count already synchronizes, so the repair may too
-    QB_CHECK(qbk::launch_gemm_ozaki(colmajor, m, n, k, S, alpha, (const q128 *)A, lda, (const q128 *)B, ldb, beta,
-                                    (q128 *)C, ldc, ws, s, ev, g_oz_slices.load() == 0));
+    // a call that ran the probe already synchronized, so the repair may too
+    if (crt)
+      QB_CHECK(qbk::launch_gemm_crt(colmajor, m, n, k, ch.t, alpha, (const q128 *)A, lda, (const q128 *)B, ldb, beta,
+                                    (q128 *)C, ldc, ws, s, ev, ch.sync));
+    else
+      QB_CHECK(qbk::launch_gemm_ozaki(colmajor, m, n, k, ch.S, alpha, (const q128 *)A, lda, (const q128 *)B, ldb,
+                                      beta, (q128 *)C, ldc, ws, s, ev, ch.sync));
     t_exact_status = (const uint32_t *)ws;
     t_oz_hdr = (const uint32_t *)ws;
     return ws_release(t_ws, s);
@@ -742,6 +797,21 @@ int qblas_set_ozaki_slices(int slices) {
   return QBLAS_OK;
 }
 int qblas_get_ozaki_slices(void) { return g_oz_slices.load(); }
+int qblas_set_ozaki_method(int method) {
+  if (method != QBLAS_OZAKI_AUTO && method != QBLAS_OZAKI_SLICES && method != QBLAS_OZAKI_CRT) return QBLAS_ERR_ARG;
+  g_oz_method.store(method);
+  return QBLAS_OK;
+}
+int qblas_get_ozaki_method(void) { return g_oz_method.load(); }
+int qblas_set_ozaki_crt_bits(int bits) {
+  if (bits != 0 && (bits < qbk::OZ_CRT_TMIN || bits > qbk::OZ_CRT_TMAX)) return QBLAS_ERR_ARG;
+  g_oz_bits.store(bits);
+  return QBLAS_OK;
+}
+int qblas_get_ozaki_crt_bits(void) { return g_oz_bits.load(); }
+int qblas_hip_ozaki_last_method(void) { return t_oz_method; }
+int qblas_hip_ozaki_last_crt_bits(void) { return t_oz_bits; }
+int qblas_hip_ozaki_last_moduli(void) { return t_oz_moduli; }
 long long qblas_hip_ozaki_repairs(void *stream) {
   if (!t_oz_hdr) return -1;
   uint32_t v = 0;

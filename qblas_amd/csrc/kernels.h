@@ -62,6 +62,16 @@ hipError_t ozaki_repair_count(const void *ws, uint32_t *out, hipStream_t st);
 // probe: scales of A and B, then for `samples` (<= 1024) exact entries ilogb|AB_ij| - (E_i + F_j)
 // as ints at ws + ozaki_probe_out_offset (INT_MAX where unusable); ozaki_slices_for turns a chosen
 // order statistic of them into a slice count
+constexpr int OZ_CRT_TMIN = 48, OZ_CRT_TMAX = 158;
+// OZAKI form, CRT method (residues modulo nq pairwise coprime moduli <= 256, one int8 GEMM per
+// modulus): t bits per scaled operand (48..158), workspace bytes (0 when t needs too many moduli)
+size_t gemm_crt_ws(int64_t m, int64_t n, int64_t k, int t);
+int crt_moduli_for(int64_t k, int t);
+hipError_t launch_gemm_crt(int colmajor, int64_t m, int64_t n, int64_t k, int t, q128 alpha, const q128 *A,
+                           int64_t lda, const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc, void *ws,
+                           hipStream_t st, hipEvent_t *ev, int sync_repair);
+// bits t of the CRT method for a probe value (as ozaki_slices_for); 0: use the slice method
+int ozaki_crt_bits_for(int64_t k, int r);
 size_t ozaki_probe_ws(int64_t m, int64_t n);
 size_t ozaki_probe_out_offset(int64_t m, int64_t n);
 hipError_t launch_ozaki_probe(int colmajor, int64_t m, int64_t n, int64_t k, const q128 *A, int64_t lda,

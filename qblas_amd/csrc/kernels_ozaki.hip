@@ -742,6 +742,34 @@ static void oz_scale(const q128 *X, int64_t R, int64_t Cn, int64_t sr, int64_t s
   k_oz_scale<<<grid, 256, 0, st>>>(X, R, Cn, sr, sc, contig, cchunk, E, F);
 }
 
+// Exact recomputation of the queued entries (header word 1 counts them, list holds (i, j)).  With
+// sync_repair the caller accepts a synchronization: the queue length is read, and when it is long
+// B's columns are read from a transposed copy (contiguous, at Bt) instead of with stride ldb.
+static hipError_t oz_repair_queue(int64_t n, int64_t k, const q128 *A, int64_t asr, int64_t asc, const q128 *B,
+                                  int64_t bsr, int64_t bsc, const int32_t *eA, const int32_t *eB, q128 alpha,
+                                  q128 beta, q128 *C, int64_t csr, int64_t csc, uint32_t *hdr, const uint32_t *list,
+                                  q128 *Bt, hipStream_t st, hipEvent_t *ev, int sync_repair) {
+  const q128 *Br = B;
+  int64_t rbsr = bsr, rbsc = bsc;
+  if (sync_repair && bsr != 1 && n > 1) {
+    uint32_t count = 0;
+    hipError_t e2 = hipMemcpyAsync(&count, hdr + 1, 4, hipMemcpyDeviceToHost, st);
+    if (e2 == hipSuccess) e2 = hipStreamSynchronize(st);
+    if (e2 != hipSuccess) return e2;
+    if (count >= 512) {
+      dim3 tg((unsigned)((n + 31) / 32), (unsigned)((k + 31) / 32));
+      k_oz_transpose<<<tg, 256, 0, st>>>(B, k, n, bsr, Bt);
+      Br = Bt;
+      rbsr = 1;
+      rbsc = k;
+    }
+  }
+  k_oz_repair<<<OZ_REPAIR_BLOCKS, OZR_BS, 0, st>>>(k, A, asr, asc, Br, rbsr, rbsc, eA, eB, alpha, beta, C, csr, csc,
+                                                   hdr, list);
+  if (ev) (void)hipEventRecord(ev[4], st);
+  return hipGetLastError();
+}
+
 hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int S, q128 alpha, const q128 *A,
                              int64_t lda, const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc, void *wsv,
                              hipStream_t st, hipEvent_t *ev, int sync_repair) {
@@ -791,28 +819,8 @@ hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int 
                                                             L.Mp, L.Np, eA, fA, eB, fB, alpha, beta, C, csr, csc, hdr,
                                                             (uint32_t *)(ws + L.list));
   if (ev) (void)hipEventRecord(ev[3], st);
-  const q128 *Br = B;
-  int64_t rbsr = bsr, rbsc = bsc;
-  if (sync_repair && bsr != 1 && n > 1) {
-    // the caller accepts a synchronization: read the queue length, and when it is long, read B's
-    // columns from a transposed copy (contiguous) instead of gathering them with stride ldb
-    uint32_t count = 0;
-    hipError_t e2 = hipMemcpyAsync(&count, hdr + 1, 4, hipMemcpyDeviceToHost, st);
-    if (e2 == hipSuccess) e2 = hipStreamSynchronize(st);
-    if (e2 != hipSuccess) return e2;
-    if (count >= 512) {
-      q128 *Bt = (q128 *)(ws + L.bt);
-      dim3 tg((unsigned)((n + 31) / 32), (unsigned)((k + 31) / 32));
-      k_oz_transpose<<<tg, 256, 0, st>>>(B, k, n, bsr, Bt);
-      Br = Bt;
-      rbsr = 1;
-      rbsc = k;
-    }
-  }
-  k_oz_repair<<<OZ_REPAIR_BLOCKS, OZR_BS, 0, st>>>(k, A, asr, asc, Br, rbsr, rbsc, eA, eB, alpha, beta, C, csr, csc,
-                                                   hdr, (const uint32_t *)(ws + L.list));
-  if (ev) (void)hipEventRecord(ev[4], st);
-  return hipGetLastError();
+  return oz_repair_queue(n, k, A, asr, asc, B, bsr, bsc, eA, eB, alpha, beta, C, csr, csc, hdr,
+                         (const uint32_t *)(ws + L.list), (q128 *)(ws + L.bt), st, ev, sync_repair);
 }
 
 size_t ozaki_probe_ws(int64_t m, int64_t n) {
@@ -863,6 +871,643 @@ hipError_t ozaki_repair_count(const void *ws, uint32_t *out, hipStream_t st) {
   hipError_t e = hipMemcpyAsync(out, (const char *)ws + 4, 4, hipMemcpyDeviceToHost, st);
   if (e != hipSuccess) return e;
   return hipStreamSynchronize(st);
+}
+
+// ==========================================================================================
+// CRT method (Ozaki scheme II: K. Ozaki, Y. Uchino, T. Imamura, "Ozaki Scheme II: A
+// GEMM-oriented emulation of floating-point matrix multiplication using an integer modular
+// technique", 2025).  Same scaling as the slice method, but the scaled integers are carried as
+// residues modulo nq pairwise coprime moduli p_q <= 256 instead of as base-256 digits:
+//
+// 1. N_il = round(a_il 2^t) with a_il = A_il 2^-E_i in (-1/4, 1/4) (|N| < 2^(t-2)), likewise
+//    N'_lj for B.  t is any bit count (not a multiple of 8).
+// 2. Residues.  r_q(N) = N mod p_q in [-p/2, p/2) as int8: from the balanced base-256 digits
+//    d_k of N (the slice method's digits with t bits), r_q = sum_k d_k (256^k mod p_q) mod p_q,
+//    the sum over digits done on the matrix cores (v_mfma_i32_32x32x32_i8 with the digits as A
+//    operand and the powers 256^k mod p_q as B operand), then reduced.  Stored in MFMA fragment
+//    order per modulus.
+// 3. Products.  One int8 GEMM per modulus, R_q = (r_q(N) r_q(N')) mod p_q (int32 accumulation is
+//    exact for k < 2^17; longer k is reduced every 2048 k blocks).
+// 4. Reconstruction.  With M = prod p_q and V_q = (M/p_q) ((M/p_q)^-1 mod p_q), X' = sum_q R_q V_q
+//    is congruent to X = sum_l N_il N'_lj mod M; M >= 4 K 2^(2t-4) >= 4|X|, so X = X' - kM with
+//    k = round(X'/M).  X' is computed exactly on the matrix cores as sum_b 256^b sum_q R_q v_qb
+//    (v_qb: balanced bytes of V_q), then carried into 11 words.
+// 5. Finish and repair as in the slice method: alpha X 2^(E_i+F_j-2t) + beta C rounded once, and
+//    certified by |AB - X 2^(E_i+F_j-2t)| <= K 2^(E_i+F_j-t-2) (representation error of both
+//    operands; nothing is dropped), otherwise recomputed exactly.
+// The GEMM count is nq (38 to 47 for t = 130..158 and k = 4096) instead of the S(S+1)/2 slice
+// pairs of the slice method at the same precision.
+// ==========================================================================================
+constexpr int CRT_QALL = 54;       // pairwise coprime moduli <= 256 (prime powers)
+constexpr int CRT_QMAX = 47;       // moduli used at most: M < 2^330, |X'| < 47 * 128 M < 2^343
+constexpr int CRT_TMIN = OZ_CRT_TMIN;  // bits of the scaled operands at least
+constexpr int CRT_TMAX = OZ_CRT_TMAX;  // and at most (|N| < 2^156 fits the 20 digits)
+constexpr int CRT_ND = 20;         // balanced base-256 digits of a scaled operand (t <= 158)
+constexpr int CRT_MW = 11;         // 32-bit words of M and of the reconstructed integer
+constexpr int CRT_TILE = 256;      // output rows and columns of one product workgroup
+constexpr int CRT_KCHUNK = 2048;   // k blocks between modular reductions of the accumulators
+
+// descending, so that the first nq moduli have the largest product
+__device__ const int32_t crt_dev_moduli[CRT_QALL] = {
+    256, 251, 243, 241, 239, 233, 229, 227, 223, 211, 199, 197, 193, 191, 181, 179, 173, 169,
+    167, 163, 157, 151, 149, 139, 137, 131, 127, 125, 121, 113, 109, 107, 103, 101, 97,  89,
+    83,  79,  73,  71,  67,  61,  59,  53,  49,  47,  43,  41,  37,  31,  29,  23,  19,  17};
+static const int32_t crt_host_moduli[CRT_QALL] = {
+    256, 251, 243, 241, 239, 233, 229, 227, 223, 211, 199, 197, 193, 191, 181, 179, 173, 169,
+    167, 163, 157, 151, 149, 139, 137, 131, 127, 125, 121, 113, 109, 107, 103, 101, 97,  89,
+    83,  79,  73,  71,  67,  61,  59,  53,  49,  47,  43,  41,  37,  31,  29,  23,  19,  17};
+
+// Per-call constants, built on the device by k_crt_tables (64 columns: moduli beyond nq are 0).
+struct crt_tab {
+  int32_t p[64];         // moduli
+  float pinv[64];        // 1 / p
+  double pinvd[64];      // 1 / p
+  int8_t cpow[64][32];   // [q][k] = 256^k mod p_q, balanced
+  int8_t vt[64][64];     // [b][q] = balanced base-256 digit b of V_q
+  uint32_t M[CRT_MW];    // product of the moduli
+  double xs[CRT_MW];     // 2^(32 (w - w0)) for words w >= w0, else 0 (w0: second-highest word of M)
+  double mtop;           // sum_w M[w] xs[w]
+  int32_t nq;
+};
+constexpr size_t CRT_TAB_BYTES = (sizeof(crt_tab) + 255) / 256 * 256;
+
+__global__ void __launch_bounds__(64) k_crt_tables(int nq, crt_tab *__restrict__ T) {
+  __shared__ uint32_t sM[CRT_MW];
+  const int q = threadIdx.x;
+  const int32_t p = (q < nq) ? crt_dev_moduli[q] : 0;
+  if (q == 0) {
+    uint32_t M[CRT_MW];
+#pragma unroll
+    for (int w = 0; w < CRT_MW; ++w) M[w] = w == 0 ? 1u : 0u;
+    for (int i = 0; i < nq; ++i) {
+      uint64_t c = 0;
+#pragma unroll
+      for (int w = 0; w < CRT_MW; ++w) {
+        const uint64_t v = (uint64_t)M[w] * (uint32_t)crt_dev_moduli[i] + c;
+        M[w] = (uint32_t)v;
+        c = v >> 32;
+      }
+    }
+#pragma unroll
+    for (int w = 0; w < CRT_MW; ++w) sM[w] = M[w];
+  }
+  __syncthreads();
+  T->p[q] = p;
+  T->pinv[q] = p ? 1.0f / (float)p : 0.0f;
+  T->pinvd[q] = p ? 1.0 / (double)p : 0.0;
+  int32_t pw = 1 % (p ? p : 1);
+  for (int k = 0; k < 32; ++k) {
+    int v = p ? pw : 0;
+    if (p && v >= (p + 1) / 2) v -= p;  // balanced: [-p/2, p/2) (p = 256: 256^0 = 1, else 0)
+    T->cpow[q][k] = (int8_t)v;
+    if (p) pw = (int32_t)(((int64_t)pw * 256) % p);
+  }
+  uint32_t V[CRT_MW];
+#pragma unroll
+  for (int w = 0; w < CRT_MW; ++w) V[w] = 0;
+  if (p) {
+    // Mq = M / p (exact), a = Mq mod p, V = Mq * a^-1 mod p (< M)
+    uint32_t Mq[CRT_MW];
+    uint64_t rem = 0;
+    for (int w = CRT_MW - 1; w >= 0; --w) {
+      const uint64_t cur = (rem << 32) | sM[w];
+      Mq[w] = (uint32_t)(cur / (uint64_t)p);
+      rem = cur % (uint64_t)p;
+    }
+    uint64_t a = 0;
+    for (int w = CRT_MW - 1; w >= 0; --w) a = ((a << 32) | Mq[w]) % (uint64_t)p;
+    uint32_t inv = 0;
+    for (uint32_t x = 1; x < (uint32_t)p; ++x)
+      if ((a * x) % (uint64_t)p == 1) {
+        inv = x;
+        break;
+      }
+    uint64_t c = 0;
+    for (int w = 0; w < CRT_MW; ++w) {
+      const uint64_t v = (uint64_t)Mq[w] * inv + c;
+      V[w] = (uint32_t)v;
+      c = v >> 32;
+    }
+  }
+  int carry = 0;
+  for (int b = 0; b < 64; ++b) {
+    const int byte = (b < 4 * CRT_MW) ? (int)((V[b >> 2] >> (8 * (b & 3))) & 0xffu) : 0;
+    int d = byte + carry;
+    carry = d >= 128;
+    if (carry) d -= 256;
+    T->vt[b][q] = (int8_t)(p ? d : 0);
+  }
+  if (q == 0) {
+    int wt = 0;
+    for (int w = 0; w < CRT_MW; ++w)
+      if (sM[w]) wt = w;
+    const int w0 = wt > 0 ? wt - 1 : 0;
+    double mt = 0;
+    for (int w = 0; w < CRT_MW; ++w) {
+      const double x = (w >= w0) ? __builtin_ldexp(1.0, 32 * (w - w0)) : 0.0;
+      T->M[w] = sM[w];
+      T->xs[w] = x;
+      mt += (double)sM[w] * x;
+    }
+    T->mtop = mt;
+    T->nq = nq;
+  }
+}
+
+// W = N + sum_{k<20} 128 256^k with N = round(x 2^(t-Es)) (two's complement, 6 words): byte k of W
+// XOR 0x80 is the balanced digit d_k of N (|N| < 2^(t-2) <= 2^156 < the bias).
+__device__ __forceinline__ void crt_scaled(q128 x, int Es, int t, uint32_t (&W)[6]) {
+  uint32_t Nw[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) Nw[i] = 0;
+  if (q_bexp(x) != 0x7fffu && !q_iszero(x)) {
+    const uq u = unpack_finite(x);
+    const int sh = u.e - 112 - Es + t;  // u.e <= Es - 3
+    uint32_t m[6] = {(uint32_t)u.ml, (uint32_t)(u.ml >> 32), (uint32_t)u.mh, (uint32_t)(u.mh >> 32), 0u, 0u};
+    if (sh >= 0) {
+      rshl<6>(Nw, m, sh);
+    } else if (sh > -115) {
+      const int pbit = -sh - 1;  // round half up of the magnitude
+      uint32_t c = 0;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) m[i] = addc32(m[i], (i == (pbit >> 5)) ? (1u << (pbit & 31)) : 0u, c, c);
+      rshr<6>(Nw, m, -sh);
+    }
+    if (u.s) {
+      uint32_t c = 1;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) Nw[i] = addc32(~Nw[i], 0u, c, c);
+    }
+  }
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) W[i] = addc32(Nw[i], i < 5 ? 0x80808080u : 0u, c, c);
+}
+
+// residue of x (|x| < 2^24) modulo p in [-p/2, p/2] (p odd) or [-128, 127] (p = 256), as a byte
+__device__ __forceinline__ uint32_t crt_red_f(int x, int p, float ip) {
+  if (p == 256) return (uint32_t)x & 0xffu;
+  const float xf = (float)x;
+  const float r = __builtin_fmaf(-__builtin_rintf(xf * ip), (float)p, xf);
+  return (uint32_t)(int)r & 0xffu;
+}
+// the same for |x| < 2^31
+__device__ __forceinline__ uint32_t crt_red_d(int x, int p, double ip) {
+  if (p == 256) return (uint32_t)x & 0xffu;
+  const double xd = (double)x;
+  const double r = __builtin_fma(-__builtin_rint(xd * ip), (double)p, xd);
+  return (uint32_t)(int)r & 0xffu;
+}
+
+// Residues of the scaled integers of one operand: rows r < R (rows of A, or columns of B), k < K,
+// element (r, k) at X[r sr + k sc], row exponent E[r] (+3).  One wave per fragment position
+// (rb, kb) of the Rb x Kb grid; out[q][rb][kb][1 KB] for q < nq, lane L's 16 bytes = row L%32,
+// k 16(L/32)..+15.  MFMA row rho of group g is the element (slot 2g + ((rho>>2)&1), byte
+// 4(rho>>3) + (rho&3)) of the fragment; the lanes holding MFMA output rows 8(v/4) + 4h + v%4
+// then hold slot 2g + h, bytes v = 0..15.
+__global__ void __launch_bounds__(256) k_crt_residues(const q128 *__restrict__ X, int64_t R, int64_t K, int64_t sr,
+                                                      int64_t sc, const int32_t *__restrict__ E, int t,
+                                                      const crt_tab *__restrict__ T, int64_t Rb, int64_t Kb,
+                                                      int8_t *__restrict__ out) {
+  const int64_t pos = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (pos >= Rb * Kb) return;  // whole waves
+  const int lane = threadIdx.x & 63, n = lane & 31, h = lane >> 5;
+  const int64_t rb = pos / Kb, kb = pos - rb * Kb;
+  const int nq = T->nq;
+  const i32x4v bc0 = *reinterpret_cast<const i32x4v *>(&T->cpow[n][16 * h]);
+  const i32x4v bc1 = *reinterpret_cast<const i32x4v *>(&T->cpow[32 + n][16 * h]);
+  const int p0 = T->p[n], p1 = T->p[32 + n];
+  const float ip0 = T->pinv[n], ip1 = T->pinv[32 + n];
+  const int64_t plane = Rb * Kb * OZ_FRAG;
+  int8_t *o0 = out + (int64_t)n * plane + pos * OZ_FRAG;
+  int8_t *o1 = out + (int64_t)(32 + n) * plane + pos * OZ_FRAG;
+  const int hh = (n >> 2) & 1, ib = 4 * (n >> 3) + (n & 3);
+  i32x16v zero;
+#pragma unroll
+  for (int v = 0; v < 16; ++v) zero[v] = 0;
+  for (int g = 0; g < 32; ++g) {
+    const int slot = 2 * g + hh;
+    const int64_t row = rb * 32 + (slot & 31), col = kb * 32 + 16 * (slot >> 5) + ib;
+    uint32_t W[6] = {0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u, 0u};
+    if (row < R && col < K) {
+      const int32_t er = E[row];
+      if (er != OZ_EZERO) crt_scaled(ldq_c(X + row * sr + col * sc), er + 3, t, W);
+    }
+    i32x4v a;
+    if (h == 0) {
+      a.x = (int)(W[0] ^ 0x80808080u);
+      a.y = (int)(W[1] ^ 0x80808080u);
+      a.z = (int)(W[2] ^ 0x80808080u);
+      a.w = (int)(W[3] ^ 0x80808080u);
+    } else {
+      a.x = (int)(W[4] ^ 0x80808080u);
+      a.y = 0;
+      a.z = 0;
+      a.w = 0;
+    }
+    const i32x16v d0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bc0, zero, 0, 0, 0);
+    const i32x16v d1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bc1, zero, 0, 0, 0);
+    if (n < nq) {
+      i32x4v w;
+      uint32_t u[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+      for (int v = 0; v < 16; ++v) u[v >> 2] |= crt_red_f(d0[v], p0, ip0) << (8 * (v & 3));
+      w.x = (int)u[0];
+      w.y = (int)u[1];
+      w.z = (int)u[2];
+      w.w = (int)u[3];
+      *reinterpret_cast<i32x4v *>(o0 + (2 * g + h) * 16) = w;
+    }
+    if (32 + n < nq) {
+      i32x4v w;
+      uint32_t u[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+      for (int v = 0; v < 16; ++v) u[v >> 2] |= crt_red_f(d1[v], p1, ip1) << (8 * (v & 3));
+      w.x = (int)u[0];
+      w.y = (int)u[1];
+      w.z = (int)u[2];
+      w.w = (int)u[3];
+      *reinterpret_cast<i32x4v *>(o1 + (2 * g + h) * 16) = w;
+    }
+  }
+}
+
+// One int8 GEMM per modulus: RC[q][mb][nb] = (RA[q] RB[q]^T) mod p_q for a 256 x 256 tile of
+// (mb, nb) blocks, 8 waves of 128 x 64 (4 x 2 MFMA blocks).  A 4-slot LDS ring of k blocks
+// (8 A + 8 B fragments, 16 KB) is filled three stages ahead with global_load_lds; each wave copies
+// 2 fragments per stage.  Workgroups of one modulus are consecutive (its residue planes stay in
+// the caches).  Output block (q, mb, nb) in MFMA result order: lane L's 16 bytes are rows
+// 8(v/4) + 4(L/32) + v%4, column L%32, v = 0..15.
+constexpr int CRT_WR = 2, CRT_WC = 4, CRT_FM = 4, CRT_FN = 2;
+constexpr int CRT_TA = CRT_WR * CRT_FM, CRT_TB = CRT_WC * CRT_FN;
+constexpr int CRT_SLOT = (CRT_TA + CRT_TB) * OZ_FRAG;
+constexpr int CRT_PER = (CRT_TA + CRT_TB) / (CRT_WR * CRT_WC);
+static_assert(CRT_TA * 32 == CRT_TILE && CRT_TB * 32 == CRT_TILE, "tile");
+static_assert((CRT_TA + CRT_TB) % (CRT_WR * CRT_WC) == 0 && 2 * CRT_PER <= 15, "copy split");
+
+__global__ void __launch_bounds__(512) k_crt_gemm(const int8_t *__restrict__ RA, const int8_t *__restrict__ RB,
+                                                  int64_t Mb, int64_t Nb, int64_t Kb, const crt_tab *__restrict__ T,
+                                                  int8_t *__restrict__ RC) {
+  __shared__ __attribute__((aligned(1024))) int8_t lds[4 * CRT_SLOT];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t tilesM = Mb / CRT_TA, tilesN = Nb / CRT_TB, per = tilesM * tilesN;
+  const int64_t q = blockIdx.x / per, tt = blockIdx.x - q * per;
+  const int64_t tm = tt / tilesN, tn = tt - tm * tilesN;
+  const int8_t *Aq = RA + (q * Mb + tm * CRT_TA) * Kb * OZ_FRAG;
+  const int8_t *Bq = RB + (q * Nb + tn * CRT_TB) * Kb * OZ_FRAG;
+  auto copy = [&](int64_t kb) {
+    if (kb >= Kb) return;
+    int8_t *d = &lds[(int)(kb & 3) * CRT_SLOT];
+#pragma unroll
+    for (int c = 0; c < CRT_PER; ++c) {
+      const int f = wave * CRT_PER + c;  // A fragments 0..TA-1, then B
+      const int8_t *s = (f < CRT_TA) ? Aq + ((int64_t)f * Kb + kb) * OZ_FRAG
+                                     : Bq + ((int64_t)(f - CRT_TA) * Kb + kb) * OZ_FRAG;
+      __builtin_amdgcn_global_load_lds((glb_void *)(s + lane * 16), (lds_void *)(d + f * OZ_FRAG), 16, 0, 0);
+    }
+  };
+  const int wr = wave / CRT_WC, wc = wave % CRT_WC;
+  const int p = T->p[q];
+  const double ipd = T->pinvd[q];
+  i32x16v acc[CRT_FM][CRT_FN];
+#pragma unroll
+  for (int a = 0; a < CRT_FM; ++a)
+#pragma unroll
+    for (int b = 0; b < CRT_FN; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
+  copy(0);
+  copy(1);
+  copy(2);
+  for (int64_t kb = 0; kb < Kb; ++kb) {
+    if (kb + 2 < Kb)
+      __builtin_amdgcn_s_waitcnt((2 * CRT_PER) | (7 << 4));  // vmcnt(2 PER) lgkmcnt(0)
+    else
+      __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_s_barrier();
+    const int8_t *base = &lds[(int)(kb & 3) * CRT_SLOT + lane * 16];
+    i32x4v fa[CRT_FM], fb[CRT_FN];
+#pragma unroll
+    for (int a = 0; a < CRT_FM; ++a) fa[a] = *reinterpret_cast<const i32x4v *>(base + (wr * CRT_FM + a) * OZ_FRAG);
+#pragma unroll
+    for (int b = 0; b < CRT_FN; ++b)
+      fb[b] = *reinterpret_cast<const i32x4v *>(base + (CRT_TA + wc * CRT_FN + b) * OZ_FRAG);
+    copy(kb + 3);
+#pragma unroll
+    for (int a = 0; a < CRT_FM; ++a)
+#pragma unroll
+      for (int b = 0; b < CRT_FN; ++b) acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
+    if ((kb + 1) % CRT_KCHUNK == 0 && kb + 1 < Kb) {
+      // |acc| <= 2048 * 32 * 2^14 = 2^30: reduce before it can overflow
+#pragma unroll
+      for (int a = 0; a < CRT_FM; ++a)
+#pragma unroll
+        for (int b = 0; b < CRT_FN; ++b)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) acc[a][b][r] = (int)(int8_t)crt_red_d(acc[a][b][r], p, ipd);
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < CRT_FM; ++a)
+#pragma unroll
+    for (int b = 0; b < CRT_FN; ++b) {
+      uint32_t u[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+      for (int v = 0; v < 16; ++v) u[v >> 2] |= crt_red_d(acc[a][b][v], p, ipd) << (8 * (v & 3));
+      const int64_t blk = (q * Mb + tm * CRT_TA + wr * CRT_FM + a) * Nb + tn * CRT_TB + wc * CRT_FN + b;
+      i32x4v o;
+      o.x = (int)u[0];
+      o.y = (int)u[1];
+      o.z = (int)u[2];
+      o.w = (int)u[3];
+      *reinterpret_cast<i32x4v *>(RC + blk * OZ_FRAG + lane * 16) = o;
+    }
+}
+
+// bytes e of the four words P[0..3] -> one word (byte j from P[j])
+__device__ __forceinline__ uint32_t crt_gather_byte(const uint32_t (&P)[16], int v, int e) {
+  return ((P[4 * v] >> (8 * e)) & 0xffu) | (((P[4 * v + 1] >> (8 * e)) & 0xffu) << 8) |
+         (((P[4 * v + 2] >> (8 * e)) & 0xffu) << 16) | (((P[4 * v + 3] >> (8 * e)) & 0xffu) << 24);
+}
+
+// Reconstruction and finish for one 32 x 32 block of C per wave.  The D^T = Vt R product has the
+// entries as columns: for the entries (row 8 rq + 4 h' + e, column n) of the block, lane L
+// (n = L%32, kh = L/32) holds the residues of moduli 32 kb + 16 kh + 0..15 (B operand, gathered
+// from product lane n + 32 h' and transposed in registers), and receives the byte sums
+// s_b = sum_q R_q v_qb for b = 8(v/4) + 4 kh + v%4 (+ 32 for the second row block): 4-byte runs
+// m = 2(v/4) + kh (+ 8).  Lanes L and L + 32 swap half their runs so that lane kh = 0 holds all
+// runs of entry e and lane kh = 1 all runs of entry e + 1.
+__global__ void __launch_bounds__(256) k_crt_finish(int64_t M, int64_t N, int64_t K, int t, int kc, int64_t Mb,
+                                                    int64_t Nb, const int8_t *__restrict__ RC,
+                                                    const crt_tab *__restrict__ T, const int32_t *__restrict__ eA,
+                                                    const uint32_t *__restrict__ fA, const int32_t *__restrict__ eB,
+                                                    const uint32_t *__restrict__ fB, q128 alpha, q128 beta,
+                                                    q128 *__restrict__ C, int64_t csr, int64_t csc,
+                                                    uint32_t *__restrict__ hdr, uint32_t *__restrict__ list) {
+  const int64_t blk = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (blk >= Mb * Nb) return;  // whole waves
+  const int lane = threadIdx.x & 63, n = lane & 31, kh = lane >> 5;
+  const int64_t bm = blk / Nb, bn = blk - bm * Nb;
+  const int nq = T->nq;
+  const int64_t plane = Mb * Nb * OZ_FRAG;
+  i32x4v aV[2][2];
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) aV[mb][kb] = *reinterpret_cast<const i32x4v *>(&T->vt[n + 32 * mb][32 * kb + 16 * kh]);
+  uint32_t Mw[CRT_MW];
+  double xsc[CRT_MW];
+#pragma unroll
+  for (int w = 0; w < CRT_MW; ++w) {
+    Mw[w] = T->M[w];
+    xsc[w] = T->xs[w];
+  }
+  const double mtop = T->mtop;
+  const bool ab_ok = (q_bexp(alpha) != 0x7fffu) && (q_bexp(beta) != 0x7fffu);
+  const bool alpha_zero = q_iszero(alpha);
+  const int ea = ab_ok && !alpha_zero ? unpack_finite(alpha).e : 0;
+  const int64_t col = bn * 32 + n;
+  i32x16v zero;
+#pragma unroll
+  for (int v = 0; v < 16; ++v) zero[v] = 0;
+  for (int hp = 0; hp < 2; ++hp) {
+    for (int rq = 0; rq < 4; ++rq) {
+      // residues of the four entries (rows 8 rq + 4 hp + e), moduli 32 kb + 16 kh + j
+      uint32_t Eo[2][4][4];
+#pragma unroll
+      for (int kb = 0; kb < 2; ++kb) {
+        uint32_t P[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          const int q = 32 * kb + 16 * kh + j;
+          P[j] = (q < nq) ? *reinterpret_cast<const uint32_t *>(RC + q * plane + blk * OZ_FRAG + (n + 32 * hp) * 16 + 4 * rq)
+                          : 0u;
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) Eo[kb][e][v] = crt_gather_byte(P, v, e);
+      }
+#pragma unroll
+      for (int e = 0; e < 4; e += 2) {
+        i32x16v D[2][2];  // [entry e / e + 1][row block]
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+          for (int mb = 0; mb < 2; ++mb) {
+            i32x4v b0, b1;
+            b0.x = (int)Eo[0][e + x][0];
+            b0.y = (int)Eo[0][e + x][1];
+            b0.z = (int)Eo[0][e + x][2];
+            b0.w = (int)Eo[0][e + x][3];
+            b1.x = (int)Eo[1][e + x][0];
+            b1.y = (int)Eo[1][e + x][1];
+            b1.z = (int)Eo[1][e + x][2];
+            b1.w = (int)Eo[1][e + x][3];
+            D[x][mb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(aV[mb][0], b0, zero, 0, 0, 0);
+            D[x][mb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(aV[mb][1], b1, D[x][mb], 0, 0, 0);
+          }
+        // runs of this lane: index a + 4 mb (a = v/4) is run 2a + kh + 8 mb
+        int64_t own[8], send[8];
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            const int64_t r0 = (int64_t)D[0][mb][4 * a] + ((int64_t)D[0][mb][4 * a + 1] << 8) +
+                               ((int64_t)D[0][mb][4 * a + 2] << 16) + ((int64_t)D[0][mb][4 * a + 3] << 24);
+            const int64_t r1 = (int64_t)D[1][mb][4 * a] + ((int64_t)D[1][mb][4 * a + 1] << 8) +
+                               ((int64_t)D[1][mb][4 * a + 2] << 16) + ((int64_t)D[1][mb][4 * a + 3] << 24);
+            own[a + 4 * mb] = kh ? r1 : r0;   // kh 0 finishes entry e, kh 1 entry e + 1
+            send[a + 4 * mb] = kh ? r0 : r1;
+          }
+        int64_t got[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const uint64_t sv = (uint64_t)send[i];
+          const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)sv, 32);
+          const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(sv >> 32), 32);
+          got[i] = (int64_t)(((uint64_t)hi << 32) | lo);
+        }
+        // X' = sum_m run_m 2^(32 m), m = 0..10 (runs 11.. are zero: V_q < M < 2^338)
+        uint32_t X[CRT_MW];
+        int64_t c = 0;
+#pragma unroll
+        for (int m = 0; m < CRT_MW; ++m) {
+          const int idx = ((m >> 1) & 3) + 4 * (m >> 3);  // position of run m in its lane's list
+          const bool mine = (m & 1) == kh;
+          const int64_t rv = (m < 16) ? (mine ? own[idx] : got[idx]) : 0;
+          const int64_t s2 = rv + c;
+          X[m] = (uint32_t)s2;
+          c = s2 >> 32;
+        }
+        // k = round(X'/M) from the words of |X'| at M's two highest (|X'| < 2^13 M, so the words
+        // above are zero); X = X' - k M
+        int64_t kq;
+        {
+          const uint32_t xs_neg = X[CRT_MW - 1] >> 31;
+          uint32_t mag[CRT_MW];
+          uint32_t cc = xs_neg;
+#pragma unroll
+          for (int w = 0; w < CRT_MW; ++w) mag[w] = addc32(xs_neg ? ~X[w] : X[w], 0u, cc, cc);
+          double top = 0;
+#pragma unroll
+          for (int w = 0; w < CRT_MW; ++w) top += (double)mag[w] * xsc[w];
+          const int64_t kmag = (int64_t)__builtin_rint(top / mtop);
+          kq = xs_neg ? -kmag : kmag;
+        }
+        int64_t bc = 0;
+#pragma unroll
+        for (int w = 0; w < CRT_MW; ++w) {
+          const int64_t s2 = (int64_t)X[w] - kq * (int64_t)Mw[w] + bc;
+          X[w] = (uint32_t)s2;
+          bc = s2 >> 32;
+        }
+        const uint32_t sgn = X[CRT_MW - 1] >> 31;
+        if (sgn) {
+          uint32_t cc = 1;
+#pragma unroll
+          for (int w = 0; w < CRT_MW; ++w) X[w] = addc32(~X[w], 0u, cc, cc);
+        }
+        // the entry
+        const int64_t row = bm * 32 + 8 * rq + 4 * hp + e + kh;
+        if (row < M && col < N) {
+          q128 *cp = C + row * csr + col * csc;
+          const q128 yv = ldq_c(cp);
+          const bool fin = !((fA[row] | fB[col]) & OZF_NONFINITE) && ab_ok && q_bexp(yv) != 0x7fffu;
+          bool done = false;
+          q128 res;
+          if (fin && alpha_zero) {
+            res = mul_packed(beta, yv);
+            done = !q_iszero(res);
+          } else if (fin && eA[row] != OZ_EZERO && eB[col] != OZ_EZERO && !rzero<CRT_MW>(X)) {
+            const int Ei = eA[row] + 3, Fj = eB[col] + 3;
+            res = finish_reg_t<CRT_MW>(X, Ei + Fj - 2 * t, sgn, 0u, 0u, true, alpha, beta, yv);
+            if (q_finite_nonzero(res)) {
+              const int bexp = ea + 1 + kc + Ei + Fj - t - 2;  // |alpha (AB - P)| < 2^bexp
+              done = bexp <= unpack_finite(res).e - 106;
+            }
+          }
+          if (done) {
+            stq_c(cp, res);
+          } else {
+            const uint32_t slot = atomicAdd(hdr + 1, 1u);
+            list[2 * (int64_t)slot] = (uint32_t)row;
+            list[2 * (int64_t)slot + 1] = (uint32_t)col;
+          }
+        }
+      }
+    }
+  }
+}
+
+struct crt_layout {
+  int64_t Mp, Np, Kb, Mb, Nb;
+  int nq;
+  size_t eA, fA, eB, fB, tab, rA, rB, rC, list, bt, total;
+};
+
+static int ceil_log2(int64_t k) {
+  int c = 0;
+  while (c < 62 && ((int64_t)1 << c) < k) ++c;
+  return c;
+}
+
+// moduli for t bits at depth k: M >= K 2^(2t-2) (so that |X| <= M/4); 0 when more than CRT_QMAX
+int crt_moduli_for(int64_t k, int t) {
+  const double need = 2.0 * t - 2.0 + ceil_log2(k < 1 ? 1 : k);
+  double bits = 0;
+  for (int q = 0; q < CRT_QMAX; ++q) {
+    bits += __builtin_log2((double)crt_host_moduli[q]);
+    if (bits >= need) return q + 1;
+  }
+  return 0;
+}
+
+static crt_layout crt_plan(int64_t m, int64_t n, int64_t k, int nq) {
+  crt_layout L;
+  L.Mp = (m + CRT_TILE - 1) / CRT_TILE * CRT_TILE;
+  L.Np = (n + CRT_TILE - 1) / CRT_TILE * CRT_TILE;
+  L.Kb = (k + 31) / 32;
+  L.Mb = L.Mp / 32;
+  L.Nb = L.Np / 32;
+  L.nq = nq;
+  auto al = [](size_t v) { return (v + 255) / 256 * 256; };
+  size_t o = OZ_HDR;
+  L.eA = o;
+  o = al(o + (size_t)L.Mp * 4);
+  L.fA = o;
+  o = al(o + (size_t)L.Mp * 4);
+  L.eB = o;
+  o = al(o + (size_t)L.Np * 4);
+  L.fB = o;
+  o = al(o + (size_t)L.Np * 4);
+  L.tab = o;
+  o = al(o + CRT_TAB_BYTES);
+  L.rA = o;
+  o = al(o + (size_t)nq * L.Mb * L.Kb * OZ_FRAG);
+  L.rB = o;
+  o = al(o + (size_t)nq * L.Nb * L.Kb * OZ_FRAG);
+  L.rC = o;
+  o = al(o + (size_t)nq * L.Mb * L.Nb * OZ_FRAG);
+  L.list = o;
+  o = al(o + (size_t)m * n * 8);
+  L.bt = o;
+  o = al(o + (size_t)n * k * 16);
+  L.total = o;
+  return L;
+}
+
+size_t gemm_crt_ws(int64_t m, int64_t n, int64_t k, int t) {
+  const int nq = crt_moduli_for(k, t);
+  return nq ? crt_plan(m, n, k, nq).total : 0;
+}
+
+// bits t for a probe value r (as ozaki_slices_for): the smallest t with 1 + (kc - 2) - t + 2 <=
+// r - 106; 0 when that needs more than CRT_TMAX bits or CRT_QMAX moduli (the slice method then)
+int ozaki_crt_bits_for(int64_t k, int r) {
+  const int kc = ceil_log2(k < 1 ? 1 : k);
+  int t = (r == 0x7fffffff) ? 136 : kc + 107 - r;
+  if (t < CRT_TMIN) t = CRT_TMIN;
+  if (t > CRT_TMAX || !crt_moduli_for(k, t)) return 0;
+  return t;
+}
+
+hipError_t launch_gemm_crt(int colmajor, int64_t m, int64_t n, int64_t k, int t, q128 alpha, const q128 *A,
+                           int64_t lda, const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc, void *wsv,
+                           hipStream_t st, hipEvent_t *ev, int sync_repair) {
+  if (t < CRT_TMIN || t > CRT_TMAX) return hipErrorInvalidValue;
+  const int nq = crt_moduli_for(k, t);
+  if (!nq) return hipErrorInvalidValue;
+  char *ws = (char *)wsv;
+  const crt_layout L = crt_plan(m, n, k, nq);
+  const int64_t asr = colmajor ? 1 : lda, asc = colmajor ? lda : 1;
+  const int64_t bsr = colmajor ? 1 : ldb, bsc = colmajor ? ldb : 1;
+  const int64_t csr = colmajor ? 1 : ldc, csc = colmajor ? ldc : 1;
+  int32_t *eA = (int32_t *)(ws + L.eA), *eB = (int32_t *)(ws + L.eB);
+  uint32_t *fA = (uint32_t *)(ws + L.fA), *fB = (uint32_t *)(ws + L.fB);
+  uint32_t *hdr = (uint32_t *)ws;
+  crt_tab *tab = (crt_tab *)(ws + L.tab);
+  int8_t *RA = (int8_t *)(ws + L.rA), *RB = (int8_t *)(ws + L.rB), *RC = (int8_t *)(ws + L.rC);
+  if (ev) (void)hipEventRecord(ev[0], st);
+  const int64_t ninit = (L.Mp > L.Np ? L.Mp : L.Np) > OZ_HDR / 4 ? (L.Mp > L.Np ? L.Mp : L.Np) : OZ_HDR / 4;
+  k_oz_init<<<(unsigned)((ninit + 255) / 256), 256, 0, st>>>(eA, fA, L.Mp, eB, fB, L.Np, hdr);
+  oz_scale(A, m, k, asr, asc, eA, fA, st);
+  oz_scale(B, n, k, bsc, bsr, eB, fB, st);
+  k_crt_tables<<<1, 64, 0, st>>>(nq, tab);
+  k_crt_residues<<<(unsigned)((L.Mb * L.Kb + 3) / 4), 256, 0, st>>>(A, m, k, asr, asc, eA, t, tab, L.Mb, L.Kb, RA);
+  k_crt_residues<<<(unsigned)((L.Nb * L.Kb + 3) / 4), 256, 0, st>>>(B, n, k, bsc, bsr, eB, t, tab, L.Nb, L.Kb, RB);
+  if (ev) (void)hipEventRecord(ev[1], st);
+  const int64_t wgs = (int64_t)nq * (L.Mb / CRT_TA) * (L.Nb / CRT_TB);
+  k_crt_gemm<<<(unsigned)wgs, 512, 0, st>>>(RA, RB, L.Mb, L.Nb, L.Kb, tab, RC);
+  if (ev) (void)hipEventRecord(ev[2], st);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_crt_finish<<<(unsigned)((L.Mb * L.Nb + 3) / 4), 256, 0, st>>>(m, n, k, t, ceil_log2(k < 1 ? 1 : k), L.Mb, L.Nb,
+                                                                   RC, tab, eA, fA, eB, fB, alpha, beta, C, csr, csc,
+                                                                   hdr, (uint32_t *)(ws + L.list));
+  if (ev) (void)hipEventRecord(ev[3], st);
+  return oz_repair_queue(n, k, A, asr, asc, B, bsr, bsc, eA, eB, alpha, beta, C, csr, csc, hdr,
+                         (const uint32_t *)(ws + L.list), (q128 *)(ws + L.bt), st, ev, sync_repair);
 }
 
 }  // namespace qbk

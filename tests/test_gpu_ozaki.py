@@ -1,4 +1,4 @@
-"""GPU parity of the OZAKI form of gemm (int8 slices on the matrix cores).
+"""GPU parity of the OZAKI form of gemm, slice method (int8 slices on the matrix cores).
 
 Contract: every entry of C = alpha*A*B + beta*C has relative error <= 2^-104 against the exact
 value (the oracle's exact big-integer accumulation of the same entry), and entries the bound
@@ -35,6 +35,7 @@ def _data(O, kind, seed, count):
 
 def _run(Q, cuda, layout, m, n, k, alpha, beta, A, lda, B, ldb, C, ldc, S):
     Q.set_ozaki_slices(S)
+    Q.set_ozaki_method(Q.OZAKI_SLICES)
     try:
         dA, dB, dC = to_dev(A), to_dev(B), to_dev(C)
         Q.dev_gemm(Q.MODE_OZAKI, layout, m, n, k, alpha, dA.data_ptr(), lda, dB.data_ptr(), ldb, beta, dC.data_ptr(),
@@ -44,6 +45,7 @@ def _run(Q, cuda, layout, m, n, k, alpha, beta, A, lda, B, ldb, C, ldc, S):
         repairs = Q.dev_ozaki_repairs()
     finally:
         Q.set_ozaki_slices(0)
+        Q.set_ozaki_method(Q.OZAKI_AUTO)
     return from_dev(dC), status, repairs
 
 
