@@ -653,6 +653,9 @@ QB_HD q128 finish_reg_t(const uint32_t (&M)[N], int lsb, uint32_t sS, uint32_t s
     return fma_packed(alpha, sq, mul_packed(beta, yv));
   }
   const uq ua = unpack_finite(alpha), ub = unpack_finite(beta), uy = unpack_finite(yv);
+  const bool z2 = ((ub.mh | ub.ml) == 0) || ((uy.mh | uy.ml) == 0);
+  // alpha = +-1 and beta * y = 0: the result is S rounded once (with alpha's sign)
+  if (z2 && ua.ml == 0 && ua.mh == HIDDEN && ua.e == 0 && !rzero<N>(M)) return rround<N>(M, lsb, sS ^ ua.s);
   // T1 = alpha * S  (N + 4 words), T2 = beta * y (8 words)
   uint32_t T1[N + 4];
   const bool z1 = rzero<N>(M) || ((ua.mh | ua.ml) == 0);
@@ -665,7 +668,6 @@ QB_HD q128 finish_reg_t(const uint32_t (&M)[N], int lsb, uint32_t sS, uint32_t s
   const int e1 = lsb + ua.e - 112;
   const uint32_t s1 = sS ^ ua.s;
   uint32_t P2[8];
-  const bool z2 = ((ub.mh | ub.ml) == 0) || ((uy.mh | uy.ml) == 0);
   if (!z2) mul113_w(P2, ub, uy);
   const int e2 = ub.e + uy.e - 224;
   const uint32_t s2 = ub.s ^ uy.s;

@@ -924,6 +924,9 @@ struct crt_tab {
   double pinvd[64];      // 1 / p
   int8_t cpow[64][32];   // [q][k] = 256^k mod p_q, balanced
   int8_t vt[64][64];     // [b][q] = balanced base-256 digit b of V_q
+  uint32_t mdiv[64];     // ceil(2^28 / p): floor(y / p) = (16 y * mdiv) >> 32 for 0 <= y < 2^20
+  int32_t half[64];      // (p - 1) / 2, 128 for p = 256
+  int32_t off[64];       // ceil(2^19 / p) p + half: x + off is in [0, 2^20) for the residue sums
   uint32_t M[CRT_MW];    // product of the moduli
   double xs[CRT_MW];     // 2^(32 (w - w0)) for words w >= w0, else 0 (w0: second-highest word of M)
   double mtop;           // sum_w M[w] xs[w]
@@ -955,6 +958,10 @@ __global__ void __launch_bounds__(64) k_crt_tables(int nq, crt_tab *__restrict__
   T->p[q] = p;
   T->pinv[q] = p ? 1.0f / (float)p : 0.0f;
   T->pinvd[q] = p ? 1.0 / (double)p : 0.0;
+  T->mdiv[q] = p ? (uint32_t)(((1u << 28) + (uint32_t)p - 1) / (uint32_t)p) : 0u;
+  const int32_t hf = p == 256 ? 128 : (p - 1) / 2;
+  T->half[q] = p ? hf : 0;
+  T->off[q] = p ? (((1 << 19) + p - 1) / p) * p + hf : 0;
   int32_t pw = 1 % (p ? p : 1);
   for (int k = 0; k < 32; ++k) {
     int v = p ? pw : 0;
@@ -1059,76 +1066,125 @@ __device__ __forceinline__ uint32_t crt_red_d(int x, int p, double ip) {
   return (uint32_t)(int)r & 0xffu;
 }
 
+// 16 residue sums y (in [0, 2^20), = x + off) of one modulus -> 16 balanced residues as bytes:
+// q = floor(y / p) by a 24-bit multiply-high, r = y - q p - half in [-128, 127]
+__device__ __forceinline__ i32x4v crt_pack16(const i32x16v &d, int p, uint32_t mdiv, int hf) {
+  int r[16];
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    const uint32_t y = (uint32_t)d[v];
+    const uint32_t q = (uint32_t)(((uint64_t)((y << 4) & 0xffffffu) * (mdiv & 0xffffffu)) >> 32);
+    r[v] = (int)y - (int)q * p - hf;
+  }
+  i32x4v w;
+  uint32_t o[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t t01 = __builtin_amdgcn_perm((uint32_t)r[4 * j + 1], (uint32_t)r[4 * j], 0x0c0c0400u);
+    const uint32_t t23 = __builtin_amdgcn_perm((uint32_t)r[4 * j + 3], (uint32_t)r[4 * j + 2], 0x0c0c0400u);
+    o[j] = __builtin_amdgcn_perm(t23, t01, 0x05040100u);
+  }
+  w.x = (int)o[0];
+  w.y = (int)o[1];
+  w.z = (int)o[2];
+  w.w = (int)o[3];
+  return w;
+}
+
 // Residues of the scaled integers of one operand: rows r < R (rows of A, or columns of B), k < K,
 // element (r, k) at X[r sr + k sc], row exponent E[r] (+3).  One wave per fragment position
 // (rb, kb) of the Rb x Kb grid; out[q][rb][kb][1 KB] for q < nq, lane L's 16 bytes = row L%32,
-// k 16(L/32)..+15.  MFMA row rho of group g is the element (slot 2g + ((rho>>2)&1), byte
-// 4(rho>>3) + (rho&3)) of the fragment; the lanes holding MFMA output rows 8(v/4) + 4h + v%4
-// then hold slot 2g + h, bytes v = 0..15.
+// k 16(L/32)..+15.  The fragment is done in two halves (k 0..15, then 16..31):
+// Phase 1: the half's 512 elements (8 per lane, loads coalesced along the contiguous dimension)
+// become balanced digits 0..19 in the wave's LDS region (element (r, c) at r*17 + c).
+// Phase 2: MFMA row rho of group g is the element (slot 2g + ((rho>>2)&1), byte 4(rho>>3) + (rho&3))
+// of the fragment, with its digits 16h..16h+15 as the A operand of lane rho + 32h; the B operand
+// holds 256^k mod p for the lane's modulus column, and the accumulator starts at off_q, so the lane
+// holding MFMA output rows 8(v/4) + 4h + v%4 holds slot 2g + h, bytes v = 0..15 as x + off.
+// Groups 0..15 use slots 0..31 (k < 16), groups 16..31 slots 32..63.
+constexpr int CRT_RS = 17;  // LDS row stride (elements) of the residue kernel
 __global__ void __launch_bounds__(256) k_crt_residues(const q128 *__restrict__ X, int64_t R, int64_t K, int64_t sr,
                                                       int64_t sc, const int32_t *__restrict__ E, int t,
                                                       const crt_tab *__restrict__ T, int64_t Rb, int64_t Kb,
                                                       int8_t *__restrict__ out) {
-  const int64_t pos = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (pos >= Rb * Kb) return;  // whole waves
-  const int lane = threadIdx.x & 63, n = lane & 31, h = lane >> 5;
+  __shared__ __attribute__((aligned(16))) uint32_t s_lo[4][32 * CRT_RS][4];
+  __shared__ uint32_t s_hi[4][32 * CRT_RS];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, n = lane & 31, h = lane >> 5;
+  const int64_t pos = (int64_t)blockIdx.x * 4 + wave;
+  if (pos >= Rb * Kb) return;  // whole waves; only wave-local LDS below
   const int64_t rb = pos / Kb, kb = pos - rb * Kb;
+  const bool kcontig = (sc == 1);
   const int nq = T->nq;
   const i32x4v bc0 = *reinterpret_cast<const i32x4v *>(&T->cpow[n][16 * h]);
   const i32x4v bc1 = *reinterpret_cast<const i32x4v *>(&T->cpow[32 + n][16 * h]);
   const int p0 = T->p[n], p1 = T->p[32 + n];
-  const float ip0 = T->pinv[n], ip1 = T->pinv[32 + n];
+  const uint32_t m0 = T->mdiv[n], m1 = T->mdiv[32 + n];
+  const int h0 = T->half[n], h1 = T->half[32 + n];
+  i32x16v init0, init1;
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    init0[v] = T->off[n];
+    init1[v] = T->off[32 + n];
+  }
   const int64_t plane = Rb * Kb * OZ_FRAG;
   int8_t *o0 = out + (int64_t)n * plane + pos * OZ_FRAG;
   int8_t *o1 = out + (int64_t)(32 + n) * plane + pos * OZ_FRAG;
   const int hh = (n >> 2) & 1, ib = 4 * (n >> 3) + (n & 3);
-  i32x16v zero;
+#pragma nounroll
+  for (int half = 0; half < 2; ++half) {
+    {
+      // element (r, c), c < 16 of this half: k contiguous -> lanes 16 apart read 4 rows of 16;
+      // rows contiguous -> lanes read 32 rows of 2 columns
+      q128 v[8];
+      int32_t ev[8];
 #pragma unroll
-  for (int v = 0; v < 16; ++v) zero[v] = 0;
-  for (int g = 0; g < 32; ++g) {
-    const int slot = 2 * g + hh;
-    const int64_t row = rb * 32 + (slot & 31), col = kb * 32 + 16 * (slot >> 5) + ib;
-    uint32_t W[6] = {0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u, 0u};
-    if (row < R && col < K) {
-      const int32_t er = E[row];
-      if (er != OZ_EZERO) crt_scaled(ldq_c(X + row * sr + col * sc), er + 3, t, W);
-    }
-    i32x4v a;
-    if (h == 0) {
-      a.x = (int)(W[0] ^ 0x80808080u);
-      a.y = (int)(W[1] ^ 0x80808080u);
-      a.z = (int)(W[2] ^ 0x80808080u);
-      a.w = (int)(W[3] ^ 0x80808080u);
-    } else {
-      a.x = (int)(W[4] ^ 0x80808080u);
-      a.y = 0;
-      a.z = 0;
-      a.w = 0;
-    }
-    const i32x16v d0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bc0, zero, 0, 0, 0);
-    const i32x16v d1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bc1, zero, 0, 0, 0);
-    if (n < nq) {
-      i32x4v w;
-      uint32_t u[4] = {0u, 0u, 0u, 0u};
+      for (int i = 0; i < 8; ++i) {
+        const int r = kcontig ? 4 * i + (lane >> 4) : n, c = kcontig ? (lane & 15) : 2 * i + h;
+        const int64_t row = rb * 32 + r, col = kb * 32 + 16 * half + c;
+        const bool in = row < R && col < K;
+        v[i] = in ? ldq_c(X + row * sr + col * sc) : q_zero(0);
+        ev[i] = in ? E[row] : OZ_EZERO;
+      }
 #pragma unroll
-      for (int v = 0; v < 16; ++v) u[v >> 2] |= crt_red_f(d0[v], p0, ip0) << (8 * (v & 3));
-      w.x = (int)u[0];
-      w.y = (int)u[1];
-      w.z = (int)u[2];
-      w.w = (int)u[3];
-      *reinterpret_cast<i32x4v *>(o0 + (2 * g + h) * 16) = w;
+      for (int i = 0; i < 8; ++i) {
+        const int r = kcontig ? 4 * i + (lane >> 4) : n, c = kcontig ? (lane & 15) : 2 * i + h;
+        uint32_t W[6] = {0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u, 0u};
+        if (ev[i] != OZ_EZERO) crt_scaled(v[i], ev[i] + 3, t, W);
+        const int e = r * CRT_RS + c;
+        i32x4v lo;
+        lo.x = (int)(W[0] ^ 0x80808080u);
+        lo.y = (int)(W[1] ^ 0x80808080u);
+        lo.z = (int)(W[2] ^ 0x80808080u);
+        lo.w = (int)(W[3] ^ 0x80808080u);
+        *reinterpret_cast<i32x4v *>(&s_lo[wave][e][0]) = lo;
+        s_hi[wave][e] = W[4] ^ 0x80808080u;
+      }
     }
-    if (32 + n < nq) {
-      i32x4v w;
-      uint32_t u[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-      for (int v = 0; v < 16; ++v) u[v >> 2] |= crt_red_f(d1[v], p1, ip1) << (8 * (v & 3));
-      w.x = (int)u[0];
-      w.y = (int)u[1];
-      w.z = (int)u[2];
-      w.w = (int)u[3];
-      *reinterpret_cast<i32x4v *>(o1 + (2 * g + h) * 16) = w;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int gg = 0; gg < 16; ++gg) {
+      const int g = 16 * half + gg;
+      const int slot = 2 * g + hh;
+      const int e = (slot & 31) * CRT_RS + ib;
+      i32x4v a;
+      if (h == 0) {
+        a = *reinterpret_cast<const i32x4v *>(&s_lo[wave][e][0]);
+      } else {
+        a.x = (int)s_hi[wave][e];
+        a.y = 0;
+        a.z = 0;
+        a.w = 0;
+      }
+      const i32x16v d0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bc0, init0, 0, 0, 0);
+      const i32x16v d1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bc1, init1, 0, 0, 0);
+      if (n < nq) *reinterpret_cast<i32x4v *>(o0 + (2 * g + h) * 16) = crt_pack16(d0, p0, m0, h0);
+      if (32 + n < nq) *reinterpret_cast<i32x4v *>(o1 + (2 * g + h) * 16) = crt_pack16(d1, p1, m1, h1);
     }
+    // the next half overwrites the region: every lane's reads of this half are done
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 
@@ -1224,10 +1280,18 @@ __global__ void __launch_bounds__(512) k_crt_gemm(const int8_t *__restrict__ RA,
     }
 }
 
-// bytes e of the four words P[0..3] -> one word (byte j from P[j])
-__device__ __forceinline__ uint32_t crt_gather_byte(const uint32_t (&P)[16], int v, int e) {
-  return ((P[4 * v] >> (8 * e)) & 0xffu) | (((P[4 * v + 1] >> (8 * e)) & 0xffu) << 8) |
-         (((P[4 * v + 2] >> (8 * e)) & 0xffu) << 16) | (((P[4 * v + 3] >> (8 * e)) & 0xffu) << 24);
+// 4 x 4 byte transpose: byte j of o_e = byte e of i_j (v_perm_b32 selects bytes 0-3 from its
+// second source and 4-7 from its first)
+__device__ __forceinline__ void crt_t4(uint32_t i0, uint32_t i1, uint32_t i2, uint32_t i3, uint32_t &o0, uint32_t &o1,
+                                       uint32_t &o2, uint32_t &o3) {
+  const uint32_t a = __builtin_amdgcn_perm(i1, i0, 0x05010400u);  // i0.0 i1.0 i0.1 i1.1
+  const uint32_t b = __builtin_amdgcn_perm(i1, i0, 0x07030602u);  // i0.2 i1.2 i0.3 i1.3
+  const uint32_t c = __builtin_amdgcn_perm(i3, i2, 0x05010400u);  // i2.0 i3.0 i2.1 i3.1
+  const uint32_t d = __builtin_amdgcn_perm(i3, i2, 0x07030602u);  // i2.2 i3.2 i2.3 i3.3
+  o0 = __builtin_amdgcn_perm(c, a, 0x05040100u);
+  o1 = __builtin_amdgcn_perm(c, a, 0x07060302u);
+  o2 = __builtin_amdgcn_perm(d, b, 0x05040100u);
+  o3 = __builtin_amdgcn_perm(d, b, 0x07060302u);
 }
 
 // Reconstruction and finish for one 32 x 32 block of C per wave.  The D^T = Vt R product has the
@@ -1237,13 +1301,19 @@ __device__ __forceinline__ uint32_t crt_gather_byte(const uint32_t (&P)[16], int
 // s_b = sum_q R_q v_qb for b = 8(v/4) + 4 kh + v%4 (+ 32 for the second row block): 4-byte runs
 // m = 2(v/4) + kh (+ 8).  Lanes L and L + 32 swap half their runs so that lane kh = 0 holds all
 // runs of entry e and lane kh = 1 all runs of entry e + 1.
+// XW: words of X = sum_l N_il N'_lj as two's complement (|X| < K 2^(2t-4) < 2^(32 XW - 1)).
+// FAST (alpha = +-1, beta = 0): the entry is X 2^(E_i+F_j-2t) rounded once, certified and stored
+// here.  Otherwise X is written to xrec (RW words per entry, row-major) for k_crt_finish_ab.
+template <int XW, bool FAST>
 __global__ void __launch_bounds__(256) k_crt_finish(int64_t M, int64_t N, int64_t K, int t, int kc, int64_t Mb,
                                                     int64_t Nb, const int8_t *__restrict__ RC,
                                                     const crt_tab *__restrict__ T, const int32_t *__restrict__ eA,
                                                     const uint32_t *__restrict__ fA, const int32_t *__restrict__ eB,
                                                     const uint32_t *__restrict__ fB, q128 alpha, q128 beta,
                                                     q128 *__restrict__ C, int64_t csr, int64_t csc,
-                                                    uint32_t *__restrict__ hdr, uint32_t *__restrict__ list) {
+                                                    uint32_t *__restrict__ hdr, uint32_t *__restrict__ list,
+                                                    uint32_t *__restrict__ xrec) {
+  constexpr int RW = (XW + 3) & ~3;
   const int64_t blk = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (blk >= Mb * Nb) return;  // whole waves
   const int lane = threadIdx.x & 63, n = lane & 31, kh = lane >> 5;
@@ -1263,14 +1333,14 @@ __global__ void __launch_bounds__(256) k_crt_finish(int64_t M, int64_t N, int64_
     xsc[w] = T->xs[w];
   }
   const double mtop = T->mtop;
-  const bool ab_ok = (q_bexp(alpha) != 0x7fffu) && (q_bexp(beta) != 0x7fffu);
-  const bool alpha_zero = q_iszero(alpha);
-  const int ea = ab_ok && !alpha_zero ? unpack_finite(alpha).e : 0;
+  const uint32_t alpha_sgn = (uint32_t)(alpha.hi >> 63);
   const int64_t col = bn * 32 + n;
   i32x16v zero;
 #pragma unroll
   for (int v = 0; v < 16; ++v) zero[v] = 0;
+#pragma nounroll
   for (int hp = 0; hp < 2; ++hp) {
+#pragma nounroll
     for (int rq = 0; rq < 4; ++rq) {
       // residues of the four entries (rows 8 rq + 4 hp + e), moduli 32 kb + 16 kh + j
       uint32_t Eo[2][4][4];
@@ -1284,26 +1354,34 @@ __global__ void __launch_bounds__(256) k_crt_finish(int64_t M, int64_t N, int64_
                           : 0u;
         }
 #pragma unroll
-        for (int e = 0; e < 4; ++e)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) Eo[kb][e][v] = crt_gather_byte(P, v, e);
+        for (int v = 0; v < 4; ++v)
+          crt_t4(P[4 * v], P[4 * v + 1], P[4 * v + 2], P[4 * v + 3], Eo[kb][0][v], Eo[kb][1][v], Eo[kb][2][v],
+                 Eo[kb][3][v]);
       }
-#pragma unroll
+#pragma nounroll
       for (int e = 0; e < 4; e += 2) {
+        // this pair's residues, selected without indexing the register array by e
+        uint32_t Ec[2][2][4];
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+          for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Ec[kb][x][v] = e ? Eo[kb][2 + x][v] : Eo[kb][x][v];
         i32x16v D[2][2];  // [entry e / e + 1][row block]
 #pragma unroll
         for (int x = 0; x < 2; ++x)
 #pragma unroll
           for (int mb = 0; mb < 2; ++mb) {
             i32x4v b0, b1;
-            b0.x = (int)Eo[0][e + x][0];
-            b0.y = (int)Eo[0][e + x][1];
-            b0.z = (int)Eo[0][e + x][2];
-            b0.w = (int)Eo[0][e + x][3];
-            b1.x = (int)Eo[1][e + x][0];
-            b1.y = (int)Eo[1][e + x][1];
-            b1.z = (int)Eo[1][e + x][2];
-            b1.w = (int)Eo[1][e + x][3];
+            b0.x = (int)Ec[0][x][0];
+            b0.y = (int)Ec[0][x][1];
+            b0.z = (int)Ec[0][x][2];
+            b0.w = (int)Ec[0][x][3];
+            b1.x = (int)Ec[1][x][0];
+            b1.y = (int)Ec[1][x][1];
+            b1.z = (int)Ec[1][x][2];
+            b1.w = (int)Ec[1][x][3];
             D[x][mb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(aV[mb][0], b0, zero, 0, 0, 0);
             D[x][mb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(aV[mb][1], b1, D[x][mb], 0, 0, 0);
           }
@@ -1362,37 +1440,45 @@ __global__ void __launch_bounds__(256) k_crt_finish(int64_t M, int64_t N, int64_
           X[w] = (uint32_t)s2;
           bc = s2 >> 32;
         }
-        const uint32_t sgn = X[CRT_MW - 1] >> 31;
-        if (sgn) {
-          uint32_t cc = 1;
-#pragma unroll
-          for (int w = 0; w < CRT_MW; ++w) X[w] = addc32(~X[w], 0u, cc, cc);
-        }
-        // the entry
         const int64_t row = bm * 32 + 8 * rq + 4 * hp + e + kh;
         if (row < M && col < N) {
-          q128 *cp = C + row * csr + col * csc;
-          const q128 yv = ldq_c(cp);
-          const bool fin = !((fA[row] | fB[col]) & OZF_NONFINITE) && ab_ok && q_bexp(yv) != 0x7fffu;
-          bool done = false;
-          q128 res;
-          if (fin && alpha_zero) {
-            res = mul_packed(beta, yv);
-            done = !q_iszero(res);
-          } else if (fin && eA[row] != OZ_EZERO && eB[col] != OZ_EZERO && !rzero<CRT_MW>(X)) {
-            const int Ei = eA[row] + 3, Fj = eB[col] + 3;
-            res = finish_reg_t<CRT_MW>(X, Ei + Fj - 2 * t, sgn, 0u, 0u, true, alpha, beta, yv);
-            if (q_finite_nonzero(res)) {
-              const int bexp = ea + 1 + kc + Ei + Fj - t - 2;  // |alpha (AB - P)| < 2^bexp
-              done = bexp <= unpack_finite(res).e - 106;
+          if (!FAST) {
+            uint32_t *xr = xrec + (row * N + col) * RW;
+#pragma unroll
+            for (int w = 0; w < RW; w += 4) {
+              i32x4v o;
+              o.x = (int)(w < XW ? X[w] : 0u);
+              o.y = (int)(w + 1 < XW ? X[w + 1] : 0u);
+              o.z = (int)(w + 2 < XW ? X[w + 2] : 0u);
+              o.w = (int)(w + 3 < XW ? X[w + 3] : 0u);
+              *reinterpret_cast<i32x4v *>(xr + w) = o;
             }
-          }
-          if (done) {
-            stq_c(cp, res);
           } else {
-            const uint32_t slot = atomicAdd(hdr + 1, 1u);
-            list[2 * (int64_t)slot] = (uint32_t)row;
-            list[2 * (int64_t)slot + 1] = (uint32_t)col;
+            const uint32_t sgn = X[CRT_MW - 1] >> 31;
+            uint32_t mag[XW];
+            uint32_t cc = sgn;
+#pragma unroll
+            for (int w = 0; w < XW; ++w) mag[w] = addc32(sgn ? ~X[w] : X[w], 0u, cc, cc);
+            q128 *cp = C + row * csr + col * csc;
+            bool done = false;
+            q128 res;
+            // beta = 0: beta * y is a zero unless y is Inf or NaN (then the exact repair decides)
+            if (!((fA[row] | fB[col]) & OZF_NONFINITE) && q_bexp(ldq_c(cp)) != 0x7fffu && eA[row] != OZ_EZERO &&
+                eB[col] != OZ_EZERO && !rzero<XW>(mag)) {
+              const int Ei = eA[row] + 3, Fj = eB[col] + 3;
+              res = rround<XW>(mag, Ei + Fj - 2 * t, sgn ^ alpha_sgn);
+              if (q_finite_nonzero(res)) {
+                const int bexp = 1 + kc + Ei + Fj - t - 2;  // |alpha (AB - P)| < 2^bexp, |alpha| = 1
+                done = bexp <= unpack_finite(res).e - 106;
+              }
+            }
+            if (done) {
+              stq_c(cp, res);
+            } else {
+              const uint32_t slot = atomicAdd(hdr + 1, 1u);
+              list[2 * (int64_t)slot] = (uint32_t)row;
+              list[2 * (int64_t)slot + 1] = (uint32_t)col;
+            }
           }
         }
       }
@@ -1400,10 +1486,68 @@ __global__ void __launch_bounds__(256) k_crt_finish(int64_t M, int64_t N, int64_
   }
 }
 
+// General alpha, beta: entry (i, j) = alpha X 2^(E_i+F_j-2t) + beta C rounded once from the record
+// of X (two's complement, XW words), certified as in k_oz_finish, else queued for the repair.
+template <int XW>
+__global__ void __launch_bounds__(256) k_crt_finish_ab(int64_t M, int64_t N, int t, int kc,
+                                                       const uint32_t *__restrict__ xrec,
+                                                       const int32_t *__restrict__ eA, const uint32_t *__restrict__ fA,
+                                                       const int32_t *__restrict__ eB, const uint32_t *__restrict__ fB,
+                                                       q128 alpha, q128 beta, q128 *__restrict__ C, int64_t csr,
+                                                       int64_t csc, uint32_t *__restrict__ hdr,
+                                                       uint32_t *__restrict__ list) {
+  constexpr int RW = (XW + 3) & ~3;
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= M * N) return;
+  const int64_t i = g / N, j = g - i * N;
+  q128 *cp = C + i * csr + j * csc;
+  const q128 yv = ldq_c(cp);
+  const bool ab_ok = (q_bexp(alpha) != 0x7fffu) && (q_bexp(beta) != 0x7fffu) && (q_bexp(yv) != 0x7fffu);
+  bool done = false;
+  q128 res;
+  const bool fin = !((fA[i] | fB[j]) & OZF_NONFINITE);
+  if (ab_ok && fin && q_iszero(alpha)) {
+    res = mul_packed(beta, yv);
+    done = !q_iszero(res);
+  } else if (ab_ok && fin && eA[i] != OZ_EZERO && eB[j] != OZ_EZERO) {
+    uint32_t X[XW];
+    const uint32_t *xr = xrec + g * RW;
+#pragma unroll
+    for (int w = 0; w < RW; w += 4) {
+      const i32x4v v = *reinterpret_cast<const i32x4v *>(xr + w);
+      if (w < XW) X[w] = (uint32_t)v.x;
+      if (w + 1 < XW) X[w + 1] = (uint32_t)v.y;
+      if (w + 2 < XW) X[w + 2] = (uint32_t)v.z;
+      if (w + 3 < XW) X[w + 3] = (uint32_t)v.w;
+    }
+    const uint32_t sgn = X[XW - 1] >> 31;
+    if (sgn) {
+      uint32_t c = 1;
+#pragma unroll
+      for (int w = 0; w < XW; ++w) X[w] = addc32(~X[w], 0u, c, c);
+    }
+    if (!rzero<XW>(X)) {
+      const int Ei = eA[i] + 3, Fj = eB[j] + 3;
+      res = finish_reg_t<XW>(X, Ei + Fj - 2 * t, sgn, 0u, 0u, true, alpha, beta, yv);
+      if (q_finite_nonzero(res)) {
+        const int bexp = unpack_finite(alpha).e + 1 + kc + Ei + Fj - t - 2;
+        done = bexp <= unpack_finite(res).e - 106;
+      }
+    }
+  }
+  if (done) {
+    stq_c(cp, res);
+  } else {
+    const uint32_t slot = atomicAdd(hdr + 1, 1u);
+    list[2 * (int64_t)slot] = (uint32_t)i;
+    list[2 * (int64_t)slot + 1] = (uint32_t)j;
+  }
+}
+
 struct crt_layout {
   int64_t Mp, Np, Kb, Mb, Nb;
   int nq;
-  size_t eA, fA, eB, fB, tab, rA, rB, rC, list, bt, total;
+  size_t eA, fA, eB, fB, tab, rA, rB, rC, xrec, list, bt, total;
 };
 
 static int ceil_log2(int64_t k) {
@@ -1449,6 +1593,8 @@ static crt_layout crt_plan(int64_t m, int64_t n, int64_t k, int nq) {
   o = al(o + (size_t)nq * L.Nb * L.Kb * OZ_FRAG);
   L.rC = o;
   o = al(o + (size_t)nq * L.Mb * L.Nb * OZ_FRAG);
+  L.xrec = o;  // records of X for general alpha, beta (12 words per entry at most)
+  o = al(o + (size_t)m * n * 48);
   L.list = o;
   o = al(o + (size_t)m * n * 8);
   L.bt = o;
@@ -1463,12 +1609,15 @@ size_t gemm_crt_ws(int64_t m, int64_t n, int64_t k, int t) {
 }
 
 // bits t for a probe value r (as ozaki_slices_for): the smallest t with 1 + (kc - 2) - t + 2 <=
-// r - 106; 0 when that needs more than CRT_TMAX bits or CRT_QMAX moduli (the slice method then)
+// r - 106, raised to the largest t the same moduli carry (precision at no cost: fewer repairs);
+// 0 when that needs more than CRT_TMAX bits or CRT_QMAX moduli (the slice method then)
 int ozaki_crt_bits_for(int64_t k, int r) {
   const int kc = ceil_log2(k < 1 ? 1 : k);
   int t = (r == 0x7fffffff) ? 136 : kc + 107 - r;
   if (t < CRT_TMIN) t = CRT_TMIN;
-  if (t > CRT_TMAX || !crt_moduli_for(k, t)) return 0;
+  const int nq = t <= CRT_TMAX ? crt_moduli_for(k, t) : 0;
+  if (!nq) return 0;
+  while (t < CRT_TMAX && crt_moduli_for(k, t + 1) == nq) ++t;
   return t;
 }
 
@@ -1502,9 +1651,30 @@ hipError_t launch_gemm_crt(int colmajor, int64_t m, int64_t n, int64_t k, int t,
   if (ev) (void)hipEventRecord(ev[2], st);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  k_crt_finish<<<(unsigned)((L.Mb * L.Nb + 3) / 4), 256, 0, st>>>(m, n, k, t, ceil_log2(k < 1 ? 1 : k), L.Mb, L.Nb,
-                                                                   RC, tab, eA, fA, eB, fB, alpha, beta, C, csr, csc,
-                                                                   hdr, (uint32_t *)(ws + L.list));
+  const int kc = ceil_log2(k < 1 ? 1 : k);
+  const int xbits = kc + 2 * t - 3;  // |X| < K 2^(2t-4) < 2^xbits; two's complement needs xbits + 1
+  const unsigned fgrid = (unsigned)((L.Mb * L.Nb + 3) / 4);
+  uint32_t *list = (uint32_t *)(ws + L.list), *xrec = (uint32_t *)(ws + L.xrec);
+  const bool fast = q_iszero(beta) && alpha.lo == 0 && (alpha.hi << 1) == (0x3fffull << 49);
+  const int64_t mn = m * n;
+#define QB_CRT_FIN(XW)                                                                                              \
+  if (fast) {                                                                                                     \
+    k_crt_finish<XW, true><<<fgrid, 256, 0, st>>>(m, n, k, t, kc, L.Mb, L.Nb, RC, tab, eA, fA, eB, fB, alpha, beta, \
+                                                  C, csr, csc, hdr, list, xrec);                                  \
+  } else {                                                                                                        \
+    k_crt_finish<XW, false><<<fgrid, 256, 0, st>>>(m, n, k, t, kc, L.Mb, L.Nb, RC, tab, eA, fA, eB, fB, alpha,     \
+                                                   beta, C, csr, csc, hdr, list, xrec);                           \
+    k_crt_finish_ab<XW><<<(unsigned)((mn + 255) / 256), 256, 0, st>>>(m, n, t, kc, xrec, eA, fA, eB, fB, alpha,    \
+                                                                      beta, C, csr, csc, hdr, list);              \
+  }
+  if (xbits < 192) {
+    QB_CRT_FIN(6)
+  } else if (xbits < 288) {
+    QB_CRT_FIN(9)
+  } else {
+    QB_CRT_FIN(CRT_MW)
+  }
+#undef QB_CRT_FIN
   if (ev) (void)hipEventRecord(ev[3], st);
   return oz_repair_queue(n, k, A, asr, asc, B, bsr, bsc, eA, eB, alpha, beta, C, csr, csc, hdr,
                          (const uint32_t *)(ws + L.list), (q128 *)(ws + L.bt), st, ev, sync_repair);
