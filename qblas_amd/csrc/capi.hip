@@ -291,18 +291,20 @@ int dev_gemv(int mode, int layout, int64_t m, int64_t n, q128 alpha, const void 
 // the precision chosen from it certifies all but about 1% of the entries, the rest being left to
 // the exact repair), INT_MAX when no sample is usable.  The probe synchronizes with the stream.
 thread_local Workspace t_pws;
+// The probe's workspace stays acquired (its scaling exponents serve the call); the caller releases
+// it after enqueuing the call.
 int oz_probe(int colmajor, int64_t m, int64_t n, int64_t k, const void *A, int64_t lda, const void *B, int64_t ldb,
-             hipStream_t s, int *out) {
+             hipStream_t s, int *out, void **pws) {
   void *ws;
   int rc = ws_acquire(t_pws, qbk::ozaki_probe_ws(m, n), s, &ws);
   if (rc) return rc;
+  *pws = ws;
   constexpr int P = 256;
   QB_CHECK(qbk::launch_ozaki_probe(colmajor, m, n, k, (const q128 *)A, lda, (const q128 *)B, ldb, P, ws, s));
   int r[P];
   QB_CHECK(hipMemcpyAsync(r, (const char *)ws + qbk::ozaki_probe_out_offset(m, n), sizeof(r), hipMemcpyDeviceToHost,
                           s));
   QB_CHECK(hipStreamSynchronize(s));
-  if ((rc = ws_release(t_pws, s))) return rc;
   std::sort(r, r + P);
   *out = r[2];
   return QBLAS_OK;
@@ -315,11 +317,13 @@ int oz_probe(int colmajor, int64_t m, int64_t n, int64_t k, const void *A, int64
 // method's (slice pairs x 64-tiles).  sync is set when the probe ran (the call synchronizes anyway).
 struct oz_choice {
   int method, S, t, sync;
+  void *probe_ws;  // the probe's workspace (held until the call is enqueued), or null
 };
 int oz_choose(int colmajor, int64_t m, int64_t n, int64_t k, const void *A, int64_t lda, const void *B, int64_t ldb,
               hipStream_t s, oz_choice *c) {
   const int method = g_oz_method.load(), fixedS = g_oz_slices.load(), fixedT = g_oz_bits.load();
   c->sync = 0;
+  c->probe_ws = nullptr;
   c->S = fixedS;
   c->t = fixedT;
   if (method == QBLAS_OZAKI_SLICES || (method == QBLAS_OZAKI_AUTO && fixedS)) {
@@ -332,7 +336,7 @@ int oz_choose(int colmajor, int64_t m, int64_t n, int64_t k, const void *A, int6
     c->method = method;
   }
   int r = 0;
-  int rc = oz_probe(colmajor, m, n, k, A, lda, B, ldb, s, &r);
+  int rc = oz_probe(colmajor, m, n, k, A, lda, B, ldb, s, &r, &c->probe_ws);
   if (rc) return rc;
   c->sync = 1;
   c->S = qbk::ozaki_slices_for(k, r);
@@ -367,6 +371,14 @@ int dev_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, q128 alpha, 
     oz_choice ch;
     int rc0 = oz_choose(colmajor, m, n, k, A, lda, B, ldb, s, &ch);
     if (rc0) return rc0;
+    // release the probe's workspace once the call's work is enqueued (its exponents are read there)
+    struct probe_hold {
+      void *ws;
+      hipStream_t s;
+      ~probe_hold() {
+        if (ws) (void)ws_release(t_pws, s);
+      }
+    } hold{ch.probe_ws, s};
     const bool crt = ch.method == QBLAS_OZAKI_CRT;
     const size_t need = crt ? qbk::gemm_crt_ws(m, n, k, ch.t) : qbk::gemm_ozaki_ws(m, n, k, ch.S);
     if (!need) {
@@ -390,10 +402,10 @@ int dev_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, q128 alpha, 
     // a call that ran the probe already synchronized, so the repair may too
     if (crt)
       QB_CHECK(qbk::launch_gemm_crt(colmajor, m, n, k, ch.t, alpha, (const q128 *)A, lda, (const q128 *)B, ldb, beta,
-                                    (q128 *)C, ldc, ws, s, ev, ch.sync));
+                                    (q128 *)C, ldc, ws, s, ev, ch.sync, ch.probe_ws));
     else
       QB_CHECK(qbk::launch_gemm_ozaki(colmajor, m, n, k, ch.S, alpha, (const q128 *)A, lda, (const q128 *)B, ldb,
-                                      beta, (q128 *)C, ldc, ws, s, ev, ch.sync));
+                                      beta, (q128 *)C, ldc, ws, s, ev, ch.sync, ch.probe_ws));
     t_exact_status = (const uint32_t *)ws;
     t_oz_hdr = (const uint32_t *)ws;
     return ws_release(t_ws, s);

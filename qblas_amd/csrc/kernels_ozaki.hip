@@ -708,10 +708,11 @@ static void oz_slice_operand(const q128 *X, int64_t R, int64_t C, int64_t sr, in
 
 template <int S>
 static void oz_launch_slices(const oz_layout &L, const q128 *A, int64_t m, int64_t k, int64_t asr, int64_t asc,
-                             const q128 *B, int64_t n, int64_t bsr, int64_t bsc, char *ws, hipStream_t st) {
-  oz_slice_operand<S>(A, m, k, asr, asc, L.Mp, (const int32_t *)(ws + L.eA), (int8_t *)(ws + L.sA), L.Kb, st);
+                             const q128 *B, int64_t n, int64_t bsr, int64_t bsc, const int32_t *eA,
+                             const int32_t *eB, char *ws, hipStream_t st) {
+  oz_slice_operand<S>(A, m, k, asr, asc, L.Mp, eA, (int8_t *)(ws + L.sA), L.Kb, st);
   // B as rows j, columns l
-  oz_slice_operand<S>(B, n, k, bsc, bsr, L.Np, (const int32_t *)(ws + L.eB), (int8_t *)(ws + L.sB), L.Kb, st);
+  oz_slice_operand<S>(B, n, k, bsc, bsr, L.Np, eB, (int8_t *)(ws + L.sB), L.Kb, st);
 }
 
 template <int S>
@@ -756,7 +757,7 @@ static hipError_t oz_repair_queue(int64_t n, int64_t k, const q128 *A, int64_t a
     hipError_t e2 = hipMemcpyAsync(&count, hdr + 1, 4, hipMemcpyDeviceToHost, st);
     if (e2 == hipSuccess) e2 = hipStreamSynchronize(st);
     if (e2 != hipSuccess) return e2;
-    if (count >= 512) {
+    if (count >= 2048) {  // the transpose (one pass over B) pays off against ~80 ns per gathered column
       dim3 tg((unsigned)((n + 31) / 32), (unsigned)((k + 31) / 32));
       k_oz_transpose<<<tg, 256, 0, st>>>(B, k, n, bsr, Bt);
       Br = Bt;
@@ -770,9 +771,33 @@ static hipError_t oz_repair_queue(int64_t n, int64_t k, const q128 *A, int64_t a
   return hipGetLastError();
 }
 
+void ozaki_probe_scales(const void *wsv, int64_t m, int64_t n, const int32_t **eA, const uint32_t **fA,
+                        const int32_t **eB, const uint32_t **fB);
+// Header and scaling exponents of a call: the probe's when it ran on the same operands (probe_ws),
+// else computed here into the call's own arrays (*eA.. point there on entry).
+static void oz_prepare(int64_t Mp, int64_t Np, int64_t m, int64_t n, int64_t k, const q128 *A, int64_t asr,
+                       int64_t asc, const q128 *B, int64_t bsr, int64_t bsc, const void *probe_ws, int32_t **eA,
+                       uint32_t **fA, int32_t **eB, uint32_t **fB, uint32_t *hdr, hipStream_t st) {
+  if (probe_ws) {
+    const int32_t *pa, *pb;
+    const uint32_t *qa, *qb;
+    ozaki_probe_scales(probe_ws, m, n, &pa, &qa, &pb, &qb);
+    *eA = const_cast<int32_t *>(pa);
+    *fA = const_cast<uint32_t *>(qa);
+    *eB = const_cast<int32_t *>(pb);
+    *fB = const_cast<uint32_t *>(qb);
+    k_oz_init<<<1, 256, 0, st>>>(*eA, *fA, 0, *eB, *fB, 0, hdr);  // header only
+    return;
+  }
+  const int64_t ninit = (Mp > Np ? Mp : Np) > OZ_HDR / 4 ? (Mp > Np ? Mp : Np) : OZ_HDR / 4;
+  k_oz_init<<<(unsigned)((ninit + 255) / 256), 256, 0, st>>>(*eA, *fA, Mp, *eB, *fB, Np, hdr);
+  oz_scale(A, m, k, asr, asc, *eA, *fA, st);
+  oz_scale(B, n, k, bsc, bsr, *eB, *fB, st);  // B columns: r = j, c = l
+}
+
 hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int S, q128 alpha, const q128 *A,
                              int64_t lda, const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc, void *wsv,
-                             hipStream_t st, hipEvent_t *ev, int sync_repair) {
+                             hipStream_t st, hipEvent_t *ev, int sync_repair, const void *probe_ws) {
   if (S < OZ_SMIN || S > OZ_SMAX) return hipErrorInvalidValue;
   char *ws = (char *)wsv;
   const oz_layout L = oz_plan(m, n, k, S);
@@ -784,13 +809,10 @@ hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int 
   uint32_t *fA = (uint32_t *)(ws + L.fA), *fB = (uint32_t *)(ws + L.fB);
   uint32_t *hdr = (uint32_t *)ws;
   if (ev) (void)hipEventRecord(ev[0], st);
-  const int64_t ninit = (L.Mp > L.Np ? L.Mp : L.Np) > OZ_HDR / 4 ? (L.Mp > L.Np ? L.Mp : L.Np) : OZ_HDR / 4;
-  k_oz_init<<<(unsigned)((ninit + 255) / 256), 256, 0, st>>>(eA, fA, L.Mp, eB, fB, L.Np, hdr);
-  oz_scale(A, m, k, asr, asc, eA, fA, st);
-  oz_scale(B, n, k, bsc, bsr, eB, fB, st);  // B columns: r = j, c = l
+  oz_prepare(L.Mp, L.Np, m, n, k, A, asr, asc, B, bsr, bsc, probe_ws, &eA, &fA, &eB, &fB, hdr, st);
   switch (S) {
 #define QB_OZ_CASE(s) \
-  case s: oz_launch_slices<s>(L, A, m, k, asr, asc, B, n, bsr, bsc, ws, st); break;
+  case s: oz_launch_slices<s>(L, A, m, k, asr, asc, B, n, bsr, bsc, eA, eB, ws, st); break;
     QB_OZ_CASE(8) QB_OZ_CASE(9) QB_OZ_CASE(10) QB_OZ_CASE(11) QB_OZ_CASE(12) QB_OZ_CASE(13) QB_OZ_CASE(14)
     QB_OZ_CASE(15) QB_OZ_CASE(16) QB_OZ_CASE(17) QB_OZ_CASE(18) QB_OZ_CASE(19) QB_OZ_CASE(20) QB_OZ_CASE(21)
     QB_OZ_CASE(22) QB_OZ_CASE(23) QB_OZ_CASE(24) QB_OZ_CASE(25) QB_OZ_CASE(26) QB_OZ_CASE(27) QB_OZ_CASE(28)
@@ -849,6 +871,17 @@ hipError_t launch_ozaki_probe(int colmajor, int64_t m, int64_t n, int64_t k, con
   k_oz_probe<<<(unsigned)(samples < 256 ? samples : 256), OZR_BS, 0, st>>>(
       m, n, k, A, asr, asc, B, bsr, bsc, eA, eB, samples, (int *)(ws + ozaki_probe_out_offset(m, n)));
   return hipGetLastError();
+}
+
+// The probe workspace's scaling exponents (m rows of A, n columns of B), valid until its next use.
+void ozaki_probe_scales(const void *wsv, int64_t m, int64_t n, const int32_t **eA, const uint32_t **fA,
+                        const int32_t **eB, const uint32_t **fB) {
+  const char *ws = (const char *)wsv;
+  auto al = [](size_t v) { return (v + 255) / 256 * 256; };
+  *eA = (const int32_t *)(ws + OZ_HDR);
+  *fA = (const uint32_t *)(ws + al(OZ_HDR + (size_t)m * 4));
+  *eB = (const int32_t *)(ws + al(OZ_HDR + (size_t)m * 4) + al((size_t)m * 4));
+  *fB = (const uint32_t *)((const char *)*eB + al((size_t)n * 4));
 }
 
 // Slice count for a probe value r (sampled entries at or above it lie >= 2^r below
@@ -962,33 +995,46 @@ __global__ void __launch_bounds__(64) k_crt_tables(int nq, crt_tab *__restrict__
   const int32_t hf = p == 256 ? 128 : (p - 1) / 2;
   T->half[q] = p ? hf : 0;
   T->off[q] = p ? (((1 << 19) + p - 1) / p) * p + hf : 0;
-  int32_t pw = 1 % (p ? p : 1);
+  const uint32_t up = p ? (uint32_t)p : 1u;
+  uint32_t pw = 1u % up;
   for (int k = 0; k < 32; ++k) {
-    int v = p ? pw : 0;
+    int v = (p && k < CRT_ND) ? (int)pw : 0;
     if (p && v >= (p + 1) / 2) v -= p;  // balanced: [-p/2, p/2) (p = 256: 256^0 = 1, else 0)
     T->cpow[q][k] = (int8_t)v;
-    if (p) pw = (int32_t)(((int64_t)pw * 256) % p);
+    pw = (pw * 256u) % up;
   }
   uint32_t V[CRT_MW];
 #pragma unroll
   for (int w = 0; w < CRT_MW; ++w) V[w] = 0;
   if (p) {
-    // Mq = M / p (exact), a = Mq mod p, V = Mq * a^-1 mod p (< M)
+    // Mq = M / p (exact), a = Mq mod p, V = Mq * a^-1 mod p (< M); 32-bit division on 16-bit
+    // limbs (remainders < 256 keep every dividend below 2^24)
     uint32_t Mq[CRT_MW];
-    uint64_t rem = 0;
+    uint32_t rem = 0;
     for (int w = CRT_MW - 1; w >= 0; --w) {
-      const uint64_t cur = (rem << 32) | sM[w];
-      Mq[w] = (uint32_t)(cur / (uint64_t)p);
-      rem = cur % (uint64_t)p;
+      const uint32_t hi = (rem << 16) | (sM[w] >> 16);
+      const uint32_t qh = hi / up;
+      rem = hi - qh * up;
+      const uint32_t lo = (rem << 16) | (sM[w] & 0xffffu);
+      const uint32_t ql = lo / up;
+      rem = lo - ql * up;
+      Mq[w] = (qh << 16) | ql;
     }
-    uint64_t a = 0;
-    for (int w = CRT_MW - 1; w >= 0; --w) a = ((a << 32) | Mq[w]) % (uint64_t)p;
-    uint32_t inv = 0;
-    for (uint32_t x = 1; x < (uint32_t)p; ++x)
-      if ((a * x) % (uint64_t)p == 1) {
-        inv = x;
-        break;
-      }
+    uint32_t a = 0;
+    for (int w = CRT_MW - 1; w >= 0; --w) {
+      a = ((a << 16) | (Mq[w] >> 16)) % up;
+      a = ((a << 16) | (Mq[w] & 0xffffu)) % up;
+    }
+    // a^-1 mod p by the extended Euclidean algorithm (gcd(a, p) = 1)
+    int r0 = p, r1 = (int)a, s0 = 0, s1 = 1;
+    while (r1 != 0) {
+      const int qq = r0 / r1, r2 = r0 - qq * r1, s2 = s0 - qq * s1;
+      r0 = r1;
+      r1 = r2;
+      s0 = s1;
+      s1 = s2;
+    }
+    const uint32_t inv = (uint32_t)(s0 < 0 ? s0 + p : s0);
     uint64_t c = 0;
     for (int w = 0; w < CRT_MW; ++w) {
       const uint64_t v = (uint64_t)Mq[w] * inv + c;
@@ -1623,7 +1669,7 @@ int ozaki_crt_bits_for(int64_t k, int r) {
 
 hipError_t launch_gemm_crt(int colmajor, int64_t m, int64_t n, int64_t k, int t, q128 alpha, const q128 *A,
                            int64_t lda, const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc, void *wsv,
-                           hipStream_t st, hipEvent_t *ev, int sync_repair) {
+                           hipStream_t st, hipEvent_t *ev, int sync_repair, const void *probe_ws) {
   if (t < CRT_TMIN || t > CRT_TMAX) return hipErrorInvalidValue;
   const int nq = crt_moduli_for(k, t);
   if (!nq) return hipErrorInvalidValue;
@@ -1638,10 +1684,7 @@ hipError_t launch_gemm_crt(int colmajor, int64_t m, int64_t n, int64_t k, int t,
   crt_tab *tab = (crt_tab *)(ws + L.tab);
   int8_t *RA = (int8_t *)(ws + L.rA), *RB = (int8_t *)(ws + L.rB), *RC = (int8_t *)(ws + L.rC);
   if (ev) (void)hipEventRecord(ev[0], st);
-  const int64_t ninit = (L.Mp > L.Np ? L.Mp : L.Np) > OZ_HDR / 4 ? (L.Mp > L.Np ? L.Mp : L.Np) : OZ_HDR / 4;
-  k_oz_init<<<(unsigned)((ninit + 255) / 256), 256, 0, st>>>(eA, fA, L.Mp, eB, fB, L.Np, hdr);
-  oz_scale(A, m, k, asr, asc, eA, fA, st);
-  oz_scale(B, n, k, bsc, bsr, eB, fB, st);
+  oz_prepare(L.Mp, L.Np, m, n, k, A, asr, asc, B, bsr, bsc, probe_ws, &eA, &fA, &eB, &fB, hdr, st);
   k_crt_tables<<<1, 64, 0, st>>>(nq, tab);
   k_crt_residues<<<(unsigned)((L.Mb * L.Kb + 3) / 4), 256, 0, st>>>(A, m, k, asr, asc, eA, t, tab, L.Mb, L.Kb, RA);
   k_crt_residues<<<(unsigned)((L.Nb * L.Kb + 3) / 4), 256, 0, st>>>(B, n, k, bsc, bsr, eB, t, tab, L.Nb, L.Kb, RB);

@@ -224,3 +224,26 @@ def test_crt_4096_cube_sampled(O, Q, cuda):
     _say(f"  max rel {rel.max():.3e} (2^{np.log2(max(rel.max(), 1e-300)):.1f}), "
          f"median 2^{np.log2(max(np.median(rel), 1e-300)):.1f}")
     assert rel.max() <= TOL
+
+
+@pytest.mark.parametrize("method", ["auto", "slices", "crt"])
+def test_repair_reads_b_through_transposed_copy(O, Q, cuda, method):
+    # RowMajor A(i, 2l) = A(i, 2l+1), B(2l+1, j) = -B(2l, j): every entry cancels exactly to zero.
+    # With beta = 0 the result is the product alone, which no bound certifies (the CRT method's
+    # exact product is zero; the slice method's is its truncation error), so all 9216 entries are
+    # queued; with the probe run (the call synchronizes) a queue of >= 2048 entries makes the
+    # repair read B's columns from a transposed copy.  Every entry must equal the oracle's value.
+    m, n, k = 96, 96, 64
+    half = O.fill(0, 91, m * (k // 2)).reshape(m, k // 2, 2)
+    A = np.repeat(half, 2, axis=1).reshape(-1, 2)
+    bh = O.fill(0, 92, (k // 2) * n).reshape(k // 2, n, 2)
+    B3 = np.repeat(bh, 2, axis=0)
+    B3[1::2, :, 1] ^= np.uint64(1 << 63)
+    B = B3.reshape(-1, 2)
+    C0 = O.fill(0, 93, m * n)
+    meth = {"auto": Q.OZAKI_AUTO, "slices": Q.OZAKI_SLICES, "crt": Q.OZAKI_CRT}[method]
+    got, status, repairs, info = _run(Q, cuda, 0, m, n, k, 1.0, 0.0, A, k, B, n, C0, n, 0, meth)
+    assert status == 0 and repairs == m * n
+    ii, jj = _all_idx(m, n)
+    rounded, rel, cand = _exact(O, 0, k, 1.0, A, k, B, n, 0.0, C0, n, got, ii, jj)
+    assert np.array_equal(cand, rounded)
