@@ -1235,17 +1235,19 @@ __global__ void __launch_bounds__(256) k_crt_residues(const q128 *__restrict__ X
 }
 
 // One int8 GEMM per modulus: RC[q][mb][nb] = (RA[q] RB[q]^T) mod p_q for a 256 x 256 tile of
-// (mb, nb) blocks, 8 waves of 128 x 64 (4 x 2 MFMA blocks).  A 4-slot LDS ring of k blocks
-// (8 A + 8 B fragments, 16 KB) is filled three stages ahead with global_load_lds; each wave copies
-// 2 fragments per stage.  Workgroups of one modulus are consecutive (its residue planes stay in
-// the caches).  Output block (q, mb, nb) in MFMA result order: lane L's 16 bytes are rows
-// 8(v/4) + 4(L/32) + v%4, column L%32, v = 0..15.
-constexpr int CRT_WR = 2, CRT_WC = 4, CRT_FM = 4, CRT_FN = 2;
+// (mb, nb) blocks, 8 waves of 128 x 64 (4 x 2 MFMA blocks).  A 4-slot LDS ring of stages of two
+// k blocks (2 x (8 A + 8 B) fragments, 32 KB) is filled three stages ahead with global_load_lds;
+// each wave copies 4 fragments per stage (two k blocks per barrier: fewer waits between MFMAs).
+// Workgroups of one modulus are consecutive (its residue planes stay in the caches).  Output block
+// (q, mb, nb) in MFMA result order: lane L's 16 bytes are rows 8(v/4) + 4(L/32) + v%4, column
+// L%32, v = 0..15.
+constexpr int CRT_WR = 2, CRT_WC = 4, CRT_FM = 4, CRT_FN = 2, CRT_KS = 2;
 constexpr int CRT_TA = CRT_WR * CRT_FM, CRT_TB = CRT_WC * CRT_FN;
-constexpr int CRT_SLOT = (CRT_TA + CRT_TB) * OZ_FRAG;
-constexpr int CRT_PER = (CRT_TA + CRT_TB) / (CRT_WR * CRT_WC);
+constexpr int CRT_SLOT = (CRT_TA + CRT_TB) * OZ_FRAG * CRT_KS;
+constexpr int CRT_PER = (CRT_TA + CRT_TB) * CRT_KS / (CRT_WR * CRT_WC);
 static_assert(CRT_TA * 32 == CRT_TILE && CRT_TB * 32 == CRT_TILE, "tile");
-static_assert((CRT_TA + CRT_TB) % (CRT_WR * CRT_WC) == 0 && 2 * CRT_PER <= 15, "copy split");
+static_assert(((CRT_TA + CRT_TB) * CRT_KS) % (CRT_WR * CRT_WC) == 0 && 2 * CRT_PER <= 15, "copy split");
+static_assert(CRT_KCHUNK % CRT_KS == 0, "reduction at stage ends");
 
 __global__ void __launch_bounds__(512) k_crt_gemm(const int8_t *__restrict__ RA, const int8_t *__restrict__ RB,
                                                   int64_t Mb, int64_t Nb, int64_t Kb, const crt_tab *__restrict__ T,
@@ -1257,15 +1259,20 @@ __global__ void __launch_bounds__(512) k_crt_gemm(const int8_t *__restrict__ RA,
   const int64_t tm = tt / tilesN, tn = tt - tm * tilesN;
   const int8_t *Aq = RA + (q * Mb + tm * CRT_TA) * Kb * OZ_FRAG;
   const int8_t *Bq = RB + (q * Nb + tn * CRT_TB) * Kb * OZ_FRAG;
-  auto copy = [&](int64_t kb) {
-    if (kb >= Kb) return;
-    int8_t *d = &lds[(int)(kb & 3) * CRT_SLOT];
+  const int64_t ns = (Kb + CRT_KS - 1) / CRT_KS;  // stages; the last may hold one k block
+  auto copy = [&](int64_t st) {
+    if (st >= ns) return;
+    int8_t *d = &lds[(int)(st & 3) * CRT_SLOT];
 #pragma unroll
     for (int c = 0; c < CRT_PER; ++c) {
-      const int f = wave * CRT_PER + c;  // A fragments 0..TA-1, then B
-      const int8_t *s = (f < CRT_TA) ? Aq + ((int64_t)f * Kb + kb) * OZ_FRAG
-                                     : Bq + ((int64_t)(f - CRT_TA) * Kb + kb) * OZ_FRAG;
-      __builtin_amdgcn_global_load_lds((glb_void *)(s + lane * 16), (lds_void *)(d + f * OZ_FRAG), 16, 0, 0);
+      const int f = wave * CRT_PER + c;  // fragment of the stage: k block kk, A 0..TA-1 then B
+      const int kk = f / (CRT_TA + CRT_TB), idx = f - kk * (CRT_TA + CRT_TB);
+      const int64_t kb = st * CRT_KS + kk;
+      if (kb < Kb) {
+        const int8_t *s = (idx < CRT_TA) ? Aq + ((int64_t)idx * Kb + kb) * OZ_FRAG
+                                         : Bq + ((int64_t)(idx - CRT_TA) * Kb + kb) * OZ_FRAG;
+        __builtin_amdgcn_global_load_lds((glb_void *)(s + lane * 16), (lds_void *)(d + f * OZ_FRAG), 16, 0, 0);
+      }
     }
   };
   const int wr = wave / CRT_WC, wc = wave % CRT_WC;
@@ -1281,25 +1288,34 @@ __global__ void __launch_bounds__(512) k_crt_gemm(const int8_t *__restrict__ RA,
   copy(0);
   copy(1);
   copy(2);
-  for (int64_t kb = 0; kb < Kb; ++kb) {
-    if (kb + 2 < Kb)
+  for (int64_t st = 0; st < ns; ++st) {
+    // the two newer stages' copies may stay in flight (a stage short of k blocks issues fewer,
+    // so only full stages are counted on: the last two stages wait for everything)
+    if (st + 2 < ns - 1)
       __builtin_amdgcn_s_waitcnt((2 * CRT_PER) | (7 << 4));  // vmcnt(2 PER) lgkmcnt(0)
     else
       __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_s_barrier();
-    const int8_t *base = &lds[(int)(kb & 3) * CRT_SLOT + lane * 16];
-    i32x4v fa[CRT_FM], fb[CRT_FN];
 #pragma unroll
-    for (int a = 0; a < CRT_FM; ++a) fa[a] = *reinterpret_cast<const i32x4v *>(base + (wr * CRT_FM + a) * OZ_FRAG);
+    for (int kk = 0; kk < CRT_KS; ++kk) {
+      const int64_t kb = st * CRT_KS + kk;
+      if (kb < Kb) {
+        const int8_t *base = &lds[(int)(st & 3) * CRT_SLOT + kk * (CRT_TA + CRT_TB) * OZ_FRAG + lane * 16];
+        i32x4v fa[CRT_FM], fb[CRT_FN];
 #pragma unroll
-    for (int b = 0; b < CRT_FN; ++b)
-      fb[b] = *reinterpret_cast<const i32x4v *>(base + (CRT_TA + wc * CRT_FN + b) * OZ_FRAG);
-    copy(kb + 3);
+        for (int a = 0; a < CRT_FM; ++a) fa[a] = *reinterpret_cast<const i32x4v *>(base + (wr * CRT_FM + a) * OZ_FRAG);
 #pragma unroll
-    for (int a = 0; a < CRT_FM; ++a)
+        for (int b = 0; b < CRT_FN; ++b)
+          fb[b] = *reinterpret_cast<const i32x4v *>(base + (CRT_TA + wc * CRT_FN + b) * OZ_FRAG);
+        if (kk == 0) copy(st + 3);
 #pragma unroll
-      for (int b = 0; b < CRT_FN; ++b) acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
-    if ((kb + 1) % CRT_KCHUNK == 0 && kb + 1 < Kb) {
+        for (int a = 0; a < CRT_FM; ++a)
+#pragma unroll
+          for (int b = 0; b < CRT_FN; ++b)
+            acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
+      }
+    }
+    if (((st + 1) * CRT_KS) % CRT_KCHUNK == 0 && (st + 1) * CRT_KS < Kb) {
       // |acc| <= 2048 * 32 * 2^14 = 2^30: reduce before it can overflow
 #pragma unroll
       for (int a = 0; a < CRT_FM; ++a)
