@@ -1239,8 +1239,9 @@ __global__ void __launch_bounds__(256) k_crt_residues(const q128 *__restrict__ X
 // k blocks (2 x (8 A + 8 B) fragments, 32 KB) is filled three stages ahead with global_load_lds;
 // each wave copies 4 fragments per stage (two k blocks per barrier: fewer waits between MFMAs).
 // Workgroups of one modulus are consecutive (its residue planes stay in the caches).  Output block
-// (q, mb, nb) in MFMA result order: lane L's 16 bytes are rows 8(v/4) + 4(L/32) + v%4, column
-// L%32, v = 0..15.
+// (q, mb, nb), 1 KB: bytes v of lane L (rows 8(v/4) + 4(L/32) + v%4, column L%32 of the MFMA
+// result) at (v/4) * 256 + L * 4 + v%4, so that the reconstruction reads rows of 4 entries of 64
+// lanes contiguously.
 constexpr int CRT_WR = 2, CRT_WC = 4, CRT_FM = 4, CRT_FN = 2, CRT_KS = 2;
 constexpr int CRT_TA = CRT_WR * CRT_FM, CRT_TB = CRT_WC * CRT_FN;
 constexpr int CRT_SLOT = (CRT_TA + CRT_TB) * OZ_FRAG * CRT_KS;
@@ -1333,12 +1334,8 @@ __global__ void __launch_bounds__(512) k_crt_gemm(const int8_t *__restrict__ RA,
 #pragma unroll
       for (int v = 0; v < 16; ++v) u[v >> 2] |= crt_red_d(acc[a][b][v], p, ipd) << (8 * (v & 3));
       const int64_t blk = (q * Mb + tm * CRT_TA + wr * CRT_FM + a) * Nb + tn * CRT_TB + wc * CRT_FN + b;
-      i32x4v o;
-      o.x = (int)u[0];
-      o.y = (int)u[1];
-      o.z = (int)u[2];
-      o.w = (int)u[3];
-      *reinterpret_cast<i32x4v *>(RC + blk * OZ_FRAG + lane * 16) = o;
+#pragma unroll
+      for (int r4 = 0; r4 < 4; ++r4) *reinterpret_cast<uint32_t *>(RC + blk * OZ_FRAG + r4 * 256 + lane * 4) = u[r4];
     }
 }
 
@@ -1412,7 +1409,7 @@ __global__ void __launch_bounds__(256) k_crt_finish(int64_t M, int64_t N, int64_
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
           const int q = 32 * kb + 16 * kh + j;
-          P[j] = (q < nq) ? *reinterpret_cast<const uint32_t *>(RC + q * plane + blk * OZ_FRAG + (n + 32 * hp) * 16 + 4 * rq)
+          P[j] = (q < nq) ? *reinterpret_cast<const uint32_t *>(RC + q * plane + blk * OZ_FRAG + rq * 256 + (n + 32 * hp) * 4)
                           : 0u;
         }
 #pragma unroll
