@@ -247,3 +247,34 @@ def test_repair_reads_b_through_transposed_copy(O, Q, cuda, method):
     ii, jj = _all_idx(m, n)
     rounded, rel, cand = _exact(O, 0, k, 1.0, A, k, B, n, 0.0, C0, n, got, ii, jj)
     assert np.array_equal(cand, rounded)
+
+
+def test_ill_conditioned_full_size_forms(O, Q, cuda):
+    # config 5 at full size: qgemm 8192 x 8192 x 16384 ColMajor, alpha = 1, beta = 0, entries
+    # ~1e-18 of the largest products.  The OZAKI form (automatic: the probe wants more than the
+    # CRT method's 158 bits, so the slice method) meets 2^-104 on 128 sampled entries; the DD and
+    # plain forms are run on the same data and their errors against the exact value recorded.
+    m, n, k = 8192, 8192, 16384
+    dA = cuda.empty((m * k, 2), dtype=cuda.int64, device="cuda:0")
+    dB = cuda.empty((k * n, 2), dtype=cuda.int64, device="cuda:0")
+    dC = cuda.zeros((m * n, 2), dtype=cuda.int64, device="cuda:0")
+    Q.dev_fill_2d(2, 5, m, k, m, 1, dA.data_ptr())
+    Q.dev_fill_2d(3, 5, n, k, k, 0, dB.data_ptr())
+    cuda.cuda.synchronize()
+    A, B = from_dev(dA), from_dev(dB)
+    C0 = np.zeros((m * n, 2), dtype=np.uint64)
+    rng = np.random.default_rng(8)
+    ii, jj = rng.integers(0, m, 128), rng.integers(0, n, 128)
+    worst = {}
+    for form, mode in (("ozaki", Q.MODE_OZAKI), ("dd", Q.MODE_DD), ("plain", Q.MODE_REFERENCE)):
+        _say(f"ill-conditioned 8192x8192x16384 ColMajor, {form}")
+        dC.zero_()
+        Q.dev_gemm(mode, 1, m, n, k, 1.0, dA.data_ptr(), m, dB.data_ptr(), k, 0.0, dC.data_ptr(), m)
+        cuda.cuda.synchronize()
+        if form == "ozaki":
+            assert Q.dev_exact_status() == 0
+            assert Q.ozaki_last_method() == Q.OZAKI_SLICES
+        _, rel, _ = _exact(O, 1, k, 1.0, A, m, B, k, 0.0, C0, m, from_dev(dC), ii, jj)
+        worst[form] = float(rel.max())
+        _say(f"  max rel err {worst[form]:.3e}")
+    assert worst["ozaki"] <= TOL
