@@ -27,7 +27,9 @@ Also reported:
   batched         4096 qgemm problems (m, n, k uniform in [8, 256]) and 16384 qgemv problems
                   (m, n uniform in [32, 1024]) with per-problem alpha, beta, each batch one grouped
                   persistent launch (tile configuration from the autotuner) replayed from a
-                  captured graph: problems/s, quad GFLOP/s or GB/s, REFERENCE and DD forms.
+                  captured graph: problems/s, quad GFLOP/s or GB/s, REFERENCE and DD forms; the
+                  qgemm batch also in the OZAKI form (slice count from the autotuner, eager calls:
+                  the form synchronizes once per call to size its workspace).
   ill_conditioned qgemm 8192 x 8192 x 16384 ColMajor whose entries cancel to ~1e-18 of the
                   largest products: OZAKI (26 slices: the bound certifies all but entries within
                   ~2^-12 of zero) and DD, with HBM in use.
@@ -285,6 +287,22 @@ def bench_batched(Q, torch, dev, stream, steps, gemm_count, gemv_count, seed=77)
         ms = timed_graph(lambda s=sh: Q.dev_gemm_batched(mode, 0, gemm_count, ddesc.data_ptr(), s))
         g[form] = {"ms": round(ms, 3), "problems_per_s": round(gemm_count / (ms * 1e-3), 1),
                    "quad_GFLOPs": round(flops / (ms * 1e-3) / 1e9, 1), "tuned_config": Q.gemm_batched_config(mode, cfg)}
+    # OZAKI form (slice count tuned on the batch's own operands); it synchronizes once per call to
+    # size its workspace, so it runs eagerly, calls back to back
+    mode = Q.MODES["ozaki"]
+    cfg = Q.dev_gemm_batched_tune(mode, 0, gemm_count, ddesc.data_ptr(), sh)
+    Q.dev_gemm_batched(mode, 0, gemm_count, ddesc.data_ptr(), sh)      # allocates the workspace
+    torch.cuda.synchronize()
+    s_ev, e_ev = events(torch)
+    s_ev.record(stream)
+    for _ in range(steps):
+        Q.dev_gemm_batched(mode, 0, gemm_count, ddesc.data_ptr(), sh)
+    e_ev.record(stream)
+    torch.cuda.synchronize()
+    ms = s_ev.elapsed_time(e_ev) / steps
+    g["ozaki"] = {"ms": round(ms, 3), "problems_per_s": round(gemm_count / (ms * 1e-3), 1),
+                  "quad_GFLOPs": round(flops / (ms * 1e-3) / 1e9, 1), "tuned_config": Q.gemm_batched_config(mode, cfg),
+                  "repaired_entries": int(Q.dev_ozaki_repairs(sh)), "graph": False}
     out["qgemm"] = g
     del pool, ab, ddesc
     torch.cuda.empty_cache()

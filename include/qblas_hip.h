@@ -202,7 +202,8 @@ int qblas_hip_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, const 
    level3.hpp:215-336 / level2.hpp:15-99 per problem, bit-identical to a loop of single calls).
    All problems share the layout.  The work is one grouped persistent kernel over the tiles (gemm)
    or rows (gemv) of all problems; the launch sequence allocates nothing when the workspace of an
-   earlier call of the same count suffices, synchronizes nothing and may be captured in a hipGraph.
+   earlier call of the same count suffices, synchronizes nothing and may be captured in a hipGraph
+   (all modes but gemm's QBLAS_MODE_OZAKI, below).
    alpha/beta are binary128 {lo, hi} words. */
 typedef struct qblas_gemm_problem {
   int64_t m, n, k;
@@ -225,19 +226,30 @@ typedef struct qblas_gemv_problem {
   uint64_t beta[2];
 } qblas_gemv_problem;
 
-/* d_problems: device array of count descriptors; gemm: modes REFERENCE and DD; gemv: REFERENCE,
-   DD and EXACT (qblas_hip_exact_status then reports products outside the accumulation window). */
+/* d_problems: device array of count descriptors; gemm: modes REFERENCE, DD and OZAKI; gemv:
+   REFERENCE, DD and EXACT (qblas_hip_exact_status then reports products outside the accumulation
+   window).
+   gemm in QBLAS_MODE_OZAKI: the slice method with one slice count S for the whole batch (the tuned
+   or selected configuration; configuration 0 is S = 17), every entry within 2^-104 relative error
+   or recomputed exactly, per problem bit-identical to qblas_hip_gemm in QBLAS_MODE_OZAKI with
+   method QBLAS_OZAKI_SLICES and slice count S.  Unlike the other modes it synchronizes the
+   stream once (the per-problem work counts are summed on the device and read back to size the
+   workspace), so it cannot be captured in a graph.  qblas_hip_ozaki_last_slices,
+   qblas_hip_ozaki_repairs and qblas_hip_exact_status report on the batch. */
 int qblas_hip_gemm_batched(int mode, int layout, int64_t count, const qblas_gemm_problem *d_problems, void *stream);
 int qblas_hip_gemv_batched(int mode, int layout, int64_t count, const qblas_gemv_problem *d_problems, void *stream);
 
-/* Autotuner for the batched gemm: times every tile configuration of the mode on a replica of the
-   batch's first problems (same shapes, synthetic data in a scratch workspace; the caller's C is not
-   touched), keeps the fastest for calls with this (mode, layout, count), and returns its index
-   (>= 0) or an error code (< 0).  Synchronizes; call it outside graph capture.  Without a tuned
-   entry the batched call uses configuration 0. */
+/* Autotuner for the batched gemm: times every configuration of the mode (tile shapes; for
+   QBLAS_MODE_OZAKI slice counts) on a replica of the batch's first 512 problems (same shapes,
+   synthetic data in a scratch workspace; QBLAS_MODE_OZAKI reads the caller's A and B, since the
+   share of entries sent to the exact repair depends on the data; the caller's C is not touched),
+   keeps the fastest for calls with this (mode, layout, count), and returns its index (>= 0) or an
+   error code (< 0).  Synchronizes; call it outside graph capture.  Without a tuned entry the
+   batched call uses configuration 0. */
 int qblas_hip_gemm_batched_tune(int mode, int layout, int64_t count, const qblas_gemm_problem *d_problems,
                                 void *stream);
-/* Description of tile configuration idx of a mode ("TMxTN ..."), or NULL. */
+/* Description of configuration idx of a mode ("TMxTN ...", or "S slices" for QBLAS_MODE_OZAKI),
+   or NULL. */
 const char *qblas_hip_gemm_batched_config(int mode, int idx);
 /* Use configuration idx for batched gemm calls with this (mode, layout, count), as the autotuner
    does with its choice.  QBLAS_ERR_ARG for an unknown mode or index. */

@@ -577,21 +577,55 @@ int tuned_config(int mode, int layout, int64_t count) {
   return it == g_tuned.end() ? 0 : it->second;
 }
 
-int check_batched(const char *what, int mode, int64_t count, const void *P) {
+int check_batched(const char *what, int mode, int64_t count, const void *P, bool gemm = false) {
   if (count < 0 || (count > 0 && (!P || ((uintptr_t)P % 8) != 0))) {
     set_err_msg(std::string(what) + ": bad count or descriptor pointer");
     return QBLAS_ERR_ARG;
   }
-  if (mode != QBLAS_MODE_REFERENCE && mode != QBLAS_MODE_DD) {
-    set_err_msg(std::string(what) + ": batched problems support QBLAS_MODE_REFERENCE and QBLAS_MODE_DD");
+  if (mode != QBLAS_MODE_REFERENCE && mode != QBLAS_MODE_DD && !(gemm && mode == QBLAS_MODE_OZAKI)) {
+    set_err_msg(std::string(what) + ": batched problems support QBLAS_MODE_REFERENCE and QBLAS_MODE_DD" +
+                (gemm ? " (gemm also QBLAS_MODE_OZAKI)" : ""));
     return QBLAS_ERR_MODE;
+  }
+  if (gemm && mode == QBLAS_MODE_OZAKI && count > 0xffffffffll) {
+    set_err_msg(std::string(what) + ": QBLAS_MODE_OZAKI takes at most 2^32 - 1 problems");
+    return QBLAS_ERR_ARG;
   }
   return QBLAS_OK;
 }
 
+int batched_configs(int mode) {
+  return mode == QBLAS_MODE_OZAKI ? qbk::ozaki_batched_configs() : qbk::gemm_batched_configs(mode == QBLAS_MODE_DD);
+}
+
+// Batched OZAKI form with S slices: the per-problem counts are summed on the device and their
+// totals read back (one synchronization) to size the workspace, then the grouped launches follow.
+int gemm_batched_ozaki(int layout, int64_t count, const qblas_gemm_problem *P, int S, hipStream_t s) {
+  void *ws, *ws2;
+  int rc;
+  if ((rc = batch_ws(qbk::gemm_ozaki_batched_scan_ws(count), &ws))) return rc;
+  QB_CHECK(qbk::launch_gemm_ozaki_batched_scan(layout != 0, count, P, ws, s));
+  int64_t tot[qbk::OZ_BATCHED_TOTALS];
+  QB_CHECK(hipMemcpyAsync(tot, (const char *)ws + qbk::gemm_ozaki_batched_totals_offset(count), sizeof(tot),
+                          hipMemcpyDeviceToHost, s));
+  QB_CHECK(hipStreamSynchronize(s));
+  if ((rc = batch_ws(qbk::gemm_ozaki_batched_ws(count, tot, S), &ws2))) return rc;
+  if (ws2 != ws) QB_CHECK(qbk::launch_gemm_ozaki_batched_scan(layout != 0, count, P, ws2, s));  // grown: new buffer
+  QB_CHECK(qbk::launch_gemm_ozaki_batched(layout != 0, count, P, S, tot, ws2, s));
+  t_exact_status = (const uint32_t *)ws2;
+  t_oz_hdr = (const uint32_t *)ws2;
+  t_oz_method = QBLAS_OZAKI_SLICES;
+  t_oz_slices = S;
+  t_oz_bits = 0;
+  t_oz_moduli = 0;
+  t_oz_ev_valid = false;
+  return QBLAS_OK;
+}
+
 int gemm_batched(int mode, int layout, int64_t count, const qblas_gemm_problem *P, int cfg, hipStream_t s) {
-  int rc = check_batched("qblas_hip_gemm_batched", mode, count, P);
+  int rc = check_batched("qblas_hip_gemm_batched", mode, count, P, true);
   if (rc || count == 0) return rc;
+  if (mode == QBLAS_MODE_OZAKI) return gemm_batched_ozaki(layout, count, P, qbk::ozaki_batched_slices(cfg), s);
   void *ws;
   if ((rc = batch_ws(qbk::batched_ws(count), &ws))) return rc;
   QB_CHECK(qbk::launch_gemm_batched(mode == QBLAS_MODE_DD, layout != 0, count, P, cfg, (int64_t)g_kc.load(), ws, s));
@@ -600,10 +634,14 @@ int gemm_batched(int mode, int layout, int64_t count, const qblas_gemm_problem *
 
 // Replica of the first min(count, 512) problems: same m, n, k, packed leading dimensions,
 // synthetic full-mantissa data; every configuration runs once untimed and three times timed.
+// QBLAS_MODE_OZAKI (configurations = slice counts): the replica reads the caller's A and B, since
+// the share of entries the certification sends to the exact repair depends on the data; only C
+// is replaced.
 int gemm_batched_tune(int mode, int layout, int64_t count, const qblas_gemm_problem *dP, hipStream_t s) {
-  int rc = check_batched("qblas_hip_gemm_batched_tune", mode, count, dP);
+  int rc = check_batched("qblas_hip_gemm_batched_tune", mode, count, dP, true);
   if (rc) return -rc;
   if (count == 0) return 0;
+  const bool oz = mode == QBLAS_MODE_OZAKI;
   const int64_t R = count < 512 ? count : 512;
   std::vector<qblas_gemm_problem> hp(R);
   QB_CHECK(hipMemcpyAsync(hp.data(), dP, R * sizeof(qblas_gemm_problem), hipMemcpyDeviceToHost, s));
@@ -611,7 +649,7 @@ int gemm_batched_tune(int mode, int layout, int64_t count, const qblas_gemm_prob
   int64_t elems = 0;
   for (auto &p : hp) {
     const int64_t m = p.m > 0 ? p.m : 0, n = p.n > 0 ? p.n : 0, k = p.k > 0 ? p.k : 0;
-    elems += m * k + k * n + m * n;
+    elems += (oz ? 0 : m * k + k * n) + m * n;
   }
   char *scratch = nullptr;
   const size_t dbytes = R * sizeof(qblas_gemm_problem);
@@ -622,12 +660,14 @@ int gemm_batched_tune(int mode, int layout, int64_t count, const qblas_gemm_prob
   int64_t off = 0;
   for (auto &p : rp) {
     const int64_t m = p.m > 0 ? p.m : 0, n = p.n > 0 ? p.n : 0, k = p.k > 0 ? p.k : 0;
-    p.A = data + off;
-    p.lda = layout ? (m > 0 ? m : 1) : (k > 0 ? k : 1);
-    off += m * k;
-    p.B = data + off;
-    p.ldb = layout ? (k > 0 ? k : 1) : (n > 0 ? n : 1);
-    off += k * n;
+    if (!oz) {
+      p.A = data + off;
+      p.lda = layout ? (m > 0 ? m : 1) : (k > 0 ? k : 1);
+      off += m * k;
+      p.B = data + off;
+      p.ldb = layout ? (k > 0 ? k : 1) : (n > 0 ? n : 1);
+      off += k * n;
+    }
     p.C = data + off;
     p.ldc = layout ? (m > 0 ? m : 1) : (n > 0 ? n : 1);
     off += m * n;
@@ -648,17 +688,21 @@ int gemm_batched_tune(int mode, int layout, int64_t count, const qblas_gemm_prob
   if ((e = qbk::launch_fill(0, 1234, elems, 0, 0, data, s)) != hipSuccess) return fail(e, "tune fill");
   if ((e = hipMemcpyAsync(dR, rp.data(), dbytes, hipMemcpyHostToDevice, s)) != hipSuccess) return fail(e, "tune copy");
   if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) return fail(e, "tune events");
-  const int ncfg = qbk::gemm_batched_configs(mode == QBLAS_MODE_DD);
+  const int ncfg = batched_configs(mode);
+  const qblas_gemm_problem *P = (const qblas_gemm_problem *)dR;
+  auto launch = [&](int c) -> hipError_t {
+    if (!oz) return qbk::launch_gemm_batched(mode == QBLAS_MODE_DD, layout != 0, R, P, c, (int64_t)g_kc.load(), ws, s);
+    if (gemm_batched_ozaki(layout, R, P, qbk::ozaki_batched_slices(c), s) != QBLAS_OK) {
+      const hipError_t le = hipGetLastError();
+      return le != hipSuccess ? le : hipErrorUnknown;
+    }
+    return hipSuccess;
+  };
   for (int c = 0; c < ncfg; ++c) {
-    const qblas_gemm_problem *P = (const qblas_gemm_problem *)dR;
-    if ((e = qbk::launch_gemm_batched(mode == QBLAS_MODE_DD, layout != 0, R, P, c, (int64_t)g_kc.load(), ws, s)) !=
-        hipSuccess)
-      return fail(e, "tune launch");
+    if ((e = launch(c)) != hipSuccess) return fail(e, "tune launch");
     if ((e = hipEventRecord(e0, s)) != hipSuccess) return fail(e, "tune record");
     for (int rep = 0; rep < 3; ++rep)
-      if ((e = qbk::launch_gemm_batched(mode == QBLAS_MODE_DD, layout != 0, R, P, c, (int64_t)g_kc.load(), ws, s)) !=
-          hipSuccess)
-        return fail(e, "tune launch");
+      if ((e = launch(c)) != hipSuccess) return fail(e, "tune launch");
     if ((e = hipEventRecord(e1, s)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess)
       return fail(e, "tune sync");
     float ms = 0;
@@ -670,6 +714,10 @@ int gemm_batched_tune(int mode, int layout, int64_t count, const qblas_gemm_prob
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (oz) {  // the replica's status and repair count are not the caller's
+    t_exact_status = nullptr;
+    t_oz_hdr = nullptr;
+  }
   QB_CHECK(hipFree(scratch));
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -891,12 +939,21 @@ int qblas_hip_gemm_batched_tune(int mode, int layout, int64_t count, const qblas
   return gemm_batched_tune(mode, layout, count, d_problems, (hipStream_t)stream);
 }
 const char *qblas_hip_gemm_batched_config(int mode, int idx) {
+  if (mode == QBLAS_MODE_OZAKI) {
+    static const std::vector<std::string> names = [] {
+      std::vector<std::string> v;
+      for (int c = 0; c < qbk::ozaki_batched_configs(); ++c)
+        v.push_back(std::to_string(qbk::ozaki_batched_slices(c)) + " slices");
+      return v;
+    }();
+    return (idx >= 0 && idx < (int)names.size()) ? names[idx].c_str() : nullptr;
+  }
   if (mode != QBLAS_MODE_REFERENCE && mode != QBLAS_MODE_DD) return nullptr;
   return qbk::gemm_batched_config_name(mode == QBLAS_MODE_DD, idx);
 }
 int qblas_hip_gemm_batched_select(int mode, int layout, int64_t count, int idx) {
-  if ((mode != QBLAS_MODE_REFERENCE && mode != QBLAS_MODE_DD) || idx < 0 ||
-      idx >= qbk::gemm_batched_configs(mode == QBLAS_MODE_DD))
+  if ((mode != QBLAS_MODE_REFERENCE && mode != QBLAS_MODE_DD && mode != QBLAS_MODE_OZAKI) || idx < 0 ||
+      idx >= batched_configs(mode))
     return QBLAS_ERR_ARG;
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tuned[std::make_tuple(mode, layout, count)] = idx;

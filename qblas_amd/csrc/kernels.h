@@ -90,4 +90,18 @@ hipError_t launch_gemm_batched(int dd_mode, int colmajor, int64_t count, const q
 size_t batched_status_offset(int64_t count);
 hipError_t launch_gemv_batched(int form, int colmajor, int64_t count, const qblas_gemv_problem *P, void *ws,
                                hipStream_t st);
+// batched OZAKI form (kernels_ozaki.hip, slice method, one slice count for the batch): the scan
+// writes per-problem prefix sums into ws (gemm_ozaki_batched_scan_ws bytes); the caller reads the
+// 9 totals at ws + gemm_ozaki_batched_totals_offset, grows ws to gemm_ozaki_batched_ws (running the
+// scan again when ws moved) and launches the rest.  ws word 0 is the status word.
+constexpr int OZ_BATCHED_TOTALS = 9;
+int ozaki_batched_configs();
+int ozaki_batched_slices(int cfg);  // slice count of configuration cfg (0 = default)
+size_t gemm_ozaki_batched_scan_ws(int64_t count);
+size_t gemm_ozaki_batched_totals_offset(int64_t count);
+hipError_t launch_gemm_ozaki_batched_scan(int colmajor, int64_t count, const qblas_gemm_problem *P, void *ws,
+                                          hipStream_t st);
+size_t gemm_ozaki_batched_ws(int64_t count, const int64_t *tot, int S);
+hipError_t launch_gemm_ozaki_batched(int colmajor, int64_t count, const qblas_gemm_problem *P, int S,
+                                     const int64_t *tot, void *ws, hipStream_t st);
 }  // namespace qbk

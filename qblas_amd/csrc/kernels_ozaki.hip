@@ -191,6 +191,10 @@ __device__ __forceinline__ void oz_digits(q128 x, int Es, uint32_t (&W)[OZ_PW]) 
   }
 }
 
+template <int S>
+__device__ __forceinline__ void oz_slice_unit(const q128 *__restrict__ X, int64_t R, int64_t C, int64_t sr,
+                                              int64_t sc, const int32_t *__restrict__ E, int8_t *__restrict__ out,
+                                              int64_t Kb, int64_t chunk, int lane);
 // One thread per (row block, k block, lane): 16 consecutive k of row 32*rb + lane%32.
 // X(r, c) = X[r*sr + c*sc] for r < R, c < C; zero digits elsewhere.
 template <int S>
@@ -199,8 +203,14 @@ __global__ void __launch_bounds__(256) k_oz_slice(const q128 *__restrict__ X, in
                                                   int64_t Kb, int64_t nthreads) {
   const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (g >= nthreads) return;
-  const int lane = (int)(g & 63);
-  const int64_t chunk = g >> 6;  // rb * Kb + kb
+  oz_slice_unit<S>(X, R, C, sr, sc, E, out, Kb, g >> 6, (int)(g & 63));
+}
+// One lane's 16 elements (row rb*32 + lane%32, k kb*32 + 16(lane/32)..+15) of fragment position
+// chunk = rb * Kb + kb: their S digits into the fragment of each slice.
+template <int S>
+__device__ __forceinline__ void oz_slice_unit(const q128 *__restrict__ X, int64_t R, int64_t C, int64_t sr,
+                                              int64_t sc, const int32_t *__restrict__ E, int8_t *__restrict__ out,
+                                              int64_t Kb, int64_t chunk, int lane) {
   const int64_t rb = chunk / Kb, kb = chunk - rb * Kb;
   const int64_t r = rb * 32 + (lane & 31);
   const int64_t c0 = kb * 32 + (lane >> 5) * 16;
@@ -310,28 +320,17 @@ __global__ void __launch_bounds__(256) k_oz_slice_rows(const q128 *__restrict__ 
 // group g and D0 <= t + u <= D1; fragments of earlier groups of the same k block are still in
 // registers.  Within a stage pairs are ordered by max(t, u), so each MFMA needs only fragments
 // read before it.  Pass sums go to record z = zbase + k pass.
+// Output tile (tm, tn) of k pass kp; the caller's workgroup (256 threads) runs it whole.
 template <int S, int D0, int D1, int NG>
-__global__ void __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(1, 1)))
-k_oz_gemm(const int8_t *__restrict__ As, const int8_t *__restrict__ Bs, int64_t Kb, int64_t Mp, int64_t Np,
-          int zbase, uint32_t *__restrict__ part) {
+__device__ __forceinline__ void oz_gemm_tile(const int8_t *__restrict__ As, const int8_t *__restrict__ Bs,
+                                             int64_t Kb, int64_t Mp, int64_t Np, int64_t tm, int64_t tn,
+                                             int64_t kp, int zbase, uint32_t *__restrict__ part) {
   constexpr int G = (S + NG - 1) / NG;
   constexpr int SLOT = 4 * G * OZ_FRAG;
   constexpr int ND = D1 - D0 + 1;
   constexpr int TMAX = (D1 < S - 1) ? D1 : S - 1;  // highest slice any pair of the range uses
   __shared__ __attribute__((aligned(1024))) int8_t lds[4 * SLOT];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  int64_t tn = blockIdx.x, tm = blockIdx.y;
-  const int64_t kp = blockIdx.z;
-  if ((gridDim.x % 16) == 0 && (gridDim.y % 16) == 0) {
-    // Rounds of 256 consecutive workgroups cover a 16x16 super-tile (row-major over super-tiles),
-    // and the 32 of a round that share an XCD (workgroup id mod 8) a 4x8 block of it: a round's
-    // slice panels stay within the infinity cache and each XCD's L2 serves 4 + 8 of them.
-    const int64_t L = blockIdx.x + (int64_t)gridDim.x * blockIdx.y;
-    const int64_t r = L >> 8, w = L & 255, xcd = w & 7, idx = w >> 3;
-    const int64_t SC = gridDim.x >> 4;
-    tm = (r / SC) * 16 + (xcd >> 1) * 4 + (idx >> 3);
-    tn = (r % SC) * 16 + (xcd & 1) * 8 + (idx & 7);
-  }
   const int64_t kb0 = kp * OZ_KPASS_BLOCKS;
   const int64_t nkb = ((kb0 + OZ_KPASS_BLOCKS < Kb) ? kb0 + OZ_KPASS_BLOCKS : Kb) - kb0;
   const int64_t nst = NG * nkb;
@@ -473,26 +472,38 @@ k_oz_gemm(const int8_t *__restrict__ As, const int8_t *__restrict__ Bs, int64_t 
   }
 }
 
+template <int S, int D0, int D1, int NG>
+__global__ void __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(1, 1)))
+k_oz_gemm(const int8_t *__restrict__ As, const int8_t *__restrict__ Bs, int64_t Kb, int64_t Mp, int64_t Np,
+          int zbase, uint32_t *__restrict__ part) {
+  int64_t tn = blockIdx.x, tm = blockIdx.y;
+  if ((gridDim.x % 16) == 0 && (gridDim.y % 16) == 0) {
+    // Rounds of 256 consecutive workgroups cover a 16x16 super-tile (row-major over super-tiles),
+    // and the 32 of a round that share an XCD (workgroup id mod 8) a 4x8 block of it: a round's
+    // slice panels stay within the infinity cache and each XCD's L2 serves 4 + 8 of them.
+    const int64_t L = blockIdx.x + (int64_t)gridDim.x * blockIdx.y;
+    const int64_t r = L >> 8, w = L & 255, xcd = w & 7, idx = w >> 3;
+    const int64_t SC = gridDim.x >> 4;
+    tm = (r / SC) * 16 + (xcd >> 1) * 4 + (idx >> 3);
+    tn = (r % SC) * 16 + (xcd & 1) * 8 + (idx & 7);
+  }
+  oz_gemm_tile<S, D0, D1, NG>(As, Bs, Kb, Mp, Np, tm, tn, blockIdx.z, zbase, part);
+}
+
 // ------------------------------------------------------------------------------------------
 // 4. finish and certification
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ bool q_finite_nonzero(q128 x) { return q_bexp(x) != 0x7fffu && !q_iszero(x); }
 
-// One thread per output (i, j), j fastest.  kfac = ceil(log2(K (S+1)/4)).
-__global__ void __launch_bounds__(256) k_oz_finish(int64_t M, int64_t N, int S, int kfac, int npass,
-                                                   const uint32_t *__restrict__ part, int64_t Mp, int64_t Np,
-                                                   const int32_t *__restrict__ eA, const uint32_t *__restrict__ fA,
-                                                   const int32_t *__restrict__ eB, const uint32_t *__restrict__ fB,
-                                                   q128 alpha, q128 beta, q128 *__restrict__ C, int64_t csr,
-                                                   int64_t csc, uint32_t *__restrict__ hdr, uint32_t *__restrict__ list) {
-  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (g >= M * N) return;
-  const int64_t i = g / N, j = g - i * N;
-  q128 *cp = C + i * csr + j * csc;
-  const q128 yv = ldq_c(cp);
+// Entry (i, j): res = alpha * (sum of its pass records) 2^(E_i+F_j-8(S+1)) + beta y rounded once;
+// true when res is certified (else the entry goes to the exact repair).  kfac = ceil(log2(K (S+1)/4)).
+__device__ __forceinline__ bool oz_finish_value(int64_t i, int64_t j, int S, int kfac, int npass,
+                                                const uint32_t *__restrict__ part, int64_t Mp, int64_t Np,
+                                                const int32_t *__restrict__ eA, const uint32_t *__restrict__ fA,
+                                                const int32_t *__restrict__ eB, const uint32_t *__restrict__ fB,
+                                                q128 alpha, q128 beta, q128 yv, q128 &res) {
   const bool ab_ok = (q_bexp(alpha) != 0x7fffu) && (q_bexp(beta) != 0x7fffu) && (q_bexp(yv) != 0x7fffu);
   bool done = false;
-  q128 res;
   const bool fin = !((fA[i] | fB[j]) & OZF_NONFINITE);
   if (ab_ok && fin && q_iszero(alpha)) {
     // alpha*AB is a zero; beta*y decides unless it is zero too (then the sign of the exact sum matters)
@@ -530,7 +541,22 @@ __global__ void __launch_bounds__(256) k_oz_finish(int64_t M, int64_t N, int S, 
       }
     }
   }
-  if (done) {
+  return done;
+}
+
+// One thread per output (i, j), j fastest.
+__global__ void __launch_bounds__(256) k_oz_finish(int64_t M, int64_t N, int S, int kfac, int npass,
+                                                   const uint32_t *__restrict__ part, int64_t Mp, int64_t Np,
+                                                   const int32_t *__restrict__ eA, const uint32_t *__restrict__ fA,
+                                                   const int32_t *__restrict__ eB, const uint32_t *__restrict__ fB,
+                                                   q128 alpha, q128 beta, q128 *__restrict__ C, int64_t csr,
+                                                   int64_t csc, uint32_t *__restrict__ hdr, uint32_t *__restrict__ list) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= M * N) return;
+  const int64_t i = g / N, j = g - i * N;
+  q128 *cp = C + i * csr + j * csc;
+  q128 res;
+  if (oz_finish_value(i, j, S, kfac, npass, part, Mp, Np, eA, fA, eB, fB, alpha, beta, ldq_c(cp), res)) {
     stq_c(cp, res);
   } else {
     const uint32_t slot = atomicAdd(hdr + 1, 1u);
@@ -596,6 +622,30 @@ __device__ __forceinline__ int oz_anchor(int ei, int ej) {
   return (ei == OZ_EZERO || ej == OZ_EZERO) ? 0 : ei + ej - 224 - (WIN_BAND + WIN_OMAX);
 }
 
+// One entry by the whole workgroup: *cp = RN(alpha sum_l ar[l asc] bc[l bsr] + beta *cp) (NaN and
+// status bit 0 when a product falls below the windows of anchor An).
+__device__ __forceinline__ void oz_repair_entry(int64_t K, const q128 *__restrict__ ar, int64_t asc,
+                                                const q128 *__restrict__ bc, int64_t bsr, int An, q128 alpha,
+                                                q128 beta, q128 *__restrict__ cp, uint32_t *__restrict__ hdr,
+                                                uint32_t (*s_w)[2 * WW + 1]) {
+  uint32_t Wl[WW], Wh[WW], flags;
+  oz_block_dot(K, ar, asc, bc, bsr, An, Wl, Wh, flags, s_w);
+  if (threadIdx.x == 0) {
+    q128 out;
+    if (flags & FL_LOST) {
+      out = q_nan();
+      atomicOr(hdr, 1u);
+    } else {
+      uint32_t Mw[SW];
+      uint32_t sign;
+      s_from_windows(Mw, sign, Wl, Wh);
+      out = finish_reg(Mw, An, sign, 0u, flags, true, alpha, beta, ldq_c(cp));
+    }
+    stq_c(cp, out);
+  }
+  __syncthreads();
+}
+
 __global__ void __launch_bounds__(OZR_BS) k_oz_repair(int64_t K, const q128 *__restrict__ A, int64_t asr, int64_t asc,
                                                       const q128 *__restrict__ B, int64_t bsr, int64_t bsc,
                                                       const int32_t *__restrict__ eA, const int32_t *__restrict__ eB,
@@ -606,24 +656,8 @@ __global__ void __launch_bounds__(OZR_BS) k_oz_repair(int64_t K, const q128 *__r
   const uint32_t count = hdr[1];
   for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) {
     const int64_t i = list[2 * (int64_t)e], j = list[2 * (int64_t)e + 1];
-    const int An = oz_anchor(eA[i], eB[j]);
-    uint32_t Wl[WW], Wh[WW], flags;
-    oz_block_dot(K, A + i * asr, asc, B + j * bsc, bsr, An, Wl, Wh, flags, s_w);
-    if (threadIdx.x == 0) {
-      q128 *cp = C + i * csr + j * csc;
-      q128 out;
-      if (flags & FL_LOST) {
-        out = q_nan();
-        atomicOr(hdr, 1u);
-      } else {
-        uint32_t Mw[SW];
-        uint32_t sign;
-        s_from_windows(Mw, sign, Wl, Wh);
-        out = finish_reg(Mw, An, sign, 0u, flags, true, alpha, beta, ldq_c(cp));
-      }
-      stq_c(cp, out);
-    }
-    __syncthreads();
+    oz_repair_entry(K, A + i * asr, asc, B + j * bsc, bsr, oz_anchor(eA[i], eB[j]), alpha, beta,
+                    C + i * csr + j * csc, hdr, s_w);
   }
 }
 
@@ -904,6 +938,321 @@ hipError_t ozaki_repair_count(const void *ws, uint32_t *out, hipStream_t st) {
   hipError_t e = hipMemcpyAsync(out, (const char *)ws + 4, 4, hipMemcpyDeviceToHost, st);
   if (e != hipSuccess) return e;
   return hipStreamSynchronize(st);
+}
+
+// ==========================================================================================
+// Batched OZAKI form (slice method): many independent problems (qblas_gemm_problem descriptors
+// in device memory), one slice count S (12..19, one diagonal range) for the whole batch.
+// k_bo_scan writes, per problem, the exclusive prefix sums of the counts below (BO_F int64 per
+// record, count + 1 records after the header); the host reads the totals (one synchronization)
+// and lays out the workspace, and every later launch is grouped over all problems: a work item
+// finds its problem by binary search in one field of the records.  Per problem the steps and the
+// per-entry result are those of the single-problem form (scaling, slicing, products, certified
+// finish, exact repair).
+// ==========================================================================================
+constexpr int BO_F = OZ_BATCHED_TOTALS;
+enum {
+  BO_ROWS = 0,  // padded rows of A (exponent arrays eA, fA)
+  BO_COLS,      // padded columns of B (eB, fB)
+  BO_FRAGA,     // fragment positions of A's slices (units of S KB)
+  BO_FRAGB,     // and of B's
+  BO_PART,      // pass-sum records
+  BO_TILES,     // 64x64 product tiles times k passes
+  BO_ENT,       // outputs m n
+  BO_SCA,       // scaling items of A: rows (RowMajor) or groups of 64 rows (ColMajor)
+  BO_SCB        // of B: groups of 64 columns (RowMajor) or columns (ColMajor)
+};
+
+struct bo_geom {
+  int64_t m, n, k, Mp, Np, Kb, npk;
+  int64_t asr, asc, bsr, bsc, csr, csc;  // element strides of A(i,l), B(l,j), C(i,j)
+};
+
+// a problem with m, n or k <= 0 has no work (C unchanged), as in the single-problem routine
+__device__ __forceinline__ bo_geom bo_geometry(const qblas_gemm_problem &g, int colmajor) {
+  bo_geom G;
+  const bool live = g.m > 0 && g.n > 0 && g.k > 0;
+  G.m = live ? g.m : 0;
+  G.n = live ? g.n : 0;
+  G.k = live ? g.k : 0;
+  G.Mp = (G.m + OZ_TILE - 1) / OZ_TILE * OZ_TILE;
+  G.Np = (G.n + OZ_TILE - 1) / OZ_TILE * OZ_TILE;
+  G.Kb = (G.k + 31) / 32;
+  G.npk = (G.Kb + OZ_KPASS_BLOCKS - 1) / OZ_KPASS_BLOCKS;
+  G.asr = colmajor ? 1 : g.lda;
+  G.asc = colmajor ? g.lda : 1;
+  G.bsr = colmajor ? 1 : g.ldb;
+  G.bsc = colmajor ? g.ldb : 1;
+  G.csr = colmajor ? 1 : g.ldc;
+  G.csc = colmajor ? g.ldc : 1;
+  return G;
+}
+
+// largest p in [0, count) with off[p][f] <= t
+__device__ __forceinline__ int64_t bo_find(const int64_t *__restrict__ off, int64_t count, int f, int64_t t) {
+  int64_t lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (off[mid * BO_F + f] <= t)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// off[0][*] = 0, off[p + 1][f] = sum of count f over problems 0..p; one block, 1024 problems a chunk
+__global__ void __launch_bounds__(1024) k_bo_scan(const qblas_gemm_problem *__restrict__ P, int64_t count,
+                                                  int colmajor, int64_t *__restrict__ off) {
+  __shared__ int64_t s[1024];
+  int64_t base[BO_F];
+#pragma unroll
+  for (int f = 0; f < BO_F; ++f) base[f] = 0;
+  if (threadIdx.x < BO_F) off[threadIdx.x] = 0;
+  for (int64_t b = 0; b < count; b += 1024) {
+    const int64_t p = b + threadIdx.x;
+    int64_t v[BO_F];
+#pragma unroll
+    for (int f = 0; f < BO_F; ++f) v[f] = 0;
+    if (p < count) {
+      const bo_geom G = bo_geometry(P[p], colmajor);
+      v[BO_ROWS] = G.Mp;
+      v[BO_COLS] = G.Np;
+      v[BO_FRAGA] = (G.Mp / 32) * G.Kb;
+      v[BO_FRAGB] = (G.Np / 32) * G.Kb;
+      v[BO_PART] = G.npk * G.Mp * G.Np;
+      v[BO_TILES] = (G.Mp / OZ_TILE) * (G.Np / OZ_TILE) * G.npk;
+      v[BO_ENT] = G.m * G.n;
+      v[BO_SCA] = colmajor ? (G.m + 63) / 64 : G.m;
+      v[BO_SCB] = colmajor ? G.n : (G.n + 63) / 64;
+    }
+#pragma unroll
+    for (int f = 0; f < BO_F; ++f) {
+      s[threadIdx.x] = v[f];
+      __syncthreads();
+      for (int o = 1; o < 1024; o <<= 1) {
+        const int64_t t = (threadIdx.x >= o) ? s[threadIdx.x - o] : 0;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+      }
+      if (p < count) off[(p + 1) * BO_F + f] = base[f] + s[threadIdx.x];
+      base[f] += s[1023];
+      __syncthreads();
+    }
+  }
+}
+
+// Scaling exponents of every problem's A rows (which = 0) or B columns (which = 1): one wave per
+// contiguous row (lanes along k), or one wave per 64 strided rows (a lane per row).
+__global__ void __launch_bounds__(256) k_bo_scale(const qblas_gemm_problem *__restrict__ P, int64_t count,
+                                                  const int64_t *__restrict__ off, int colmajor, int which,
+                                                  int64_t items, int32_t *__restrict__ E, uint32_t *__restrict__ F) {
+  const int lane = threadIdx.x & 63;
+  const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= items) return;  // whole waves
+  const int fs = which ? BO_SCB : BO_SCA;
+  const int64_t p = bo_find(off, count, fs, w);
+  const bo_geom G = bo_geometry(P[p], colmajor);
+  const q128 *X = (const q128 *)(which ? P[p].B : P[p].A);
+  const int64_t R = which ? G.n : G.m, sr = which ? G.bsc : G.asr, sc = which ? G.bsr : G.asc;
+  const int64_t ro = off[p * BO_F + (which ? BO_COLS : BO_ROWS)];
+  const int64_t u = w - off[p * BO_F + fs];
+  const bool contig = which ? colmajor : !colmajor;
+  int e = OZ_EZERO;
+  uint32_t fl = 0;
+  if (contig) {
+    const q128 *row = X + u * sr;
+    for (int64_t c = lane; c < G.k; c += 64) e = max(e, oz_ilogb(ldq_c(row + c * sc), fl));
+    e = wave_max(e);
+    fl = wave_or(fl);
+    if (lane == 0) {
+      E[ro + u] = e;
+      F[ro + u] = fl;
+    }
+  } else {
+    const int64_t r = u * 64 + lane;
+    if (r < R) {
+      for (int64_t c = 0; c < G.k; ++c) e = max(e, oz_ilogb(ldq_c(X + r * sr + c * sc), fl));
+      E[ro + r] = e;
+      F[ro + r] = fl;
+    }
+  }
+}
+
+// slices of every problem's A (which = 0) or B (which = 1): one thread per (fragment position, lane)
+template <int S>
+__global__ void __launch_bounds__(256) k_bo_slice(const qblas_gemm_problem *__restrict__ P, int64_t count,
+                                                  const int64_t *__restrict__ off, int colmajor, int which,
+                                                  int64_t nthreads, const int32_t *__restrict__ E,
+                                                  int8_t *__restrict__ out) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= nthreads) return;
+  const int64_t chunk = g >> 6;
+  const int fs = which ? BO_FRAGB : BO_FRAGA;
+  const int64_t p = bo_find(off, count, fs, chunk);
+  const bo_geom G = bo_geometry(P[p], colmajor);
+  const q128 *X = (const q128 *)(which ? P[p].B : P[p].A);
+  const int64_t R = which ? G.n : G.m, sr = which ? G.bsc : G.asr, sc = which ? G.bsr : G.asc;
+  const int64_t f0 = off[p * BO_F + fs];
+  oz_slice_unit<S>(X, R, G.k, sr, sc, E + off[p * BO_F + (which ? BO_COLS : BO_ROWS)], out + f0 * (int64_t)(S * OZ_FRAG),
+                   G.Kb, chunk - f0, (int)(g & 63));
+}
+
+// one 64x64 output tile of one k pass per workgroup
+template <int S>
+__global__ void __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(1, 1)))
+k_bo_gemm(const qblas_gemm_problem *__restrict__ P, int64_t count, const int64_t *__restrict__ off, int colmajor,
+          const int8_t *__restrict__ As, const int8_t *__restrict__ Bs, uint32_t *__restrict__ part) {
+  const int64_t t = blockIdx.x;
+  const int64_t p = bo_find(off, count, BO_TILES, t);
+  const bo_geom G = bo_geometry(P[p], colmajor);
+  const int64_t *o = off + p * BO_F;
+  const int64_t u = t - o[BO_TILES];
+  const int64_t TN = G.Np / OZ_TILE, TM = G.Mp / OZ_TILE;
+  const int64_t tn = u % TN, tm = (u / TN) % TM, kp = u / (TN * TM);
+  oz_gemm_tile<S, 0, S - 1, 2>(As + o[BO_FRAGA] * (int64_t)(S * OZ_FRAG), Bs + o[BO_FRAGB] * (int64_t)(S * OZ_FRAG),
+                               G.Kb, G.Mp, G.Np, tm, tn, kp, 0, part + o[BO_PART] * OZ_PREC);
+}
+
+// one thread per output of every problem; uncertified entries queue (problem, i, j)
+__global__ void __launch_bounds__(256) k_bo_finish(const qblas_gemm_problem *__restrict__ P, int64_t count,
+                                                   const int64_t *__restrict__ off, int colmajor, int S, int64_t total,
+                                                   const uint32_t *__restrict__ part, const int32_t *__restrict__ eA,
+                                                   const uint32_t *__restrict__ fA, const int32_t *__restrict__ eB,
+                                                   const uint32_t *__restrict__ fB, uint32_t *__restrict__ hdr,
+                                                   uint32_t *__restrict__ list) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= total) return;
+  const int64_t p = bo_find(off, count, BO_ENT, g);
+  const qblas_gemm_problem &pr = P[p];
+  const bo_geom G = bo_geometry(pr, colmajor);
+  const int64_t *o = off + p * BO_F;
+  const int64_t u = g - o[BO_ENT];
+  const int64_t i = u / G.n, j = u - i * G.n;
+  int kfac = 0;  // ceil(log2(K (S+1) / 4))
+  while ((4ll << kfac) < G.k * (S + 1)) ++kfac;
+  q128 *cp = (q128 *)pr.C + i * G.csr + j * G.csc;
+  q128 res;
+  if (oz_finish_value(i, j, S, kfac, (int)G.npk, part + o[BO_PART] * OZ_PREC, G.Mp, G.Np, eA + o[BO_ROWS],
+                      fA + o[BO_ROWS], eB + o[BO_COLS], fB + o[BO_COLS], q_make(pr.alpha[0], pr.alpha[1]),
+                      q_make(pr.beta[0], pr.beta[1]), ldq_c(cp), res)) {
+    stq_c(cp, res);
+  } else {
+    const uint32_t slot = atomicAdd(hdr + 1, 1u);
+    list[4 * (int64_t)slot] = (uint32_t)p;
+    list[4 * (int64_t)slot + 1] = (uint32_t)i;
+    list[4 * (int64_t)slot + 2] = (uint32_t)j;
+  }
+}
+
+// exact recomputation of the queued entries, one workgroup per entry (grid-stride)
+__global__ void __launch_bounds__(OZR_BS) k_bo_repair(const qblas_gemm_problem *__restrict__ P, int64_t count,
+                                                      const int64_t *__restrict__ off, int colmajor,
+                                                      const int32_t *__restrict__ eA, const int32_t *__restrict__ eB,
+                                                      uint32_t *__restrict__ hdr, const uint32_t *__restrict__ list) {
+  __shared__ uint32_t s_w[OZR_BS / 64][2 * WW + 1];
+  const uint32_t nq = hdr[1];
+  for (uint32_t e = blockIdx.x; e < nq; e += gridDim.x) {
+    const int64_t p = list[4 * (int64_t)e], i = list[4 * (int64_t)e + 1], j = list[4 * (int64_t)e + 2];
+    const qblas_gemm_problem &pr = P[p];
+    const bo_geom G = bo_geometry(pr, colmajor);
+    const int64_t *o = off + p * BO_F;
+    oz_repair_entry(G.k, (const q128 *)pr.A + i * G.asr, G.asc, (const q128 *)pr.B + j * G.bsc, G.bsr,
+                    oz_anchor(eA[o[BO_ROWS] + i], eB[o[BO_COLS] + j]), q_make(pr.alpha[0], pr.alpha[1]),
+                    q_make(pr.beta[0], pr.beta[1]), (q128 *)pr.C + i * G.csr + j * G.csc, hdr, s_w);
+  }
+}
+
+struct bo_layout {
+  size_t eA, fA, eB, fB, sA, sB, part, list, total;
+};
+static size_t bo_records_end(int64_t count) { return (OZ_HDR + (size_t)(count + 1) * BO_F * 8 + 255) / 256 * 256; }
+static bo_layout bo_plan(int64_t count, const int64_t *tot, int S) {
+  auto al = [](size_t v) { return (v + 255) / 256 * 256; };
+  bo_layout L;
+  size_t o = bo_records_end(count);
+  L.eA = o;
+  o = al(o + (size_t)tot[BO_ROWS] * 4);
+  L.fA = o;
+  o = al(o + (size_t)tot[BO_ROWS] * 4);
+  L.eB = o;
+  o = al(o + (size_t)tot[BO_COLS] * 4);
+  L.fB = o;
+  o = al(o + (size_t)tot[BO_COLS] * 4);
+  L.sA = o;
+  o = al(o + (size_t)tot[BO_FRAGA] * S * OZ_FRAG);
+  L.sB = o;
+  o = al(o + (size_t)tot[BO_FRAGB] * S * OZ_FRAG);
+  L.part = o;
+  o = al(o + (size_t)tot[BO_PART] * OZ_PREC * 4);
+  L.list = o;
+  o = al(o + (size_t)tot[BO_ENT] * 16);
+  L.total = o;
+  return L;
+}
+
+static const int BO_SLICES[] = {17, 16, 15, 14, 18, 19};
+int ozaki_batched_configs() { return (int)(sizeof(BO_SLICES) / sizeof(BO_SLICES[0])); }
+int ozaki_batched_slices(int cfg) { return (cfg >= 0 && cfg < ozaki_batched_configs()) ? BO_SLICES[cfg] : 0; }
+
+size_t gemm_ozaki_batched_scan_ws(int64_t count) { return bo_records_end(count); }
+size_t gemm_ozaki_batched_totals_offset(int64_t count) { return OZ_HDR + (size_t)count * BO_F * 8; }
+
+hipError_t launch_gemm_ozaki_batched_scan(int colmajor, int64_t count, const qblas_gemm_problem *P, void *ws,
+                                          hipStream_t st) {
+  k_bo_scan<<<1, 1024, 0, st>>>(P, count, colmajor, (int64_t *)((char *)ws + OZ_HDR));
+  return hipGetLastError();
+}
+
+size_t gemm_ozaki_batched_ws(int64_t count, const int64_t *tot, int S) { return bo_plan(count, tot, S).total; }
+
+template <int S>
+static void bo_launch(int colmajor, int64_t count, const qblas_gemm_problem *P, const int64_t *tot, const int64_t *off,
+                      const bo_layout &L, char *ws, hipStream_t st) {
+  int32_t *eA = (int32_t *)(ws + L.eA), *eB = (int32_t *)(ws + L.eB);
+  uint32_t *fA = (uint32_t *)(ws + L.fA), *fB = (uint32_t *)(ws + L.fB);
+  int8_t *sA = (int8_t *)(ws + L.sA), *sB = (int8_t *)(ws + L.sB);
+  uint32_t *part = (uint32_t *)(ws + L.part), *hdr = (uint32_t *)ws, *list = (uint32_t *)(ws + L.list);
+  const int64_t ninit = tot[BO_ROWS] > tot[BO_COLS] ? tot[BO_ROWS] : tot[BO_COLS];
+  const int64_t ni = ninit > OZ_HDR / 4 ? ninit : OZ_HDR / 4;
+  k_oz_init<<<(unsigned)((ni + 255) / 256), 256, 0, st>>>(eA, fA, tot[BO_ROWS], eB, fB, tot[BO_COLS], hdr);
+  if (tot[BO_SCA] > 0)
+    k_bo_scale<<<(unsigned)((tot[BO_SCA] + 3) / 4), 256, 0, st>>>(P, count, off, colmajor, 0, tot[BO_SCA], eA, fA);
+  if (tot[BO_SCB] > 0)
+    k_bo_scale<<<(unsigned)((tot[BO_SCB] + 3) / 4), 256, 0, st>>>(P, count, off, colmajor, 1, tot[BO_SCB], eB, fB);
+  const int64_t ta = tot[BO_FRAGA] * 64, tb = tot[BO_FRAGB] * 64;
+  k_bo_slice<S><<<(unsigned)((ta + 255) / 256), 256, 0, st>>>(P, count, off, colmajor, 0, ta, eA, sA);
+  k_bo_slice<S><<<(unsigned)((tb + 255) / 256), 256, 0, st>>>(P, count, off, colmajor, 1, tb, eB, sB);
+  k_bo_gemm<S><<<(unsigned)tot[BO_TILES], 256, 0, st>>>(P, count, off, colmajor, sA, sB, part);
+  k_bo_finish<<<(unsigned)((tot[BO_ENT] + 255) / 256), 256, 0, st>>>(P, count, off, colmajor, S, tot[BO_ENT], part,
+                                                                       eA, fA, eB, fB, hdr, list);
+  k_bo_repair<<<OZ_REPAIR_BLOCKS, OZR_BS, 0, st>>>(P, count, off, colmajor, eA, eB, hdr, list);
+}
+
+// After launch_gemm_ozaki_batched_scan on the same workspace: tot = the totals it wrote (read back
+// by the caller), ws at least gemm_ozaki_batched_ws(count, tot, S) bytes.  Header word 0 of ws is
+// the status word (bit 0: a repaired entry fell outside the exact windows and was set to NaN).
+hipError_t launch_gemm_ozaki_batched(int colmajor, int64_t count, const qblas_gemm_problem *P, int S,
+                                     const int64_t *tot, void *wsv, hipStream_t st) {
+  if (tot[BO_ENT] == 0) return hipSuccess;
+  // grids: one block per tile, per 256 entries and per 4 fragment positions
+  const int64_t lim = 0x7fffffffll;
+  if (tot[BO_TILES] > lim || (tot[BO_ENT] + 255) / 256 > lim || (tot[BO_FRAGA] + 3) / 4 > lim ||
+      (tot[BO_FRAGB] + 3) / 4 > lim || (tot[BO_SCA] + 3) / 4 > lim || (tot[BO_SCB] + 3) / 4 > lim ||
+      count > 0xffffffffll || tot[BO_ENT] > 0xffffffffll)
+    return hipErrorInvalidValue;
+  char *ws = (char *)wsv;
+  const bo_layout L = bo_plan(count, tot, S);
+  const int64_t *off = (const int64_t *)(ws + OZ_HDR);
+  switch (S) {
+#define QB_BO_CASE(s) \
+  case s: bo_launch<s>(colmajor, count, P, tot, off, L, ws, st); break;
+    QB_BO_CASE(14) QB_BO_CASE(15) QB_BO_CASE(16) QB_BO_CASE(17) QB_BO_CASE(18) QB_BO_CASE(19)
+#undef QB_BO_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 // ==========================================================================================

@@ -1,0 +1,195 @@
+"""GPU parity of the batched OZAKI qgemm (slice method, one slice count for the whole batch).
+
+Each problem of the batch must equal, bit for bit, the single-problem OZAKI form run with the
+same slice count (method SLICES). The batched kernels run the same per-entry computation:
+  - the same scaling, digits and pass sums;
+  - the same certification and exact repair.
+Sampled entries of every problem must also be within 2^-104 relative error of the oracle's exact
+value (exact zeros exactly). Coverage:
+  - both layouts, padded leading dimensions, empty problems;
+  - every slice-count configuration;
+  - a problem with k > 4096 (two int32 accumulation passes);
+  - rows that cancel to exact zeros and to tiny values (the exact repair path);
+  - the autotuner, which must leave the caller's C untouched and pick a valid configuration.
+"""
+import numpy as np
+import pytest
+
+from helpers import from_dev
+from test_gpu_batched import Pool, make_gemm_batch
+
+pytestmark = pytest.mark.gpu
+
+TOL = 2.0 ** -104
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _strides(layout, ld):
+    return (ld, 1) if layout == 0 else (1, ld)
+
+
+def _restore(pool, cuda, C0, idx_of):
+    for p, c in enumerate(C0):
+        o = pool.offsets[idx_of(p)]
+        pool.dev[o:o + c.shape[0]] = cuda.from_numpy(c.view(np.int64)).to("cuda:0")
+
+
+def _check_oracle(O, layout, dims, arrays, C0, got_of, rng, per_problem=300):
+    worst = 0.0
+    for p, (m, n, k, lda, ldb, ldc, a, b) in enumerate(dims):
+        got = got_of(p)
+        if not (m and n and k):
+            assert np.array_equal(got, C0[p]), ("empty problem changed", p)
+            continue
+        ii, jj = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
+        ii, jj = ii.ravel(), jj.ravel()
+        if ii.shape[0] > per_problem:
+            sel = rng.choice(ii.shape[0], per_problem, replace=False)
+            ii, jj = ii[sel], jj[sel]
+        cs = _strides(layout, ldc)
+        cand = got[ii * cs[0] + jj * cs[1]]
+        _, rel = O.exact_gemm_samples(ii, jj, k, a, arrays[3 * p], _strides(layout, lda), arrays[3 * p + 1],
+                                      _strides(layout, ldb), b, C0[p], cs, cand=cand)
+        assert np.all(rel <= TOL), (p, (m, n, k), float(rel.max()))
+        worst = max(worst, float(rel.max()))
+    return worst
+
+
+def _single_problem(Q, cuda, layout, dims, arrays, S):
+    """C of every problem from the single-problem OZAKI form with S slices (method SLICES)."""
+    Q.set_ozaki_method(Q.OZAKI_SLICES)
+    Q.set_ozaki_slices(S)
+    out = []
+    try:
+        for p, (m, n, k, lda, ldb, ldc, a, b) in enumerate(dims):
+            dA = cuda.from_numpy(arrays[3 * p].view(np.int64)).to("cuda:0")
+            dB = cuda.from_numpy(arrays[3 * p + 1].view(np.int64)).to("cuda:0")
+            dC = cuda.from_numpy(arrays[3 * p + 2].view(np.int64)).to("cuda:0")
+            Q.dev_gemm(Q.MODE_OZAKI, layout, m, n, k, a, dA.data_ptr(), lda, dB.data_ptr(), ldb, b,
+                       dC.data_ptr(), ldc)
+            cuda.cuda.synchronize()
+            out.append(from_dev(dC))
+    finally:
+        Q.set_ozaki_slices(0)
+        Q.set_ozaki_method(Q.OZAKI_AUTO)
+    return out
+
+
+def test_ozaki_batched_every_slice_count(O, Q, cuda):
+    configs = [i for i in range(16) if Q.gemm_batched_config(Q.MODE_OZAKI, i)]
+    assert len(configs) >= 4 and Q.gemm_batched_config(Q.MODE_OZAKI, len(configs)) is None
+    rng = np.random.default_rng(5)
+    for layout in (0, 1):
+        dims, arrays, pool, ddesc = make_gemm_batch(O, cuda, layout, 24, 1, 150, 61 + layout)
+        C0 = [arrays[3 * p + 2] for p in range(len(dims))]
+        for cfg in configs:
+            name = Q.gemm_batched_config(Q.MODE_OZAKI, cfg)
+            S = int(name.split()[0])
+            _say(f"ozaki batched layout={layout} cfg={cfg} ({name})")
+            _restore(pool, cuda, C0, lambda p: 3 * p + 2)
+            Q.gemm_batched_select(Q.MODE_OZAKI, layout, len(dims), cfg)
+            Q.dev_gemm_batched(Q.MODE_OZAKI, layout, len(dims), ddesc.data_ptr())
+            cuda.cuda.synchronize()
+            assert Q.dev_exact_status() == 0
+            assert Q.ozaki_last_slices() == S
+            got = [pool.get(3 * p + 2, C0[p].shape[0]) for p in range(len(dims))]
+            worst = _check_oracle(O, layout, dims, arrays, C0, lambda p: got[p], rng)
+            _say(f"  max rel err {worst:.3e}, repaired entries {Q.dev_ozaki_repairs()}")
+            if cfg in (0, configs[-1]):
+                single = _single_problem(Q, cuda, layout, dims, arrays, S)
+                for p in range(len(dims)):
+                    assert np.array_equal(got[p], single[p]), ("batched != single-problem", layout, cfg, p, dims[p][:3])
+        Q.gemm_batched_select(Q.MODE_OZAKI, layout, len(dims), 0)
+
+
+def _cancel_batch(O, cuda, layout):
+    """Problems whose rows cancel, with beta = 0 so that alpha AB alone decides the certification:
+    A = [X, -X] along k against B = [Y; Y] makes AB exactly zero, and A = [X, -X, D] against
+    B = [Y; Y; Z] with tiny D (|D| < 2^-80) leaves AB = D Z, far below the scale of the rows; plus
+    a problem with k = 4200 (two accumulation passes) and beta != 0."""
+    import qblas_amd as Q
+    specs = [(40, 33, 64, "zero"), (50, 70, 96, "tiny"), (17, 9, 4200, "plain"), (64, 64, 130, "tiny")]
+    dims, arrays = [], []
+    zero = np.zeros(2, np.uint64)
+    for p, (m, n, k, kind) in enumerate(specs):
+        d = 8 if kind == "tiny" else 0
+        h = (k - d) // 2
+        X = O.fill(0, 900 + p, m * h).reshape(m, h, 2)
+        Y = O.fill(0, 950 + p, h * n).reshape(h, n, 2)
+        if kind == "plain":
+            A = O.fill(0, 990 + p, m * k).reshape(m, k, 2)
+            Bm = O.fill(0, 995 + p, k * n).reshape(k, n, 2)
+        else:
+            negX = X.copy()
+            negX[..., 1] ^= np.uint64(1 << 63)
+            A, Bm = [X, negX], [Y, Y]
+            if d:
+                A.append(O.fill(1, 980 + p, m * d, -90, -80).reshape(m, d, 2))
+                Bm.append(O.fill(0, 985 + p, d * n).reshape(d, n, 2))
+            A = np.concatenate(A, axis=1)
+            Bm = np.concatenate(Bm, axis=0)
+        C = O.fill(0, 970 + p, m * n).reshape(m, n, 2)
+        if layout == 0:
+            a_arr, lda = np.ascontiguousarray(A).reshape(-1, 2), k
+            b_arr, ldb = np.ascontiguousarray(Bm).reshape(-1, 2), n
+            c_arr, ldc = np.ascontiguousarray(C).reshape(-1, 2), n
+        else:
+            a_arr, lda = np.ascontiguousarray(A.transpose(1, 0, 2)).reshape(-1, 2), m
+            b_arr, ldb = np.ascontiguousarray(Bm.transpose(1, 0, 2)).reshape(-1, 2), k
+            c_arr, ldc = np.ascontiguousarray(C.transpose(1, 0, 2)).reshape(-1, 2), m
+        arrays += [a_arr, b_arr, c_arr]
+        ab = O.fill(0, 40 + p, 2)
+        dims.append((m, n, k, lda, ldb, ldc, ab[0], ab[1] if kind == "plain" else zero))
+    pool = Pool(cuda, arrays)
+    desc = np.zeros(len(dims), Q.GEMM_PROBLEM)
+    for p, (m, n, k, lda, ldb, ldc, a, b) in enumerate(dims):
+        desc[p] = (m, n, k, lda, ldb, ldc, pool.ptr(3 * p), pool.ptr(3 * p + 1), pool.ptr(3 * p + 2), a, b, 0)
+    ddesc = cuda.from_numpy(desc.view(np.uint8)).to("cuda:0")
+    return dims, arrays, pool, ddesc
+
+
+def test_ozaki_batched_cancellation_and_long_k(O, Q, cuda):
+    rng = np.random.default_rng(6)
+    for layout in (0, 1):
+        dims, arrays, pool, ddesc = _cancel_batch(O, cuda, layout)
+        C0 = [arrays[3 * p + 2] for p in range(len(dims))]
+        _say(f"ozaki batched cancellation layout={layout}")
+        Q.dev_gemm_batched(Q.MODE_OZAKI, layout, len(dims), ddesc.data_ptr())
+        cuda.cuda.synchronize()
+        assert Q.dev_exact_status() == 0
+        repaired = Q.dev_ozaki_repairs()
+        assert repaired >= 40 * 33, repaired      # every entry of the exactly cancelling problem
+        got = [pool.get(3 * p + 2, C0[p].shape[0]) for p in range(len(dims))]
+        worst = _check_oracle(O, layout, dims, arrays, C0, lambda p: got[p], rng, per_problem=400)
+        _say(f"  max rel err {worst:.3e}, repaired entries {repaired}")
+        S = Q.ozaki_last_slices()
+        single = _single_problem(Q, cuda, layout, dims, arrays, S)
+        for p in range(len(dims)):
+            assert np.array_equal(got[p], single[p]), ("batched != single-problem", layout, p, dims[p][:3])
+
+
+def test_ozaki_batched_tuner(O, Q, cuda):
+    layout = 0
+    dims, arrays, pool, ddesc = make_gemm_batch(O, cuda, layout, 48, 8, 200, 71, empties=False)
+    before = pool.dev.clone()
+    best = Q.dev_gemm_batched_tune(Q.MODE_OZAKI, layout, len(dims), ddesc.data_ptr())
+    cuda.cuda.synchronize()
+    assert Q.gemm_batched_config(Q.MODE_OZAKI, best), best
+    assert cuda.equal(pool.dev, before), "the tuner wrote the caller's matrices"
+    _say(f"tuned slice count: {Q.gemm_batched_config(Q.MODE_OZAKI, best)}")
+    Q.dev_gemm_batched(Q.MODE_OZAKI, layout, len(dims), ddesc.data_ptr())
+    cuda.cuda.synchronize()
+    assert Q.dev_exact_status() == 0
+    C0 = [arrays[3 * p + 2] for p in range(len(dims))]
+    got = [pool.get(3 * p + 2, C0[p].shape[0]) for p in range(len(dims))]
+    _check_oracle(O, layout, dims, arrays, C0, lambda p: got[p], np.random.default_rng(7), per_problem=200)
+    Q.gemm_batched_select(Q.MODE_OZAKI, layout, len(dims), 0)
+
+
+def test_ozaki_batched_rejected_for_gemv(Q, cuda):
+    desc = cuda.zeros(64, dtype=cuda.uint8, device="cuda:0")
+    with pytest.raises(RuntimeError):
+        Q.dev_gemv_batched(Q.MODE_OZAKI, 0, 1, desc.data_ptr())
