@@ -193,3 +193,124 @@ def test_ozaki_batched_rejected_for_gemv(Q, cuda):
     desc = cuda.zeros(64, dtype=cuda.uint8, device="cuda:0")
     with pytest.raises(RuntimeError):
         Q.dev_gemv_batched(Q.MODE_OZAKI, 0, 1, desc.data_ptr())
+
+
+def _quantiles(v):
+    v = np.sort(np.asarray(v, dtype=np.float64))
+    q = lambda f: float(v[min(len(v) - 1, int(f * len(v)))])
+    return {"median": q(0.5), "p90": q(0.9), "p99": q(0.99), "max": float(v[-1])}
+
+
+def test_config4_full_size_error_distribution(O, Q, cuda):
+    """The batched workload at full size: 4096 qgemm problems (m, n, k uniform in [8, 256]) and
+    16384 qgemv problems (m, n uniform in [32, 1024]), RowMajor, entries and per-problem alpha, beta
+    full-mantissa uniform [-1, 1) (device generator).  Per problem, the largest relative error of
+    sampled entries against the oracle's exact value; its distribution over the problems is
+    printed for every form.  OZAKI gemm stays within 2^-104 and EXACT gemv is correctly rounded
+    on every sample."""
+    torch = cuda
+    rng = np.random.default_rng(77)
+    # ---- gemm batch
+    count = 4096
+    dims = rng.integers(8, 257, size=(count, 3))
+    sizes = dims[:, 0] * dims[:, 2] + dims[:, 2] * dims[:, 1] + dims[:, 0] * dims[:, 1]
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    pool = torch.empty((int(offs[-1]), 2), dtype=torch.int64, device="cuda:0")
+    Q.dev_fill(Q.FILL_UNIFORM_FULL, 77, pool.shape[0], pool.data_ptr())
+    ab = torch.empty((2 * count, 2), dtype=torch.int64, device="cuda:0")
+    Q.dev_fill(Q.FILL_UNIFORM_FULL, 78, ab.shape[0], ab.data_ptr())
+    torch.cuda.synchronize()
+    abh = from_dev(ab)
+    host = from_dev(pool)
+    desc = np.zeros(count, Q.GEMM_PROBLEM)
+    base = pool.data_ptr()
+    S_PER = 4
+    samples = []
+    for p in range(count):
+        m, n, k = (int(v) for v in dims[p])
+        o = int(offs[p])
+        desc[p] = (m, n, k, k, n, n, base + 16 * o, base + 16 * (o + m * k), base + 16 * (o + m * k + k * n),
+                   abh[2 * p], abh[2 * p + 1], 0)
+        samples.append((rng.integers(0, m, S_PER), rng.integers(0, n, S_PER)))
+    ddesc = torch.from_numpy(desc.view(np.uint8)).to("cuda:0")
+    cpos = np.concatenate([offs[p] + dims[p, 0] * dims[p, 2] + dims[p, 2] * dims[p, 1] +
+                           samples[p][0] * dims[p, 1] + samples[p][1] for p in range(count)])
+    cidx = torch.from_numpy(cpos).to("cuda:0")
+    pool0 = pool.clone()
+    for form in ("ozaki", "dd", "reference"):
+        mode = Q.MODES[form]
+        pool.copy_(pool0)
+        if form == "ozaki":
+            best = Q.dev_gemm_batched_tune(mode, 0, count, ddesc.data_ptr())
+            _say(f"config 4 gemm: tuned {Q.gemm_batched_config(mode, best)}")
+        Q.dev_gemm_batched(mode, 0, count, ddesc.data_ptr())
+        torch.cuda.synchronize()
+        if form == "ozaki":
+            assert Q.dev_exact_status() == 0
+        got = from_dev(pool[cidx])
+        worst = []
+        for p in range(count):
+            m, n, k = (int(v) for v in dims[p])
+            o = int(offs[p])
+            A, B = host[o:o + m * k], host[o + m * k:o + m * k + k * n]
+            C0 = host[o + m * k + k * n:o + m * k + k * n + m * n]
+            _, rel = O.exact_gemm_samples(samples[p][0], samples[p][1], k, abh[2 * p], A, (k, 1), B, (n, 1),
+                                          abh[2 * p + 1], C0, (n, 1), cand=got[S_PER * p:S_PER * (p + 1)])
+            worst.append(float(rel.max()))
+        qs = _quantiles(worst)
+        _say(f"config 4 gemm {form}: per-problem max rel err over {S_PER} samples: " +
+             ", ".join(f"{key} {val:.3e}" for key, val in qs.items()))
+        if form == "ozaki":
+            assert qs["max"] <= TOL
+    del pool, pool0, host, cidx
+    torch.cuda.empty_cache()
+
+    # ---- gemv batch (A is 73 GB on the device; one sampled row per problem is copied back)
+    count = 16384
+    mn = rng.integers(32, 1025, size=(count, 2))
+    asz = mn[:, 0] * mn[:, 1]
+    offs = np.concatenate([[0], np.cumsum(asz + mn[:, 1] + mn[:, 0])]).astype(np.int64)
+    pool = torch.empty((int(offs[-1]), 2), dtype=torch.int64, device="cuda:0")
+    Q.dev_fill(Q.FILL_UNIFORM_FULL, 79, pool.shape[0], pool.data_ptr())
+    ab = torch.empty((2 * count, 2), dtype=torch.int64, device="cuda:0")
+    Q.dev_fill(Q.FILL_UNIFORM_FULL, 80, ab.shape[0], ab.data_ptr())
+    torch.cuda.synchronize()
+    abh = from_dev(ab)
+    desc = np.zeros(count, Q.GEMV_PROBLEM)
+    base = pool.data_ptr()
+    rows, arow, xv = rng.integers(0, mn[:, 0]), [], []
+    for p in range(count):
+        m, n = int(mn[p, 0]), int(mn[p, 1])
+        o = int(offs[p])
+        desc[p] = (m, n, n, 1, 1, base + 16 * o, base + 16 * (o + m * n), base + 16 * (o + m * n + n),
+                   abh[2 * p], abh[2 * p + 1])
+        i = int(rows[p])
+        arow.append(from_dev(pool[o + i * n:o + (i + 1) * n]))
+        xv.append(from_dev(pool[o + m * n:o + m * n + n]))
+    ddesc = torch.from_numpy(desc.view(np.uint8)).to("cuda:0")
+    ypos = np.concatenate([offs[p] + asz[p] + mn[p, 1] + np.arange(mn[p, 0]) for p in range(count)])
+    yidx = torch.from_numpy(ypos).to("cuda:0")
+    y0 = pool[yidx].clone()
+    spos = torch.from_numpy((offs[:-1] + asz + mn[:, 1] + rows).astype(np.int64)).to("cuda:0")
+    y0s = from_dev(pool[spos])
+    zero_i = np.zeros(1, np.int64)
+    for form in ("exact", "dd", "reference"):
+        mode = Q.MODES[form]
+        pool[yidx] = y0
+        Q.dev_gemv_batched(mode, 0, count, ddesc.data_ptr())
+        torch.cuda.synchronize()
+        if form == "exact":
+            assert Q.dev_exact_status() == 0
+        got = from_dev(pool[spos])
+        worst, mism = [], 0
+        for p in range(count):
+            n = int(mn[p, 1])
+            rounded, rel = O.exact_gemm_samples(zero_i, zero_i, n, abh[2 * p], arow[p], (n, 1), xv[p], (1, 0),
+                                                abh[2 * p + 1], y0s[p:p + 1], (0, 0), cand=got[p:p + 1])
+            worst.append(float(rel[0]))
+            mism += int(not np.array_equal(rounded[0], got[p]))
+        qs = _quantiles(worst)
+        _say(f"config 4 gemv {form}: rel err of one sampled row per problem: " +
+             ", ".join(f"{key} {val:.3e}" for key, val in qs.items()) + f"; not correctly rounded: {mism}")
+        if form == "exact":
+            assert mism == 0
