@@ -76,6 +76,7 @@ def lib():
         L.or_rand64.restype = ctypes.c_uint64
         L.or_rand64.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
         L.or_fill.argtypes = [ctypes.c_int, ctypes.c_uint64, sz, ctypes.c_int, ctypes.c_int, P]
+        L.or_fill_range.argtypes = [ctypes.c_int, ctypes.c_uint64, sz, sz, ctypes.c_int, ctypes.c_int, P]
         L.or_fill_pair_y.argtypes = [ctypes.c_uint64, sz, P]
         L.or_q2d.restype = ctypes.c_double
         L.or_q2d.argtypes = [Q128]
@@ -266,9 +267,10 @@ KIND_LOG_UNIFORM = 1    # sign +-, exponent uniform in [lo, hi), 112 random frac
 KIND_CANCEL_PAIRS = 2   # +-(1e20*u) pairs on every 8-block's first two slots, kind 0 elsewhere
 
 
-def fill(kind, seed, n, lo_exp=-30, hi_exp=30):
+def fill(kind, seed, n, lo_exp=-30, hi_exp=30, start=0):
+    """Values start .. start + n - 1 of the seeded sequence of the given kind."""
     out = np.empty((n, 2), dtype=np.uint64)
-    lib().or_fill(kind, seed, n, lo_exp, hi_exp, _ptr(out))
+    lib().or_fill_range(kind, seed, start, n, lo_exp, hi_exp, _ptr(out))
     return out
 
 

@@ -678,30 +678,36 @@ static double u11(uint64_t r) { return (double)(r >> 11) * 0x1p-52 - 1.0; }
    (u in [1, 2), exact in double), formed by an exact binary128 multiply (53 x 47 bits); entries with
    i % 8 >= 2 are instead kind-0 values (the O(1) residue).  The matching y vector repeats
    y_{i+1} = y_i on the pairs so that the pair products cancel exactly. */
-void or_fill(int kind, uint64_t seed, size_t n, int lo_exp, int hi_exp, q128 *out) {
+void or_fill_range(int kind, uint64_t seed, size_t start, size_t n, int lo_exp, int hi_exp, q128 *out) {
     #pragma omp parallel for schedule(static)
-    for (size_t i = 0; i < n; ++i) {
+    for (size_t t = 0; t < n; ++t) {
+        const size_t i = start + t;
+        q128 *o = out + t;
         if (kind == 0) {
             double a = u11(or_rand64(seed, 2 * i)), b = u11(or_rand64(seed, 2 * i + 1));
-            out[i] = f2q((__float128)a + (__float128)b * 0x1p-60Q);
+            *o = f2q((__float128)a + (__float128)b * 0x1p-60Q);
         } else if (kind == 1) {
             uint64_t r0 = or_rand64(seed, 3 * i), r1 = or_rand64(seed, 3 * i + 1), r2 = or_rand64(seed, 3 * i + 2);
             uint32_t span = (uint32_t)(hi_exp - lo_exp);
             int e = lo_exp + (int)((uint32_t)r0 % span);
             q128 v; v.lo = r2; v.hi = (r0 & (1ull << 63)) | ((uint64_t)(e + BIAS) << 48) | (r1 >> 16);
-            out[i] = v;
+            *o = v;
         } else {
             size_t p = i & ~(size_t)1;
             if ((p % 8) < 2) {
                 double u = 1.0 + (double)(or_rand64(seed, p) >> 12) * 0x1p-52;
                 __float128 v = (__float128)u * (__float128)1e20;
-                out[i] = f2q((i & 1) ? -v : v);
+                *o = f2q((i & 1) ? -v : v);
             } else {
                 double a = u11(or_rand64(seed, 2 * i)), b = u11(or_rand64(seed, 2 * i + 1));
-                out[i] = f2q((__float128)a + (__float128)b * 0x1p-60Q);
+                *o = f2q((__float128)a + (__float128)b * 0x1p-60Q);
             }
         }
     }
+}
+/* the first n values of a sequence (or_fill_range writes values start .. start + n - 1) */
+void or_fill(int kind, uint64_t seed, size_t n, int lo_exp, int hi_exp, q128 *out) {
+    or_fill_range(kind, seed, 0, n, lo_exp, hi_exp, out);
 }
 /* y vector for the cancellation variant: kind-0 values from the stream seed + 1, with
    y_{i+1} = y_i on each pair (p % 8 < 2) */

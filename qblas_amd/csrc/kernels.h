@@ -10,6 +10,19 @@
 namespace qbk {
 using qb::q128;
 
+// One launch dimension holds at most 2^32 - 1 work-items: the dispatch packet's grid size is a
+// 32-bit work-item count, and a larger launch covers only the count modulo 2^32, without an
+// error.  Element-wise kernels therefore stride over their elements from a capped grid
+// (stride_blocks); the launchers of the other kernels check grid_fits and fail otherwise.
+constexpr int64_t QB_GRID_ITEMS_MAX = 0xffffffffll;
+inline bool grid_fits(int64_t blocks, int64_t block_size) {
+  return blocks >= 0 && blocks <= QB_GRID_ITEMS_MAX / block_size;
+}
+inline unsigned stride_blocks(int64_t items, int block_size) {
+  const int64_t b = (items + block_size - 1) / block_size;
+  return (unsigned)(b < 65536 ? b : 65536);
+}
+
 // reference order (kernels_ref.hip)
 hipError_t launch_dot_ref(const q128 *x, int64_t incx, const q128 *y, int64_t incy, int64_t n, int T,
                           q128 *part /* 2*T */, q128 *out, hipStream_t st);

@@ -119,11 +119,12 @@ __global__ void __launch_bounds__(256) k_dot_dd_final(const dd *__restrict__ par
 // alpha*x_j like level2.hpp:71; RowMajor uses scale = 1).
 __global__ void k_vec_to_dd(int64_t n, q128 scale, int use_scale, const q128 *__restrict__ x, int64_t incx,
                             dd *__restrict__ xs) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  q128 v = ldq_cached(x + j * incx);
-  if (use_scale) v = mul_packed(scale, v);
-  st_dd(xs + j, q_to_dd(v));
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+    q128 v = ldq_cached(x + j * incx);
+    if (use_scale) v = mul_packed(scale, v);
+    st_dd(xs + j, q_to_dd(v));
+  }
 }
 
 // RowMajor: R rows per wave (R = 2, and 1 for the last row of an odd m), lanes stride along the
@@ -223,20 +224,21 @@ __global__ void __launch_bounds__(GC_BS) k_gemv_col_dd_partial(int64_t m, int64_
 // y_i = RN(beta * y_i) + RN(sum of the chunk partials), the partials read 8 at a time
 __global__ void k_gemv_col_dd_final(int64_t m, int nchunks, const dd *__restrict__ part, q128 beta,
                                     q128 *__restrict__ y, int64_t incy) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  dd s = dd_zero();
-  int c = 0;
-  for (; c + 7 < nchunks; c += 8) {
-    dd t[8];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+    dd s = dd_zero();
+    int c = 0;
+    for (; c + 7 < nchunks; c += 8) {
+      dd t[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) t[u] = ld_dd(part + (int64_t)(c + u) * m + i);
+      for (int u = 0; u < 8; ++u) t[u] = ld_dd(part + (int64_t)(c + u) * m + i);
 #pragma unroll
-    for (int u = 0; u < 8; ++u) s = dd_add(s, t[u]);
+      for (int u = 0; u < 8; ++u) s = dd_add(s, t[u]);
+    }
+    for (; c < nchunks; ++c) s = dd_add(s, ld_dd(part + (int64_t)c * m + i));
+    q128 *yi = y + i * incy;
+    stq(yi, add_packed(mul_packed(beta, ldq_cached(yi)), dd_to_q(s)));
   }
-  for (; c < nchunks; ++c) s = dd_add(s, ld_dd(part + (int64_t)c * m + i));
-  q128 *yi = y + i * incy;
-  stq(yi, add_packed(mul_packed(beta, ldq_cached(yi)), dd_to_q(s)));
 }
 
 // ------------------------------------------------------------------------------------------
@@ -249,10 +251,11 @@ constexpr int GD_K = 16;   // k step staged in LDS
 // out[r * Cp + c] = dd(X[r * ld + c]) for r < R, c < C; zeros in the padding up to Rp x Cp
 __global__ void k_rows_to_dd(const q128 *__restrict__ X, int64_t R, int64_t C, int64_t ld, dd *__restrict__ out,
                              int64_t Rp, int64_t Cp) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= Rp * Cp) return;
-  const int64_t r = g / Cp, c = g - r * Cp;
-  st_dd(out + g, (r < R && c < C) ? q_to_dd(ldq(X + r * ld + c)) : dd_zero());
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < Rp * Cp; g += stride) {
+    const int64_t r = g / Cp, c = g - r * Cp;
+    st_dd(out + g, (r < R && c < C) ? q_to_dd(ldq(X + r * ld + c)) : dd_zero());
+  }
 }
 
 // C entry = RN(alpha * RN(acc) + RN(beta * c)), the reference's epilogue (level3.hpp:297-300);
@@ -369,7 +372,8 @@ hipError_t launch_gemv_dd(int colmajor, int64_t m, int64_t n, q128 alpha, const 
                           const q128 *x, int64_t incx, q128 beta, q128 *y, int64_t incy, void *ws,
                           hipStream_t st) {
   dd *xs = (dd *)ws;
-  hipLaunchKernelGGL(k_vec_to_dd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, alpha, colmajor, x, incx,
+  if (colmajor && !grid_fits((m + GC_BS - 1) / GC_BS, GC_BS)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_vec_to_dd, dim3(stride_blocks(n, 256)), dim3(256), 0, st, n, alpha, colmajor, x, incx,
                      xs);
   if (!colmajor) {
     const int64_t pairs = (m + 1) / 2;  // two rows per wave and step
@@ -384,7 +388,7 @@ hipError_t launch_gemv_dd(int colmajor, int64_t m, int64_t n, q128 alpha, const 
     dd *part = xs + n;
     hipLaunchKernelGGL(k_gemv_col_dd_partial, dim3((unsigned)((m + GC_BS - 1) / GC_BS), (unsigned)nch_eff),
                        dim3(GC_BS), 0, st, m, n, chunk, A, lda, xs, part);
-    hipLaunchKernelGGL(k_gemv_col_dd_final, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, m, nch_eff, part,
+    hipLaunchKernelGGL(k_gemv_col_dd_final, dim3(stride_blocks(m, 256)), dim3(256), 0, st, m, nch_eff, part,
                        beta, y, incy);
   }
   return hipGetLastError();
@@ -413,8 +417,9 @@ hipError_t launch_gemm_dd(int colmajor, int64_t m, int64_t n, int64_t k, q128 al
   }
   const int64_t Mp = round_up(m, GD_T), Np = round_up(n, GD_T), Kp = round_up(k, GD_K);
   dd *Ad = (dd *)ws, *Bd = Ad + Mp * Kp;
-  k_rows_to_dd<<<(unsigned)((Mp * Kp + 255) / 256), 256, 0, st>>>(A, m, k, lda, Ad, Mp, Kp);
-  k_rows_to_dd<<<(unsigned)((Kp * Np + 255) / 256), 256, 0, st>>>(B, k, n, ldb, Bd, Kp, Np);
+  if (!grid_fits(Np / GD_T, 256)) return hipErrorInvalidValue;
+  k_rows_to_dd<<<stride_blocks(Mp * Kp, 256), 256, 0, st>>>(A, m, k, lda, Ad, Mp, Kp);
+  k_rows_to_dd<<<stride_blocks(Kp * Np, 256), 256, 0, st>>>(B, k, n, ldb, Bd, Kp, Np);
   dim3 grid((unsigned)(Np / GD_T), (unsigned)(Mp / GD_T));
   k_gemm_dd<<<grid, 256, 0, st>>>(m, n, Kp, Ad, Bd, Np, alpha, beta, C, ldc);
   return hipGetLastError();

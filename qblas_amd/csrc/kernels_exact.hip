@@ -109,41 +109,43 @@ __global__ void __launch_bounds__(XR_BS) k_gemv_row_exact(int64_t m, int64_t n, 
                                                           const q128 *__restrict__ A, int64_t lda,
                                                           const q128 *__restrict__ x, int64_t incx,
                                                           part_t *__restrict__ parts) {
-  const int64_t wid = (int64_t)blockIdx.x * (XR_BS / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (wid >= m * nch) return;  // whole waves
-  const int64_t row = wid / nch;
-  const int c = (int)(wid - row * nch);
-  const int64_t j0 = (int64_t)c * chunk;
-  const int64_t j1 = (j0 + chunk < n) ? j0 + chunk : n;
-  const q128 *r = A + row * lda;
-  // anchor from the row's first elements, so every column chunk of the row shares it
-  int e0 = EXP_NONE;
-  if (lane < n) e0 = pair_exp(ldq_c(r + lane), ldq_c(x + (int64_t)lane * incx));
-  const int An = wave_anchor(e0);
-  uint32_t Wl[WW], Wh[WW];
+  const int64_t nw = (int64_t)gridDim.x * (XR_BS / 64);
+  // whole waves stride over the (row, chunk) items
+  for (int64_t wid = (int64_t)blockIdx.x * (XR_BS / 64) + (threadIdx.x >> 6); wid < m * nch; wid += nw) {
+    const int64_t row = wid / nch;
+    const int c = (int)(wid - row * nch);
+    const int64_t j0 = (int64_t)c * chunk;
+    const int64_t j1 = (j0 + chunk < n) ? j0 + chunk : n;
+    const q128 *r = A + row * lda;
+    // anchor from the row's first elements, so every column chunk of the row shares it
+    int e0 = EXP_NONE;
+    if (lane < n) e0 = pair_exp(ldq_c(r + lane), ldq_c(x + (int64_t)lane * incx));
+    const int An = wave_anchor(e0);
+    uint32_t Wl[WW], Wh[WW];
 #pragma unroll
-  for (int k = 0; k < WW; ++k) {
-    Wl[k] = 0;
-    Wh[k] = 0;
-  }
-  uint32_t flags = 0;
-  int64_t j = j0 + lane;
-  if (incx == 1) {
-    for (; j + 64 < j1; j += 128) {
-      const q128 a0 = ldq_nt(r + j), a1 = ldq_nt(r + j + 64);
-      const q128 b0 = ldq_c(x + j), b1 = ldq_c(x + j + 64);
-      accumulate(Wl, Wh, An, a0, b0, flags);
-      accumulate(Wl, Wh, An, a1, b1, flags);
+    for (int k = 0; k < WW; ++k) {
+      Wl[k] = 0;
+      Wh[k] = 0;
     }
-    if (j < j1) accumulate(Wl, Wh, An, ldq_nt(r + j), ldq_c(x + j), flags);
-  } else {
-    for (; j < j1; j += 64) accumulate(Wl, Wh, An, ldq_nt(r + j), ldq_c(x + j * incx), flags);
+    uint32_t flags = 0;
+    int64_t j = j0 + lane;
+    if (incx == 1) {
+      for (; j + 64 < j1; j += 128) {
+        const q128 a0 = ldq_nt(r + j), a1 = ldq_nt(r + j + 64);
+        const q128 b0 = ldq_c(x + j), b1 = ldq_c(x + j + 64);
+        accumulate(Wl, Wh, An, a0, b0, flags);
+        accumulate(Wl, Wh, An, a1, b1, flags);
+      }
+      if (j < j1) accumulate(Wl, Wh, An, ldq_nt(r + j), ldq_c(x + j), flags);
+    } else {
+      for (; j < j1; j += 64) accumulate(Wl, Wh, An, ldq_nt(r + j), ldq_c(x + j * incx), flags);
+    }
+    wave_sum_win(Wl);
+    wave_sum_win(Wh);
+    flags = wave_or(flags);
+    if (lane == 0) put_parts(parts + 2 * ((int64_t)c * m + row), An, flags, Wl, Wh);
   }
-  wave_sum_win(Wl);
-  wave_sum_win(Wh);
-  flags = wave_or(flags);
-  if (lane == 0) put_parts(parts + 2 * ((int64_t)c * m + row), An, flags, Wl, Wh);
 }
 
 // ColMajor: lane = row i, blockIdx.y = column chunk; per-lane anchor.
@@ -311,40 +313,41 @@ __global__ void __launch_bounds__(64) k_exact_finish_dot(const col_rec *__restri
 __global__ void __launch_bounds__(64) k_exact_finish_rows(int64_t m, int nch, const part_t *__restrict__ parts,
                                                          q128 alpha, q128 beta, q128 *__restrict__ y, int64_t incy,
                                                          uint32_t *__restrict__ status) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  uint32_t lo[WW], hi[WW];
-#pragma unroll
-  for (int k = 0; k < WW; ++k) {
-    lo[k] = 0;
-    hi[k] = 0;
-  }
-  uint32_t fl = 0;
-  const int A0 = parts[2 * i].A;
-  for (int c = 0; c < nch; ++c) {
-    const part_t *p = parts + 2 * ((int64_t)c * m + i);
-    fl |= p[0].flags | p[1].flags;
-    if (p[0].A != A0) fl |= FL_LOST;  // chunks of a row share its anchor by construction
-    uint32_t X[WW], Y[WW];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+    uint32_t lo[WW], hi[WW];
 #pragma unroll
     for (int k = 0; k < WW; ++k) {
-      X[k] = p[0].W[k];
-      Y[k] = p[1].W[k];
+      lo[k] = 0;
+      hi[k] = 0;
     }
-    win_add_win(lo, X);
-    win_add_win(hi, Y);
+    uint32_t fl = 0;
+    const int A0 = parts[2 * i].A;
+    for (int c = 0; c < nch; ++c) {
+      const part_t *p = parts + 2 * ((int64_t)c * m + i);
+      fl |= p[0].flags | p[1].flags;
+      if (p[0].A != A0) fl |= FL_LOST;  // chunks of a row share its anchor by construction
+      uint32_t X[WW], Y[WW];
+#pragma unroll
+      for (int k = 0; k < WW; ++k) {
+        X[k] = p[0].W[k];
+        Y[k] = p[1].W[k];
+      }
+      win_add_win(lo, X);
+      win_add_win(hi, Y);
+    }
+    q128 *yi = y + i * incy;
+    q128 r;
+    if (fl & FL_LOST) {
+      r = q_nan();
+      atomicOr(status, 1u);
+    } else {
+      uint32_t M[SW], sign;
+      s_from_windows(M, sign, lo, hi);
+      r = finish_reg(M, A0, sign, 0u, fl, true, alpha, beta, ldq_c(yi));
+    }
+    stq_c(yi, r);
   }
-  q128 *yi = y + i * incy;
-  q128 r;
-  if (fl & FL_LOST) {
-    r = q_nan();
-    atomicOr(status, 1u);
-  } else {
-    uint32_t M[SW], sign;
-    s_from_windows(M, sign, lo, hi);
-    r = finish_reg(M, A0, sign, 0u, fl, true, alpha, beta, ldq_c(yi));
-  }
-  stq_c(yi, r);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -413,13 +416,14 @@ hipError_t launch_gemv_exact(int colmajor, int64_t m, int64_t n, q128 alpha, con
   if (e != hipSuccess) return e;
   if (!colmajor) {
     const int64_t waves = m * nch;
-    hipLaunchKernelGGL(k_gemv_row_exact, dim3((unsigned)((waves + XR_BS / 64 - 1) / (XR_BS / 64))), dim3(XR_BS), 0,
+    hipLaunchKernelGGL(k_gemv_row_exact, dim3(stride_blocks(waves, XR_BS / 64)), dim3(XR_BS), 0,
                        st, m, n, chunk, nch, A, lda, x, incx, parts);
   } else {
+    if (!grid_fits((m + XC_BS - 1) / XC_BS, XC_BS)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_gemv_col_exact, dim3((unsigned)((m + XC_BS - 1) / XC_BS), (unsigned)nch), dim3(XC_BS), 0,
                        st, m, n, chunk, A, lda, x, incx, parts);
   }
-  hipLaunchKernelGGL(k_exact_finish_rows, dim3((unsigned)((m + 63) / 64)), dim3(64), 0, st, m, nch, parts, alpha,
+  hipLaunchKernelGGL(k_exact_finish_rows, dim3(stride_blocks(m, 64)), dim3(64), 0, st, m, nch, parts, alpha,
                      beta, y, incy, status);
   return hipGetLastError();
 }

@@ -551,17 +551,18 @@ __global__ void __launch_bounds__(256) k_oz_finish(int64_t M, int64_t N, int S, 
                                                    const int32_t *__restrict__ eB, const uint32_t *__restrict__ fB,
                                                    q128 alpha, q128 beta, q128 *__restrict__ C, int64_t csr,
                                                    int64_t csc, uint32_t *__restrict__ hdr, uint32_t *__restrict__ list) {
-  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (g >= M * N) return;
-  const int64_t i = g / N, j = g - i * N;
-  q128 *cp = C + i * csr + j * csc;
-  q128 res;
-  if (oz_finish_value(i, j, S, kfac, npass, part, Mp, Np, eA, fA, eB, fB, alpha, beta, ldq_c(cp), res)) {
-    stq_c(cp, res);
-  } else {
-    const uint32_t slot = atomicAdd(hdr + 1, 1u);
-    list[2 * (int64_t)slot] = (uint32_t)i;
-    list[2 * (int64_t)slot + 1] = (uint32_t)j;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x; g < M * N; g += stride) {
+    const int64_t i = g / N, j = g - i * N;
+    q128 *cp = C + i * csr + j * csc;
+    q128 res;
+    if (oz_finish_value(i, j, S, kfac, npass, part, Mp, Np, eA, fA, eB, fB, alpha, beta, ldq_c(cp), res)) {
+      stq_c(cp, res);
+    } else {
+      const uint32_t slot = atomicAdd(hdr + 1, 1u);
+      list[2 * (int64_t)slot] = (uint32_t)i;
+      list[2 * (int64_t)slot + 1] = (uint32_t)j;
+    }
   }
 }
 
@@ -763,6 +764,15 @@ static void oz_launch_products(const oz_layout &L, char *ws, hipStream_t st) {
   }
 }
 
+// every launch of a call fits the 32-bit grid size: scaling (up to one wave per row), slicing
+// (64 threads per 32 x 32 fragment position; row-contiguous: 4 k blocks per workgroup along x),
+// products (Np / 64 workgroups along x)
+static bool oz_grids_fit(int64_t Mp, int64_t Np, int64_t Kb) {
+  const int64_t R = Mp > Np ? Mp : Np;
+  return grid_fits((R + 3) / 4, 256) && grid_fits((R / 32) * Kb * 64 / 256 + 1, 256) &&
+         grid_fits((Kb + OZS_KB - 1) / OZS_KB, 256);
+}
+
 static void oz_scale(const q128 *X, int64_t R, int64_t Cn, int64_t sr, int64_t sc, int32_t *E, uint32_t *F,
                      hipStream_t st) {
   const int contig = (sc == 1);
@@ -835,6 +845,7 @@ hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int 
   if (S < OZ_SMIN || S > OZ_SMAX) return hipErrorInvalidValue;
   char *ws = (char *)wsv;
   const oz_layout L = oz_plan(m, n, k, S);
+  if (!oz_grids_fit(L.Mp, L.Np, L.Kb)) return hipErrorInvalidValue;
   // element strides: A(i,l), B(l,j), C(i,j)
   const int64_t asr = colmajor ? 1 : lda, asc = colmajor ? lda : 1;
   const int64_t bsr = colmajor ? 1 : ldb, bsc = colmajor ? ldb : 1;
@@ -871,7 +882,7 @@ hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int 
   int kfac = 0;
   while ((double)(1ull << (kfac < 62 ? kfac : 62)) < kb && kfac < 80) ++kfac;
   const int64_t mn = m * n;
-  k_oz_finish<<<(unsigned)((mn + 255) / 256), 256, 0, st>>>(m, n, S, kfac, (int)L.npass, (const uint32_t *)(ws + L.part),
+  k_oz_finish<<<stride_blocks(mn, 256), 256, 0, st>>>(m, n, S, kfac, (int)L.npass, (const uint32_t *)(ws + L.part),
                                                             L.Mp, L.Np, eA, fA, eB, fB, alpha, beta, C, csr, csc, hdr,
                                                             (uint32_t *)(ws + L.list));
   if (ev) (void)hipEventRecord(ev[3], st);
@@ -898,6 +909,7 @@ hipError_t launch_ozaki_probe(int colmajor, int64_t m, int64_t n, int64_t k, con
   const int64_t asr = colmajor ? 1 : lda, asc = colmajor ? lda : 1;
   const int64_t bsr = colmajor ? 1 : ldb, bsc = colmajor ? ldb : 1;
   const int64_t ninit = (m > n ? m : n) > OZ_HDR / 4 ? (m > n ? m : n) : OZ_HDR / 4;
+  if (!grid_fits((ninit + 3) / 4, 256)) return hipErrorInvalidValue;
   k_oz_init<<<(unsigned)((ninit + 255) / 256), 256, 0, st>>>(eA, fA, m, eB, fB, n, hdr);
   oz_scale(A, m, k, asr, asc, eA, fA, st);
   oz_scale(B, n, k, bsc, bsr, eB, fB, st);
@@ -1236,11 +1248,13 @@ static void bo_launch(int colmajor, int64_t count, const qblas_gemm_problem *P, 
 hipError_t launch_gemm_ozaki_batched(int colmajor, int64_t count, const qblas_gemm_problem *P, int S,
                                      const int64_t *tot, void *wsv, hipStream_t st) {
   if (tot[BO_ENT] == 0) return hipSuccess;
-  // grids: one block per tile, per 256 entries and per 4 fragment positions
-  const int64_t lim = 0x7fffffffll;
-  if (tot[BO_TILES] > lim || (tot[BO_ENT] + 255) / 256 > lim || (tot[BO_FRAGA] + 3) / 4 > lim ||
-      (tot[BO_FRAGB] + 3) / 4 > lim || (tot[BO_SCA] + 3) / 4 > lim || (tot[BO_SCB] + 3) / 4 > lim ||
-      count > 0xffffffffll || tot[BO_ENT] > 0xffffffffll)
+  // grids: one workgroup per tile, one thread per entry and per (fragment position, lane), one
+  // wave per scaling item; queued entries are numbered in 32 bits
+  const int64_t ni = tot[BO_ROWS] > tot[BO_COLS] ? tot[BO_ROWS] : tot[BO_COLS];
+  if (!grid_fits(tot[BO_TILES], 256) || !grid_fits((tot[BO_ENT] + 255) / 256, 256) ||
+      !grid_fits(tot[BO_FRAGA] / 4 + 1, 256) || !grid_fits(tot[BO_FRAGB] / 4 + 1, 256) ||
+      !grid_fits((tot[BO_SCA] + 3) / 4, 256) || !grid_fits((tot[BO_SCB] + 3) / 4, 256) ||
+      !grid_fits((ni + 255) / 256 + 1, 256) || count > 0xffffffffll)
     return hipErrorInvalidValue;
   char *ws = (char *)wsv;
   const bo_layout L = bo_plan(count, tot, S);
@@ -1905,50 +1919,51 @@ __global__ void __launch_bounds__(256) k_crt_finish_ab(int64_t M, int64_t N, int
                                                        int64_t csc, uint32_t *__restrict__ hdr,
                                                        uint32_t *__restrict__ list) {
   constexpr int RW = (XW + 3) & ~3;
-  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (g >= M * N) return;
-  const int64_t i = g / N, j = g - i * N;
-  q128 *cp = C + i * csr + j * csc;
-  const q128 yv = ldq_c(cp);
-  const bool ab_ok = (q_bexp(alpha) != 0x7fffu) && (q_bexp(beta) != 0x7fffu) && (q_bexp(yv) != 0x7fffu);
-  bool done = false;
-  q128 res;
-  const bool fin = !((fA[i] | fB[j]) & OZF_NONFINITE);
-  if (ab_ok && fin && q_iszero(alpha)) {
-    res = mul_packed(beta, yv);
-    done = !q_iszero(res);
-  } else if (ab_ok && fin && eA[i] != OZ_EZERO && eB[j] != OZ_EZERO) {
-    uint32_t X[XW];
-    const uint32_t *xr = xrec + g * RW;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x; g < M * N; g += stride) {
+    const int64_t i = g / N, j = g - i * N;
+    q128 *cp = C + i * csr + j * csc;
+    const q128 yv = ldq_c(cp);
+    const bool ab_ok = (q_bexp(alpha) != 0x7fffu) && (q_bexp(beta) != 0x7fffu) && (q_bexp(yv) != 0x7fffu);
+    bool done = false;
+    q128 res;
+    const bool fin = !((fA[i] | fB[j]) & OZF_NONFINITE);
+    if (ab_ok && fin && q_iszero(alpha)) {
+      res = mul_packed(beta, yv);
+      done = !q_iszero(res);
+    } else if (ab_ok && fin && eA[i] != OZ_EZERO && eB[j] != OZ_EZERO) {
+      uint32_t X[XW];
+      const uint32_t *xr = xrec + g * RW;
 #pragma unroll
-    for (int w = 0; w < RW; w += 4) {
-      const i32x4v v = *reinterpret_cast<const i32x4v *>(xr + w);
-      if (w < XW) X[w] = (uint32_t)v.x;
-      if (w + 1 < XW) X[w + 1] = (uint32_t)v.y;
-      if (w + 2 < XW) X[w + 2] = (uint32_t)v.z;
-      if (w + 3 < XW) X[w + 3] = (uint32_t)v.w;
-    }
-    const uint32_t sgn = X[XW - 1] >> 31;
-    if (sgn) {
-      uint32_t c = 1;
+      for (int w = 0; w < RW; w += 4) {
+        const i32x4v v = *reinterpret_cast<const i32x4v *>(xr + w);
+        if (w < XW) X[w] = (uint32_t)v.x;
+        if (w + 1 < XW) X[w + 1] = (uint32_t)v.y;
+        if (w + 2 < XW) X[w + 2] = (uint32_t)v.z;
+        if (w + 3 < XW) X[w + 3] = (uint32_t)v.w;
+      }
+      const uint32_t sgn = X[XW - 1] >> 31;
+      if (sgn) {
+        uint32_t c = 1;
 #pragma unroll
-      for (int w = 0; w < XW; ++w) X[w] = addc32(~X[w], 0u, c, c);
-    }
-    if (!rzero<XW>(X)) {
-      const int Ei = eA[i] + 3, Fj = eB[j] + 3;
-      res = finish_reg_t<XW>(X, Ei + Fj - 2 * t, sgn, 0u, 0u, true, alpha, beta, yv);
-      if (q_finite_nonzero(res)) {
-        const int bexp = unpack_finite(alpha).e + 1 + kc + Ei + Fj - t - 2;
-        done = bexp <= unpack_finite(res).e - 106;
+        for (int w = 0; w < XW; ++w) X[w] = addc32(~X[w], 0u, c, c);
+      }
+      if (!rzero<XW>(X)) {
+        const int Ei = eA[i] + 3, Fj = eB[j] + 3;
+        res = finish_reg_t<XW>(X, Ei + Fj - 2 * t, sgn, 0u, 0u, true, alpha, beta, yv);
+        if (q_finite_nonzero(res)) {
+          const int bexp = unpack_finite(alpha).e + 1 + kc + Ei + Fj - t - 2;
+          done = bexp <= unpack_finite(res).e - 106;
+        }
       }
     }
-  }
-  if (done) {
-    stq_c(cp, res);
-  } else {
-    const uint32_t slot = atomicAdd(hdr + 1, 1u);
-    list[2 * (int64_t)slot] = (uint32_t)i;
-    list[2 * (int64_t)slot + 1] = (uint32_t)j;
+    if (done) {
+      stq_c(cp, res);
+    } else {
+      const uint32_t slot = atomicAdd(hdr + 1, 1u);
+      list[2 * (int64_t)slot] = (uint32_t)i;
+      list[2 * (int64_t)slot + 1] = (uint32_t)j;
+    }
   }
 }
 
@@ -2045,13 +2060,16 @@ hipError_t launch_gemm_crt(int colmajor, int64_t m, int64_t n, int64_t k, int t,
   uint32_t *hdr = (uint32_t *)ws;
   crt_tab *tab = (crt_tab *)(ws + L.tab);
   int8_t *RA = (int8_t *)(ws + L.rA), *RB = (int8_t *)(ws + L.rB), *RC = (int8_t *)(ws + L.rC);
+  const int64_t wgs = (int64_t)nq * (L.Mb / CRT_TA) * (L.Nb / CRT_TB);
+  if (!oz_grids_fit(L.Mp, L.Np, L.Kb) || !grid_fits((L.Mb * L.Kb + 3) / 4, 256) ||
+      !grid_fits((L.Nb * L.Kb + 3) / 4, 256) || !grid_fits(wgs, 512) || !grid_fits((L.Mb * L.Nb + 3) / 4, 256))
+    return hipErrorInvalidValue;
   if (ev) (void)hipEventRecord(ev[0], st);
   oz_prepare(L.Mp, L.Np, m, n, k, A, asr, asc, B, bsr, bsc, probe_ws, &eA, &fA, &eB, &fB, hdr, st);
   k_crt_tables<<<1, 64, 0, st>>>(nq, tab);
   k_crt_residues<<<(unsigned)((L.Mb * L.Kb + 3) / 4), 256, 0, st>>>(A, m, k, asr, asc, eA, t, tab, L.Mb, L.Kb, RA);
   k_crt_residues<<<(unsigned)((L.Nb * L.Kb + 3) / 4), 256, 0, st>>>(B, n, k, bsc, bsr, eB, t, tab, L.Nb, L.Kb, RB);
   if (ev) (void)hipEventRecord(ev[1], st);
-  const int64_t wgs = (int64_t)nq * (L.Mb / CRT_TA) * (L.Nb / CRT_TB);
   k_crt_gemm<<<(unsigned)wgs, 512, 0, st>>>(RA, RB, L.Mb, L.Nb, L.Kb, tab, RC);
   if (ev) (void)hipEventRecord(ev[2], st);
   hipError_t e = hipGetLastError();
@@ -2069,7 +2087,7 @@ hipError_t launch_gemm_crt(int colmajor, int64_t m, int64_t n, int64_t k, int t,
   } else {                                                                                                        \
     k_crt_finish<XW, false><<<fgrid, 256, 0, st>>>(m, n, k, t, kc, L.Mb, L.Nb, RC, tab, eA, fA, eB, fB, alpha,     \
                                                    beta, C, csr, csc, hdr, list, xrec);                           \
-    k_crt_finish_ab<XW><<<(unsigned)((mn + 255) / 256), 256, 0, st>>>(m, n, t, kc, xrec, eA, fA, eB, fB, alpha,    \
+    k_crt_finish_ab<XW><<<stride_blocks(mn, 256), 256, 0, st>>>(m, n, t, kc, xrec, eA, fA, eB, fB, alpha,    \
                                                                       beta, C, csr, csc, hdr, list);              \
   }
   if (xbits < 192) {

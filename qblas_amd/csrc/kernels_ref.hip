@@ -144,20 +144,22 @@ __global__ void k_gemv_row_ref(int64_t m, int64_t n, q128 alpha, const q128 *__r
 // ColMajor: ax[j] = mul(alpha, x_j) once per column (level2.hpp:71).
 __global__ void k_scale_vec(int64_t n, q128 alpha, const q128 *__restrict__ x, int64_t incx,
                             q128 *__restrict__ ax) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n) st(ax + j, mul_packed(alpha, ld(x + j * incx)));
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride)
+    st(ax + j, mul_packed(alpha, ld(x + j * incx)));
 }
 // ColMajor: y_i = mul(beta, y_i) (level2.hpp:63-66), then y_i = fma(A_ij, ax_j, y_i) for
 // j = 0..n-1 (level2.hpp:69-80); lane i owns row i, so loads of A columns coalesce.
 __global__ void k_gemv_col_ref(int64_t m, int64_t n, const q128 *__restrict__ A, int64_t lda,
                                const q128 *__restrict__ ax, q128 beta, q128 *__restrict__ y, int64_t incy) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  q128 *yi = y + i * incy;
-  chain ch;
-  chain_set(ch, mul_packed(beta, ld(yi)));
-  for (int64_t j = 0; j < n; ++j) chain_fma(ch, ld(A + j * lda + i), ld(ax + j));
-  st(yi, chain_get(ch));
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+    q128 *yi = y + i * incy;
+    chain ch;
+    chain_set(ch, mul_packed(beta, ld(yi)));
+    for (int64_t j = 0; j < n; ++j) chain_fma(ch, ld(A + j * lda + i), ld(ax + j));
+    st(yi, chain_get(ch));
+  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -237,20 +239,21 @@ __device__ q128 gen_value(int kind, uint64_t seed, int64_t i, int lo_exp, int hi
 }
 
 __global__ void k_fill(int kind, uint64_t seed, int64_t n, int lo_exp, int hi_exp, q128 *__restrict__ out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  st(out + i, gen_value(kind, seed, i, lo_exp, hi_exp));
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    st(out + i, gen_value(kind, seed, i, lo_exp, hi_exp));
 }
 
 // value number r*cols + c of the generator stored at element (r, c) of a rows x cols matrix
 // (RowMajor r*ld + c or ColMajor r + c*ld)
 __global__ void k_fill_2d(int kind, uint64_t seed, int64_t rows, int64_t cols, int64_t ld, int colmajor, int lo_exp,
                           int hi_exp, q128 *__restrict__ out) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= rows * cols) return;
-  // walk the storage order so that writes coalesce
-  const int64_t r = colmajor ? g % rows : g / cols, c = colmajor ? g / rows : g % cols;
-  st(out + (colmajor ? r + c * ld : r * ld + c), gen_value(kind, seed, r * cols + c, lo_exp, hi_exp));
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < rows * cols; g += stride) {
+    // walk the storage order so that writes coalesce
+    const int64_t r = colmajor ? g % rows : g / cols, c = colmajor ? g / rows : g % cols;
+    st(out + (colmajor ? r + c * ld : r * ld + c), gen_value(kind, seed, r * cols + c, lo_exp, hi_exp));
+  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -281,11 +284,12 @@ hipError_t launch_gemv_ref(int colmajor, int64_t m, int64_t n, q128 alpha, const
   if (!colmajor) {
     const int64_t lanes = (incx == 1) ? 2 * m : m;
     const int64_t blocks = (lanes + threads - 1) / threads;
+    if (!grid_fits(blocks, threads)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_gemv_row_ref, dim3((unsigned)blocks), dim3(threads), 0, st, m, n, alpha, A, lda, x,
                        incx, beta, y, incy);
   } else {
-    hipLaunchKernelGGL(k_scale_vec, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, alpha, x, incx, ax);
-    hipLaunchKernelGGL(k_gemv_col_ref, dim3((unsigned)((m + threads - 1) / threads)), dim3(threads), 0, st, m, n,
+    hipLaunchKernelGGL(k_scale_vec, dim3(stride_blocks(n, 256)), dim3(256), 0, st, n, alpha, x, incx, ax);
+    hipLaunchKernelGGL(k_gemv_col_ref, dim3(stride_blocks(m, threads)), dim3(threads), 0, st, m, n,
                        A, lda, ax, beta, y, incy);
   }
   return hipGetLastError();
@@ -294,6 +298,7 @@ hipError_t launch_gemv_ref(int colmajor, int64_t m, int64_t n, q128 alpha, const
 hipError_t launch_gemm_ref(int colmajor, int64_t m, int64_t n, int64_t k, int64_t kc, q128 alpha, const q128 *A,
                            int64_t lda, const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc,
                            hipStream_t st) {
+  if (!grid_fits((n + 15) / 16, 256)) return hipErrorInvalidValue;
   dim3 grid((unsigned)((n + 15) / 16), (unsigned)((m + 15) / 16));
   hipLaunchKernelGGL(k_gemm_ref, grid, dim3(256), 0, st, colmajor, m, n, k, kc, alpha, A, lda, B, ldb, beta, C,
                      ldc);
@@ -301,18 +306,18 @@ hipError_t launch_gemm_ref(int colmajor, int64_t m, int64_t n, int64_t k, int64_
 }
 
 __global__ void k_zero(q128 *__restrict__ p, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) st(p + i, q_zero(0));
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) st(p + i, q_zero(0));
 }
 hipError_t launch_zero(q128 *p, int64_t n, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+  hipLaunchKernelGGL(k_zero, dim3(stride_blocks(n, 256)), dim3(256), 0, st, p, n);
   return hipGetLastError();
 }
 
 hipError_t launch_fill(int kind, uint64_t seed, int64_t n, int lo_exp, int hi_exp, q128 *out, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, kind, seed, n, lo_exp, hi_exp,
+  hipLaunchKernelGGL(k_fill, dim3(stride_blocks(n, 256)), dim3(256), 0, st, kind, seed, n, lo_exp, hi_exp,
                      out);
   return hipGetLastError();
 }
@@ -322,7 +327,7 @@ hipError_t launch_fill_2d(int kind, uint64_t seed, int64_t rows, int64_t cols, i
                           int hi_exp, q128 *out, hipStream_t st) {
   const int64_t n = rows * cols;
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fill_2d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, kind, seed, rows, cols, ld,
+  hipLaunchKernelGGL(k_fill_2d, dim3(stride_blocks(n, 256)), dim3(256), 0, st, kind, seed, rows, cols, ld,
                      colmajor, lo_exp, hi_exp, out);
   return hipGetLastError();
 }
