@@ -61,7 +61,7 @@ __device__ __forceinline__ int64_t cdiv(int64_t a, int64_t b) { return (a + b - 
 // work items of problem p:
 //   kind 0 (gemm): tiles of tm x tn outputs (none when m, n or k is 0: level3.hpp:221-222)
 //   kind 1 (gemv, REFERENCE): groups of 64 lanes: 32 rows (RowMajor, incx == 1) or 64 rows
-//   kind 2 (gemv, DD): RowMajor 4 rows per wave (16 lanes per row); ColMajor 64 rows per wave
+//   kind 2 (gemv, DD): RowMajor 4 rows per wave (lanes along the columns); ColMajor 64 rows per wave
 //   kind 3 (gemv, EXACT): RowMajor one row per wave; ColMajor 64 rows per wave
 // (gemv has no work when m or n is 0: level2.hpp:21-22, :59-60)
 __device__ __forceinline__ int64_t batch_items(const void *P, int kind, int64_t p, int tm, int tn, int colmajor) {
@@ -287,8 +287,8 @@ __global__ void __launch_bounds__(256) k_gemv_batched_ref(const qblas_gemv_probl
 }
 
 // ------------------------------------------------------------------------------------------
-// gemv, DD: RowMajor 16 lanes per row, 4 rows per wave (lanes along the row, butterfly sum over
-// the 16); ColMajor one lane per row over the columns (coalesced column reads).
+// gemv, DD: RowMajor 4 rows per wave with the lanes along the columns (butterfly sums over the
+// wave); ColMajor one lane per row over the columns (coalesced column reads).
 // y = fma(alpha, RN(sum), mul(beta, y)).
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ dd bshfl_xor_dd(dd v, int msk) {
@@ -298,13 +298,22 @@ __device__ __forceinline__ dd bshfl_xor_dd(dd v, int msk) {
   return r;
 }
 
+// Each wave takes a contiguous range of work items (equal counts per wave) and steps through the
+// problems linearly, so the problem lookup is one search per wave.
+// RowMajor: an item is 4 rows; lanes stride along the columns, 2 column steps of 64 per iteration
+// (8 loads of A in flight per lane, each converted x element serving the 4 rows), dd partials per
+// lane, butterfly wave sums, and lane r finishes row r.
+// ColMajor: an item is 64 rows, one per lane, over the columns (2 accumulators).
 __global__ void __launch_bounds__(256) k_gemv_batched_dd(const qblas_gemv_problem *__restrict__ P, int64_t count,
                                                          const int64_t *__restrict__ start, int colmajor) {
   const int lane = threadIdx.x & 63;
   const int64_t total = start[count];
-  const int64_t waves = (int64_t)gridDim.x * 4;
-  for (int64_t u = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); u < total; u += waves) {
-    const int64_t p = find_problem(start, count, u);
+  const int64_t waves = (int64_t)gridDim.x * 4, w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t per = (total + waves - 1) / waves;
+  const int64_t u0 = w * per, u1 = (u0 + per < total) ? u0 + per : total;
+  int64_t p = (u0 < total) ? find_problem(start, count, u0) : 0;
+  for (int64_t u = u0; u < u1; ++u) {
+    while (start[p + 1] <= u) ++p;  // the problem holding item u (empty problems are skipped)
     const qblas_gemv_problem &v = P[p];
     const int64_t m = v.m, n = v.n, lda = v.lda, incx = v.incx, incy = v.incy;
     const int64_t ul = u - start[p];
@@ -313,24 +322,45 @@ __global__ void __launch_bounds__(256) k_gemv_batched_dd(const qblas_gemv_proble
     q128 *y = (q128 *)v.y;
     const q128 alpha = qv(v.alpha), beta = qv(v.beta);
     if (!colmajor) {
-      const int64_t i = ul * 4 + (lane >> 4);
-      const int sub = lane & 15;
-      dd acc0 = dd_zero(), acc1 = dd_zero();
-      if (i < m) {
-        const q128 *row = A + i * lda;
-        int64_t j = sub;
-        for (; j + 16 < n; j += 32) {
-          acc0 = dd_fma(q_to_dd(bld(row + j)), q_to_dd(bld(x + j * incx)), acc0);
-          acc1 = dd_fma(q_to_dd(bld(row + j + 16)), q_to_dd(bld(x + (j + 16) * incx)), acc1);
-        }
-        if (j < n) acc0 = dd_fma(q_to_dd(bld(row + j)), q_to_dd(bld(x + j * incx)), acc0);
-      }
-      dd s = dd_add(acc0, acc1);
+      const int64_t i0 = ul * 4;
+      const int R = (m - i0 < 4) ? (int)(m - i0) : 4;  // rows of this item (wave-uniform)
+      dd acc[4][2];
 #pragma unroll
-      for (int msk = 8; msk >= 1; msk >>= 1) s = dd_add(s, bshfl_xor_dd(s, msk));
-      if (i < m && sub == 0) {
-        q128 *yi = y + i * incy;
-        bst(yi, fma_packed(alpha, dd_to_q(s), mul_packed(beta, bld(yi))));
+      for (int r = 0; r < 4; ++r) acc[r][0] = acc[r][1] = dd_zero();
+      for (int64_t j = lane; j < n; j += 128) {
+        const bool h1 = j + 64 < n;
+        q128 a0[4], a1[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (r < R) {
+            const q128 *row = A + (i0 + r) * lda;
+            a0[r] = bld(row + j);
+            a1[r] = h1 ? bld(row + j + 64) : q_zero(0);
+          }
+        }
+        const dd x0 = q_to_dd(bld(x + j * incx));
+        const dd x1 = h1 ? q_to_dd(bld(x + (j + 64) * incx)) : dd_zero();
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (r < R) {
+            acc[r][0] = dd_fma(q_to_dd(a0[r]), x0, acc[r][0]);
+            acc[r][1] = dd_fma(q_to_dd(a1[r]), x1, acc[r][1]);
+          }
+        }
+      }
+      dd mine = dd_zero();
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (r < R) {
+          dd s = dd_add(acc[r][0], acc[r][1]);
+#pragma unroll
+          for (int msk = 32; msk >= 1; msk >>= 1) s = dd_add(s, bshfl_xor_dd(s, msk));
+          if (lane == r) mine = s;
+        }
+      }
+      if (lane < R) {
+        q128 *yi = y + (i0 + lane) * incy;
+        bst(yi, fma_packed(alpha, dd_to_q(mine), mul_packed(beta, bld(yi))));
       }
     } else {
       const int64_t i = ul * 64 + lane;

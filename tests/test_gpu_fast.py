@@ -105,9 +105,10 @@ def test_dot_exact_full_size_and_properties(O, Q, cuda):
 
 
 def test_workspace_reuse_across_streams(O, Q, cuda):
-    # every device routine of a host thread shares one workspace: a DD dot on one stream and a DD
-    # ColMajor gemv issued right after on another both use it, and the library orders the second
-    # after the first, so each result equals its single-stream value bit for bit
+    # a DD dot on one stream and a DD ColMajor gemv issued right after on another, both with
+    # scratch memory from the library: each result equals its single-stream value bit for bit.
+    # Then six streams in turn (more than the library keeps buffers for, so buffers are dropped
+    # and reallocated between calls), each call checked the same way.
     n, m, k = 1 << 22, 2048, 2048
     x, y = _vecs(O, "full", n, 61)
     A, v, w0 = O.fill(0, 62, m * k), O.fill(0, 63, k), O.fill(0, 64, m)
@@ -127,6 +128,12 @@ def test_workspace_reuse_across_streams(O, Q, cuda):
         _say("dd dot then dd gemv on two streams")
         r, w = run(s1.cuda_stream, s2.cuda_stream)
         assert np.array_equal(r, ref_r) and np.array_equal(w, ref_w)
+    streams = [cuda.cuda.Stream() for _ in range(6)]
+    for rnd in range(2):
+        _say(f"dd dot then dd gemv over six streams, round {rnd}")
+        for i in range(6):
+            r, w = run(streams[i].cuda_stream, streams[(i + 1) % 6].cuda_stream)
+            assert np.array_equal(r, ref_r) and np.array_equal(w, ref_w), (rnd, i)
 
 
 GEMV_CASES = [(0, 1, 1, 1.0, 0.0), (0, 7, 300, 1.5, -0.5), (0, 64, 1000, "q", "q"), (0, 300, 33, 1.0, 1.0),
