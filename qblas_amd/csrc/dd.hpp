@@ -37,6 +37,28 @@ QB_HD double ldexp_i(double v, int e) {
 
 // binary128 -> double-double (see header comment).  NaN/Inf map to NaN/Inf in hi.
 QB_HD dd q_to_dd(q128 a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // Branch-free, the same bits as the general code below (checked on every exponent with random,
+  // tie and carry low parts): hi is the double with the top 53 bits (exponent 52, assembled
+  // directly) scaled by ldexp; lo is the low 60 bits as two exact 32-bit halves joined by one
+  // rounding FMA (= the correctly rounded conversion), scaled by ldexp; signs are OR-ed in.
+  const uint32_t be = q_bexp(a);
+  const uint64_t sbit = a.hi & 0x8000000000000000ull;
+  const int e = (int)be - QBIAS;
+  const uint64_t frac52 = ((a.hi & FRAC_HI_MASK) << 4) | (a.lo >> 60);
+  const double top = __builtin_bit_cast(double, ((uint64_t)(1023 + 52) << 52) | frac52);
+  const double hi0 = ldexp_i(top, e - 52);
+  const uint64_t rest = a.lo & 0x0fffffffffffffffull;
+  const double rl = __builtin_fma((double)(uint32_t)(rest >> 32), 4294967296.0, (double)(uint32_t)rest);
+  const double lo0 = ldexp_i(rl, e - 112);
+  const bool special = be == 0 || be == 0x7fffu;
+  const bool nanq = be == 0x7fffu && (((a.hi & FRAC_HI_MASK) | a.lo) != 0);
+  dd r;
+  r.hi = be == 0x7fffu ? (nanq ? __builtin_nan("") : __builtin_bit_cast(double, sbit | 0x7ff0000000000000ull))
+                       : __builtin_bit_cast(double, (be == 0 ? 0ull : __builtin_bit_cast(uint64_t, hi0)) | sbit);
+  r.lo = special ? 0.0 : __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, lo0) | sbit);
+  return r;
+#else
   dd r;
   const uint32_t be = q_bexp(a);
   const double sgn = (a.hi >> 63) ? -1.0 : 1.0;
@@ -61,6 +83,7 @@ QB_HD dd q_to_dd(q128 a) {
   // rest < 2^60: convert with round-to-nearest (the i64->f64 conversion is correctly rounded)
   r.lo = sgn * ldexp_i((double)rest, e - 112);
   return r;
+#endif
 }
 
 QB_HD void two_sum(double a, double b, double &s, double &e) {

@@ -1754,7 +1754,9 @@ __global__ void __launch_bounds__(256) k_crt_finish(int64_t M, int64_t N, int64_
     Mw[w] = T->M[w];
     xsc[w] = T->xs[w];
   }
-  const double mtop = T->mtop;
+  // X'/M is within 1/4 of the integer k (|X| <= M/4), so a product with the reciprocal rounds to
+  // the same k as the quotient: its relative error (~2^-52) times |X'/M| < 2^13 is far below 1/4
+  const double inv_mtop = 1.0 / T->mtop;
   const uint32_t alpha_sgn = (uint32_t)(alpha.hi >> 63);
   const int64_t col = bn * 32 + n;
   i32x16v zero;
@@ -1842,7 +1844,7 @@ __global__ void __launch_bounds__(256) k_crt_finish(int64_t M, int64_t N, int64_
         }
         // k = round(X'/M) from the words of |X'| at M's two highest (|X'| < 2^13 M, so the words
         // above are zero); X = X' - k M
-        int64_t kq;
+        int32_t kq;  // |k| <= 128 nq < 2^13
         {
           const uint32_t xs_neg = X[CRT_MW - 1] >> 31;
           uint32_t mag[CRT_MW];
@@ -1852,13 +1854,13 @@ __global__ void __launch_bounds__(256) k_crt_finish(int64_t M, int64_t N, int64_
           double top = 0;
 #pragma unroll
           for (int w = 0; w < CRT_MW; ++w) top += (double)mag[w] * xsc[w];
-          const int64_t kmag = (int64_t)__builtin_rint(top / mtop);
+          const int32_t kmag = (int32_t)__builtin_rint(top * inv_mtop);
           kq = xs_neg ? -kmag : kmag;
         }
         int64_t bc = 0;
 #pragma unroll
         for (int w = 0; w < CRT_MW; ++w) {
-          const int64_t s2 = (int64_t)X[w] - kq * (int64_t)Mw[w] + bc;
+          const int64_t s2 = (int64_t)X[w] - (int64_t)kq * (int64_t)Mw[w] + bc;
           X[w] = (uint32_t)s2;
           bc = s2 >> 32;
         }
