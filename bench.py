@@ -14,7 +14,8 @@ in HBM before the timed region; one step = one qgemm call.
 Also reported:
   qgemm_forms     the same problem in the plain soft-binary128 form (the reference's summation
                   order and blocking, one correctly rounded FMA at a time), the DD form (double-
-                  double on FP64 FMA) and the OZAKI form.
+                  double on FP64 FMA; fp64: its FP64 rate from the kernel's 30 flops / 27 FP64
+                  instructions per multiply-add, against 78.6 TFLOP/s) and the OZAKI form.
   memory_bound    qdot n = 2^24 and qgemv 8192 x 8192 RowMajor / ColMajor (alpha = 1.5,
                   beta = -0.5, entries a + b 2^-60) in the DD and EXACT forms: GB/s against the
                   8 TB/s HBM peak.  Algorithmic bytes: dot 32 n, gemv 16 (m n + n + 2 m).
@@ -50,6 +51,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 INT8_PEAK_TOPS = 5000.0
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 vector (an FMA counts 2 flops)
+# k_gemm_dd's loop body per multiply-add (gfx950 code of this build): TwoProd (1 v_mul_f64 +
+# 1 v_fma_f64), the two cross terms (2 v_fma_f64) and the accurate double-double addition
+# (23 v_add_f64): 27 FP64 instructions, 30 flops
+DD_GEMM_FP64_INSTS_PER_MAC = 27
+DD_GEMM_FP64_FLOPS_PER_MAC = 30
 METRIC = "quad-equiv GFLOP/s, qgemm 4096³ at ≤2^-104 rel err; qdot/qgemv HBM GB/s"
 
 
@@ -548,6 +555,13 @@ def main():
                                                        Q.OZAKI_SLICES, 0))
     forms["dd"] = bench_form_once(Q, torch, dev, stream, N, Q.MODE_DD)
     forms["dd"]["speedup_vs_plain"] = None
+    macs = float(N) ** 3
+    fp64_tflops = macs * DD_GEMM_FP64_FLOPS_PER_MAC / (forms["dd"]["ms"] * 1e-3) / 1e12
+    forms["dd"]["fp64"] = {"flops_per_mac": DD_GEMM_FP64_FLOPS_PER_MAC,
+                           "instructions_per_mac": DD_GEMM_FP64_INSTS_PER_MAC,
+                           "TFLOP/s": round(fp64_tflops, 1), "frac_of_peak": round(fp64_tflops / FP64_PEAK_TFLOPS, 4),
+                           "issue_frac": round(macs * DD_GEMM_FP64_INSTS_PER_MAC / (forms["dd"]["ms"] * 1e-3) /
+                                               (FP64_PEAK_TFLOPS / 2 * 1e12), 4)}
     if not args.no_plain:
         forms["plain"] = bench_plain(Q, torch, dev, stream, N)
         forms["dd"]["speedup_vs_plain"] = round(forms["plain"]["ms"] / forms["dd"]["ms"], 1)

@@ -1483,7 +1483,11 @@ __device__ __forceinline__ i32x4v crt_pack16(const i32x16v &d, int p, uint32_t m
   for (int v = 0; v < 16; ++v) {
     const uint32_t y = (uint32_t)d[v];
     const uint32_t q = (uint32_t)(((uint64_t)((y << 4) & 0xffffffu) * (mdiv & 0xffffffu)) >> 32);
-    r[v] = (int)y - (int)q * p - hf;
+    // q < 2^20 / p and p <= 256: q p + half is one full-rate 24-bit multiply-add (the compiler
+    // otherwise picks the 64-bit v_mad_u64_u32)
+    uint32_t qp;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(qp) : "v"(q), "v"((uint32_t)p), "v"((uint32_t)hf));
+    r[v] = (int)(y - qp);
   }
   i32x4v w;
   uint32_t o[4];
@@ -1502,8 +1506,8 @@ __device__ __forceinline__ i32x4v crt_pack16(const i32x16v &d, int p, uint32_t m
 
 // Residues of the scaled integers of one operand: rows r < R (rows of A, or columns of B), k < K,
 // element (r, k) at X[r sr + k sc], row exponent E[r] (+3).  One wave per fragment position
-// (rb, kb) of the Rb x Kb grid; out[q][rb][kb][1 KB] for q < nq, lane L's 16 bytes = row L%32,
-// k 16(L/32)..+15.  The fragment is done in two halves (k 0..15, then 16..31):
+// (rb, kb) of the Rb x Kb grid; plane q < nq at out + q * plane holds [rb][kb][1 KB], lane L's 16
+// bytes = row L%32, k 16(L/32)..+15.  The fragment is done in two halves (k 0..15, then 16..31):
 // Phase 1: the half's 512 elements (8 per lane, loads coalesced along the contiguous dimension)
 // become balanced digits 0..19 in the wave's LDS region (element (r, c) at r*17 + c).
 // Phase 2: MFMA row rho of group g is the element (slot 2g + ((rho>>2)&1), byte 4(rho>>3) + (rho&3))
@@ -1515,7 +1519,7 @@ constexpr int CRT_RS = 17;  // LDS row stride (elements) of the residue kernel
 __global__ void __launch_bounds__(256) k_crt_residues(const q128 *__restrict__ X, int64_t R, int64_t K, int64_t sr,
                                                       int64_t sc, const int32_t *__restrict__ E, int t,
                                                       const crt_tab *__restrict__ T, int64_t Rb, int64_t Kb,
-                                                      int8_t *__restrict__ out) {
+                                                      int64_t plane, int8_t *__restrict__ out) {
   __shared__ __attribute__((aligned(16))) uint32_t s_lo[4][32 * CRT_RS][4];
   __shared__ uint32_t s_hi[4][32 * CRT_RS];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, n = lane & 31, h = lane >> 5;
@@ -1535,7 +1539,6 @@ __global__ void __launch_bounds__(256) k_crt_residues(const q128 *__restrict__ X
     init0[v] = T->off[n];
     init1[v] = T->off[32 + n];
   }
-  const int64_t plane = Rb * Kb * OZ_FRAG;
   int8_t *o0 = out + (int64_t)n * plane + pos * OZ_FRAG;
   int8_t *o1 = out + (int64_t)(32 + n) * plane + pos * OZ_FRAG;
   const int hh = (n >> 2) & 1, ib = 4 * (n >> 3) + (n & 3);
@@ -1597,8 +1600,8 @@ __global__ void __launch_bounds__(256) k_crt_residues(const q128 *__restrict__ X
   }
 }
 
-// One int8 GEMM per modulus: RC[q][mb][nb] = (RA[q] RB[q]^T) mod p_q for a 256 x 256 tile of
-// (mb, nb) blocks, 8 waves of 128 x 64 (4 x 2 MFMA blocks).  A 4-slot LDS ring of stages of two
+// One int8 GEMM per modulus: RC[q][mb][nb] = (RA[q] RB[q]^T) mod p_q (planes pA, pB, pC bytes apart)
+// for a 256 x 256 tile of (mb, nb) blocks, 8 waves of 128 x 64 (4 x 2 MFMA blocks).  A 4-slot LDS ring of stages of two
 // k blocks (2 x (8 A + 8 B) fragments, 32 KB) is filled three stages ahead with global_load_lds;
 // each wave copies 4 fragments per stage (two k blocks per barrier: fewer waits between MFMAs).
 // Workgroups of one modulus are consecutive (its residue planes stay in the caches).  Output block
@@ -1614,15 +1617,16 @@ static_assert(((CRT_TA + CRT_TB) * CRT_KS) % (CRT_WR * CRT_WC) == 0 && 2 * CRT_P
 static_assert(CRT_KCHUNK % CRT_KS == 0, "reduction at stage ends");
 
 __global__ void __launch_bounds__(512) k_crt_gemm(const int8_t *__restrict__ RA, const int8_t *__restrict__ RB,
-                                                  int64_t Mb, int64_t Nb, int64_t Kb, const crt_tab *__restrict__ T,
+                                                  int64_t Mb, int64_t Nb, int64_t Kb, int64_t pA, int64_t pB,
+                                                  int64_t pC, const crt_tab *__restrict__ T,
                                                   int8_t *__restrict__ RC) {
   __shared__ __attribute__((aligned(1024))) int8_t lds[4 * CRT_SLOT];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t tilesM = Mb / CRT_TA, tilesN = Nb / CRT_TB, per = tilesM * tilesN;
   const int64_t q = blockIdx.x / per, tt = blockIdx.x - q * per;
   const int64_t tm = tt / tilesN, tn = tt - tm * tilesN;
-  const int8_t *Aq = RA + (q * Mb + tm * CRT_TA) * Kb * OZ_FRAG;
-  const int8_t *Bq = RB + (q * Nb + tn * CRT_TB) * Kb * OZ_FRAG;
+  const int8_t *Aq = RA + q * pA + tm * CRT_TA * Kb * OZ_FRAG;
+  const int8_t *Bq = RB + q * pB + tn * CRT_TB * Kb * OZ_FRAG;
   const int64_t ns = (Kb + CRT_KS - 1) / CRT_KS;  // stages; the last may hold one k block
   auto copy = [&](int64_t st) {
     if (st >= ns) return;
@@ -1696,9 +1700,10 @@ __global__ void __launch_bounds__(512) k_crt_gemm(const int8_t *__restrict__ RA,
       uint32_t u[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
       for (int v = 0; v < 16; ++v) u[v >> 2] |= crt_red_d(acc[a][b][v], p, ipd) << (8 * (v & 3));
-      const int64_t blk = (q * Mb + tm * CRT_TA + wr * CRT_FM + a) * Nb + tn * CRT_TB + wc * CRT_FN + b;
+      const int64_t blk = (tm * CRT_TA + wr * CRT_FM + a) * Nb + tn * CRT_TB + wc * CRT_FN + b;
 #pragma unroll
-      for (int r4 = 0; r4 < 4; ++r4) *reinterpret_cast<uint32_t *>(RC + blk * OZ_FRAG + r4 * 256 + lane * 4) = u[r4];
+      for (int r4 = 0; r4 < 4; ++r4)
+        *reinterpret_cast<uint32_t *>(RC + q * pC + blk * OZ_FRAG + r4 * 256 + lane * 4) = u[r4];
     }
 }
 
@@ -1728,7 +1733,7 @@ __device__ __forceinline__ void crt_t4(uint32_t i0, uint32_t i1, uint32_t i2, ui
 // here.  Otherwise X is written to xrec (RW words per entry, row-major) for k_crt_finish_ab.
 template <int XW, bool FAST>
 __global__ void __launch_bounds__(256) k_crt_finish(int64_t M, int64_t N, int64_t K, int t, int kc, int64_t Mb,
-                                                    int64_t Nb, const int8_t *__restrict__ RC,
+                                                    int64_t Nb, int64_t plane, const int8_t *__restrict__ RC,
                                                     const crt_tab *__restrict__ T, const int32_t *__restrict__ eA,
                                                     const uint32_t *__restrict__ fA, const int32_t *__restrict__ eB,
                                                     const uint32_t *__restrict__ fB, q128 alpha, q128 beta,
@@ -1741,7 +1746,6 @@ __global__ void __launch_bounds__(256) k_crt_finish(int64_t M, int64_t N, int64_
   const int lane = threadIdx.x & 63, n = lane & 31, kh = lane >> 5;
   const int64_t bm = blk / Nb, bn = blk - bm * Nb;
   const int nq = T->nq;
-  const int64_t plane = Mb * Nb * OZ_FRAG;
   i32x4v aV[2][2];
 #pragma unroll
   for (int mb = 0; mb < 2; ++mb)
@@ -1971,6 +1975,7 @@ __global__ void __launch_bounds__(256) k_crt_finish_ab(int64_t M, int64_t N, int
 
 struct crt_layout {
   int64_t Mp, Np, Kb, Mb, Nb;
+  int64_t pA, pB, pC;  // byte strides of the per-modulus planes of RA, RB, RC
   int nq;
   size_t eA, fA, eB, fB, tab, rA, rB, rC, xrec, list, bt, total;
 };
@@ -1992,6 +1997,7 @@ int crt_moduli_for(int64_t k, int t) {
   return 0;
 }
 
+constexpr int64_t CRT_PLANE_PAD = 4096;
 static crt_layout crt_plan(int64_t m, int64_t n, int64_t k, int nq) {
   crt_layout L;
   L.Mp = (m + CRT_TILE - 1) / CRT_TILE * CRT_TILE;
@@ -2012,12 +2018,18 @@ static crt_layout crt_plan(int64_t m, int64_t n, int64_t k, int nq) {
   o = al(o + (size_t)L.Np * 4);
   L.tab = o;
   o = al(o + CRT_TAB_BYTES);
+  // Planes a power of two apart (16 MB at 4096^2) put the same offset of every modulus on the same
+  // memory channels: the residue kernel's stores (one offset in 32 planes per instruction) ran at
+  // 2.0 TB/s that way and at 5.0 TB/s with 4 KB between the planes' starting channels.
+  L.pA = L.Mb * L.Kb * OZ_FRAG + CRT_PLANE_PAD;
+  L.pB = L.Nb * L.Kb * OZ_FRAG + CRT_PLANE_PAD;
+  L.pC = L.Mb * L.Nb * OZ_FRAG + CRT_PLANE_PAD;
   L.rA = o;
-  o = al(o + (size_t)nq * L.Mb * L.Kb * OZ_FRAG);
+  o = al(o + (size_t)nq * L.pA);
   L.rB = o;
-  o = al(o + (size_t)nq * L.Nb * L.Kb * OZ_FRAG);
+  o = al(o + (size_t)nq * L.pB);
   L.rC = o;
-  o = al(o + (size_t)nq * L.Mb * L.Nb * OZ_FRAG);
+  o = al(o + (size_t)nq * L.pC);
   L.xrec = o;  // records of X for general alpha, beta (12 words per entry at most)
   o = al(o + (size_t)m * n * 48);
   L.list = o;
@@ -2069,10 +2081,10 @@ hipError_t launch_gemm_crt(int colmajor, int64_t m, int64_t n, int64_t k, int t,
   if (ev) (void)hipEventRecord(ev[0], st);
   oz_prepare(L.Mp, L.Np, m, n, k, A, asr, asc, B, bsr, bsc, probe_ws, &eA, &fA, &eB, &fB, hdr, st);
   k_crt_tables<<<1, 64, 0, st>>>(nq, tab);
-  k_crt_residues<<<(unsigned)((L.Mb * L.Kb + 3) / 4), 256, 0, st>>>(A, m, k, asr, asc, eA, t, tab, L.Mb, L.Kb, RA);
-  k_crt_residues<<<(unsigned)((L.Nb * L.Kb + 3) / 4), 256, 0, st>>>(B, n, k, bsc, bsr, eB, t, tab, L.Nb, L.Kb, RB);
+  k_crt_residues<<<(unsigned)((L.Mb * L.Kb + 3) / 4), 256, 0, st>>>(A, m, k, asr, asc, eA, t, tab, L.Mb, L.Kb, L.pA, RA);
+  k_crt_residues<<<(unsigned)((L.Nb * L.Kb + 3) / 4), 256, 0, st>>>(B, n, k, bsc, bsr, eB, t, tab, L.Nb, L.Kb, L.pB, RB);
   if (ev) (void)hipEventRecord(ev[1], st);
-  k_crt_gemm<<<(unsigned)wgs, 512, 0, st>>>(RA, RB, L.Mb, L.Nb, L.Kb, tab, RC);
+  k_crt_gemm<<<(unsigned)wgs, 512, 0, st>>>(RA, RB, L.Mb, L.Nb, L.Kb, L.pA, L.pB, L.pC, tab, RC);
   if (ev) (void)hipEventRecord(ev[2], st);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -2084,10 +2096,10 @@ hipError_t launch_gemm_crt(int colmajor, int64_t m, int64_t n, int64_t k, int t,
   const int64_t mn = m * n;
 #define QB_CRT_FIN(XW)                                                                                              \
   if (fast) {                                                                                                     \
-    k_crt_finish<XW, true><<<fgrid, 256, 0, st>>>(m, n, k, t, kc, L.Mb, L.Nb, RC, tab, eA, fA, eB, fB, alpha, beta, \
+    k_crt_finish<XW, true><<<fgrid, 256, 0, st>>>(m, n, k, t, kc, L.Mb, L.Nb, L.pC, RC, tab, eA, fA, eB, fB, alpha, beta, \
                                                   C, csr, csc, hdr, list, xrec);                                  \
   } else {                                                                                                        \
-    k_crt_finish<XW, false><<<fgrid, 256, 0, st>>>(m, n, k, t, kc, L.Mb, L.Nb, RC, tab, eA, fA, eB, fB, alpha,     \
+    k_crt_finish<XW, false><<<fgrid, 256, 0, st>>>(m, n, k, t, kc, L.Mb, L.Nb, L.pC, RC, tab, eA, fA, eB, fB, alpha,     \
                                                    beta, C, csr, csc, hdr, list, xrec);                           \
     k_crt_finish_ab<XW><<<stride_blocks(mn, 256), 256, 0, st>>>(m, n, t, kc, xrec, eA, fA, eB, fB, alpha,    \
                                                                       beta, C, csr, csc, hdr, list);              \
