@@ -1477,11 +1477,11 @@ __device__ __forceinline__ uint32_t crt_red_d(int x, int p, double ip) {
 
 // 16 residue sums y (in [0, 2^20), = x + off) of one modulus -> 16 balanced residues as bytes:
 // q = floor(y / p) by a 24-bit multiply-high, r = y - q p - half in [-128, 127]
-__device__ __forceinline__ i32x4v crt_pack16(const i32x16v &d, int p, uint32_t mdiv, int hf) {
+__device__ __forceinline__ i32x4v crt_pack16(const i32x16v &d, int p, uint32_t mdiv, int hf, uint32_t off) {
   int r[16];
 #pragma unroll
   for (int v = 0; v < 16; ++v) {
-    const uint32_t y = (uint32_t)d[v];
+    const uint32_t y = (uint32_t)d[v] + off;
     const uint32_t q = (uint32_t)(((uint64_t)((y << 4) & 0xffffffu) * (mdiv & 0xffffffu)) >> 32);
     // q < 2^20 / p and p <= 256: q p + half is one full-rate 24-bit multiply-add (the compiler
     // otherwise picks the 64-bit v_mad_u64_u32)
@@ -1533,12 +1533,10 @@ __global__ void __launch_bounds__(256) k_crt_residues(const q128 *__restrict__ X
   const int p0 = T->p[n], p1 = T->p[32 + n];
   const uint32_t m0 = T->mdiv[n], m1 = T->mdiv[32 + n];
   const int h0 = T->half[n], h1 = T->half[32 + n];
-  i32x16v init0, init1;
+  const uint32_t off0 = (uint32_t)T->off[n], off1 = (uint32_t)T->off[32 + n];
+  i32x16v zero;
 #pragma unroll
-  for (int v = 0; v < 16; ++v) {
-    init0[v] = T->off[n];
-    init1[v] = T->off[32 + n];
-  }
+  for (int v = 0; v < 16; ++v) zero[v] = 0;
   int8_t *o0 = out + (int64_t)n * plane + pos * OZ_FRAG;
   int8_t *o1 = out + (int64_t)(32 + n) * plane + pos * OZ_FRAG;
   const int hh = (n >> 2) & 1, ib = 4 * (n >> 3) + (n & 3);
@@ -1588,10 +1586,10 @@ __global__ void __launch_bounds__(256) k_crt_residues(const q128 *__restrict__ X
         a.z = 0;
         a.w = 0;
       }
-      const i32x16v d0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bc0, init0, 0, 0, 0);
-      const i32x16v d1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bc1, init1, 0, 0, 0);
-      if (n < nq) *reinterpret_cast<i32x4v *>(o0 + (2 * g + h) * 16) = crt_pack16(d0, p0, m0, h0);
-      if (32 + n < nq) *reinterpret_cast<i32x4v *>(o1 + (2 * g + h) * 16) = crt_pack16(d1, p1, m1, h1);
+      const i32x16v d0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bc0, zero, 0, 0, 0);
+      const i32x16v d1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bc1, zero, 0, 0, 0);
+      if (n < nq) *reinterpret_cast<i32x4v *>(o0 + (2 * g + h) * 16) = crt_pack16(d0, p0, m0, h0, off0);
+      if (32 + n < nq) *reinterpret_cast<i32x4v *>(o1 + (2 * g + h) * 16) = crt_pack16(d1, p1, m1, h1, off1);
     }
     // the next half overwrites the region: every lane's reads of this half are done
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
