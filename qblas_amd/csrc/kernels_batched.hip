@@ -298,82 +298,85 @@ __device__ __forceinline__ dd bshfl_xor_dd(dd v, int msk) {
   return r;
 }
 
-// Each wave takes a contiguous range of work items (equal counts per wave) and steps through the
-// problems linearly, so the problem lookup is one search per wave.
+// Waves take chunks of BGV_CHUNK consecutive work items, chunk c by wave c mod waves, so that the
+// waves running at one time work on neighbouring memory; within a chunk the problems are stepped
+// through linearly (one search per chunk).
 // RowMajor: an item is 4 rows; lanes stride along the columns, 2 column steps of 64 per iteration
 // (8 loads of A in flight per lane, each converted x element serving the 4 rows), dd partials per
 // lane, butterfly wave sums, and lane r finishes row r.
 // ColMajor: an item is 64 rows, one per lane, over the columns (2 accumulators).
+constexpr int BGV_CHUNK = 4;  // work items a wave takes in a row
 __global__ void __launch_bounds__(256) k_gemv_batched_dd(const qblas_gemv_problem *__restrict__ P, int64_t count,
                                                          const int64_t *__restrict__ start, int colmajor) {
   const int lane = threadIdx.x & 63;
   const int64_t total = start[count];
   const int64_t waves = (int64_t)gridDim.x * 4, w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int64_t per = (total + waves - 1) / waves;
-  const int64_t u0 = w * per, u1 = (u0 + per < total) ? u0 + per : total;
-  int64_t p = (u0 < total) ? find_problem(start, count, u0) : 0;
-  for (int64_t u = u0; u < u1; ++u) {
-    while (start[p + 1] <= u) ++p;  // the problem holding item u (empty problems are skipped)
-    const qblas_gemv_problem &v = P[p];
-    const int64_t m = v.m, n = v.n, lda = v.lda, incx = v.incx, incy = v.incy;
-    const int64_t ul = u - start[p];
-    const q128 *A = (const q128 *)v.A;
-    const q128 *x = (const q128 *)v.x;
-    q128 *y = (q128 *)v.y;
-    const q128 alpha = qv(v.alpha), beta = qv(v.beta);
-    if (!colmajor) {
-      const int64_t i0 = ul * 4;
-      const int R = (m - i0 < 4) ? (int)(m - i0) : 4;  // rows of this item (wave-uniform)
-      dd acc[4][2];
+  for (int64_t c0 = w * BGV_CHUNK; c0 < total; c0 += waves * BGV_CHUNK) {
+    const int64_t c1 = (c0 + BGV_CHUNK < total) ? c0 + BGV_CHUNK : total;
+    int64_t p = find_problem(start, count, c0);
+    for (int64_t u = c0; u < c1; ++u) {
+      while (start[p + 1] <= u) ++p;  // the problem holding item u (empty problems are skipped)
+      const qblas_gemv_problem &v = P[p];
+      const int64_t m = v.m, n = v.n, lda = v.lda, incx = v.incx, incy = v.incy;
+      const int64_t ul = u - start[p];
+      const q128 *A = (const q128 *)v.A;
+      const q128 *x = (const q128 *)v.x;
+      q128 *y = (q128 *)v.y;
+      const q128 alpha = qv(v.alpha), beta = qv(v.beta);
+      if (!colmajor) {
+        const int64_t i0 = ul * 4;
+        const int R = (m - i0 < 4) ? (int)(m - i0) : 4;  // rows of this item (wave-uniform)
+        dd acc[4][2];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[r][0] = acc[r][1] = dd_zero();
-      for (int64_t j = lane; j < n; j += 128) {
-        const bool h1 = j + 64 < n;
-        q128 a0[4], a1[4];
+        for (int r = 0; r < 4; ++r) acc[r][0] = acc[r][1] = dd_zero();
+        for (int64_t j = lane; j < n; j += 128) {
+          const bool h1 = j + 64 < n;
+          q128 a0[4], a1[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            if (r < R) {
+              const q128 *row = A + (i0 + r) * lda;
+              a0[r] = bld(row + j);
+              a1[r] = h1 ? bld(row + j + 64) : q_zero(0);
+            }
+          }
+          const dd x0 = q_to_dd(bld(x + j * incx));
+          const dd x1 = h1 ? q_to_dd(bld(x + (j + 64) * incx)) : dd_zero();
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            if (r < R) {
+              acc[r][0] = dd_fma(q_to_dd(a0[r]), x0, acc[r][0]);
+              acc[r][1] = dd_fma(q_to_dd(a1[r]), x1, acc[r][1]);
+            }
+          }
+        }
+        dd mine = dd_zero();
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           if (r < R) {
-            const q128 *row = A + (i0 + r) * lda;
-            a0[r] = bld(row + j);
-            a1[r] = h1 ? bld(row + j + 64) : q_zero(0);
+            dd s = dd_add(acc[r][0], acc[r][1]);
+#pragma unroll
+            for (int msk = 32; msk >= 1; msk >>= 1) s = dd_add(s, bshfl_xor_dd(s, msk));
+            if (lane == r) mine = s;
           }
         }
-        const dd x0 = q_to_dd(bld(x + j * incx));
-        const dd x1 = h1 ? q_to_dd(bld(x + (j + 64) * incx)) : dd_zero();
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          if (r < R) {
-            acc[r][0] = dd_fma(q_to_dd(a0[r]), x0, acc[r][0]);
-            acc[r][1] = dd_fma(q_to_dd(a1[r]), x1, acc[r][1]);
+        if (lane < R) {
+          q128 *yi = y + (i0 + lane) * incy;
+          bst(yi, fma_packed(alpha, dd_to_q(mine), mul_packed(beta, bld(yi))));
+        }
+      } else {
+        const int64_t i = ul * 64 + lane;
+        if (i < m) {
+          dd acc0 = dd_zero(), acc1 = dd_zero();
+          int64_t j = 0;
+          for (; j + 1 < n; j += 2) {
+            acc0 = dd_fma(q_to_dd(bld(A + j * lda + i)), q_to_dd(bld(x + j * incx)), acc0);
+            acc1 = dd_fma(q_to_dd(bld(A + (j + 1) * lda + i)), q_to_dd(bld(x + (j + 1) * incx)), acc1);
           }
+          if (j < n) acc0 = dd_fma(q_to_dd(bld(A + j * lda + i)), q_to_dd(bld(x + j * incx)), acc0);
+          q128 *yi = y + i * incy;
+          bst(yi, fma_packed(alpha, dd_to_q(dd_add(acc0, acc1)), mul_packed(beta, bld(yi))));
         }
-      }
-      dd mine = dd_zero();
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        if (r < R) {
-          dd s = dd_add(acc[r][0], acc[r][1]);
-#pragma unroll
-          for (int msk = 32; msk >= 1; msk >>= 1) s = dd_add(s, bshfl_xor_dd(s, msk));
-          if (lane == r) mine = s;
-        }
-      }
-      if (lane < R) {
-        q128 *yi = y + (i0 + lane) * incy;
-        bst(yi, fma_packed(alpha, dd_to_q(mine), mul_packed(beta, bld(yi))));
-      }
-    } else {
-      const int64_t i = ul * 64 + lane;
-      if (i < m) {
-        dd acc0 = dd_zero(), acc1 = dd_zero();
-        int64_t j = 0;
-        for (; j + 1 < n; j += 2) {
-          acc0 = dd_fma(q_to_dd(bld(A + j * lda + i)), q_to_dd(bld(x + j * incx)), acc0);
-          acc1 = dd_fma(q_to_dd(bld(A + (j + 1) * lda + i)), q_to_dd(bld(x + (j + 1) * incx)), acc1);
-        }
-        if (j < n) acc0 = dd_fma(q_to_dd(bld(A + j * lda + i)), q_to_dd(bld(x + j * incx)), acc0);
-        q128 *yi = y + i * incy;
-        bst(yi, fma_packed(alpha, dd_to_q(dd_add(acc0, acc1)), mul_packed(beta, bld(yi))));
       }
     }
   }

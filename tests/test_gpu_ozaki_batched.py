@@ -9,7 +9,7 @@ value (exact zeros exactly). Coverage:
   - both layouts, padded leading dimensions, empty problems;
   - every slice-count configuration;
   - a problem with k > 4096 (two int32 accumulation passes);
-  - rows that cancel to exact zeros and to tiny values (the exact repair path);
+  - rows that cancel to exact zeros and to tiny values (the exact repair path), and infinities;
   - the autotuner, which must leave the caller's C untouched and pick a valid configuration.
 """
 import numpy as np
@@ -111,7 +111,8 @@ def _cancel_batch(O, cuda, layout):
     B = [Y; Y; Z] with tiny D (|D| < 2^-80) leaves AB = D Z, far below the scale of the rows; plus
     a problem with k = 4200 (two accumulation passes) and beta != 0."""
     import qblas_amd as Q
-    specs = [(40, 33, 64, "zero"), (50, 70, 96, "tiny"), (17, 9, 4200, "plain"), (64, 64, 130, "tiny")]
+    specs = [(40, 33, 64, "zero"), (50, 70, 96, "tiny"), (17, 9, 4200, "plain"), (64, 64, 130, "tiny"),
+             (20, 30, 50, "inf")]
     dims, arrays = [], []
     zero = np.zeros(2, np.uint64)
     for p, (m, n, k, kind) in enumerate(specs):
@@ -119,9 +120,12 @@ def _cancel_batch(O, cuda, layout):
         h = (k - d) // 2
         X = O.fill(0, 900 + p, m * h).reshape(m, h, 2)
         Y = O.fill(0, 950 + p, h * n).reshape(h, n, 2)
-        if kind == "plain":
+        if kind in ("plain", "inf"):
             A = O.fill(0, 990 + p, m * k).reshape(m, k, 2)
             Bm = O.fill(0, 995 + p, k * n).reshape(k, n, 2)
+            if kind == "inf":  # +Inf at A[3, 7] and -Inf at A[11, 0]: rows 3 and 11 become +-Inf or NaN
+                A[3, 7] = (0, 0x7fff000000000000)
+                A[11, 0] = (0, 0xffff000000000000)
         else:
             negX = X.copy()
             negX[..., 1] ^= np.uint64(1 << 63)
@@ -142,7 +146,7 @@ def _cancel_batch(O, cuda, layout):
             c_arr, ldc = np.ascontiguousarray(C.transpose(1, 0, 2)).reshape(-1, 2), m
         arrays += [a_arr, b_arr, c_arr]
         ab = O.fill(0, 40 + p, 2)
-        dims.append((m, n, k, lda, ldb, ldc, ab[0], ab[1] if kind == "plain" else zero))
+        dims.append((m, n, k, lda, ldb, ldc, ab[0], ab[1] if kind in ("plain", "inf") else zero))
     pool = Pool(cuda, arrays)
     desc = np.zeros(len(dims), Q.GEMM_PROBLEM)
     for p, (m, n, k, lda, ldb, ldc, a, b) in enumerate(dims):
@@ -163,8 +167,14 @@ def test_ozaki_batched_cancellation_and_long_k(O, Q, cuda):
         repaired = Q.dev_ozaki_repairs()
         assert repaired >= 40 * 33, repaired      # every entry of the exactly cancelling problem
         got = [pool.get(3 * p + 2, C0[p].shape[0]) for p in range(len(dims))]
-        worst = _check_oracle(O, layout, dims, arrays, C0, lambda p: got[p], rng, per_problem=400)
+        finite = len(dims) - 1        # the last problem holds the infinities (the oracle takes finite inputs)
+        worst = _check_oracle(O, layout, dims[:finite], arrays, C0, lambda p: got[p], rng, per_problem=400)
         _say(f"  max rel err {worst:.3e}, repaired entries {repaired}")
+        m, n = dims[finite][:2]
+        cs = _strides(layout, dims[finite][5])
+        bexp = lambda i, j: (int(got[finite][i * cs[0] + j * cs[1]][1]) >> 48) & 0x7FFF
+        assert all(bexp(i, j) == 0x7FFF for i in (3, 11) for j in range(n)), "Inf rows are not Inf/NaN"
+        assert all(bexp(i, j) != 0x7FFF for i in (0, 5, m - 1) for j in range(n)), "finite rows hit"
         S = Q.ozaki_last_slices()
         single = _single_problem(Q, cuda, layout, dims, arrays, S)
         for p in range(len(dims)):
