@@ -52,7 +52,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 INT8_PEAK_TOPS = 5000.0
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 vector (an FMA counts 2 flops)
-# k_gemm_dd's loop body per multiply-add (gfx950 code of this build): TwoProd (1 v_mul_f64 +
+# k_gemm_dd's loop body per multiply-add (the library's gfx950 code): TwoProd (1 v_mul_f64 +
 # 1 v_fma_f64), the two cross terms (2 v_fma_f64) and the accurate double-double addition
 # (23 v_add_f64): 27 FP64 instructions, 30 flops
 DD_GEMM_FP64_INSTS_PER_MAC = 27
@@ -509,7 +509,7 @@ def main():
         print(json.dumps(summarize_pmc(*args.summarize_pmc), indent=1))
         sys.exit(0)
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "bench.py measures one GPU; --gpus must be 1", "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
     import qblas_amd as Q

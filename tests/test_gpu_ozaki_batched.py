@@ -79,13 +79,13 @@ def _single_problem(Q, cuda, layout, dims, arrays, S):
 
 
 def test_ozaki_batched_every_slice_count(O, Q, cuda):
-    configs = [i for i in range(16) if Q.gemm_batched_config(Q.MODE_OZAKI, i)]
-    assert len(configs) >= 4 and Q.gemm_batched_config(Q.MODE_OZAKI, len(configs)) is None
+    cfgs = [i for i in range(16) if Q.gemm_batched_config(Q.MODE_OZAKI, i)]
+    assert len(cfgs) >= 4 and Q.gemm_batched_config(Q.MODE_OZAKI, len(cfgs)) is None
     rng = np.random.default_rng(5)
     for layout in (0, 1):
         dims, arrays, pool, ddesc = make_gemm_batch(O, cuda, layout, 24, 1, 150, 61 + layout)
         C0 = [arrays[3 * p + 2] for p in range(len(dims))]
-        for cfg in configs:
+        for cfg in cfgs:
             name = Q.gemm_batched_config(Q.MODE_OZAKI, cfg)
             S = int(name.split()[0])
             _say(f"ozaki batched layout={layout} cfg={cfg} ({name})")
@@ -98,7 +98,7 @@ def test_ozaki_batched_every_slice_count(O, Q, cuda):
             got = [pool.get(3 * p + 2, C0[p].shape[0]) for p in range(len(dims))]
             worst = _check_oracle(O, layout, dims, arrays, C0, lambda p: got[p], rng)
             _say(f"  max rel err {worst:.3e}, repaired entries {Q.dev_ozaki_repairs()}")
-            if cfg in (0, configs[-1]):
+            if cfg in (0, cfgs[-1]):
                 single = _single_problem(Q, cuda, layout, dims, arrays, S)
                 for p in range(len(dims)):
                     assert np.array_equal(got[p], single[p]), ("batched != single-problem", layout, cfg, p, dims[p][:3])
@@ -211,7 +211,7 @@ def _quantiles(v):
     return {"median": q(0.5), "p90": q(0.9), "p99": q(0.99), "max": float(v[-1])}
 
 
-def test_config4_full_size_error_distribution(O, Q, cuda):
+def test_batched_full_size_error_distribution(O, Q, cuda):
     """The batched workload at full size: 4096 qgemm problems (m, n, k uniform in [8, 256]) and
     16384 qgemv problems (m, n uniform in [32, 1024]), RowMajor, entries and per-problem alpha, beta
     full-mantissa uniform [-1, 1) (device generator).  Per problem, the largest relative error of
@@ -252,7 +252,7 @@ def test_config4_full_size_error_distribution(O, Q, cuda):
         pool.copy_(pool0)
         if form == "ozaki":
             best = Q.dev_gemm_batched_tune(mode, 0, count, ddesc.data_ptr())
-            _say(f"config 4 gemm: tuned {Q.gemm_batched_config(mode, best)}")
+            _say(f"4096 gemm problems: tuned {Q.gemm_batched_config(mode, best)}")
         Q.dev_gemm_batched(mode, 0, count, ddesc.data_ptr())
         torch.cuda.synchronize()
         if form == "ozaki":
@@ -268,7 +268,7 @@ def test_config4_full_size_error_distribution(O, Q, cuda):
                                           abh[2 * p + 1], C0, (n, 1), cand=got[S_PER * p:S_PER * (p + 1)])
             worst.append(float(rel.max()))
         qs = _quantiles(worst)
-        _say(f"config 4 gemm {form}: per-problem max rel err over {S_PER} samples: " +
+        _say(f"4096 gemm problems, {form}: per-problem max rel err over {S_PER} samples: " +
              ", ".join(f"{key} {val:.3e}" for key, val in qs.items()))
         if form == "ozaki":
             assert qs["max"] <= TOL
@@ -320,7 +320,7 @@ def test_config4_full_size_error_distribution(O, Q, cuda):
             worst.append(float(rel[0]))
             mism += int(not np.array_equal(rounded[0], got[p]))
         qs = _quantiles(worst)
-        _say(f"config 4 gemv {form}: rel err of one sampled row per problem: " +
+        _say(f"16384 gemv problems, {form}: rel err of one sampled row per problem: " +
              ", ".join(f"{key} {val:.3e}" for key, val in qs.items()) + f"; not correctly rounded: {mism}")
         if form == "exact":
             assert mism == 0
