@@ -374,15 +374,14 @@ int oz_probe(int colmajor, int64_t m, int64_t n, int64_t k, const void *A, int64
 // method; CRT bits the CRT method); otherwise the probe's value sets both: the slice count S and
 // the CRT bits t whose certification bounds pass the probe's entries.  AUTO takes the CRT method
 // when t fits (<= 158 bits) and its padded GEMM work (moduli x 256-tiles) is below the slice
-// method's (slice pairs x 64-tiles).  sync is set when the probe ran (the call synchronizes anyway).
+// method's (slice pairs x 64-tiles).
 struct oz_choice {
-  int method, S, t, sync;
+  int method, S, t;
   void *probe_ws;  // the probe's workspace (held until the call is enqueued), or null
 };
 int oz_choose(int colmajor, int64_t m, int64_t n, int64_t k, const void *A, int64_t lda, const void *B, int64_t ldb,
               hipStream_t s, oz_choice *c) {
   const int method = g_oz_method.load(), fixedS = g_oz_slices.load(), fixedT = g_oz_bits.load();
-  c->sync = 0;
   c->probe_ws = nullptr;
   c->S = fixedS;
   c->t = fixedT;
@@ -398,7 +397,6 @@ int oz_choose(int colmajor, int64_t m, int64_t n, int64_t k, const void *A, int6
   int r = 0;
   int rc = oz_probe(colmajor, m, n, k, A, lda, B, ldb, s, &r, &c->probe_ws);
   if (rc) return rc;
-  c->sync = 1;
   c->S = qbk::ozaki_slices_for(k, r);
   c->t = qbk::ozaki_crt_bits_for(k, r);
   if (c->method == QBLAS_OZAKI_SLICES) return QBLAS_OK;
@@ -459,13 +457,12 @@ int dev_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, q128 alpha, 
       ev = t_oz_ev;
     }
     t_oz_ev_valid = ev != nullptr;
-    // a call that ran the probe already synchronized, so the repair may too
     if (crt)
       QB_CHECK(qbk::launch_gemm_crt(colmajor, m, n, k, ch.t, alpha, (const q128 *)A, lda, (const q128 *)B, ldb, beta,
-                                    (q128 *)C, ldc, ws, s, ev, ch.sync, ch.probe_ws));
+                                    (q128 *)C, ldc, ws, s, ev, ch.probe_ws));
     else
       QB_CHECK(qbk::launch_gemm_ozaki(colmajor, m, n, k, ch.S, alpha, (const q128 *)A, lda, (const q128 *)B, ldb,
-                                      beta, (q128 *)C, ldc, ws, s, ev, ch.sync, ch.probe_ws));
+                                      beta, (q128 *)C, ldc, ws, s, ev, ch.probe_ws));
     t_exact_status = (const uint32_t *)ws;
     t_oz_hdr = (const uint32_t *)ws;
     return ws_release(t_ws, s);

@@ -70,7 +70,6 @@ hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int 
                              int64_t lda, const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc, void *ws,
                              hipStream_t st, hipEvent_t *ev /* null, or 5 events: start, after slicing, after
                              the slice products, after the finish, after the repair */,
-                             int sync_repair /* 1: may synchronize to choose how the repair reads B */,
                              const void *probe_ws /* null, or the workspace of a probe of the same operands:
                              its scaling exponents are used (it must stay intact until the call's work ends) */);
 hipError_t ozaki_repair_count(const void *ws, uint32_t *out, hipStream_t st);
@@ -84,7 +83,7 @@ size_t gemm_crt_ws(int64_t m, int64_t n, int64_t k, int t);
 int crt_moduli_for(int64_t k, int t);
 hipError_t launch_gemm_crt(int colmajor, int64_t m, int64_t n, int64_t k, int t, q128 alpha, const q128 *A,
                            int64_t lda, const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc, void *ws,
-                           hipStream_t st, hipEvent_t *ev, int sync_repair, const void *probe_ws);
+                           hipStream_t st, hipEvent_t *ev, const void *probe_ws);
 // bits t of the CRT method for a probe value (as ozaki_slices_for); 0: use the slice method
 int ozaki_crt_bits_for(int64_t k, int r);
 size_t ozaki_probe_ws(int64_t m, int64_t n);

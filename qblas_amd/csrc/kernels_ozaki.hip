@@ -647,17 +647,26 @@ __device__ __forceinline__ void oz_repair_entry(int64_t K, const q128 *__restric
   __syncthreads();
 }
 
+// Queued entries at least this many: B's strided columns are read from a transposed copy (one pass
+// over B pays off against ~80 ns per gathered column).
+constexpr uint32_t OZ_REPAIR_TRANSPOSE = 2048;
+
+// Bt: null, or the transposed copy of B (N x K row-major) that k_oz_transpose writes when the queue
+// holds at least OZ_REPAIR_TRANSPOSE entries; the same count selects it here.
 __global__ void __launch_bounds__(OZR_BS) k_oz_repair(int64_t K, const q128 *__restrict__ A, int64_t asr, int64_t asc,
                                                       const q128 *__restrict__ B, int64_t bsr, int64_t bsc,
-                                                      const int32_t *__restrict__ eA, const int32_t *__restrict__ eB,
-                                                      q128 alpha, q128 beta, q128 *__restrict__ C, int64_t csr,
-                                                      int64_t csc, uint32_t *__restrict__ hdr,
-                                                      const uint32_t *__restrict__ list) {
+                                                      const q128 *__restrict__ Bt, const int32_t *__restrict__ eA,
+                                                      const int32_t *__restrict__ eB, q128 alpha, q128 beta,
+                                                      q128 *__restrict__ C, int64_t csr, int64_t csc,
+                                                      uint32_t *__restrict__ hdr, const uint32_t *__restrict__ list) {
   __shared__ uint32_t s_w[OZR_BS / 64][2 * WW + 1];
   const uint32_t count = hdr[1];
+  const bool useT = Bt != nullptr && count >= OZ_REPAIR_TRANSPOSE;
+  const q128 *Br = useT ? Bt : B;
+  const int64_t rbsr = useT ? 1 : bsr, rbsc = useT ? K : bsc;
   for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) {
     const int64_t i = list[2 * (int64_t)e], j = list[2 * (int64_t)e + 1];
-    oz_repair_entry(K, A + i * asr, asc, B + j * bsc, bsr, oz_anchor(eA[i], eB[j]), alpha, beta,
+    oz_repair_entry(K, A + i * asr, asc, Br + j * rbsc, rbsr, oz_anchor(eA[i], eB[j]), alpha, beta,
                     C + i * csr + j * csc, hdr, s_w);
   }
 }
@@ -704,10 +713,12 @@ __global__ void __launch_bounds__(OZR_BS) k_oz_probe(int64_t M, int64_t N, int64
   }
 }
 
-// out[j * K + l] = B[l * ldb + j] (K x N row-major -> N x K row-major), 32 x 32 tiles via LDS
+// out[j * K + l] = B[l * ldb + j] (K x N row-major -> N x K row-major), 32 x 32 tiles via LDS; only
+// when the repair queue (header word 1) holds at least OZ_REPAIR_TRANSPOSE entries
 __global__ void __launch_bounds__(256) k_oz_transpose(const q128 *__restrict__ B, int64_t K, int64_t N, int64_t ldb,
-                                                      q128 *__restrict__ out) {
+                                                      const uint32_t *__restrict__ hdr, q128 *__restrict__ out) {
   __shared__ uint4 t[32][33];
+  if (hdr[1] < OZ_REPAIR_TRANSPOSE) return;  // uniform: the whole grid skips the copy
   const int64_t l0 = (int64_t)blockIdx.y * 32, j0 = (int64_t)blockIdx.x * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
 #pragma unroll
@@ -787,30 +798,20 @@ static void oz_scale(const q128 *X, int64_t R, int64_t Cn, int64_t sr, int64_t s
   k_oz_scale<<<grid, 256, 0, st>>>(X, R, Cn, sr, sc, contig, cchunk, E, F);
 }
 
-// Exact recomputation of the queued entries (header word 1 counts them, list holds (i, j)).  With
-// sync_repair the caller accepts a synchronization: the queue length is read, and when it is long
-// B's columns are read from a transposed copy (contiguous, at Bt) instead of with stride ldb.
+// Exact recomputation of the queued entries (header word 1 counts them, list holds (i, j)).  When
+// B's columns are strided, a transposed copy of B (at Bt) is made first if the queue is long; the
+// copy and the repair read the queue length on the device, so nothing is synchronized.
 static hipError_t oz_repair_queue(int64_t n, int64_t k, const q128 *A, int64_t asr, int64_t asc, const q128 *B,
                                   int64_t bsr, int64_t bsc, const int32_t *eA, const int32_t *eB, q128 alpha,
                                   q128 beta, q128 *C, int64_t csr, int64_t csc, uint32_t *hdr, const uint32_t *list,
-                                  q128 *Bt, hipStream_t st, hipEvent_t *ev, int sync_repair) {
-  const q128 *Br = B;
-  int64_t rbsr = bsr, rbsc = bsc;
-  if (sync_repair && bsr != 1 && n > 1) {
-    uint32_t count = 0;
-    hipError_t e2 = hipMemcpyAsync(&count, hdr + 1, 4, hipMemcpyDeviceToHost, st);
-    if (e2 == hipSuccess) e2 = hipStreamSynchronize(st);
-    if (e2 != hipSuccess) return e2;
-    if (count >= 2048) {  // the transpose (one pass over B) pays off against ~80 ns per gathered column
-      dim3 tg((unsigned)((n + 31) / 32), (unsigned)((k + 31) / 32));
-      k_oz_transpose<<<tg, 256, 0, st>>>(B, k, n, bsr, Bt);
-      Br = Bt;
-      rbsr = 1;
-      rbsc = k;
-    }
+                                  q128 *Bt, hipStream_t st, hipEvent_t *ev) {
+  const bool strided = bsr != 1 && n > 1;
+  if (strided) {
+    dim3 tg((unsigned)((n + 31) / 32), (unsigned)((k + 31) / 32));
+    k_oz_transpose<<<tg, 256, 0, st>>>(B, k, n, bsr, hdr, Bt);
   }
-  k_oz_repair<<<OZ_REPAIR_BLOCKS, OZR_BS, 0, st>>>(k, A, asr, asc, Br, rbsr, rbsc, eA, eB, alpha, beta, C, csr, csc,
-                                                   hdr, list);
+  k_oz_repair<<<OZ_REPAIR_BLOCKS, OZR_BS, 0, st>>>(k, A, asr, asc, B, bsr, bsc, strided ? Bt : nullptr, eA, eB,
+                                                   alpha, beta, C, csr, csc, hdr, list);
   if (ev) (void)hipEventRecord(ev[4], st);
   return hipGetLastError();
 }
@@ -841,7 +842,7 @@ static void oz_prepare(int64_t Mp, int64_t Np, int64_t m, int64_t n, int64_t k, 
 
 hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int S, q128 alpha, const q128 *A,
                              int64_t lda, const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc, void *wsv,
-                             hipStream_t st, hipEvent_t *ev, int sync_repair, const void *probe_ws) {
+                             hipStream_t st, hipEvent_t *ev, const void *probe_ws) {
   if (S < OZ_SMIN || S > OZ_SMAX) return hipErrorInvalidValue;
   char *ws = (char *)wsv;
   const oz_layout L = oz_plan(m, n, k, S);
@@ -887,7 +888,7 @@ hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int 
                                                             (uint32_t *)(ws + L.list));
   if (ev) (void)hipEventRecord(ev[3], st);
   return oz_repair_queue(n, k, A, asr, asc, B, bsr, bsc, eA, eB, alpha, beta, C, csr, csc, hdr,
-                         (const uint32_t *)(ws + L.list), (q128 *)(ws + L.bt), st, ev, sync_repair);
+                         (const uint32_t *)(ws + L.list), (q128 *)(ws + L.bt), st, ev);
 }
 
 size_t ozaki_probe_ws(int64_t m, int64_t n) {
@@ -2058,7 +2059,7 @@ int ozaki_crt_bits_for(int64_t k, int r) {
 
 hipError_t launch_gemm_crt(int colmajor, int64_t m, int64_t n, int64_t k, int t, q128 alpha, const q128 *A,
                            int64_t lda, const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc, void *wsv,
-                           hipStream_t st, hipEvent_t *ev, int sync_repair, const void *probe_ws) {
+                           hipStream_t st, hipEvent_t *ev, const void *probe_ws) {
   if (t < CRT_TMIN || t > CRT_TMAX) return hipErrorInvalidValue;
   const int nq = crt_moduli_for(k, t);
   if (!nq) return hipErrorInvalidValue;
@@ -2112,7 +2113,7 @@ hipError_t launch_gemm_crt(int colmajor, int64_t m, int64_t n, int64_t k, int t,
 #undef QB_CRT_FIN
   if (ev) (void)hipEventRecord(ev[3], st);
   return oz_repair_queue(n, k, A, asr, asc, B, bsr, bsc, eA, eB, alpha, beta, C, csr, csc, hdr,
-                         (const uint32_t *)(ws + L.list), (q128 *)(ws + L.bt), st, ev, sync_repair);
+                         (const uint32_t *)(ws + L.list), (q128 *)(ws + L.bt), st, ev);
 }
 
 }  // namespace qbk
