@@ -31,6 +31,8 @@ Also reported:
                   captured graph: problems/s, quad GFLOP/s or GB/s, REFERENCE and DD forms; the
                   qgemm batch also in the OZAKI form (slice count from the autotuner, eager calls:
                   the form synchronizes once per call to size its workspace).
+  host_abi        the reference's C ABI with host buffers (pageable and pinned): the same qgemm,
+                  qgemv RowMajor and qdot with their PCIe copies in and out, wall-clock per call.
   ill_conditioned qgemm 8192 x 8192 x 16384 ColMajor whose entries cancel to ~1e-18 of the
                   largest products: OZAKI (26 slices: the bound certifies all but entries within
                   ~2^-12 of zero) and DD, with HBM in use.
@@ -77,6 +79,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     p.add_argument("--no-batched", action="store_true", help="skip the batched problems")
     p.add_argument("--no-ill", action="store_true", help="skip the ill-conditioned 8192x8192x16384 qgemm")
+    p.add_argument("--no-host", action="store_true", help="skip the host-buffer C ABI calls")
     p.add_argument("--no-ill-plain", action="store_true",
                    help="skip the plain form (~10 s) of the ill-conditioned qgemm")
     p.add_argument("--batch-gemm", type=int, default=4096, help="problems in the batched qgemm")
@@ -352,6 +355,68 @@ def bench_batched(Q, torch, dev, stream, steps, gemm_count, gemv_count, seed=77)
     return out
 
 
+def bench_host_abi(Q, torch, dev, N, m, n, reps):
+    """The reference's C ABI with host buffers (quadblas_qgemm in the OZAKI form, quadblas_qgemv
+    RowMajor and quadblas_qdot in the DD form, same workloads as above): each call copies its
+    operands to HBM and its result back over PCIe and returns when the result is in host memory,
+    as a drop-in caller's call does.  Wall-clock per call (median of `reps` after one warm-up),
+    with pageable host buffers (what a caller's malloc gives) and with pinned ones (hipHostMalloc,
+    torch's pin_memory).  pcie_bytes: the operands in plus the result out."""
+    import numpy as np
+    sh = torch.cuda.current_stream().cuda_stream
+    gA = torch.empty((N * N, 2), dtype=torch.int64, device=dev)
+    gB = torch.empty((N * N, 2), dtype=torch.int64, device=dev)
+    Q.dev_fill(Q.FILL_LOG_UNIFORM, 11, N * N, gA.data_ptr(), -30, 30, stream=sh)
+    Q.dev_fill(Q.FILL_LOG_UNIFORM, 12, N * N, gB.data_ptr(), -30, 30, stream=sh)
+    gM = torch.empty((m * m, 2), dtype=torch.int64, device=dev)
+    gv = torch.empty((m, 2), dtype=torch.int64, device=dev)
+    gx = torch.empty((n, 2), dtype=torch.int64, device=dev)
+    gy = torch.empty((n, 2), dtype=torch.int64, device=dev)
+    for t, seed in ((gM, 3), (gv, 4), (gx, 1), (gy, 2)):
+        Q.dev_fill(Q.FILL_UNIFORM_FULL, seed, t.shape[0], t.data_ptr(), stream=sh)
+    torch.cuda.synchronize()
+    saved = Q.get_mode()
+
+    def timed(f):
+        f()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            f()
+            ts.append(time.perf_counter() - t0)
+        return statistics.median(ts) * 1e3
+
+    out = {}
+    for pinned in (False, True):
+        def host(t):
+            h = torch.empty(t.shape, dtype=t.dtype, pin_memory=pinned)
+            h.copy_(t)
+            return h
+        hA, hB, hM, hv, hx, hy = (host(t) for t in (gA, gB, gM, gv, gx, gy))
+        hC = torch.zeros((N * N, 2), dtype=torch.int64, pin_memory=pinned)
+        hw = torch.zeros((m, 2), dtype=torch.int64, pin_memory=pinned)
+        a, b, c, M_, v, w, x, y = (t.numpy().view(np.uint64) for t in (hA, hB, hC, hM, hv, hw, hx, hy))
+        r = {}
+        Q.set_mode("ozaki")
+        ms = timed(lambda: Q.qgemm("R", "N", "N", N, N, N, 1.0, a, N, b, N, 0.0, c, N))
+        pb = 16.0 * 4 * N * N   # A, B, C in; C out
+        r["qgemm"] = {"ms": round(ms, 2), "quad_GFLOPs": round(2.0 * N ** 3 / (ms * 1e-3) / 1e9, 1),
+                      "pcie_bytes": pb, "GB/s": round(pb / (ms * 1e-3) / 1e9, 1)}
+        Q.set_mode("dd")
+        ms = timed(lambda: Q.qgemv("R", "N", m, m, 1.5, M_, m, v, 1, -0.5, w, 1))
+        pb = 16.0 * (m * m + m + 2 * m)
+        r["qgemv_rowmajor"] = {"ms": round(ms, 2), "pcie_bytes": pb, "GB/s": round(pb / (ms * 1e-3) / 1e9, 1)}
+        ms = timed(lambda: Q.qdot(n, x, 1, y, 1))
+        pb = 32.0 * n
+        r["qdot"] = {"ms": round(ms, 2), "pcie_bytes": pb, "GB/s": round(pb / (ms * 1e-3) / 1e9, 1)}
+        out["pinned" if pinned else "pageable"] = r
+        del hA, hB, hC, hM, hv, hw, hx, hy, a, b, c, M_, v, w, x, y
+    Q.set_mode(saved)
+    del gA, gB, gM, gv, gx, gy
+    torch.cuda.empty_cache()
+    return out
+
+
 def bench_ill_conditioned(Q, torch, dev, stream, M, N, K, slices, plain=True):
     """qgemm M x N x K ColMajor, alpha = 1, beta = 0: rows of A carry +-1e20 u pairs along k and
     the columns of B equal values on each pair (qblas_hip_fill_2d kinds 2 and 3), so the exact
@@ -575,6 +640,9 @@ def main():
     batched = None
     if not args.no_batched:
         batched = bench_batched(Q, torch, dev, stream, max(args.steps, 3), args.batch_gemm, args.batch_gemv)
+    host_abi = None
+    if not args.no_host:
+        host_abi = bench_host_abi(Q, torch, dev, N, args.m, args.n_dot, 3)
     cpu = cpu_mem = None
     if not args.no_cpu:
         cpu = cpu_baseline_gemm(N, args.cpu_rows)
@@ -591,7 +659,7 @@ def main():
                                                               if oz["method"] == "crt" else f" ({S} slices)"),
                    "error_bound": "each entry <= 2^-105.9 rel (certified) or correctly rounded (repaired)"},
         "roofline": roofline, "cpu_baseline": cpu, "qgemm_forms": forms, "memory_bound": memory,
-        "cpu_baseline_memory_bound": cpu_mem, "batched": batched, "ill_conditioned": ill, "ozaki_status": oz["status"], "wall_s": round(oz["wall_s"], 3),
+        "cpu_baseline_memory_bound": cpu_mem, "batched": batched, "ill_conditioned": ill, "host_abi": host_abi, "ozaki_status": oz["status"], "wall_s": round(oz["wall_s"], 3),
     }
     print(json.dumps(line))
 

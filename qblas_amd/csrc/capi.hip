@@ -21,6 +21,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -251,6 +252,210 @@ int host_stream(hipStream_t *s) {
   return QBLAS_OK;
 }
 
+// the host gemm's second stream (C's copy in) and its two events
+thread_local hipStream_t t_stream2 = nullptr;
+thread_local hipEvent_t t_ev_ab = nullptr, t_ev_c = nullptr;
+int host_stream2(hipStream_t *s) {
+  if (!t_stream2) QB_CHECK(hipStreamCreateWithFlags(&t_stream2, hipStreamNonBlocking));
+  if (!t_ev_ab) QB_CHECK(hipEventCreateWithFlags(&t_ev_ab, hipEventDisableTiming));
+  if (!t_ev_c) QB_CHECK(hipEventCreateWithFlags(&t_ev_c, hipEventDisableTiming));
+  *s = t_stream2;
+  return QBLAS_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// host <-> device copies of the host-pointer entry points
+// ---------------------------------------------------------------------------------------------
+// A caller's pinned buffer (hipHostMalloc) is copied by one DMA.  A pageable one (what malloc
+// gives) goes through pinned staging slots: up to stage_workers() threads (QBLAS_STAGE_WORKERS,
+// 1..16, default 8; the calling thread is one of them) take interleaved chunks of stage_chunk()
+// bytes (QBLAS_STAGE_CHUNK_MB, 1..64, default 8), each copying a chunk between the caller's buffer
+// and one of its STAGE_SLOTS slots with memcpy while the DMA engine moves its other slot, and
+// waiting on the slot's event before reusing it.  The slots (128 MiB pinned with the defaults)
+// are allocated at the thread's first staged copy and kept.  Measured on MI355X: 50 GB/s host to
+// device from pageable memory (56 GB/s from pinned; the runtime's own pageable path, which stages
+// on one thread, 28 GB/s); 4 to 16 workers give the same rate, 8 and 16 MiB chunks beat 2 and 4
+// MiB (per-copy overhead).  Spans below STAGE_DIRECT use one plain copy.
+constexpr int STAGE_WORKERS = 16;  // at most
+constexpr int STAGE_SLOTS = 2;
+constexpr size_t STAGE_DIRECT = (size_t)2 << 20;
+struct StageRing {
+  bool ready = false;
+  void *slot[STAGE_WORKERS][STAGE_SLOTS] = {};
+  hipEvent_t ev[STAGE_WORKERS][STAGE_SLOTS] = {};
+  bool pending[STAGE_WORKERS][STAGE_SLOTS] = {};  // a DMA from the slot may still be running
+};
+thread_local StageRing t_ring;
+
+size_t stage_chunk() {
+  static const size_t n = [] {
+    const char *e = getenv("QBLAS_STAGE_CHUNK_MB");
+    const int v = e ? atoi(e) : 8;
+    return (size_t)(v < 1 ? 1 : (v > 64 ? 64 : v)) << 20;
+  }();
+  return n;
+}
+
+int stage_workers() {
+  static const int n = [] {
+    const char *e = getenv("QBLAS_STAGE_WORKERS");
+    const int v = e ? atoi(e) : 8;
+    return v < 1 ? 1 : (v > STAGE_WORKERS ? STAGE_WORKERS : v);
+  }();
+  return n;
+}
+
+int stage_ring() {
+  if (t_ring.ready) return QBLAS_OK;
+  for (int w = 0; w < stage_workers(); ++w)
+    for (int k = 0; k < STAGE_SLOTS; ++k) {
+      if (!t_ring.slot[w][k]) QB_CHECK(hipHostMalloc(&t_ring.slot[w][k], stage_chunk(), hipHostMallocDefault));
+      // default flags: the event's completion also makes a device-to-host copy visible to the host
+      if (!t_ring.ev[w][k]) QB_CHECK(hipEventCreateWithFlags(&t_ring.ev[w][k], hipEventDisableTiming));
+    }
+  t_ring.ready = true;
+  return QBLAS_OK;
+}
+
+bool host_pinned(const void *p) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();  // pageable memory may be reported as an error: not one of ours
+    return false;
+  }
+  return a.type == hipMemoryTypeHost;
+}
+
+// chunks w, w + nw, ... of [0, bytes) by worker w, through the calling thread's ring R (the
+// workers are other threads); returns the first HIP error
+hipError_t stage_chunks(StageRing &R, int w, int nw, int dev, char *d, char *h, size_t bytes, bool h2d,
+                        hipStream_t s) {
+  hipError_t e = hipSetDevice(dev);
+  if (e != hipSuccess) return e;
+  const size_t chunk = stage_chunk();
+  const size_t nch = (bytes + chunk - 1) / chunk;
+  auto span = [&](size_t c, size_t *off) {
+    *off = c * chunk;
+    return std::min(chunk, bytes - *off);
+  };
+  if (h2d) {
+    int i = 0;
+    for (size_t c = (size_t)w; c < nch; c += (size_t)nw, ++i) {
+      const int k = i % STAGE_SLOTS;
+      if (R.pending[w][k] && (e = hipEventSynchronize(R.ev[w][k])) != hipSuccess) return e;
+      R.pending[w][k] = false;
+      size_t off;
+      const size_t len = span(c, &off);
+      memcpy(R.slot[w][k], h + off, len);
+      if ((e = hipMemcpyAsync(d + off, R.slot[w][k], len, hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+      if ((e = hipEventRecord(R.ev[w][k], s)) != hipSuccess) return e;
+      R.pending[w][k] = true;
+    }
+    return hipSuccess;
+  }
+  // device to host: the DMAs of the first STAGE_SLOTS chunks go out at once; each wait is
+  // followed by the host copy out of the slot and the DMA of the chunk that slot takes next
+  std::vector<size_t> mine;
+  for (size_t c = (size_t)w; c < nch; c += (size_t)nw) mine.push_back(c);
+  auto issue = [&](size_t i) -> hipError_t {
+    const int k = (int)(i % STAGE_SLOTS);
+    hipError_t r;
+    if (R.pending[w][k] && (r = hipEventSynchronize(R.ev[w][k])) != hipSuccess) return r;
+    size_t off;
+    const size_t len = span(mine[i], &off);
+    if ((r = hipMemcpyAsync(R.slot[w][k], d + off, len, hipMemcpyDeviceToHost, s)) != hipSuccess) return r;
+    if ((r = hipEventRecord(R.ev[w][k], s)) != hipSuccess) return r;
+    R.pending[w][k] = true;
+    return hipSuccess;
+  };
+  for (size_t i = 0; i < mine.size() && i < (size_t)STAGE_SLOTS; ++i)
+    if ((e = issue(i)) != hipSuccess) return e;
+  for (size_t i = 0; i < mine.size(); ++i) {
+    const int k = (int)(i % STAGE_SLOTS);
+    if ((e = hipEventSynchronize(R.ev[w][k])) != hipSuccess) return e;
+    R.pending[w][k] = false;
+    size_t off;
+    const size_t len = span(mine[i], &off);
+    memcpy(h + off, R.slot[w][k], len);
+    if (i + STAGE_SLOTS < mine.size() && (e = issue(i + STAGE_SLOTS)) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// Copy `bytes` between a caller's host buffer and device memory on stream s.  Host to device
+// returns with the DMAs queued on s (later work on s is ordered after them); device to host
+// returns once the data is in the host buffer.
+int copy_host(void *dev_p, void *host_p, size_t bytes, bool h2d, hipStream_t s) {
+  if (!bytes) return QBLAS_OK;
+  if (bytes < STAGE_DIRECT || host_pinned(host_p)) {
+    QB_CHECK(h2d ? hipMemcpyAsync(dev_p, host_p, bytes, hipMemcpyHostToDevice, s)
+                 : hipMemcpyAsync(host_p, dev_p, bytes, hipMemcpyDeviceToHost, s));
+    if (!h2d) QB_CHECK(hipStreamSynchronize(s));
+    return QBLAS_OK;
+  }
+  int rc = stage_ring();
+  if (rc) return rc;
+  StageRing &R = t_ring;
+  int dev = 0;
+  QB_CHECK(hipGetDevice(&dev));
+  const size_t nch = (bytes + stage_chunk() - 1) / stage_chunk();
+  const int nw = (int)std::min<size_t>(nch, (size_t)stage_workers());
+  hipError_t err[STAGE_WORKERS];
+  std::vector<std::thread> th;
+  for (int w = 1; w < nw; ++w)
+    th.emplace_back([&, w] { err[w] = stage_chunks(R, w, nw, dev, (char *)dev_p, (char *)host_p, bytes, h2d, s); });
+  err[0] = stage_chunks(R, 0, nw, dev, (char *)dev_p, (char *)host_p, bytes, h2d, s);
+  for (auto &t : th) t.join();
+  for (int w = 0; w < nw; ++w)
+    if (err[w] != hipSuccess) {
+      set_err(h2d ? "host to device copy" : "device to host copy", err[w]);
+      return QBLAS_ERR_HIP;
+    }
+  return QBLAS_OK;
+}
+
+// A host-to-device copy staged by background threads while the calling thread goes on enqueueing
+// work; once join() returns, every DMA of the copy is enqueued on its stream.
+struct AsyncH2D {
+  std::vector<std::thread> th;
+  hipError_t err[STAGE_WORKERS] = {};
+  int nw = 0;
+  hipError_t join() {
+    for (auto &t : th) t.join();
+    th.clear();
+    for (int w = 0; w < nw; ++w)
+      if (err[w] != hipSuccess) return err[w];
+    return hipSuccess;
+  }
+  ~AsyncH2D() {
+    for (auto &t : th)
+      if (t.joinable()) t.join();
+  }
+};
+
+int copy_h2d_async(AsyncH2D &a, void *dev_p, const void *host_p, size_t bytes, hipStream_t s) {
+  if (!bytes) return QBLAS_OK;
+  if (bytes < STAGE_DIRECT || host_pinned(host_p)) {
+    QB_CHECK(hipMemcpyAsync(dev_p, host_p, bytes, hipMemcpyHostToDevice, s));
+    return QBLAS_OK;
+  }
+  int rc = stage_ring();
+  if (rc) return rc;
+  StageRing *R = &t_ring;
+  int dev = 0;
+  QB_CHECK(hipGetDevice(&dev));
+  const size_t nch = (bytes + stage_chunk() - 1) / stage_chunk();
+  const int nw = (int)std::min<size_t>(nch, (size_t)stage_workers());
+  a.nw = nw;
+  for (int w = 0; w < nw; ++w) {
+    a.err[w] = hipSuccess;
+    a.th.emplace_back([&a, R, w, nw, dev, dev_p, host_p, bytes, s] {
+      a.err[w] = stage_chunks(*R, w, nw, dev, (char *)dev_p, (char *)host_p, bytes, true, s);
+    });
+  }
+  return QBLAS_OK;
+}
+
 bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
 
 q128 load_q(const void *p) {
@@ -413,8 +618,12 @@ int oz_choose(int colmajor, int64_t m, int64_t n, int64_t k, const void *A, int6
   return QBLAS_OK;
 }
 
+// gate: null, or the hook that makes s wait for C's copy in (called before the first kernel that
+// reads C is enqueued; the OZAKI form enqueues its probe, scaling, residues or slices and products
+// first)
 int dev_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, q128 alpha, const void *A, int64_t lda,
-             const void *B, int64_t ldb, q128 beta, void *C, int64_t ldc, hipStream_t s) {
+             const void *B, int64_t ldb, q128 beta, void *C, int64_t ldc, hipStream_t s,
+             const qbk::c_gate *gate = nullptr) {
   if (m < 0 || n < 0 || k < 0 || lda < 0 || ldb < 0 || ldc < 0) {
     set_err_msg("qblas_hip_gemm: negative size or leading dimension");
     return QBLAS_ERR_ARG;
@@ -425,6 +634,7 @@ int dev_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, q128 alpha, 
   }
   if (m == 0 || n == 0 || k == 0) return QBLAS_OK;  // level3.hpp:221-222
   const int colmajor = layout != 0;
+  if (gate && mode != QBLAS_MODE_OZAKI) QB_CHECK(gate->fn(gate->arg, s));
   if (mode == QBLAS_MODE_OZAKI) {
     oz_choice ch;
     int rc0 = oz_choose(colmajor, m, n, k, A, lda, B, ldb, s, &ch);
@@ -459,10 +669,10 @@ int dev_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, q128 alpha, 
     t_oz_ev_valid = ev != nullptr;
     if (crt)
       QB_CHECK(qbk::launch_gemm_crt(colmajor, m, n, k, ch.t, alpha, (const q128 *)A, lda, (const q128 *)B, ldb, beta,
-                                    (q128 *)C, ldc, ws, s, ev, ch.probe_ws));
+                                    (q128 *)C, ldc, ws, s, ev, ch.probe_ws, gate));
     else
       QB_CHECK(qbk::launch_gemm_ozaki(colmajor, m, n, k, ch.S, alpha, (const q128 *)A, lda, (const q128 *)B, ldb,
-                                      beta, (q128 *)C, ldc, ws, s, ev, ch.probe_ws));
+                                      beta, (q128 *)C, ldc, ws, s, ev, ch.probe_ws, gate));
     t_exact_status = (const uint32_t *)ws;
     t_oz_hdr = (const uint32_t *)ws;
     return ws_release(t_ws, s);
@@ -518,8 +728,7 @@ size_t span_bytes(int64_t count, int64_t inc) { return count <= 0 ? 0 : (size_t)
 int stage_in(int slot, const void *host, size_t bytes, hipStream_t s, void **dev) {
   int rc = ws_acquire(t_stage[slot], bytes ? bytes : 16, s, dev);
   if (rc) return rc;
-  if (bytes) QB_CHECK(hipMemcpyAsync(*dev, host, bytes, hipMemcpyHostToDevice, s));
-  return QBLAS_OK;
+  return copy_host(*dev, (void *)host, bytes, true, s);
 }
 
 int host_dot(int64_t n, const void *x, int64_t incx, const void *y, int64_t incy, q128 *out) {
@@ -564,8 +773,7 @@ int host_gemv(int layout, int64_t m, int64_t n, q128 alpha, const void *A, int64
   if ((rc = stage_in(1, x, xbytes, s, &dx))) return rc;
   if ((rc = stage_in(2, y, ybytes, s, &dy))) return rc;
   if ((rc = dev_gemv(vec_mode(), layout, m, n, alpha, dA, lda, dx, incx, beta, dy, incy, s))) return rc;
-  QB_CHECK(hipMemcpyAsync(y, dy, ybytes, hipMemcpyDeviceToHost, s));
-  QB_CHECK(hipStreamSynchronize(s));
+  if ((rc = copy_host(dy, y, ybytes, false, s))) return rc;
   for (int i = 0; i < 3; ++i) ws_release(t_stage[i], s);
   return check_exact_status(s);
 }
@@ -593,10 +801,33 @@ int host_gemm(int layout, int64_t m, int64_t n, int64_t k, q128 alpha, const voi
   void *dA, *dB, *dC;
   if ((rc = stage_in(0, A, abytes, s, &dA))) return rc;
   if ((rc = stage_in(1, B, bbytes, s, &dB))) return rc;
-  if ((rc = stage_in(2, C, cbytes, s, &dC))) return rc;
-  if ((rc = dev_gemm(mat_mode(), layout, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc, s))) return rc;
-  QB_CHECK(hipMemcpyAsync(C, dC, cbytes, hipMemcpyDeviceToHost, s));
-  QB_CHECK(hipStreamSynchronize(s));
+  // C goes in after A and B, on a second stream and by background threads, while the kernels
+  // that do not read C run on s (the OZAKI form's probe, residues or slices, and products); the
+  // gate makes s wait for C's copy before the first kernel that reads it
+  hipStream_t s2;
+  if ((rc = host_stream2(&s2))) return rc;
+  if ((rc = ws_acquire(t_stage[2], cbytes, s, &dC))) return rc;
+  QB_CHECK(hipEventRecord(t_ev_ab, s));
+  QB_CHECK(hipStreamWaitEvent(s2, t_ev_ab, 0));
+  AsyncH2D cin;
+  if ((rc = copy_h2d_async(cin, dC, C, cbytes, s2))) return rc;
+  struct GateArg {
+    AsyncH2D *a;
+    hipStream_t s2;
+    bool called;
+  } ga{&cin, s2, false};
+  const qbk::c_gate gate{[](void *p, hipStream_t st) -> hipError_t {
+                           auto *g = (GateArg *)p;
+                           g->called = true;
+                           hipError_t e = g->a->join();
+                           if (e == hipSuccess) e = hipEventRecord(t_ev_c, g->s2);
+                           if (e == hipSuccess) e = hipStreamWaitEvent(st, t_ev_c, 0);
+                           return e;
+                         },
+                         &ga};
+  if ((rc = dev_gemm(mat_mode(), layout, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc, s, &gate))) return rc;
+  if (!ga.called) QB_CHECK(gate.fn(gate.arg, s));  // C is copied back even if no kernel read it
+  if ((rc = copy_host(dC, C, cbytes, false, s))) return rc;
   for (int i = 0; i < 3; ++i) ws_release(t_stage[i], s);
   return check_exact_status(s);
 }
@@ -797,8 +1028,7 @@ int host_axpy(int64_t n, q128 alpha, const void *x, int64_t incx, void *y, int64
   if ((rc = stage_in(0, x, xb, s, &dx))) return rc;
   if ((rc = stage_in(1, y, yb, s, &dy))) return rc;
   if ((rc = dev_axpy(n, alpha, dx, incx, dy, incy, s))) return rc;
-  QB_CHECK(hipMemcpyAsync(y, dy, yb, hipMemcpyDeviceToHost, s));
-  QB_CHECK(hipStreamSynchronize(s));
+  if ((rc = copy_host(dy, y, yb, false, s))) return rc;
   for (int i = 0; i < 2; ++i) ws_release(t_stage[i], s);
   return QBLAS_OK;
 }

@@ -62,6 +62,13 @@ hipError_t launch_gemv_exact(int colmajor, int64_t m, int64_t n, q128 alpha, con
                              const q128 *x, int64_t incx, q128 beta, q128 *y, int64_t incy, void *ws,
                              hipStream_t st);
 
+// Called by the gemm launchers right before they enqueue the first kernel that reads C: a caller
+// still copying C in on another stream makes st wait for that copy there.
+struct c_gate {
+  hipError_t (*fn)(void *arg, hipStream_t st);
+  void *arg;
+};
+
 // OZAKI form of gemm (kernels_ozaki.hip): S int8 slices per operand.  ws[0] (u32) is the status
 // word as for the exact form; ws[1] counts the entries the call recomputed exactly.
 constexpr int OZ_SMIN = 8, OZ_SMAX = 28;
@@ -71,7 +78,8 @@ hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int 
                              hipStream_t st, hipEvent_t *ev /* null, or 5 events: start, after slicing, after
                              the slice products, after the finish, after the repair */,
                              const void *probe_ws /* null, or the workspace of a probe of the same operands:
-                             its scaling exponents are used (it must stay intact until the call's work ends) */);
+                             its scaling exponents are used (it must stay intact until the call's work ends) */,
+                             const c_gate *gate = nullptr);
 hipError_t ozaki_repair_count(const void *ws, uint32_t *out, hipStream_t st);
 // probe: scales of A and B, then for `samples` (<= 1024) exact entries ilogb|AB_ij| - (E_i + F_j)
 // as ints at ws + ozaki_probe_out_offset (INT_MAX where unusable); ozaki_slices_for turns a chosen
@@ -83,7 +91,7 @@ size_t gemm_crt_ws(int64_t m, int64_t n, int64_t k, int t);
 int crt_moduli_for(int64_t k, int t);
 hipError_t launch_gemm_crt(int colmajor, int64_t m, int64_t n, int64_t k, int t, q128 alpha, const q128 *A,
                            int64_t lda, const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc, void *ws,
-                           hipStream_t st, hipEvent_t *ev, const void *probe_ws);
+                           hipStream_t st, hipEvent_t *ev, const void *probe_ws, const c_gate *gate = nullptr);
 // bits t of the CRT method for a probe value (as ozaki_slices_for); 0: use the slice method
 int ozaki_crt_bits_for(int64_t k, int r);
 size_t ozaki_probe_ws(int64_t m, int64_t n);

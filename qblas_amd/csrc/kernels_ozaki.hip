@@ -842,7 +842,7 @@ static void oz_prepare(int64_t Mp, int64_t Np, int64_t m, int64_t n, int64_t k, 
 
 hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int S, q128 alpha, const q128 *A,
                              int64_t lda, const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc, void *wsv,
-                             hipStream_t st, hipEvent_t *ev, const void *probe_ws) {
+                             hipStream_t st, hipEvent_t *ev, const void *probe_ws, const c_gate *gate) {
   if (S < OZ_SMIN || S > OZ_SMAX) return hipErrorInvalidValue;
   char *ws = (char *)wsv;
   const oz_layout L = oz_plan(m, n, k, S);
@@ -883,6 +883,7 @@ hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int 
   int kfac = 0;
   while ((double)(1ull << (kfac < 62 ? kfac : 62)) < kb && kfac < 80) ++kfac;
   const int64_t mn = m * n;
+  if (gate && (e = gate->fn(gate->arg, st)) != hipSuccess) return e;
   k_oz_finish<<<stride_blocks(mn, 256), 256, 0, st>>>(m, n, S, kfac, (int)L.npass, (const uint32_t *)(ws + L.part),
                                                             L.Mp, L.Np, eA, fA, eB, fB, alpha, beta, C, csr, csc, hdr,
                                                             (uint32_t *)(ws + L.list));
@@ -2059,7 +2060,7 @@ int ozaki_crt_bits_for(int64_t k, int r) {
 
 hipError_t launch_gemm_crt(int colmajor, int64_t m, int64_t n, int64_t k, int t, q128 alpha, const q128 *A,
                            int64_t lda, const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc, void *wsv,
-                           hipStream_t st, hipEvent_t *ev, const void *probe_ws) {
+                           hipStream_t st, hipEvent_t *ev, const void *probe_ws, const c_gate *gate) {
   if (t < CRT_TMIN || t > CRT_TMAX) return hipErrorInvalidValue;
   const int nq = crt_moduli_for(k, t);
   if (!nq) return hipErrorInvalidValue;
@@ -2093,6 +2094,7 @@ hipError_t launch_gemm_crt(int colmajor, int64_t m, int64_t n, int64_t k, int t,
   uint32_t *list = (uint32_t *)(ws + L.list), *xrec = (uint32_t *)(ws + L.xrec);
   const bool fast = q_iszero(beta) && alpha.lo == 0 && (alpha.hi << 1) == (0x3fffull << 49);
   const int64_t mn = m * n;
+  if (gate && (e = gate->fn(gate->arg, st)) != hipSuccess) return e;
 #define QB_CRT_FIN(XW)                                                                                              \
   if (fast) {                                                                                                     \
     k_crt_finish<XW, true><<<fgrid, 256, 0, st>>>(m, n, k, t, kc, L.Mb, L.Nb, L.pC, RC, tab, eA, fA, eB, fB, alpha, beta, \
