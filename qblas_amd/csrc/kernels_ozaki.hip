@@ -135,7 +135,15 @@ __global__ void __launch_bounds__(256) k_oz_scale(const q128 *__restrict__ X, in
     const int lane = threadIdx.x & 63;
     if (r >= R) return;  // whole waves
     const q128 *row = X + r * sr;
-    for (int64_t c = c0 + lane; c < c1; c += 64) e = max(e, oz_ilogb(ldq_c(row + c), fl));
+    int64_t c = c0 + lane;
+    for (; c + 192 < c1; c += 256) {  // four loads in flight per lane
+      q128 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = ldq_c(row + c + 64 * u);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) e = max(e, oz_ilogb(v[u], fl));
+    }
+    for (; c < c1; c += 64) e = max(e, oz_ilogb(ldq_c(row + c), fl));
     e = wave_max(e);
     fl = wave_or(fl);
     if (lane == 0) {
@@ -145,7 +153,15 @@ __global__ void __launch_bounds__(256) k_oz_scale(const q128 *__restrict__ X, in
   } else {
     const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (r >= R) return;
-    for (int64_t c = c0; c < c1; ++c) e = max(e, oz_ilogb(ldq_c(X + r * sr + c * sc), fl));
+    int64_t c = c0;
+    for (; c + 3 < c1; c += 4) {
+      q128 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = ldq_c(X + r * sr + (c + u) * sc);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) e = max(e, oz_ilogb(v[u], fl));
+    }
+    for (; c < c1; ++c) e = max(e, oz_ilogb(ldq_c(X + r * sr + c * sc), fl));
     if (e != OZ_EZERO) atomicMax(E + r, e);
     if (fl) atomicOr(F + r, fl);
   }
@@ -1433,33 +1449,37 @@ __global__ void __launch_bounds__(64) k_crt_tables(int nq, crt_tab *__restrict__
 }
 
 // W = N + sum_{k<20} 128 256^k with N = round(x 2^(t-Es)) (two's complement, 6 words): byte k of W
-// XOR 0x80 is the balanced digit d_k of N (|N| < 2^(t-2) <= 2^156 < the bias).
+// XOR 0x80 is the balanced digit d_k of N (|N| < 2^(t-2) <= 2^156 < the bias).  N rounds the
+// magnitude half up.  With x = (-1)^s m 2^(e-112) and sh = e - 112 - Es + t <= 43 (e <= Es - 3,
+// t <= 158): T = floor(m 2^(sh+1)) is m 2^44 shifted right by 43 - sh (one shift, no branch), and
+// |N| = floor(T / 2) + (T & 1); the sign and the bias are added in one carry chain:
+// W = (floor(T / 2) XOR -s) + bias + (s XOR (T & 1)).
 __device__ __forceinline__ void crt_scaled(q128 x, int Es, int t, uint32_t (&W)[6]) {
-  uint32_t Nw[6];
+  const bool live = q_bexp(x) != 0x7fffu && !q_iszero(x);
+  const uq u = unpack_finite(x);
+  const uint64_t ml = live ? u.ml : 0ull, mh = live ? u.mh : 0ull;
+  const uint32_t s = live ? u.s : 0u;
+  int rs = live ? 43 - (u.e - 112 - Es + t) : 0;
+  rs = rs < 0 ? 0 : (rs > 191 ? 191 : rs);
+  uint32_t b[6] = {0u, (uint32_t)(ml << 12), (uint32_t)(ml >> 20), (uint32_t)(ml >> 52) | (uint32_t)(mh << 12),
+                   (uint32_t)(mh >> 20), 0u};  // m 2^44 (157 bits)
+  const int ws = rs >> 5, bs = rs & 31;
 #pragma unroll
-  for (int i = 0; i < 6; ++i) Nw[i] = 0;
-  if (q_bexp(x) != 0x7fffu && !q_iszero(x)) {
-    const uq u = unpack_finite(x);
-    const int sh = u.e - 112 - Es + t;  // u.e <= Es - 3
-    uint32_t m[6] = {(uint32_t)u.ml, (uint32_t)(u.ml >> 32), (uint32_t)u.mh, (uint32_t)(u.mh >> 32), 0u, 0u};
-    if (sh >= 0) {
-      rshl<6>(Nw, m, sh);
-    } else if (sh > -115) {
-      const int pbit = -sh - 1;  // round half up of the magnitude
-      uint32_t c = 0;
+  for (int step = 4; step >= 1; step >>= 1) {
+    const bool on = (ws & step) != 0;
 #pragma unroll
-      for (int i = 0; i < 6; ++i) m[i] = addc32(m[i], (i == (pbit >> 5)) ? (1u << (pbit & 31)) : 0u, c, c);
-      rshr<6>(Nw, m, -sh);
-    }
-    if (u.s) {
-      uint32_t c = 1;
-#pragma unroll
-      for (int i = 0; i < 6; ++i) Nw[i] = addc32(~Nw[i], 0u, c, c);
-    }
+    for (int i = 0; i < 6; ++i) b[i] = on ? ((i + step < 6) ? b[i + step] : 0u) : b[i];
   }
-  uint32_t c = 0;
+  uint32_t T[6];
 #pragma unroll
-  for (int i = 0; i < 6; ++i) W[i] = addc32(Nw[i], i < 5 ? 0x80808080u : 0u, c, c);
+  for (int i = 0; i < 6; ++i) T[i] = (uint32_t)(((((uint64_t)((i + 1 < 6) ? b[i + 1] : 0u)) << 32) | b[i]) >> bs);
+  const uint32_t g = T[0] & 1u, smask = 0u - s;
+  uint32_t c = s ^ g;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    const uint32_t th = (uint32_t)(((((uint64_t)((i + 1 < 6) ? T[i + 1] : 0u)) << 32) | T[i]) >> 1);
+    W[i] = addc32(th ^ smask, i < 5 ? 0x80808080u : 0u, c, c);
+  }
 }
 
 // residue of x (|x| < 2^24) modulo p in [-p/2, p/2] (p odd) or [-128, 127] (p = 256), as a byte
@@ -1516,6 +1536,11 @@ __device__ __forceinline__ i32x4v crt_pack16(const i32x16v &d, int p, uint32_t m
 // of the fragment, with its digits 16h..16h+15 as the A operand of lane rho + 32h; the B operand
 // holds 256^k mod p for the lane's modulus column, and the accumulator starts at off_q, so the lane
 // holding MFMA output rows 8(v/4) + 4h + v%4 holds slot 2g + h, bytes v = 0..15 as x + off.
+// Moduli 32.. come from the swapped product (the powers as the A operand, the digits as B): lane
+// (c, h) then holds element c's sums for moduli 32 + 8(v/4) + 4h + v%4, so only the moduli below
+// nq are reduced (4 or 8 per lane instead of 16); the four lanes of a quad hold four consecutive
+// bytes of one slot, and a byte transpose across the quad (DPP broadcasts) gives each lane the
+// dword of one modulus.
 // Groups 0..15 use slots 0..31 (k < 16), groups 16..31 slots 32..63.
 constexpr int CRT_RS = 17;  // LDS row stride (elements) of the residue kernel
 __global__ void __launch_bounds__(256) k_crt_residues(const q128 *__restrict__ X, int64_t R, int64_t K, int64_t sr,
@@ -1524,6 +1549,17 @@ __global__ void __launch_bounds__(256) k_crt_residues(const q128 *__restrict__ X
                                                       int64_t plane, int8_t *__restrict__ out) {
   __shared__ __attribute__((aligned(16))) uint32_t s_lo[4][32 * CRT_RS][4];
   __shared__ uint32_t s_hi[4][32 * CRT_RS];
+  __shared__ i32x4v s_cst[32];  // moduli 32..63: p, mdiv, half, off
+  if (threadIdx.x < 32) {
+    const int q = 32 + (int)threadIdx.x;
+    i32x4v c;
+    c.x = T->p[q];
+    c.y = (int)T->mdiv[q];
+    c.z = T->half[q];
+    c.w = T->off[q];
+    s_cst[threadIdx.x] = c;
+  }
+  __syncthreads();  // every wave of the block is still here
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, n = lane & 31, h = lane >> 5;
   const int64_t pos = (int64_t)blockIdx.x * 4 + wave;
   if (pos >= Rb * Kb) return;  // whole waves; only wave-local LDS below
@@ -1532,15 +1568,18 @@ __global__ void __launch_bounds__(256) k_crt_residues(const q128 *__restrict__ X
   const int nq = T->nq;
   const i32x4v bc0 = *reinterpret_cast<const i32x4v *>(&T->cpow[n][16 * h]);
   const i32x4v bc1 = *reinterpret_cast<const i32x4v *>(&T->cpow[32 + n][16 * h]);
-  const int p0 = T->p[n], p1 = T->p[32 + n];
-  const uint32_t m0 = T->mdiv[n], m1 = T->mdiv[32 + n];
-  const int h0 = T->half[n], h1 = T->half[32 + n];
-  const uint32_t off0 = (uint32_t)T->off[n], off1 = (uint32_t)T->off[32 + n];
+  const int p0 = T->p[n];
+  const uint32_t m0 = T->mdiv[n];
+  const int h0 = T->half[n];
+  const uint32_t off0 = (uint32_t)T->off[n];
+  // quad byte transpose: lane j of a quad takes byte j of each quad lane's word
+  const uint32_t qj = (uint32_t)(lane & 3);
+  const uint32_t sel_lo = 0x0c0c0000u | ((4u + qj) << 8) | qj, sel_hi = ((4u + qj) << 24) | (qj << 16) | 0x0c0cu;
+  const int64_t st_off = pos * OZ_FRAG + 4 * (n >> 3);
   i32x16v zero;
 #pragma unroll
   for (int v = 0; v < 16; ++v) zero[v] = 0;
   int8_t *o0 = out + (int64_t)n * plane + pos * OZ_FRAG;
-  int8_t *o1 = out + (int64_t)(32 + n) * plane + pos * OZ_FRAG;
   const int hh = (n >> 2) & 1, ib = 4 * (n >> 3) + (n & 3);
 #pragma nounroll
   for (int half = 0; half < 2; ++half) {
@@ -1589,9 +1628,36 @@ __global__ void __launch_bounds__(256) k_crt_residues(const q128 *__restrict__ X
         a.w = 0;
       }
       const i32x16v d0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bc0, zero, 0, 0, 0);
-      const i32x16v d1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bc1, zero, 0, 0, 0);
       if (n < nq) *reinterpret_cast<i32x4v *>(o0 + (2 * g + h) * 16) = crt_pack16(d0, p0, m0, h0, off0);
-      if (32 + n < nq) *reinterpret_cast<i32x4v *>(o1 + (2 * g + h) * 16) = crt_pack16(d1, p1, m1, h1, off1);
+      if (nq > 32) {  // uniform
+        const i32x16v d1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(bc1, a, zero, 0, 0, 0);
+        // this lane's element: slot 2g + ((n >> 2) & 1), byte 4 (n >> 3) + (n & 3)
+        int8_t *ob = out + st_off + (2 * g + ((n >> 2) & 1)) * 16;
+#pragma unroll
+        for (int vg = 0; vg < 2; ++vg) {
+          if (32 + 8 * vg >= nq) break;  // uniform
+          uint32_t r[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const i32x4v c = s_cst[8 * vg + 4 * h + j];
+            const uint32_t y = (uint32_t)d1[4 * vg + j] + (uint32_t)c.w;
+            const uint32_t q = (uint32_t)(((uint64_t)((y << 4) & 0xffffffu) * ((uint32_t)c.y & 0xffffffu)) >> 32);
+            uint32_t qp;
+            asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(qp) : "v"(q), "v"((uint32_t)c.x), "v"((uint32_t)c.z));
+            r[j] = y - qp;
+          }
+          const uint32_t t01 = __builtin_amdgcn_perm(r[1], r[0], 0x0c0c0400u);
+          const uint32_t t23 = __builtin_amdgcn_perm(r[3], r[2], 0x0c0c0400u);
+          const uint32_t R = __builtin_amdgcn_perm(t23, t01, 0x05040100u);  // moduli +0..3 of this element
+          const uint32_t x0 = (uint32_t)__builtin_amdgcn_mov_dpp((int)R, 0x00, 0xf, 0xf, false);
+          const uint32_t x1 = (uint32_t)__builtin_amdgcn_mov_dpp((int)R, 0x55, 0xf, 0xf, false);
+          const uint32_t x2 = (uint32_t)__builtin_amdgcn_mov_dpp((int)R, 0xaa, 0xf, 0xf, false);
+          const uint32_t x3 = (uint32_t)__builtin_amdgcn_mov_dpp((int)R, 0xff, 0xf, 0xf, false);
+          const uint32_t word = __builtin_amdgcn_perm(x1, x0, sel_lo) | __builtin_amdgcn_perm(x3, x2, sel_hi);
+          const int qi = 32 + 8 * vg + 4 * h + (int)qj;
+          if (qi < nq) *reinterpret_cast<uint32_t *>(ob + (int64_t)qi * plane) = word;
+        }
+      }
     }
     // the next half overwrites the region: every lane's reads of this half are done
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
