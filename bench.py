@@ -634,15 +634,17 @@ def main():
             if key in forms:
                 forms[key]["speedup_vs_plain"] = round(forms["plain"]["ms"] / forms[key]["ms"], 1)
     torch.cuda.empty_cache()
+    # the host-buffer calls run before the legs that allocate tens of GB: on a box where 74 GB had
+    # been allocated and freed, the copy of C in no longer overlapped the qgemm's kernels
+    host_abi = None
+    if not args.no_host:
+        host_abi = bench_host_abi(Q, torch, dev, N, args.m, args.n_dot, 3)
     ill = None
     if not args.no_ill:
         ill = bench_ill_conditioned(Q, torch, dev, stream, 8192, 8192, 16384, 26, plain=not args.no_ill_plain)
     batched = None
     if not args.no_batched:
         batched = bench_batched(Q, torch, dev, stream, max(args.steps, 3), args.batch_gemm, args.batch_gemv)
-    host_abi = None
-    if not args.no_host:
-        host_abi = bench_host_abi(Q, torch, dev, N, args.m, args.n_dot, 3)
     cpu = cpu_mem = None
     if not args.no_cpu:
         cpu = cpu_baseline_gemm(N, args.cpu_rows)
