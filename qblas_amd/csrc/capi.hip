@@ -558,6 +558,7 @@ int dev_gemv(int mode, int layout, int64_t m, int64_t n, q128 alpha, const void 
 thread_local Workspace t_pws;
 // The probe's workspace stays acquired (its scaling exponents serve the call); the caller releases
 // it after enqueuing the call.
+thread_local int *t_probe_out = nullptr;  // pinned: the probe kernel's results
 int oz_probe(int colmajor, int64_t m, int64_t n, int64_t k, const void *A, int64_t lda, const void *B, int64_t ldb,
              hipStream_t s, int *out, void **pws) {
   void *ws;
@@ -565,11 +566,13 @@ int oz_probe(int colmajor, int64_t m, int64_t n, int64_t k, const void *A, int64
   if (rc) return rc;
   *pws = ws;
   constexpr int P = 256;
-  QB_CHECK(qbk::launch_ozaki_probe(colmajor, m, n, k, (const q128 *)A, lda, (const q128 *)B, ldb, P, ws, s));
-  int r[P];
-  QB_CHECK(hipMemcpyAsync(r, (const char *)ws + qbk::ozaki_probe_out_offset(m, n), sizeof(r), hipMemcpyDeviceToHost,
-                          s));
+  // the probe writes its P results straight into pinned host memory (no copy after the kernel)
+  if (!t_probe_out) QB_CHECK(hipHostMalloc((void **)&t_probe_out, P * sizeof(int), hipHostMallocDefault));
+  QB_CHECK(qbk::launch_ozaki_probe(colmajor, m, n, k, (const q128 *)A, lda, (const q128 *)B, ldb, P, ws, s,
+                                   t_probe_out));
   QB_CHECK(hipStreamSynchronize(s));
+  int r[P];
+  memcpy(r, t_probe_out, sizeof(r));
   std::sort(r, r + P);
   *out = r[2];
   return QBLAS_OK;

@@ -97,7 +97,8 @@ int ozaki_crt_bits_for(int64_t k, int r);
 size_t ozaki_probe_ws(int64_t m, int64_t n);
 size_t ozaki_probe_out_offset(int64_t m, int64_t n);
 hipError_t launch_ozaki_probe(int colmajor, int64_t m, int64_t n, int64_t k, const q128 *A, int64_t lda,
-                              const q128 *B, int64_t ldb, int samples, void *ws, hipStream_t st);
+                              const q128 *B, int64_t ldb, int samples, void *ws, hipStream_t st,
+                              int *out = nullptr /* null: ws + ozaki_probe_out_offset; else e.g. pinned host memory */);
 int ozaki_slices_for(int64_t k, int r);
 
 // batched problems (kernels_batched.hip); ws holds count + 1 prefix sums

@@ -916,7 +916,7 @@ size_t ozaki_probe_out_offset(int64_t m, int64_t n) {
 }
 
 hipError_t launch_ozaki_probe(int colmajor, int64_t m, int64_t n, int64_t k, const q128 *A, int64_t lda,
-                              const q128 *B, int64_t ldb, int samples, void *wsv, hipStream_t st) {
+                              const q128 *B, int64_t ldb, int samples, void *wsv, hipStream_t st, int *out) {
   char *ws = (char *)wsv;
   auto al = [](size_t v) { return (v + 255) / 256 * 256; };
   uint32_t *hdr = (uint32_t *)ws;
@@ -933,7 +933,7 @@ hipError_t launch_ozaki_probe(int colmajor, int64_t m, int64_t n, int64_t k, con
   oz_scale(B, n, k, bsc, bsr, eB, fB, st);
   if (samples > OZ_PROBE_MAX) samples = OZ_PROBE_MAX;
   k_oz_probe<<<(unsigned)(samples < 256 ? samples : 256), OZR_BS, 0, st>>>(
-      m, n, k, A, asr, asc, B, bsr, bsc, eA, eB, samples, (int *)(ws + ozaki_probe_out_offset(m, n)));
+      m, n, k, A, asr, asc, B, bsr, bsc, eA, eB, samples, out ? out : (int *)(ws + ozaki_probe_out_offset(m, n)));
   return hipGetLastError();
 }
 
