@@ -559,6 +559,29 @@ thread_local Workspace t_pws;
 // The probe's workspace stays acquired (its scaling exponents serve the call); the caller releases
 // it after enqueuing the call.
 thread_local int *t_probe_out = nullptr;  // pinned: the probe kernel's results
+
+// The CRT method's tables depend on the modulus count only: built once per (device, count) by this
+// thread, waited for once, then read by every later call (read-only, so any stream may use them).
+thread_local std::map<std::pair<int, int>, void *> t_crt_tabs;
+int crt_tables(int nq, hipStream_t s, const void **out) {
+  int dev = 0;
+  QB_CHECK(hipGetDevice(&dev));
+  auto it = t_crt_tabs.find({dev, nq});
+  if (it == t_crt_tabs.end()) {
+    void *p = nullptr;
+    QB_CHECK(hipMalloc(&p, qbk::crt_tab_bytes()));
+    hipError_t e = qbk::launch_crt_tables(nq, p, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      (void)hipFree(p);
+      set_err("crt tables", e);
+      return QBLAS_ERR_HIP;
+    }
+    it = t_crt_tabs.emplace(std::make_pair(dev, nq), p).first;
+  }
+  *out = it->second;
+  return QBLAS_OK;
+}
 int oz_probe(int colmajor, int64_t m, int64_t n, int64_t k, const void *A, int64_t lda, const void *B, int64_t ldb,
              hipStream_t s, int *out, void **pws) {
   void *ws;
@@ -670,9 +693,11 @@ int dev_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, q128 alpha, 
       ev = t_oz_ev;
     }
     t_oz_ev_valid = ev != nullptr;
+    const void *tab = nullptr;
+    if (crt && (rc = crt_tables(t_oz_moduli, s, &tab))) return rc;
     if (crt)
       QB_CHECK(qbk::launch_gemm_crt(colmajor, m, n, k, ch.t, alpha, (const q128 *)A, lda, (const q128 *)B, ldb, beta,
-                                    (q128 *)C, ldc, ws, s, ev, ch.probe_ws, gate));
+                                    (q128 *)C, ldc, ws, s, ev, ch.probe_ws, gate, tab));
     else
       QB_CHECK(qbk::launch_gemm_ozaki(colmajor, m, n, k, ch.S, alpha, (const q128 *)A, lda, (const q128 *)B, ldb,
                                       beta, (q128 *)C, ldc, ws, s, ev, ch.probe_ws, gate));

@@ -91,7 +91,13 @@ size_t gemm_crt_ws(int64_t m, int64_t n, int64_t k, int t);
 int crt_moduli_for(int64_t k, int t);
 hipError_t launch_gemm_crt(int colmajor, int64_t m, int64_t n, int64_t k, int t, q128 alpha, const q128 *A,
                            int64_t lda, const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc, void *ws,
-                           hipStream_t st, hipEvent_t *ev, const void *probe_ws, const c_gate *gate = nullptr);
+                           hipStream_t st, hipEvent_t *ev, const void *probe_ws, const c_gate *gate = nullptr,
+                           const void *tab = nullptr /* null, or the tables of crt_moduli_for(k, t) moduli built by
+                           launch_crt_tables and complete before the call's work */);
+// the CRT method's per-modulus constants for nq moduli (crt_tab_bytes() bytes at out); they depend
+// on nq only
+size_t crt_tab_bytes();
+hipError_t launch_crt_tables(int nq, void *out, hipStream_t st);
 // bits t of the CRT method for a probe value (as ozaki_slices_for); 0: use the slice method
 int ozaki_crt_bits_for(int64_t k, int r);
 size_t ozaki_probe_ws(int64_t m, int64_t n);

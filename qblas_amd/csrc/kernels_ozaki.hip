@@ -2124,9 +2124,17 @@ int ozaki_crt_bits_for(int64_t k, int r) {
   return t;
 }
 
+size_t crt_tab_bytes() { return CRT_TAB_BYTES; }
+hipError_t launch_crt_tables(int nq, void *out, hipStream_t st) {
+  if (nq < 1 || nq > CRT_QALL) return hipErrorInvalidValue;
+  k_crt_tables<<<1, 64, 0, st>>>(nq, (crt_tab *)out);
+  return hipGetLastError();
+}
+
 hipError_t launch_gemm_crt(int colmajor, int64_t m, int64_t n, int64_t k, int t, q128 alpha, const q128 *A,
                            int64_t lda, const q128 *B, int64_t ldb, q128 beta, q128 *C, int64_t ldc, void *wsv,
-                           hipStream_t st, hipEvent_t *ev, const void *probe_ws, const c_gate *gate) {
+                           hipStream_t st, hipEvent_t *ev, const void *probe_ws, const c_gate *gate,
+                           const void *tab_in) {
   if (t < CRT_TMIN || t > CRT_TMAX) return hipErrorInvalidValue;
   const int nq = crt_moduli_for(k, t);
   if (!nq) return hipErrorInvalidValue;
@@ -2138,7 +2146,7 @@ hipError_t launch_gemm_crt(int colmajor, int64_t m, int64_t n, int64_t k, int t,
   int32_t *eA = (int32_t *)(ws + L.eA), *eB = (int32_t *)(ws + L.eB);
   uint32_t *fA = (uint32_t *)(ws + L.fA), *fB = (uint32_t *)(ws + L.fB);
   uint32_t *hdr = (uint32_t *)ws;
-  crt_tab *tab = (crt_tab *)(ws + L.tab);
+  crt_tab *tab = tab_in ? (crt_tab *)tab_in : (crt_tab *)(ws + L.tab);
   int8_t *RA = (int8_t *)(ws + L.rA), *RB = (int8_t *)(ws + L.rB), *RC = (int8_t *)(ws + L.rC);
   const int64_t wgs = (int64_t)nq * (L.Mb / CRT_TA) * (L.Nb / CRT_TB);
   if (!oz_grids_fit(L.Mp, L.Np, L.Kb) || !grid_fits((L.Mb * L.Kb + 3) / 4, 256) ||
@@ -2146,7 +2154,7 @@ hipError_t launch_gemm_crt(int colmajor, int64_t m, int64_t n, int64_t k, int t,
     return hipErrorInvalidValue;
   if (ev) (void)hipEventRecord(ev[0], st);
   oz_prepare(L.Mp, L.Np, m, n, k, A, asr, asc, B, bsr, bsc, probe_ws, &eA, &fA, &eB, &fB, hdr, st);
-  k_crt_tables<<<1, 64, 0, st>>>(nq, tab);
+  if (!tab_in) k_crt_tables<<<1, 64, 0, st>>>(nq, tab);
   k_crt_residues<<<(unsigned)((L.Mb * L.Kb + 3) / 4), 256, 0, st>>>(A, m, k, asr, asc, eA, t, tab, L.Mb, L.Kb, L.pA, RA);
   k_crt_residues<<<(unsigned)((L.Nb * L.Kb + 3) / 4), 256, 0, st>>>(B, n, k, bsc, bsr, eB, t, tab, L.Nb, L.Kb, L.pB, RB);
   if (ev) (void)hipEventRecord(ev[1], st);
