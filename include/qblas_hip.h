@@ -4,7 +4,11 @@
  *
  * Part 1 keeps the reference's C entry points byte-for-byte (same names, argument order and
  * meaning, same return types); they take HOST pointers to Sleef_quad / __float128 arrays, copy
- * them to the GPU, run the kernels and copy results back before returning.
+ * them to the GPU, run the kernels and copy results back before returning.  A pinned buffer
+ * (hipHostMalloc) is copied by one DMA; a pageable one of 2 MiB or more is staged by up to
+ * QBLAS_STAGE_WORKERS (default 8) library threads through pinned slots of QBLAS_STAGE_CHUNK_MB
+ * (default 8) MiB, two per thread, allocated at a calling thread's first staged copy and kept.
+ * quadblas_qgemm copies C in on a second stream while the kernels that do not read C run.
  * Part 2 carries the C++ template API's full-precision entry points (binary128 alpha/beta and a
  * binary128 dot result), which include/quadblas/quadblas.hpp wraps as QuadBLAS::dot/gemv/gemm
  * and Vector/Matrix.
