@@ -305,16 +305,31 @@ int stage_workers() {
   return n;
 }
 
-int stage_ring() {
-  if (t_ring.ready) return QBLAS_OK;
+// false when the slots cannot be had (pinned memory exhausted): the caller then copies directly
+bool stage_ring() {
+  if (t_ring.ready) return true;
   for (int w = 0; w < stage_workers(); ++w)
     for (int k = 0; k < STAGE_SLOTS; ++k) {
-      if (!t_ring.slot[w][k]) QB_CHECK(hipHostMalloc(&t_ring.slot[w][k], stage_chunk(), hipHostMallocDefault));
-      // default flags: the event's completion also makes a device-to-host copy visible to the host
-      if (!t_ring.ev[w][k]) QB_CHECK(hipEventCreateWithFlags(&t_ring.ev[w][k], hipEventDisableTiming));
+      // default event flags: completion also makes a device-to-host copy visible to the host
+      if (!t_ring.slot[w][k]) {
+        void *p = nullptr;
+        if (hipHostMalloc(&p, stage_chunk(), hipHostMallocDefault) != hipSuccess) {
+          (void)hipGetLastError();  // not an error of a later launch
+          return false;
+        }
+        t_ring.slot[w][k] = p;
+      }
+      if (!t_ring.ev[w][k]) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+          (void)hipGetLastError();
+          return false;
+        }
+        t_ring.ev[w][k] = e;
+      }
     }
   t_ring.ready = true;
-  return QBLAS_OK;
+  return true;
 }
 
 bool host_pinned(const void *p) {
@@ -387,25 +402,31 @@ hipError_t stage_chunks(StageRing &R, int w, int nw, int dev, char *d, char *h, 
 // returns once the data is in the host buffer.
 int copy_host(void *dev_p, void *host_p, size_t bytes, bool h2d, hipStream_t s) {
   if (!bytes) return QBLAS_OK;
-  if (bytes < STAGE_DIRECT || host_pinned(host_p)) {
+  if (bytes < STAGE_DIRECT || host_pinned(host_p) || !stage_ring()) {
     QB_CHECK(h2d ? hipMemcpyAsync(dev_p, host_p, bytes, hipMemcpyHostToDevice, s)
                  : hipMemcpyAsync(host_p, dev_p, bytes, hipMemcpyDeviceToHost, s));
     if (!h2d) QB_CHECK(hipStreamSynchronize(s));
     return QBLAS_OK;
   }
-  int rc = stage_ring();
-  if (rc) return rc;
   StageRing &R = t_ring;
   int dev = 0;
   QB_CHECK(hipGetDevice(&dev));
   const size_t nch = (bytes + stage_chunk() - 1) / stage_chunk();
   const int nw = (int)std::min<size_t>(nch, (size_t)stage_workers());
   hipError_t err[STAGE_WORKERS];
-  std::vector<std::thread> th;
+  std::thread th[STAGE_WORKERS];
+  bool spawned[STAGE_WORKERS] = {};
+  for (int w = 1; w < nw; ++w) {
+    try {
+      th[w] = std::thread([&, w] { err[w] = stage_chunks(R, w, nw, dev, (char *)dev_p, (char *)host_p, bytes, h2d, s); });
+      spawned[w] = true;
+    } catch (...) {  // no thread to be had: this share runs on the calling thread below
+    }
+  }
+  for (int w = 0; w < nw; ++w)
+    if (!spawned[w]) err[w] = stage_chunks(R, w, nw, dev, (char *)dev_p, (char *)host_p, bytes, h2d, s);
   for (int w = 1; w < nw; ++w)
-    th.emplace_back([&, w] { err[w] = stage_chunks(R, w, nw, dev, (char *)dev_p, (char *)host_p, bytes, h2d, s); });
-  err[0] = stage_chunks(R, 0, nw, dev, (char *)dev_p, (char *)host_p, bytes, h2d, s);
-  for (auto &t : th) t.join();
+    if (spawned[w]) th[w].join();
   for (int w = 0; w < nw; ++w)
     if (err[w] != hipSuccess) {
       set_err(h2d ? "host to device copy" : "device to host copy", err[w]);
@@ -417,12 +438,12 @@ int copy_host(void *dev_p, void *host_p, size_t bytes, bool h2d, hipStream_t s) 
 // A host-to-device copy staged by background threads while the calling thread goes on enqueueing
 // work; once join() returns, every DMA of the copy is enqueued on its stream.
 struct AsyncH2D {
-  std::vector<std::thread> th;
+  std::thread th[STAGE_WORKERS];
   hipError_t err[STAGE_WORKERS] = {};
   int nw = 0;
   hipError_t join() {
-    for (auto &t : th) t.join();
-    th.clear();
+    for (auto &t : th)
+      if (t.joinable()) t.join();
     for (int w = 0; w < nw; ++w)
       if (err[w] != hipSuccess) return err[w];
     return hipSuccess;
@@ -435,12 +456,10 @@ struct AsyncH2D {
 
 int copy_h2d_async(AsyncH2D &a, void *dev_p, const void *host_p, size_t bytes, hipStream_t s) {
   if (!bytes) return QBLAS_OK;
-  if (bytes < STAGE_DIRECT || host_pinned(host_p)) {
+  if (bytes < STAGE_DIRECT || host_pinned(host_p) || !stage_ring()) {
     QB_CHECK(hipMemcpyAsync(dev_p, host_p, bytes, hipMemcpyHostToDevice, s));
     return QBLAS_OK;
   }
-  int rc = stage_ring();
-  if (rc) return rc;
   StageRing *R = &t_ring;
   int dev = 0;
   QB_CHECK(hipGetDevice(&dev));
@@ -449,9 +468,13 @@ int copy_h2d_async(AsyncH2D &a, void *dev_p, const void *host_p, size_t bytes, h
   a.nw = nw;
   for (int w = 0; w < nw; ++w) {
     a.err[w] = hipSuccess;
-    a.th.emplace_back([&a, R, w, nw, dev, dev_p, host_p, bytes, s] {
+    try {
+      a.th[w] = std::thread([&a, R, w, nw, dev, dev_p, host_p, bytes, s] {
+        a.err[w] = stage_chunks(*R, w, nw, dev, (char *)dev_p, (char *)host_p, bytes, true, s);
+      });
+    } catch (...) {  // no thread to be had: this share runs now, on the calling thread
       a.err[w] = stage_chunks(*R, w, nw, dev, (char *)dev_p, (char *)host_p, bytes, true, s);
-    });
+    }
   }
   return QBLAS_OK;
 }
