@@ -123,3 +123,53 @@ def test_host_axpy_staged(O, Q, cuda, pinned):
                                 ctypes.c_void_p(hy.ctypes.data), incy)
         assert Q.last_error() in ("", None)
         assert np.array_equal(hy, want)
+
+
+def test_host_calls_from_two_threads(O, Q, cuda):
+    # each host thread has its own stream, staging slots and workspaces: staged gemv (ColMajor,
+    # 22 MiB of A, y staged both ways) and dot calls from two threads at once give the same bits
+    # as the device-pointer routines
+    import threading
+    m, n, incy = 200003, 7, 2
+    A = O.fill(0, 71, m * n)
+    x = O.fill(0, 72, n)
+    y0 = O.fill(0, 73, (m - 1) * incy + 1)
+    dA, dx, dy = to_dev(A), to_dev(x), to_dev(y0)
+    Q.dev_gemv(Q.MODE_DD, 1, m, n, 1.5, dA.data_ptr(), m, dx.data_ptr(), 1, -0.5, dy.data_ptr(), incy)
+    cuda.cuda.synchronize()
+    want_y = from_dev(dy)
+    nd = 300001
+    u, v = O.fill(0, 74, nd), O.fill(0, 75, nd)
+    du, dv, dr = to_dev(u), to_dev(v), dev_empty(1)
+    Q.dev_dot(Q.MODE_DD, nd, du.data_ptr(), 1, dv.data_ptr(), 1, 1, dr.data_ptr())
+    cuda.cuda.synchronize()
+    want_d = pair(from_dev(dr)[0])
+    _say("host gemv and dot from two threads")
+    errors = []
+
+    def gemv_worker():
+        try:
+            for _ in range(4):
+                hy = y0.copy()
+                Q.gemv(1, m, n, 1.5, A, m, x, 1, -0.5, hy, incy)
+                if not np.array_equal(hy, want_y):
+                    errors.append("gemv")
+        except Exception as e:  # noqa: BLE001 - reported below
+            errors.append(repr(e))
+
+    def dot_worker():
+        try:
+            for _ in range(8):
+                if pair(Q.dot(nd, u, 1, v, 1)) != want_d:
+                    errors.append("dot")
+        except Exception as e:  # noqa: BLE001
+            errors.append(repr(e))
+
+    with _mode(Q, "dd"):
+        ts = [threading.Thread(target=gemv_worker), threading.Thread(target=dot_worker)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join(timeout=100)
+        assert not any(t.is_alive() for t in ts)
+    assert errors == []
