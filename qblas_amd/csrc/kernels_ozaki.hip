@@ -803,12 +803,13 @@ static bool oz_grids_fit(int64_t Mp, int64_t Np, int64_t Kb) {
 static void oz_scale(const q128 *X, int64_t R, int64_t Cn, int64_t sr, int64_t sc, int32_t *E, uint32_t *F,
                      hipStream_t st) {
   const int contig = (sc == 1);
-  // enough blocks to fill the chip: split the reduced dimension into chunks
+  // enough blocks to fill the chip: split the reduced dimension into chunks (about 8192 blocks;
+  // 2048 blocks of at least 64 columns took 9 us more for the two operands at 4096^2)
   const int64_t rblocks = contig ? (R + 3) / 4 : (R + 255) / 256;
-  int64_t nch = (2048 + rblocks - 1) / rblocks;
+  int64_t nch = (8192 + rblocks - 1) / rblocks;
   if (nch < 1) nch = 1;
   int64_t cchunk = (Cn + nch - 1) / nch;
-  if (cchunk < 64) cchunk = 64;
+  if (cchunk < 16) cchunk = 16;
   nch = (Cn + cchunk - 1) / cchunk;
   dim3 grid((unsigned)rblocks, (unsigned)nch);
   k_oz_scale<<<grid, 256, 0, st>>>(X, R, Cn, sr, sc, contig, cchunk, E, F);
