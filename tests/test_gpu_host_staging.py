@@ -1,11 +1,11 @@
 """Host-pointer entry points over the pipelined staging of caller buffers.
 
-A pageable host span of 2 MiB or more is copied in 4 MiB chunks by up to 8 threads through pinned
+A pageable host span of 2 MiB or more is copied in 8 MiB chunks by up to 8 threads through pinned
 staging slots; a pinned one (hipHostMalloc, torch's pin_memory) by one DMA.  Each case runs the
 host-pointer entry point and the device-pointer one (or the oracle) on the same data and requires
 the same bits, including the entries between the rows of C (ldc > n) and between strided elements
 of y, which the host call writes back unchanged:
-  - spans of 2.3 to 16 MiB, so the chunks split unevenly over the workers and the last is short;
+  - spans of 2.3 to 22 MiB: one to three chunks over as many workers, the last chunk short;
   - outputs (C, y) large enough to be staged on the way back too;
   - pageable and pinned caller buffers;
   - a second call right after the first, reusing the staging slots.
