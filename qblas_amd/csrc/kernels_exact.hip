@@ -323,7 +323,29 @@ __global__ void __launch_bounds__(64) k_exact_finish_rows(int64_t m, int nch, co
     }
     uint32_t fl = 0;
     const int A0 = parts[2 * i].A;
-    for (int c = 0; c < nch; ++c) {
+    int c = 0;
+    // four chunks' records loaded before they are added: one row per thread leaves few waves, so
+    // the loads must overlap (window additions are exact, so their order does not matter)
+    for (; c + 3 < nch; c += 4) {
+      uint32_t X[4][WW], Y[4][WW];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const part_t *p = parts + 2 * ((int64_t)(c + u) * m + i);
+        fl |= p[0].flags | p[1].flags;
+        if (p[0].A != A0) fl |= FL_LOST;
+#pragma unroll
+        for (int k = 0; k < WW; ++k) {
+          X[u][k] = p[0].W[k];
+          Y[u][k] = p[1].W[k];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        win_add_win(lo, X[u]);
+        win_add_win(hi, Y[u]);
+      }
+    }
+    for (; c < nch; ++c) {
       const part_t *p = parts + 2 * ((int64_t)c * m + i);
       fl |= p[0].flags | p[1].flags;
       if (p[0].A != A0) fl |= FL_LOST;  // chunks of a row share its anchor by construction
