@@ -17,7 +17,9 @@
  *
  * Binary128 values cross the ABI as 16-byte little-endian images of __float128 ({lo, hi} 64-bit
  * words), 16-byte aligned.  No function throws; failures are reported through
- * quadblas_hip_last_error() (thread-local) and leave outputs unchanged.
+ * quadblas_hip_last_error() (thread-local) and leave outputs unchanged, except that the
+ * functions returning a value return NaN (quadblas_qdot, quadblas_qnrm2; quadblas_dot_q and
+ * quadblas_nrm2_q write a binary128 NaN).
  */
 #ifndef QBLAS_HIP_H
 #define QBLAS_HIP_H
@@ -115,8 +117,10 @@ enum {
   QBLAS_ERR_ARG = 1,      /* invalid argument (negative size/stride, misaligned pointer, ...) */
   QBLAS_ERR_HIP = 2,      /* a HIP runtime call failed */
   QBLAS_ERR_MODE = 3,     /* mode not available for this routine */
-  QBLAS_ERR_RANGE = 4     /* EXACT form (and the exact repair of the OZAKI form): product exponents
-                             too far apart for the accumulation window */
+  QBLAS_ERR_RANGE = 4     /* EXACT form and the OZAKI form's exact repair: a term fell outside the
+                             full-range accumulator.  Cannot happen for binary128 inputs: products
+                             too far apart for the fast accumulation windows are recomputed on a
+                             full-range path (qblas_hip_exact_wide_items counts them) */
 };
 
 /* Mode used by Parts 1 and 2 (default QBLAS_MODE_REFERENCE; environment QBLAS_MODE overrides at
@@ -174,10 +178,18 @@ int qblas_hip_phase_times(float *ms);
 const char *quadblas_hip_last_error(void);
 void qblas_clear_error(void);
 
-/* After a device-API call in QBLAS_MODE_EXACT on this thread: synchronizes the stream and returns
-   1 if a product fell outside the accumulation window (the output was set to NaN), 0 if the
-   result is exact, -1 on a HIP error.  Host-pointer entry points check this themselves. */
+/* After a device-API call in QBLAS_MODE_EXACT or QBLAS_MODE_OZAKI on this thread (the last such
+   call; any later host-pointer call replaces it): synchronizes the stream and returns 1 if a term
+   fell outside the full-range accumulator (the output was set to NaN; cannot happen for binary128
+   inputs), 0 if the result is exact or certified, -1 on a HIP error or when a later call on the
+   same stream has reused the call's workspace.  Host-pointer entry points check this themselves. */
 int qblas_hip_exact_status(void *stream);
+
+/* After the same kind of call: synchronizes the stream and returns how many items (dots, gemv
+   rows, GEMM entries of the exact repair) were recomputed on the full-range path because their
+   products lay too far apart for the accumulation windows (about 2^+-124 around an anchor); -1 if
+   unknown (as for qblas_hip_exact_status).  A dot counts as one item. */
+long long qblas_hip_exact_wide_items(void *stream);
 
 /* dot: *d_result (device, 16 bytes) = x . y.  T: the reference thread count for
    QBLAS_MODE_REFERENCE (ignored otherwise).  stream: hipStream_t or NULL. */
@@ -206,8 +218,13 @@ int qblas_hip_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, const 
    level3.hpp:215-336 / level2.hpp:15-99 per problem, bit-identical to a loop of single calls).
    All problems share the layout.  The work is one grouped persistent kernel over the tiles (gemm)
    or rows (gemv) of all problems; the launch sequence allocates nothing when the workspace of an
-   earlier call of the same count suffices, synchronizes nothing and may be captured in a hipGraph
-   (all modes but gemm's QBLAS_MODE_OZAKI, below).
+   earlier call of the same count on the same stream suffices, synchronizes nothing and may be
+   captured in a hipGraph (all modes but gemm's QBLAS_MODE_OZAKI, below).  Workspaces are kept per
+   (device, stream) of the calling thread; one used while its stream was capturing is never freed
+   (so the graph stays valid).  A capture cannot allocate: the capturing stream uses its own
+   buffer if large enough, else takes over another stream's (the least recently used large enough
+   one), else the call fails with QBLAS_ERR_ARG -- make the same call once, on any stream of the
+   thread, before capturing.
    alpha/beta are binary128 {lo, hi} words. */
 typedef struct qblas_gemm_problem {
   int64_t m, n, k;
@@ -231,8 +248,8 @@ typedef struct qblas_gemv_problem {
 } qblas_gemv_problem;
 
 /* d_problems: device array of count descriptors; gemm: modes REFERENCE, DD and OZAKI; gemv:
-   REFERENCE, DD and EXACT (qblas_hip_exact_status then reports products outside the accumulation
-   window).
+   REFERENCE, DD and EXACT (rows whose products lie too far apart for the accumulation windows are
+   found again by a second walk over the batch and computed on the full-range path).
    gemm in QBLAS_MODE_OZAKI: the slice method with one slice count S for the whole batch (the tuned
    or selected configuration; configuration 0 is S = 17), every entry within 2^-104 relative error
    or recomputed exactly, per problem bit-identical to qblas_hip_gemm in QBLAS_MODE_OZAKI with

@@ -2,6 +2,8 @@
 //
 // Same names, signatures and semantics as the reference's C++ API for dot / gemv / gemm:
 //   enum class Layout                          include/quadblas/core/types.hpp:8-12
+//   configuration constants                    include/quadblas/core/constants.hpp:10-18
+//   VERSION, VERSION_MAJOR/MINOR/PATCH         include/quadblas/quadblas.hpp:54-59
 //   QuadBLAS::dot                              include/quadblas/algorithms/level1.hpp:80-137
 //   QuadBLAS::axpy                             include/quadblas/algorithms/level1.hpp:190-223
 //   QuadBLAS::gemv                             include/quadblas/algorithms/level2.hpp:85-99
@@ -33,7 +35,15 @@ typedef struct {
   uint64_t x, y;
 } Sleef_uint64_2t;
 typedef __float128 Sleef_quad;
+#endif
+#ifndef SLEEF_QUAD_C
+#if defined(__STRICT_ANSI__)
+// strict ISO mode has no Q literal suffix: the literal goes through double, as the reference's
+// own fallback (core/platform.hpp:21-23, Sleef_cast_from_doubleq1)
+#define SLEEF_QUAD_C(x) ((Sleef_quad)(x))
+#else
 #define SLEEF_QUAD_C(x) (x##Q)
+#endif
 #endif
 
 static_assert(sizeof(Sleef_quad) == 16, "Sleef_quad must be IEEE binary128");
@@ -41,6 +51,26 @@ static_assert(sizeof(Sleef_quad) == 16, "Sleef_quad must be IEEE binary128");
 namespace QuadBLAS {
 
 enum class Layout { RowMajor, ColMajor };
+
+// ---- configuration constants, with the reference's values ------------------------------------
+// Here they describe the REFERENCE evaluation form, which keeps the reference's order:
+// VECTOR_SIZE lane chains per dot (two), PARALLEL_THRESHOLD the length from which a dot is cut
+// into one chunk per thread, GEMM_BLOCK_SIZE the largest m, n, k of the unblocked GEMM, and
+// L1_CACHE_SIZE the cache size whose blocking gives kc = 126 (qblas_set_kc changes kc).
+// ALIGNMENT is the byte alignment of aligned_alloc and of quadblas_is_aligned.
+constexpr size_t VECTOR_SIZE = 2;
+constexpr size_t CACHE_LINE_SIZE = 64;
+constexpr size_t L1_CACHE_SIZE = 32768;
+constexpr size_t L2_CACHE_SIZE = 262144;
+constexpr size_t PARALLEL_THRESHOLD = 500;
+constexpr size_t GEMM_BLOCK_SIZE = 64;
+constexpr size_t ALIGNMENT = 32;
+
+// ---- version of the API this library implements ------------------------------------------------
+constexpr const char *VERSION = "1.0.0";
+constexpr int VERSION_MAJOR = 1;
+constexpr int VERSION_MINOR = 0;
+constexpr int VERSION_PATCH = 0;
 
 template <Layout layout>
 class Matrix;
@@ -51,12 +81,12 @@ class Vector;
 inline int get_num_threads() { return quadblas_get_num_threads(); }
 inline void set_num_threads(int num_threads) { quadblas_set_num_threads(num_threads); }
 
-// ---- 32-byte aligned host buffers (nullptr on failure) -------------------------------------
+// ---- ALIGNMENT-byte aligned host buffers (nullptr on failure) ---------------------------------
 template <typename T>
 inline T *aligned_alloc(size_t count) {
   void *p = nullptr;
-  const size_t bytes = ((count * sizeof(T) + 31) / 32) * 32;
-  if (posix_memalign(&p, 32, bytes ? bytes : 32) != 0) return nullptr;
+  const size_t bytes = ((count * sizeof(T) + ALIGNMENT - 1) / ALIGNMENT) * ALIGNMENT;
+  if (posix_memalign(&p, ALIGNMENT, bytes ? bytes : ALIGNMENT) != 0) return nullptr;
   return static_cast<T *>(p);
 }
 template <typename T>
@@ -88,11 +118,13 @@ inline void gemm(Layout layout, size_t m, size_t n, size_t k, Sleef_quad alpha, 
 }
 
 // ---- containers -------------------------------------------------------------------------------
-// Owning (zero-filled, 32-byte aligned) or viewing a strided host array; move-only.
+// Owning (zero-filled, 32-byte aligned) or viewing a strided host array; move-only.  The sized
+// constructor is not explicit, as in the reference (cpp_classes.hpp:28): `VectorRowMajor v = n;`
+// compiles.
 template <Layout layout>
 class Vector {
  public:
-  explicit Vector(size_t size) : data_(aligned_alloc<Sleef_quad>(size)), size_(size), stride_(1), owns_(true) {
+  Vector(size_t size) : data_(aligned_alloc<Sleef_quad>(size)), size_(size), stride_(1), owns_(true) {
     if (data_) std::fill(data_, data_ + size, Sleef_quad(0));
   }
   Vector(Sleef_quad *data, size_t size, size_t stride = 1) : data_(data), size_(size), stride_(stride), owns_(false) {}

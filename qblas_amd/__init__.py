@@ -92,6 +92,7 @@ def lib():
         "qblas_hip_fill": (I, [I, ctypes.c_uint64, I64, I, I, P, P]),
         "qblas_hip_fill_2d": (I, [I, ctypes.c_uint64, I64, I64, I64, I, I, I, P, P]),
         "qblas_hip_exact_status": (I, [P]),
+        "qblas_hip_exact_wide_items": (ctypes.c_longlong, [P]),
         "qblas_set_ozaki_slices": (I, [I]),
         "qblas_get_ozaki_slices": (I, []),
         "qblas_hip_ozaki_repairs": (ctypes.c_longlong, [P]),
@@ -314,8 +315,16 @@ def dev_gemm(mode, layout, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, stream=
 
 
 def dev_exact_status(stream=0):
-    """1 if the last EXACT-form device call of this thread lost a product (output NaN), else 0."""
+    """1 if a term of the last EXACT/OZAKI-form device call of this thread fell outside the
+    full-range accumulator (output NaN; cannot happen for binary128 inputs), 0 if the result is
+    exact, -1 on an error (including a workspace reused by a later call on the stream)."""
     return lib().qblas_hip_exact_status(stream)
+
+
+def dev_exact_wide_items(stream=0):
+    """Items (dots, rows, GEMM entries) the last EXACT- or OZAKI-form call on this thread computed
+    on the full-range path (products too far apart for the accumulation windows); -1 if unknown."""
+    return lib().qblas_hip_exact_wide_items(stream)
 
 
 def dev_ozaki_repairs(stream=0):

@@ -164,6 +164,11 @@ int ws_release(Workspace &w, hipStream_t s) {
 // runs in order, so a stream reusing its own buffer needs no event (an event record costs a few
 // microseconds of queue time per call, 2% of a memory-bound gemv).  A buffer grows after its
 // stream has drained; the least recently used entry is dropped after a device synchronization.
+// A buffer handed out while its stream was capturing a graph may be referenced by that graph's
+// replays: it is never freed (growth and eviction retire it instead).  A capture allows neither
+// synchronization nor hipMalloc: a capturing stream without a large enough buffer takes over the
+// least recently used large enough buffer of another stream (which gets a new one on its next
+// call); with none, the call fails.
 constexpr int WS_STREAMS = 4;
 struct StreamWorkspaces {
   struct Entry {
@@ -171,13 +176,38 @@ struct StreamWorkspaces {
     int dev = -1;
     void *p = nullptr;
     size_t cap = 0;
-    uint64_t used = 0;  // 0: free slot
+    uint64_t used = 0;  // 0: free slot; else the stamp of the latest acquisition
+    bool captured = false;
   } e[WS_STREAMS];
   uint64_t clock = 0;
 };
-int ws_acquire(StreamWorkspaces &W, size_t bytes, hipStream_t s, void **out) {
+std::mutex g_retired_mu;
+std::vector<void *> g_retired;  // buffers a captured graph may use: kept until the process ends
+void ws_retire(void *p) {
+  std::lock_guard<std::mutex> lk(g_retired_mu);
+  g_retired.push_back(p);
+}
+bool stream_capturing(hipStream_t s) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return cs == hipStreamCaptureStatusActive;
+}
+// Where a call's status words live: valid while the workspace entry has not been handed out again
+// (its stamp unchanged) -- a later call on that stream reuses, regrows or drops the buffer.
+struct StatusRef {
+  const uint32_t *p = nullptr;
+  const StreamWorkspaces::Entry *entry = nullptr;
+  uint64_t stamp = 0;
+  bool valid() const { return p && entry && entry->used == stamp; }
+};
+int ws_acquire(StreamWorkspaces &W, size_t bytes, hipStream_t s, void **out,
+               const StreamWorkspaces::Entry **used_entry = nullptr) {
   int dev = 0;
   QB_CHECK(hipGetDevice(&dev));
+  const bool capturing = stream_capturing(s);
   StreamWorkspaces::Entry *hit = nullptr, *lru = &W.e[0];
   for (auto &x : W.e) {
     if (x.used && x.s == s && x.dev == dev) {
@@ -186,16 +216,37 @@ int ws_acquire(StreamWorkspaces &W, size_t bytes, hipStream_t s, void **out) {
     }
     if (x.used < lru->used) lru = &x;
   }
+  if (capturing && (!hit || bytes > hit->cap)) {
+    // no allocation is allowed during a capture: the least recently used large enough buffer of
+    // this device moves to the capturing stream (its former stream gets a new one next time)
+    StreamWorkspaces::Entry *donor = nullptr;
+    for (auto &x : W.e)
+      if (&x != hit && x.used && x.dev == dev && x.p && x.cap >= bytes && (!donor || x.used < donor->used)) donor = &x;
+    if (!donor) {
+      set_err_msg("no workspace large enough during graph capture: make the same call once before capturing");
+      return QBLAS_ERR_ARG;
+    }
+    if (hit) {
+      if (hit->p) ws_retire(hit->p);
+      *hit = StreamWorkspaces::Entry();
+    }
+    donor->s = s;
+    hit = donor;
+  }
   if (!hit) {
     hit = lru;
-    if (hit->p) {  // drop the least recently used buffer once every stream of its device is done
-      if (hit->dev != dev) QB_CHECK(hipSetDevice(hit->dev));
-      hipError_t e = hipDeviceSynchronize();
-      if (e == hipSuccess) e = hipFree(hit->p);
-      if (hit->dev != dev) (void)hipSetDevice(dev);
-      if (e != hipSuccess) {
-        set_err("workspace", e);
-        return QBLAS_ERR_HIP;
+    if (hit->p) {
+      if (hit->captured || capturing) {
+        ws_retire(hit->p);  // no synchronization possible (capture) or a graph may still use it
+      } else {  // drop the least recently used buffer once every stream of its device is done
+        if (hit->dev != dev) QB_CHECK(hipSetDevice(hit->dev));
+        hipError_t e = hipDeviceSynchronize();
+        if (e == hipSuccess) e = hipFree(hit->p);
+        if (hit->dev != dev) (void)hipSetDevice(dev);
+        if (e != hipSuccess) {
+          set_err("workspace", e);
+          return QBLAS_ERR_HIP;
+        }
       }
     }
     *hit = StreamWorkspaces::Entry();
@@ -204,24 +255,32 @@ int ws_acquire(StreamWorkspaces &W, size_t bytes, hipStream_t s, void **out) {
   }
   if (bytes > hit->cap) {
     if (hit->p) {
-      QB_CHECK(hipStreamSynchronize(s));  // the stream's earlier calls may still read the buffer
-      QB_CHECK(hipFree(hit->p));
+      if (hit->captured || capturing) {  // a graph may use it, or no synchronization is allowed
+        ws_retire(hit->p);
+      } else {
+        QB_CHECK(hipStreamSynchronize(s));  // the stream's earlier calls may still read the buffer
+        QB_CHECK(hipFree(hit->p));
+      }
       hit->p = nullptr;
       hit->cap = 0;
+      hit->captured = false;
     }
     const size_t want = bytes < 4096 ? 4096 : bytes;
     QB_CHECK(hipMalloc(&hit->p, want));
     hit->cap = want;
   }
+  if (capturing) hit->captured = true;
   hit->used = ++W.clock;
   *out = hit->p;
+  if (used_entry) *used_entry = hit;
   return QBLAS_OK;
 }
 int ws_release(StreamWorkspaces &, hipStream_t) { return QBLAS_OK; }
 
 thread_local StreamWorkspaces t_ws;
-thread_local const uint32_t *t_exact_status = nullptr;  // status word of the last EXACT-form call
-thread_local const uint32_t *t_oz_hdr = nullptr;        // header of the last OZAKI-form call
+thread_local StreamWorkspaces t_bws;                    // batched calls: prefix sums, OZAKI batch arrays
+thread_local StatusRef t_exact_status;                  // status words of the last EXACT/OZAKI-form call
+thread_local StatusRef t_oz_hdr;                        // header of the last OZAKI-form call
 std::atomic<int> g_phase_timing{0};                     // record phase events in OZAKI calls
 thread_local hipEvent_t t_oz_ev[5] = {};
 thread_local bool t_oz_ev_valid = false;
@@ -230,18 +289,44 @@ thread_local int t_oz_method = 0;                       // method of the last OZ
 thread_local int t_oz_bits = 0;                         // CRT bits of the last OZAKI call
 thread_local int t_oz_moduli = 0;                       // CRT moduli of the last OZAKI call
 
-// After a synchronous EXACT-form call: report a product that fell outside its window.
+// the status words at `words` of workspace entry `e` (its latest acquisition) become the
+// thread's EXACT status (and the OZAKI header when oz)
+void set_status(const void *words, const StreamWorkspaces::Entry *e, bool oz) {
+  StatusRef r;
+  r.p = (const uint32_t *)words;
+  r.entry = e;
+  r.stamp = e ? e->used : 0;
+  t_exact_status = r;
+  if (oz) t_oz_hdr = r;
+}
+
+// After an EXACT- or OZAKI-form call on this thread: report a term that fell outside the
+// full-range accumulator (cannot happen for binary128 inputs).  QBLAS_OK when there is no such
+// call (a host-pointer entry point clears the reference first, so it sees only its own call).
 int check_exact_status(hipStream_t s) {
-  if (!t_exact_status) return QBLAS_OK;
+  if (!t_exact_status.p) return QBLAS_OK;
+  if (!t_exact_status.valid()) {
+    t_exact_status = StatusRef();
+    set_err_msg("exact status: the call's workspace was reused by a later call on its stream");
+    return QBLAS_ERR_ARG;
+  }
   uint32_t st = 0;
-  QB_CHECK(hipMemcpyAsync(&st, t_exact_status, 4, hipMemcpyDeviceToHost, s));
+  QB_CHECK(hipMemcpyAsync(&st, t_exact_status.p, 4, hipMemcpyDeviceToHost, s));
   QB_CHECK(hipStreamSynchronize(s));
-  t_exact_status = nullptr;
   if (st & 1u) {
-    set_err_msg("exact form: a product lies outside the accumulation window; the output was set to NaN");
+    set_err_msg("exact form: a term fell outside the full-range accumulator; the output was set to NaN");
     return QBLAS_ERR_RANGE;
   }
   return QBLAS_OK;
+}
+// words 2 of the same status area: items recomputed on the full-range path by that call
+long long exact_wide_items(hipStream_t s) {
+  if (!t_exact_status.valid()) return -1;
+  uint32_t v = 0;
+  if (hipMemcpyAsync(&v, t_exact_status.p + 2, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return -1;
+  return (long long)v;
 }
 thread_local Workspace t_stage[4];     // host-API operand staging
 thread_local hipStream_t t_stream = nullptr;
@@ -513,10 +598,11 @@ int dev_dot(int mode, int64_t n, const void *x, int64_t incx, const void *y, int
   }
   if (mode == QBLAS_MODE_EXACT) {
     void *ws = nullptr;
-    int rc = ws_acquire(t_ws, qbk::dot_exact_ws(n), s, &ws);
+    const StreamWorkspaces::Entry *we = nullptr;
+    int rc = ws_acquire(t_ws, qbk::dot_exact_ws(n), s, &ws, &we);
     if (rc) return rc;
     QB_CHECK(qbk::launch_dot_exact((const q128 *)x, incx, (const q128 *)y, incy, n, ws, (q128 *)res, s));
-    t_exact_status = (const uint32_t *)ws;
+    set_status(ws, we, false);
     return ws_release(t_ws, s);
   }
   if (mode != QBLAS_MODE_REFERENCE) {
@@ -552,11 +638,12 @@ int dev_gemv(int mode, int layout, int64_t m, int64_t n, q128 alpha, const void 
   }
   if (mode == QBLAS_MODE_EXACT) {
     void *ws = nullptr;
-    int rc = ws_acquire(t_ws, qbk::gemv_exact_ws(colmajor, m, n), s, &ws);
+    const StreamWorkspaces::Entry *we = nullptr;
+    int rc = ws_acquire(t_ws, qbk::gemv_exact_ws(colmajor, m, n), s, &ws, &we);
     if (rc) return rc;
     QB_CHECK(qbk::launch_gemv_exact(colmajor, m, n, alpha, (const q128 *)A, lda, (const q128 *)x, incx, beta,
                                     (q128 *)y, incy, ws, s));
-    t_exact_status = (const uint32_t *)ws;
+    set_status(ws, we, false);
     return ws_release(t_ws, s);
   }
   if (mode != QBLAS_MODE_REFERENCE) {
@@ -707,7 +794,8 @@ int dev_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, q128 alpha, 
     t_oz_bits = crt ? ch.t : 0;
     t_oz_moduli = crt ? qbk::crt_moduli_for(k, ch.t) : 0;
     void *ws;
-    int rc = ws_acquire(t_ws, need, s, &ws);
+    const StreamWorkspaces::Entry *we = nullptr;
+    int rc = ws_acquire(t_ws, need, s, &ws, &we);
     if (rc) return rc;
     hipEvent_t *ev = nullptr;
     if (g_phase_timing.load()) {
@@ -724,8 +812,7 @@ int dev_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, q128 alpha, 
     else
       QB_CHECK(qbk::launch_gemm_ozaki(colmajor, m, n, k, ch.S, alpha, (const q128 *)A, lda, (const q128 *)B, ldb,
                                       beta, (q128 *)C, ldc, ws, s, ev, ch.probe_ws, gate));
-    t_exact_status = (const uint32_t *)ws;
-    t_oz_hdr = (const uint32_t *)ws;
+    set_status(ws, we, true);
     return ws_release(t_ws, s);
   }
   if (mode == QBLAS_MODE_DD) {
@@ -791,6 +878,7 @@ int host_dot(int64_t n, const void *x, int64_t incx, const void *y, int64_t incy
     set_err_msg("quadblas dot: negative stride");
     return QBLAS_ERR_ARG;
   }
+  t_exact_status = StatusRef();  // check_exact_status below sees this call only
   hipStream_t s;
   int rc = host_stream(&s);
   if (rc) return rc;
@@ -814,6 +902,7 @@ int host_gemv(int layout, int64_t m, int64_t n, q128 alpha, const void *A, int64
     set_err_msg("quadblas gemv: negative stride");
     return QBLAS_ERR_ARG;
   }
+  t_exact_status = StatusRef();  // check_exact_status below sees this call only
   hipStream_t s;
   int rc = host_stream(&s);
   if (rc) return rc;
@@ -836,6 +925,7 @@ int host_gemm(int layout, int64_t m, int64_t n, int64_t k, q128 alpha, const voi
     set_err_msg("quadblas gemm: negative leading dimension");
     return QBLAS_ERR_ARG;
   }
+  t_exact_status = StatusRef();  // check_exact_status below sees this call only
   hipStream_t s;
   int rc = host_stream(&s);
   if (rc) return rc;
@@ -884,27 +974,13 @@ int host_gemm(int layout, int64_t m, int64_t n, int64_t k, q128 alpha, const voi
 }
 
 // ---------------------------------------------------------------------------------------------
-// batched problems: a per-thread prefix-sum buffer that only grows (so repeated calls of the same
-// count launch kernels and nothing else, and can be captured in a graph), and the tuned tile
-// configuration per (mode, layout, count)
+// batched problems: grow-only buffers per (device, stream) of the calling thread (t_bws: the
+// prefix sums, the EXACT status words, the OZAKI batch's arrays), so repeated calls of the same
+// count launch kernels and nothing else, and a buffer used in a captured graph is never freed;
+// and the tuned tile configuration per (mode, layout, count)
 // ---------------------------------------------------------------------------------------------
-thread_local void *t_bws = nullptr;
-thread_local size_t t_bws_size = 0;
-
-int batch_ws(size_t bytes, void **out) {
-  if (bytes > t_bws_size) {
-    if (t_bws) {
-      QB_CHECK(hipDeviceSynchronize());  // an earlier batched call may still read it
-      QB_CHECK(hipFree(t_bws));
-      t_bws = nullptr;
-      t_bws_size = 0;
-    }
-    const size_t want = bytes < 4096 ? 4096 : bytes;
-    QB_CHECK(hipMalloc(&t_bws, want));
-    t_bws_size = want;
-  }
-  *out = t_bws;
-  return QBLAS_OK;
+int batch_ws(size_t bytes, hipStream_t s, void **out, const StreamWorkspaces::Entry **e = nullptr) {
+  return ws_acquire(t_bws, bytes, s, out, e);
 }
 
 std::mutex g_tune_mu;
@@ -941,18 +1017,18 @@ int batched_configs(int mode) {
 // totals read back (one synchronization) to size the workspace, then the grouped launches follow.
 int gemm_batched_ozaki(int layout, int64_t count, const qblas_gemm_problem *P, int S, hipStream_t s) {
   void *ws, *ws2;
+  const StreamWorkspaces::Entry *we = nullptr;
   int rc;
-  if ((rc = batch_ws(qbk::gemm_ozaki_batched_scan_ws(count), &ws))) return rc;
+  if ((rc = batch_ws(qbk::gemm_ozaki_batched_scan_ws(count), s, &ws))) return rc;
   QB_CHECK(qbk::launch_gemm_ozaki_batched_scan(layout != 0, count, P, ws, s));
   int64_t tot[qbk::OZ_BATCHED_TOTALS];
   QB_CHECK(hipMemcpyAsync(tot, (const char *)ws + qbk::gemm_ozaki_batched_totals_offset(count), sizeof(tot),
                           hipMemcpyDeviceToHost, s));
   QB_CHECK(hipStreamSynchronize(s));
-  if ((rc = batch_ws(qbk::gemm_ozaki_batched_ws(count, tot, S), &ws2))) return rc;
+  if ((rc = batch_ws(qbk::gemm_ozaki_batched_ws(count, tot, S), s, &ws2, &we))) return rc;
   if (ws2 != ws) QB_CHECK(qbk::launch_gemm_ozaki_batched_scan(layout != 0, count, P, ws2, s));  // grown: new buffer
   QB_CHECK(qbk::launch_gemm_ozaki_batched(layout != 0, count, P, S, tot, ws2, s));
-  t_exact_status = (const uint32_t *)ws2;
-  t_oz_hdr = (const uint32_t *)ws2;
+  set_status(ws2, we, true);
   t_oz_method = QBLAS_OZAKI_SLICES;
   t_oz_slices = S;
   t_oz_bits = 0;
@@ -966,7 +1042,7 @@ int gemm_batched(int mode, int layout, int64_t count, const qblas_gemm_problem *
   if (rc || count == 0) return rc;
   if (mode == QBLAS_MODE_OZAKI) return gemm_batched_ozaki(layout, count, P, qbk::ozaki_batched_slices(cfg), s);
   void *ws;
-  if ((rc = batch_ws(qbk::batched_ws(count), &ws))) return rc;
+  if ((rc = batch_ws(qbk::batched_ws(count), s, &ws))) return rc;
   QB_CHECK(qbk::launch_gemm_batched(mode == QBLAS_MODE_DD, layout != 0, count, P, cfg, (int64_t)g_kc.load(), ws, s));
   return QBLAS_OK;
 }
@@ -983,8 +1059,14 @@ int gemm_batched_tune(int mode, int layout, int64_t count, const qblas_gemm_prob
   const bool oz = mode == QBLAS_MODE_OZAKI;
   const int64_t R = count < 512 ? count : 512;
   std::vector<qblas_gemm_problem> hp(R);
-  QB_CHECK(hipMemcpyAsync(hp.data(), dP, R * sizeof(qblas_gemm_problem), hipMemcpyDeviceToHost, s));
-  QB_CHECK(hipStreamSynchronize(s));
+  {
+    hipError_t e = hipMemcpyAsync(hp.data(), dP, R * sizeof(qblas_gemm_problem), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      set_err("tune: reading the descriptors", e);
+      return -QBLAS_ERR_HIP;
+    }
+  }
   int64_t elems = 0;
   for (auto &p : hp) {
     const int64_t m = p.m > 0 ? p.m : 0, n = p.n > 0 ? p.n : 0, k = p.k > 0 ? p.k : 0;
@@ -993,7 +1075,13 @@ int gemm_batched_tune(int mode, int layout, int64_t count, const qblas_gemm_prob
   char *scratch = nullptr;
   const size_t dbytes = R * sizeof(qblas_gemm_problem);
   const size_t bytes = (size_t)elems * 16 + dbytes + qbk::batched_ws(R) + 256;
-  QB_CHECK(hipMalloc(&scratch, bytes));
+  {
+    const hipError_t e = hipMalloc(&scratch, bytes);
+    if (e != hipSuccess) {
+      set_err("tune: scratch allocation", e);
+      return -QBLAS_ERR_HIP;
+    }
+  }
   std::vector<qblas_gemm_problem> rp(hp);
   q128 *data = (q128 *)scratch;
   int64_t off = 0;
@@ -1054,10 +1142,16 @@ int gemm_batched_tune(int mode, int layout, int64_t count, const qblas_gemm_prob
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (oz) {  // the replica's status and repair count are not the caller's
-    t_exact_status = nullptr;
-    t_oz_hdr = nullptr;
+    t_exact_status = StatusRef();
+    t_oz_hdr = StatusRef();
   }
-  QB_CHECK(hipFree(scratch));
+  {
+    const hipError_t e = hipFree(scratch);
+    if (e != hipSuccess) {
+      set_err("tune: freeing the scratch", e);
+      return -QBLAS_ERR_HIP;
+    }
+  }
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tuned[std::make_tuple(mode, layout, count)] = best;
@@ -1093,6 +1187,7 @@ int host_nrm2(int64_t n, const void *x, int64_t incx, q128 *out) {
     set_err_msg("quadblas nrm2: negative stride");
     return QBLAS_ERR_ARG;
   }
+  t_exact_status = StatusRef();  // check_exact_status below sees this call only
   hipStream_t s;
   int rc = host_stream(&s);
   if (rc) return rc;
@@ -1161,16 +1256,19 @@ int quadblas_get_num_threads(void) { return cur_threads(); }
 const char *quadblas_get_version(void) { return "QuadBLAS 1.0.0 - MI355X (gfx950) HIP backend"; }
 int quadblas_is_aligned(const void *ptr) { return ((uintptr_t)ptr % 32u) == 0; }
 
+// On failure the binary128 result is NaN (as quadblas_qdot's double), never a stale value.
 void quadblas_dot_q(size_t n, const void *x, size_t incx, const void *y, size_t incy, void *result) {
   q128 r;
-  if (host_dot((int64_t)n, x, (int64_t)incx, y, (int64_t)incy, &r) == QBLAS_OK) memcpy(result, &r, 16);
+  if (host_dot((int64_t)n, x, (int64_t)incx, y, (int64_t)incy, &r) != QBLAS_OK) r = qb::q_nan();
+  memcpy(result, &r, 16);
 }
 void quadblas_axpy_q(size_t n, const void *alpha, const void *x, size_t incx, void *y, size_t incy) {
   host_axpy((int64_t)n, load_q(alpha), x, (int64_t)incx, y, (int64_t)incy);
 }
 void quadblas_nrm2_q(size_t n, const void *x, size_t incx, void *result) {
   q128 r;
-  if (host_nrm2((int64_t)n, x, (int64_t)incx, &r) == QBLAS_OK) memcpy(result, &r, 16);
+  if (host_nrm2((int64_t)n, x, (int64_t)incx, &r) != QBLAS_OK) r = qb::q_nan();
+  memcpy(result, &r, 16);
 }
 void quadblas_gemv_q(int layout, size_t m, size_t n, const void *alpha, const void *A, size_t lda, const void *x,
                      size_t incx, const void *beta, void *y, size_t incy) {
@@ -1211,11 +1309,12 @@ int qblas_hip_ozaki_last_method(void) { return t_oz_method; }
 int qblas_hip_ozaki_last_crt_bits(void) { return t_oz_bits; }
 int qblas_hip_ozaki_last_moduli(void) { return t_oz_moduli; }
 long long qblas_hip_ozaki_repairs(void *stream) {
-  if (!t_oz_hdr) return -1;
+  if (!t_oz_hdr.valid()) return -1;
   uint32_t v = 0;
-  if (qbk::ozaki_repair_count(t_oz_hdr, &v, (hipStream_t)stream) != hipSuccess) return -1;
+  if (qbk::ozaki_repair_count(t_oz_hdr.p, &v, (hipStream_t)stream) != hipSuccess) return -1;
   return (long long)v;
 }
+long long qblas_hip_exact_wide_items(void *stream) { return exact_wide_items((hipStream_t)stream); }
 int qblas_hip_ozaki_last_slices(void) { return t_oz_slices; }
 void qblas_hip_set_phase_timing(int on) { g_phase_timing.store(on ? 1 : 0); }
 int qblas_hip_phase_times(float *ms) {
@@ -1266,10 +1365,12 @@ int qblas_hip_gemv_batched(int mode, int layout, int64_t count, const qblas_gemv
   int rc = check_batched("qblas_hip_gemv_batched", exact ? QBLAS_MODE_REFERENCE : mode, count, d_problems);
   if (rc || count == 0) return rc;
   void *ws;
-  if ((rc = batch_ws(qbk::batched_ws(count), &ws))) return rc;
+  const StreamWorkspaces::Entry *we = nullptr;
+  const hipStream_t s = (hipStream_t)stream;
+  if ((rc = batch_ws(qbk::batched_ws(count), s, &ws, &we))) return rc;
   const int form = exact ? 2 : (mode == QBLAS_MODE_DD ? 1 : 0);
-  QB_CHECK(qbk::launch_gemv_batched(form, layout != 0, count, d_problems, ws, (hipStream_t)stream));
-  if (exact) t_exact_status = (const uint32_t *)((const char *)ws + qbk::batched_status_offset(count));
+  QB_CHECK(qbk::launch_gemv_batched(form, layout != 0, count, d_problems, ws, s));
+  if (exact) set_status((const char *)ws + qbk::batched_status_offset(count), we, false);
   return QBLAS_OK;
 }
 int qblas_hip_gemm_batched_tune(int mode, int layout, int64_t count, const qblas_gemm_problem *d_problems,

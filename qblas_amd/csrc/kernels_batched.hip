@@ -17,12 +17,13 @@
 //                   y = fma(A_ij, mul(alpha, x_j), y) over j.
 //   DD gemm / gemv  double-double accumulation (dd.hpp), C = fma(alpha, RN(acc), mul(beta, C)).
 //   EXACT gemv      RN(alpha * sum_j A_ij x_j + beta * y_i), the exact value rounded once (exact.hpp);
-//                   a product outside the accumulation windows makes y_i NaN and sets bit 0 of
-//                   the status word.
+//                   a row with a product outside its accumulation windows is left to
+//                   k_gemv_batched_exact_wide, which finds it again and takes the full-range path
+//                   (wide.hpp).
 // Tile shapes are template parameters; the host picks one configuration per call (the
 // autotuner in capi.hip measures them).
 #include "dd.hpp"
-#include "exact_dev.hpp"
+#include "wide.hpp"
 #include "kernels.h"
 
 using namespace qb;
@@ -387,17 +388,14 @@ __global__ void __launch_bounds__(256) k_gemv_batched_dd(const qblas_gemv_proble
 // ColMajor one lane per row over the columns (anchor from the row's first regular product)
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ q128 exact_row_out(const uint32_t (&Wl)[WW], const uint32_t (&Wh)[WW], int An,
-                                              uint32_t flags, q128 alpha, q128 beta, q128 yv, uint32_t *status) {
-  if (flags & FL_LOST) {
-    atomicOr(status, 1u);
-    return q_nan();
-  }
+                                              uint32_t flags, q128 alpha, q128 beta, q128 yv) {
   uint32_t Mw[SW];
   uint32_t sign;
   s_from_windows(Mw, sign, Wl, Wh);
   return finish_reg(Mw, An, sign, 0u, flags, true, alpha, beta, yv);
 }
 
+// status[1] != 0 after this kernel: some row was left for the full-range path (its y unwritten)
 __global__ void __launch_bounds__(256) k_gemv_batched_exact(const qblas_gemv_problem *__restrict__ P, int64_t count,
                                                             const int64_t *__restrict__ start, int colmajor,
                                                             uint32_t *__restrict__ status) {
@@ -432,7 +430,11 @@ __global__ void __launch_bounds__(256) k_gemv_batched_exact(const qblas_gemv_pro
       flags = wave_or(flags);
       if (lane == 0) {
         q128 *yi = y + i * incy;
-        stq_c(yi, exact_row_out(Wl, Wh, An, flags, alpha, beta, ldq_c(yi), status));
+        if (flags & FL_LOST) {
+          atomicOr(status + 1, 1u);
+        } else {
+          stq_c(yi, exact_row_out(Wl, Wh, An, flags, alpha, beta, ldq_c(yi)));
+        }
       }
     } else {
       const int64_t i = ul * 64 + lane;
@@ -442,9 +444,95 @@ __global__ void __launch_bounds__(256) k_gemv_batched_exact(const qblas_gemv_pro
         const int An = (e0 == EXP_NONE) ? 0 : e0 - WIN2_CENTER;
         for (int64_t j = 0; j < n; ++j) accumulate(Wl, Wh, An, ldq_c(A + j * lda + i), ldq_c(x + j * incx), flags);
         q128 *yi = y + i * incy;
-        stq_c(yi, exact_row_out(Wl, Wh, An, flags, alpha, beta, ldq_c(yi), status));
+        if (flags & FL_LOST) {
+          atomicOr(status + 1, 1u);
+        } else {
+          stq_c(yi, exact_row_out(Wl, Wh, An, flags, alpha, beta, ldq_c(yi)));
+        }
       }
     }
+  }
+}
+
+// whether a product of the pair misses the two windows of anchor An (the fast path's test)
+__device__ __forceinline__ bool pair_misses(q128 a, q128 b, int An) {
+  uq x, y;
+  uint32_t fl = 0;
+  const int c = classify_pair(a, b, x, y, fl);
+  if (c != 0 && c != 3) return false;
+  const int o = prod_lsb(x, y) - An;
+  return o < WIN_OMIN || o > WIN_BAND + WIN_OMAX;
+}
+
+// After k_gemv_batched_exact: when it left rows (status[1]), the same walk over the work items
+// finds them again (same anchors, same window test), four waves' rows at a time into an LDS list,
+// and the workgroup computes each on the full-range path.  Returns at once otherwise.
+__global__ void __launch_bounds__(256) k_gemv_batched_exact_wide(const qblas_gemv_problem *__restrict__ P,
+                                                                 int64_t count, const int64_t *__restrict__ start,
+                                                                 int colmajor, uint32_t *__restrict__ status) {
+  __shared__ kul_t K;
+  __shared__ int64_t s_p[256], s_i[256];
+  __shared__ int s_n;
+  if (status[1] == 0) return;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t total = start[count];
+  kul_clear_all(K);
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  for (int64_t g = blockIdx.x; g * 4 < total; g += gridDim.x) {  // uniform over the workgroup
+    const int64_t u = g * 4 + wv;
+    if (u < total) {
+      const int64_t p = find_problem(start, count, u);
+      const qblas_gemv_problem &v = P[p];
+      const int64_t m = v.m, n = v.n, lda = v.lda, incx = v.incx;
+      const int64_t ul = u - start[p];
+      const q128 *A = (const q128 *)v.A;
+      const q128 *x = (const q128 *)v.x;
+      if (!colmajor) {
+        const q128 *row = A + ul * lda;
+        int e0 = EXP_NONE;
+        if (lane < n) e0 = pair_exp(ldq_c(row + lane), ldq_c(x + (int64_t)lane * incx));
+        const int An = wave_anchor(e0);
+        bool miss = false;
+        for (int64_t j = lane; j < n; j += 64) miss |= pair_misses(ldq_c(row + j), ldq_c(x + j * incx), An);
+        if (__any(miss) && lane == 0) {
+          const int q = atomicAdd(&s_n, 1);
+          s_p[q] = p;
+          s_i[q] = ul;
+        }
+      } else {
+        const int64_t i = ul * 64 + lane;
+        if (i < m) {
+          int e0 = EXP_NONE;
+          for (int64_t j = 0; j < n && j < 4 && e0 == EXP_NONE; ++j) e0 = pair_exp(ldq_c(A + j * lda + i), ldq_c(x + j * incx));
+          const int An = (e0 == EXP_NONE) ? 0 : e0 - WIN2_CENTER;
+          bool miss = false;
+          for (int64_t j = 0; j < n && !miss; ++j) miss = pair_misses(ldq_c(A + j * lda + i), ldq_c(x + j * incx), An);
+          if (miss) {
+            const int q = atomicAdd(&s_n, 1);
+            s_p[q] = p;
+            s_i[q] = i;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    const int nl = s_n;
+    for (int q = 0; q < nl; ++q) {
+      const qblas_gemv_problem &v = P[s_p[q]];
+      const int64_t i = s_i[q];
+      const q128 *A = (const q128 *)v.A;
+      q128 *yi = (q128 *)v.y + i * v.incy;
+      const q128 r = wide_item(K, v.n, colmajor ? A + i : A + i * v.lda, colmajor ? v.lda : 1, (const q128 *)v.x,
+                               v.incx, true, qv(v.alpha), qv(v.beta), ldq_c(yi), status);
+      if (threadIdx.x == 0) {
+        stq_c(yi, r);
+        atomicAdd(status + 2, 1u);  // rows on the full-range path
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
   }
 }
 
@@ -478,7 +566,7 @@ size_t batched_ws(int64_t count) { return ((size_t)(count + 1) * sizeof(int64_t)
 size_t batched_status_offset(int64_t count) { return ((size_t)(count + 1) * sizeof(int64_t) + 255) / 256 * 256; }
 
 __global__ void k_batch_status_zero(uint32_t *s) {
-  if (threadIdx.x == 0) s[0] = 0;
+  if (threadIdx.x < 4) s[threadIdx.x] = 0;
 }
 
 hipError_t launch_gemm_batched(int dd_mode, int colmajor, int64_t count, const qblas_gemm_problem *P, int cfg,
@@ -519,6 +607,7 @@ hipError_t launch_gemv_batched(int form, int colmajor, int64_t count, const qbla
     uint32_t *status = (uint32_t *)((char *)ws + batched_status_offset(count));
     k_batch_status_zero<<<1, 64, 0, st>>>(status);
     k_gemv_batched_exact<<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, status);
+    k_gemv_batched_exact_wide<<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, status);
   }
   return hipGetLastError();
 }

@@ -12,10 +12,11 @@
 //   k_exact_finish_dot     sums the column records, ripples carries once, rounds once.
 //   k_exact_finish_rows    one thread per row: sums the row's chunk windows (shared anchor),
 //                          forms alpha*S + beta*y exactly in registers and rounds into y.
-// Loads of the streamed operands are coalesced 128-bit loads.  If any product falls outside its
-// window the result is not exact: the output is set to NaN and bit 0 of the status word is set
-// (the host entry points report it as an error).
-#include "exact_dev.hpp"
+// Loads of the streamed operands are coalesced 128-bit loads.  A dot or row with a product outside
+// its lane windows (products more than ~2^124 from the anchor) is recomputed on the full-range
+// path (wide.hpp): the dot by the finish kernel's whole grid, rows by k_exact_wide_rows from a
+// queue that the row finish fills.  Header word 2 counts the items so recomputed.
+#include "wide.hpp"
 #include "kernels.h"
 
 using namespace qb;
@@ -269,50 +270,193 @@ __global__ void __launch_bounds__(XF_BS) k_exact_combine(const part_t *__restric
   }
 }
 
-__global__ void __launch_bounds__(64) k_exact_finish_dot(const col_rec *__restrict__ recs, int nrec,
-                                                         q128 *__restrict__ out, uint32_t *__restrict__ status) {
+// Finish of the dot: block 0 sums the column records, ripples the carries once and rounds.  When
+// a product fell outside its block's windows, or block anchors lay too far apart for the combine,
+// the whole grid (XW_BLOCKS workgroups) recomputes the dot on the full-range path instead: block
+// b accumulates its slice of the elements in LDS (wide.hpp), normalises it and writes its digits
+// to record b; the last block to finish (a ticket in header word 3) adds the records digit by digit,
+// rounds once and writes the result (header word 2 counts the call as recomputed).
+constexpr int XW_BLOCKS = 128;
+struct dot_rec {
+  int lo, t, neg;
+  uint32_t flags;
+  uint32_t D[KUL_N];
+};
+
+__global__ void __launch_bounds__(256) k_exact_finish_dot(const col_rec *__restrict__ recs, int nrec,
+                                                          const q128 *__restrict__ x, int64_t incx,
+                                                          const q128 *__restrict__ y, int64_t incy, int64_t n,
+                                                          dot_rec *__restrict__ wrec, q128 *__restrict__ out,
+                                                          uint32_t *__restrict__ hdr) {
+  __shared__ kul_t K;
   __shared__ uint64_t col[ACCW];
-  if (threadIdx.x < ACCW) {
-    uint64_t s = 0;
-    for (int b = 0; b < nrec; ++b) s += recs[b].c[threadIdx.x];  // < 2^45 for nrec <= 32
-    col[threadIdx.x] = s;
+  __shared__ int s_last;
+  uint32_t lost = 0;
+  for (int b = 0; b < nrec; ++b) lost |= recs[b].lost;
+  if (recs[0].flags & FL_LOST) lost = 1;
+  if (!lost) {
+    if (blockIdx.x != 0) return;
+    if (threadIdx.x < ACCW) {
+      uint64_t s = 0;
+      for (int b = 0; b < nrec; ++b) s += recs[b].c[threadIdx.x];  // < 2^45 for nrec <= 32
+      col[threadIdx.x] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const uint32_t fl = recs[0].flags;
+    const int amin = recs[0].amin;
+    q128 r;
+    if (amin == EXP_NONE) {
+      r = (fl & (FL_NAN | FL_PINF | FL_NINF)) ? special_result(fl) : q_zero(0);
+    } else {
+      wide_acc wa;
+      wide_init(wa);
+      wa.any = true;
+      wa.base = amin - 32;
+      uint64_t carry = 0;
+      for (int k = 0; k < ACCW; ++k) {
+        const uint64_t v = col[k] + carry;  // two's complement words: the top carry is dropped
+        wa.w[k] = (uint32_t)v;
+        carry = v >> 32;
+      }
+      uint32_t M[SW], sign, sticky;
+      int lsb;
+      compact_wide(wa, M, lsb, sign, sticky);
+      r = finish_reg(M, lsb, sign, sticky, fl, false, q_one(), q_zero(0), q_zero(0));
+    }
+    stq_c(out, r);
+    return;
+  }
+  // full-range path: block b takes elements [b*chunk, (b+1)*chunk)
+  kul_clear_all(K);
+  __syncthreads();
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t l0 = (int64_t)blockIdx.x * chunk;
+  const int64_t l1 = (l0 + chunk < n) ? l0 + chunk : n;
+  int e0 = EXP_NONE;
+  if (l0 + (int64_t)threadIdx.x < l1) e0 = pair_exp(ldq_c(x + (l0 + threadIdx.x) * incx), ldq_c(y + (l0 + threadIdx.x) * incy));
+  const int An = wave_anchor(e0);
+  uint32_t Wl[WW], Wh[WW];
+#pragma unroll
+  for (int k = 0; k < WW; ++k) {
+    Wl[k] = 0;
+    Wh[k] = 0;
+  }
+  uint32_t flags = 0;
+  int tlo = KUL_N, thi = -1;
+  const uq none = unpack_finite(q_zero(0));
+  for (int64_t s0 = l0; s0 < l1; s0 += KUL_SEG) {
+    const int64_t s1 = (l1 - s0 > KUL_SEG) ? s0 + KUL_SEG : l1;
+    kul_accumulate(K, s0, s1, x, incx, y, incy, An, Wl, Wh, false, none, tlo, thi, flags);
+    if (s1 < l1) {
+      kul_publish_range(K, tlo, thi);
+      tlo = KUL_N;
+      thi = -1;
+      __syncthreads();
+      if (threadIdx.x == 0) (void)kul_normalize(K);
+      __syncthreads();
+    }
+  }
+  wave_sum_win(Wl);
+  wave_sum_win(Wh);
+  flags = wave_or(flags);
+  if ((threadIdx.x & 63) == 0) {
+    kul_add_window(K, Wl, An, false, none, tlo, thi, flags);
+    kul_add_window(K, Wh, An + WIN_BAND, false, none, tlo, thi, flags);
+    if (flags) atomicOr(&K.flags, flags);
+  }
+  kul_publish_range(K, tlo, thi);
+  __syncthreads();
+  dot_rec *my = wrec + blockIdx.x;
+  if (threadIdx.x == 0) {
+    const int t = kul_normalize(K);
+    my->lo = K.lo;
+    my->t = t;
+    my->neg = K.neg;
+    my->flags = K.flags;
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  uint32_t fl = recs[0].flags, lost = 0;
-  for (int b = 0; b < nrec; ++b) lost |= recs[b].lost;
-  const int amin = recs[0].amin;
-  if (lost) fl |= FL_LOST;
-  q128 r;
-  if (fl & FL_LOST) {
-    r = q_nan();
-    atomicOr(status, 1u);
-  } else if (amin == EXP_NONE) {
-    r = (fl & (FL_NAN | FL_PINF | FL_NINF)) ? special_result(fl) : q_zero(0);
-  } else {
-    wide_acc wa;
-    wide_init(wa);
-    wa.any = true;
-    wa.base = amin - 32;
-    uint64_t carry = 0;
-    for (int k = 0; k < ACCW; ++k) {
-      const uint64_t v = col[k] + carry;  // two's complement words: the top carry is dropped
-      wa.w[k] = (uint32_t)v;
-      carry = v >> 32;
+  {
+    const int lo = K.lo, hi = K.hi;
+    for (int i = lo + (int)threadIdx.x; i <= hi; i += blockDim.x) {
+      my->D[i] = (uint32_t)K.L[i];
+      K.L[i] = 0;
     }
-    uint32_t M[SW], sign, sticky;
-    int lsb;
-    compact_wide(wa, M, lsb, sign, sticky);
-    r = finish_reg(M, lsb, sign, sticky, fl, false, q_one(), q_zero(0), q_zero(0));
   }
-  stq_c(out, r);
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t ticket = __hip_atomic_fetch_add(hdr + 3, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = ticket == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  // last block: digit sums of all records (each < 2^32 * gridDim.x), then one rounding
+  __shared__ int s_lo, s_hi;
+  __shared__ uint32_t s_fl;
+  if (threadIdx.x == 0) {
+    s_lo = KUL_N;
+    s_hi = -1;
+    s_fl = 0;
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+    const dot_rec *r = wrec + b;
+    const int lo = __hip_atomic_load(&r->lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int t = __hip_atomic_load(&r->t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int ng = __hip_atomic_load(&r->neg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t f = __hip_atomic_load(&r->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int top = ng ? t + 1 : t;
+    if (lo <= top) {
+      atomicMin(&s_lo, lo);
+      atomicMax(&s_hi, top);
+    }
+    if (f) atomicOr(&s_fl, f);
+  }
+  __syncthreads();
+  const int glo = s_lo, ghi = min(s_hi, KUL_N - 1);
+  for (int i = glo + (int)threadIdx.x; i <= ghi; i += blockDim.x) {
+    int64_t s = 0;
+    for (int b = 0; b < (int)gridDim.x; ++b) {
+      const dot_rec *r = wrec + b;
+      const int lo = r->lo, t = r->t;
+      if (i >= lo && i <= t) s += (int64_t)r->D[i];
+      if (r->neg && i == t + 1) s -= 1;
+    }
+    K.L[i] = (unsigned long long)s;
+  }
+  if (threadIdx.x == 0) {
+    K.lo = glo;
+    K.hi = ghi;
+    K.flags = s_fl;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int t = kul_normalize(K);
+    const uint32_t fl = K.flags;
+    q128 r;
+    if (fl & FL_LOST) {
+      r = q_nan();
+      atomicOr(hdr, 1u);
+    } else if (fl & (FL_NAN | FL_PINF | FL_NINF)) {
+      r = special_result(fl);
+    } else {
+      bool z;
+      r = kul_round(K, t, z);
+    }
+    stq_c(out, r);
+    hdr[2] = 1;  // the dot took the full-range path
+    hdr[3] = 0;
+  }
 }
 
 // one thread per row: sums the row's chunk windows (they share the row's anchor), then
-// RN(alpha*S + beta*y) in registers
+// RN(alpha*S + beta*y) in registers.  A row with a product outside its windows is queued (header
+// word 2 counts the queue) and left for k_exact_wide_rows; its y is not written here.
 __global__ void __launch_bounds__(64) k_exact_finish_rows(int64_t m, int nch, const part_t *__restrict__ parts,
                                                          q128 alpha, q128 beta, q128 *__restrict__ y, int64_t incy,
-                                                         uint32_t *__restrict__ status) {
+                                                         uint32_t *__restrict__ hdr, int64_t *__restrict__ queue) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
     uint32_t lo[WW], hi[WW];
@@ -359,16 +503,36 @@ __global__ void __launch_bounds__(64) k_exact_finish_rows(int64_t m, int nch, co
       win_add_win(hi, Y);
     }
     q128 *yi = y + i * incy;
-    q128 r;
     if (fl & FL_LOST) {
-      r = q_nan();
-      atomicOr(status, 1u);
+      const uint32_t slot = atomicAdd(hdr + 2, 1u);
+      queue[slot] = i;
     } else {
       uint32_t M[SW], sign;
       s_from_windows(M, sign, lo, hi);
-      r = finish_reg(M, A0, sign, 0u, fl, true, alpha, beta, ldq_c(yi));
+      stq_c(yi, finish_reg(M, A0, sign, 0u, fl, true, alpha, beta, ldq_c(yi)));
     }
-    stq_c(yi, r);
+  }
+}
+
+// The queued rows on the full-range path, one workgroup per row (grid-stride over the queue; the
+// whole grid returns at once when the queue is empty).
+constexpr int XW_ROW_BLOCKS = 256;
+__global__ void __launch_bounds__(256) k_exact_wide_rows(int colmajor, int64_t n, const q128 *__restrict__ A,
+                                                        int64_t lda, const q128 *__restrict__ x, int64_t incx,
+                                                        q128 alpha, q128 beta, q128 *__restrict__ y, int64_t incy,
+                                                        uint32_t *__restrict__ hdr,
+                                                        const int64_t *__restrict__ queue) {
+  __shared__ kul_t K;
+  const uint32_t count = hdr[2];
+  if (blockIdx.x >= count) return;
+  kul_clear_all(K);
+  __syncthreads();
+  for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) {
+    const int64_t i = queue[e];
+    const q128 *row = colmajor ? A + i : A + i * lda;
+    q128 *yi = y + i * incy;
+    const q128 r = wide_item(K, n, row, colmajor ? lda : 1, x, incx, true, alpha, beta, ldq_c(yi), hdr);
+    if (threadIdx.x == 0) stq_c(yi, r);
   }
 }
 
@@ -399,14 +563,20 @@ static void gemv_exact_split(int colmajor, int64_t m, int64_t n, int &nch, int64
   chunk = (n + want - 1) / want;
   nch = (int)((n + chunk - 1) / chunk);
 }
-size_t dot_exact_ws(int64_t n) {
+// workspace of the dot: [header 256 B | partials | column records | full-range records]
+static size_t dot_exact_recs_off(int64_t n) {
   return 256 + 2 * (size_t)dot_exact_blocks(n) * sizeof(part_t) + XF_BLOCKS * sizeof(col_rec);
 }
-size_t gemv_exact_ws(int colmajor, int64_t m, int64_t n) {
+size_t dot_exact_ws(int64_t n) { return (dot_exact_recs_off(n) + 255) / 256 * 256 + XW_BLOCKS * sizeof(dot_rec); }
+// workspace of gemv: [header 256 B | partials | queue of rows for the full-range path]
+static size_t gemv_exact_queue_off(int colmajor, int64_t m, int64_t n) {
   int nch;
   int64_t chunk;
   gemv_exact_split(colmajor, m, n, nch, chunk);
   return 256 + 2 * (size_t)nch * (size_t)m * sizeof(part_t);
+}
+size_t gemv_exact_ws(int colmajor, int64_t m, int64_t n) {
+  return gemv_exact_queue_off(colmajor, m, n) + (size_t)m * sizeof(int64_t);
 }
 
 // workspace: [status (u32) | pad to 256 | partials]
@@ -422,7 +592,9 @@ hipError_t launch_dot_exact(const q128 *x, int64_t incx, const q128 *y, int64_t 
   const int nparts = 2 * blocks;
   const int cblocks = nparts < XF_BLOCKS * XF_BS ? (nparts + XF_BS - 1) / XF_BS : XF_BLOCKS;
   hipLaunchKernelGGL(k_exact_combine, dim3(cblocks), dim3(XF_BS), 0, st, parts, nparts, recs);
-  hipLaunchKernelGGL(k_exact_finish_dot, dim3(1), dim3(64), 0, st, recs, cblocks, out, status);
+  dot_rec *wrec = (dot_rec *)((char *)ws + (dot_exact_recs_off(n) + 255) / 256 * 256);
+  hipLaunchKernelGGL(k_exact_finish_dot, dim3(XW_BLOCKS), dim3(256), 0, st, recs, cblocks, x, incx, y, incy, n, wrec,
+                     out, status);
   return hipGetLastError();
 }
 
@@ -445,8 +617,12 @@ hipError_t launch_gemv_exact(int colmajor, int64_t m, int64_t n, q128 alpha, con
     hipLaunchKernelGGL(k_gemv_col_exact, dim3((unsigned)((m + XC_BS - 1) / XC_BS), (unsigned)nch), dim3(XC_BS), 0,
                        st, m, n, chunk, A, lda, x, incx, parts);
   }
+  int64_t *queue = (int64_t *)((char *)ws + gemv_exact_queue_off(colmajor, m, n));
   hipLaunchKernelGGL(k_exact_finish_rows, dim3(stride_blocks(m, 64)), dim3(64), 0, st, m, nch, parts, alpha,
-                     beta, y, incy, status);
+                     beta, y, incy, status, queue);
+  const unsigned wb = (unsigned)(m < XW_ROW_BLOCKS ? m : XW_ROW_BLOCKS);
+  hipLaunchKernelGGL(k_exact_wide_rows, dim3(wb), dim3(256), 0, st, colmajor, n, A, lda, x, incx, alpha, beta, y,
+                     incy, status, queue);
   return hipGetLastError();
 }
 

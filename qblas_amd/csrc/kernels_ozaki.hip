@@ -27,9 +27,10 @@
 //    (representation error of both operands plus the dropped diagonals t+u >= S): an entry is
 //    written only if alpha times that bound is <= 2^-106 |result|.
 // 5. Repair.  Every other entry (cancellation below the bound, zero sums, Inf/NaN operands) is
-//    queued and recomputed by the EXACT form's exact accumulation (correctly rounded).
+//    queued and recomputed by the EXACT form's exact accumulation (correctly rounded); an entry
+//    whose products lie too far apart for the windows takes the full-range path (wide.hpp).
 // So the result of every entry has relative error <= 2^-105.9 or is correctly rounded.
-#include "exact_dev.hpp"
+#include "wide.hpp"
 #include "kernels.h"
 
 namespace qbk {
@@ -87,7 +88,7 @@ static oz_layout oz_plan(int64_t m, int64_t n, int64_t k, int S) {
   L.part = o;
   o = al(o + (size_t)L.npass * L.Mp * L.Np * OZ_PREC * 4);
   L.list = o;
-  o = al(o + (size_t)m * n * 8);
+  o = al(o + (size_t)m * n * 12);  // (i, j) pairs, then the full-range queue
   L.bt = o;  // B^T (n x k) for the repair of many entries when B's columns are strided
   o = al(o + (size_t)n * k * 16);
   L.total = o;
@@ -588,8 +589,8 @@ __global__ void __launch_bounds__(256) k_oz_finish(int64_t M, int64_t N, int S, 
 // One workgroup per queued entry (grid-stride over the queue): exact accumulation of
 // sum_l A_il B_lj in two windows per lane with a block-uniform anchor placing the largest
 // possible product (from the row and column maxima) at the top of the high window, then the
-// exact alpha*S + beta*C rounded once.  A product below the low window sets bit 0 of the
-// status word and the entry becomes NaN (as in the EXACT form).
+// exact alpha*S + beta*C rounded once.  An entry with a product below the low window is
+// recomputed on the full-range path by the same workgroup.
 constexpr int OZR_BS = 256;
 
 // Exact sum_l A_il B_lj by one workgroup: two windows per lane (anchor An), wave and block
@@ -639,26 +640,24 @@ __device__ __forceinline__ int oz_anchor(int ei, int ej) {
   return (ei == OZ_EZERO || ej == OZ_EZERO) ? 0 : ei + ej - 224 - (WIN_BAND + WIN_OMAX);
 }
 
-// One entry by the whole workgroup: *cp = RN(alpha sum_l ar[l asc] bc[l bsr] + beta *cp) (NaN and
-// status bit 0 when a product falls below the windows of anchor An).
+// One entry by the whole workgroup: *cp = RN(alpha sum_l ar[l asc] bc[l bsr] + beta *cp).  An entry
+// with a product outside the windows of anchor An is left unwritten: its queue index e goes to
+// the full-range queue wq (header word 2 counts it) for k_oz_wide / k_bo_wide.
 __device__ __forceinline__ void oz_repair_entry(int64_t K, const q128 *__restrict__ ar, int64_t asc,
                                                 const q128 *__restrict__ bc, int64_t bsr, int An, q128 alpha,
                                                 q128 beta, q128 *__restrict__ cp, uint32_t *__restrict__ hdr,
-                                                uint32_t (*s_w)[2 * WW + 1]) {
+                                                uint32_t (*s_w)[2 * WW + 1], uint32_t e, uint32_t *__restrict__ wq) {
   uint32_t Wl[WW], Wh[WW], flags;
   oz_block_dot(K, ar, asc, bc, bsr, An, Wl, Wh, flags, s_w);
   if (threadIdx.x == 0) {
-    q128 out;
     if (flags & FL_LOST) {
-      out = q_nan();
-      atomicOr(hdr, 1u);
+      wq[atomicAdd(hdr + 2, 1u)] = e;
     } else {
       uint32_t Mw[SW];
       uint32_t sign;
       s_from_windows(Mw, sign, Wl, Wh);
-      out = finish_reg(Mw, An, sign, 0u, flags, true, alpha, beta, ldq_c(cp));
+      stq_c(cp, finish_reg(Mw, An, sign, 0u, flags, true, alpha, beta, ldq_c(cp)));
     }
-    stq_c(cp, out);
   }
   __syncthreads();
 }
@@ -674,7 +673,8 @@ __global__ void __launch_bounds__(OZR_BS) k_oz_repair(int64_t K, const q128 *__r
                                                       const q128 *__restrict__ Bt, const int32_t *__restrict__ eA,
                                                       const int32_t *__restrict__ eB, q128 alpha, q128 beta,
                                                       q128 *__restrict__ C, int64_t csr, int64_t csc,
-                                                      uint32_t *__restrict__ hdr, const uint32_t *__restrict__ list) {
+                                                      uint32_t *__restrict__ hdr, const uint32_t *__restrict__ list,
+                                                      uint32_t *__restrict__ wq) {
   __shared__ uint32_t s_w[OZR_BS / 64][2 * WW + 1];
   const uint32_t count = hdr[1];
   const bool useT = Bt != nullptr && count >= OZ_REPAIR_TRANSPOSE;
@@ -683,7 +683,35 @@ __global__ void __launch_bounds__(OZR_BS) k_oz_repair(int64_t K, const q128 *__r
   for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) {
     const int64_t i = list[2 * (int64_t)e], j = list[2 * (int64_t)e + 1];
     oz_repair_entry(K, A + i * asr, asc, Br + j * rbsc, rbsr, oz_anchor(eA[i], eB[j]), alpha, beta,
-                    C + i * csr + j * csc, hdr, s_w);
+                    C + i * csr + j * csc, hdr, s_w, e, wq);
+  }
+}
+
+// Entries whose products lie too far apart for the repair's windows (queue wq of indices into
+// list, header word 2), one workgroup each on the full-range path (wide.hpp); the whole grid
+// returns at once when there are none.
+constexpr int OZ_WIDE_BLOCKS = 256;
+__global__ void __launch_bounds__(256) k_oz_wide(int64_t K, const q128 *__restrict__ A, int64_t asr, int64_t asc,
+                                                 const q128 *__restrict__ B, int64_t bsr, int64_t bsc,
+                                                 const q128 *__restrict__ Bt, q128 alpha, q128 beta,
+                                                 q128 *__restrict__ C, int64_t csr, int64_t csc,
+                                                 uint32_t *__restrict__ hdr, const uint32_t *__restrict__ list,
+                                                 const uint32_t *__restrict__ wq) {
+  __shared__ kul_t Kl;
+  const uint32_t nw = hdr[2];
+  if (blockIdx.x >= nw) return;
+  const uint32_t count = hdr[1];
+  const bool useT = Bt != nullptr && count >= OZ_REPAIR_TRANSPOSE;
+  const q128 *Br = useT ? Bt : B;
+  const int64_t rbsr = useT ? 1 : bsr, rbsc = useT ? K : bsc;
+  kul_clear_all(Kl);
+  __syncthreads();
+  for (uint32_t w = blockIdx.x; w < nw; w += gridDim.x) {
+    const uint32_t e = wq[w];
+    const int64_t i = list[2 * (int64_t)e], j = list[2 * (int64_t)e + 1];
+    q128 *cp = C + i * csr + j * csc;
+    const q128 r = wide_item(Kl, K, A + i * asr, asc, Br + j * rbsc, rbsr, true, alpha, beta, ldq_c(cp), hdr);
+    if (threadIdx.x == 0) stq_c(cp, r);
   }
 }
 
@@ -818,17 +846,20 @@ static void oz_scale(const q128 *X, int64_t R, int64_t Cn, int64_t sr, int64_t s
 // Exact recomputation of the queued entries (header word 1 counts them, list holds (i, j)).  When
 // B's columns are strided, a transposed copy of B (at Bt) is made first if the queue is long; the
 // copy and the repair read the queue length on the device, so nothing is synchronized.
-static hipError_t oz_repair_queue(int64_t n, int64_t k, const q128 *A, int64_t asr, int64_t asc, const q128 *B,
-                                  int64_t bsr, int64_t bsc, const int32_t *eA, const int32_t *eB, q128 alpha,
-                                  q128 beta, q128 *C, int64_t csr, int64_t csc, uint32_t *hdr, const uint32_t *list,
-                                  q128 *Bt, hipStream_t st, hipEvent_t *ev) {
+static hipError_t oz_repair_queue(int64_t m, int64_t n, int64_t k, const q128 *A, int64_t asr, int64_t asc,
+                                  const q128 *B, int64_t bsr, int64_t bsc, const int32_t *eA, const int32_t *eB,
+                                  q128 alpha, q128 beta, q128 *C, int64_t csr, int64_t csc, uint32_t *hdr,
+                                  uint32_t *list, q128 *Bt, hipStream_t st, hipEvent_t *ev) {
+  uint32_t *wq = list + 2 * m * n;  // full-range queue after the (i, j) list
   const bool strided = bsr != 1 && n > 1;
   if (strided) {
     dim3 tg((unsigned)((n + 31) / 32), (unsigned)((k + 31) / 32));
     k_oz_transpose<<<tg, 256, 0, st>>>(B, k, n, bsr, hdr, Bt);
   }
   k_oz_repair<<<OZ_REPAIR_BLOCKS, OZR_BS, 0, st>>>(k, A, asr, asc, B, bsr, bsc, strided ? Bt : nullptr, eA, eB,
-                                                   alpha, beta, C, csr, csc, hdr, list);
+                                                   alpha, beta, C, csr, csc, hdr, list, wq);
+  k_oz_wide<<<OZ_WIDE_BLOCKS, 256, 0, st>>>(k, A, asr, asc, B, bsr, bsc, strided ? Bt : nullptr, alpha, beta, C, csr,
+                                            csc, hdr, list, wq);
   if (ev) (void)hipEventRecord(ev[4], st);
   return hipGetLastError();
 }
@@ -905,8 +936,8 @@ hipError_t launch_gemm_ozaki(int colmajor, int64_t m, int64_t n, int64_t k, int 
                                                             L.Mp, L.Np, eA, fA, eB, fB, alpha, beta, C, csr, csc, hdr,
                                                             (uint32_t *)(ws + L.list));
   if (ev) (void)hipEventRecord(ev[3], st);
-  return oz_repair_queue(n, k, A, asr, asc, B, bsr, bsc, eA, eB, alpha, beta, C, csr, csc, hdr,
-                         (const uint32_t *)(ws + L.list), (q128 *)(ws + L.bt), st, ev);
+  return oz_repair_queue(m, n, k, A, asr, asc, B, bsr, bsc, eA, eB, alpha, beta, C, csr, csc, hdr,
+                         (uint32_t *)(ws + L.list), (q128 *)(ws + L.bt), st, ev);
 }
 
 size_t ozaki_probe_ws(int64_t m, int64_t n) {
@@ -1181,7 +1212,8 @@ __global__ void __launch_bounds__(256) k_bo_finish(const qblas_gemm_problem *__r
 __global__ void __launch_bounds__(OZR_BS) k_bo_repair(const qblas_gemm_problem *__restrict__ P, int64_t count,
                                                       const int64_t *__restrict__ off, int colmajor,
                                                       const int32_t *__restrict__ eA, const int32_t *__restrict__ eB,
-                                                      uint32_t *__restrict__ hdr, const uint32_t *__restrict__ list) {
+                                                      uint32_t *__restrict__ hdr, const uint32_t *__restrict__ list,
+                                                      uint32_t *__restrict__ wq) {
   __shared__ uint32_t s_w[OZR_BS / 64][2 * WW + 1];
   const uint32_t nq = hdr[1];
   for (uint32_t e = blockIdx.x; e < nq; e += gridDim.x) {
@@ -1191,7 +1223,28 @@ __global__ void __launch_bounds__(OZR_BS) k_bo_repair(const qblas_gemm_problem *
     const int64_t *o = off + p * BO_F;
     oz_repair_entry(G.k, (const q128 *)pr.A + i * G.asr, G.asc, (const q128 *)pr.B + j * G.bsc, G.bsr,
                     oz_anchor(eA[o[BO_ROWS] + i], eB[o[BO_COLS] + j]), q_make(pr.alpha[0], pr.alpha[1]),
-                    q_make(pr.beta[0], pr.beta[1]), (q128 *)pr.C + i * G.csr + j * G.csc, hdr, s_w);
+                    q_make(pr.beta[0], pr.beta[1]), (q128 *)pr.C + i * G.csr + j * G.csc, hdr, s_w, e, wq);
+  }
+}
+
+// the batch's entries on the full-range path (queue wq, header word 2), one workgroup each
+__global__ void __launch_bounds__(256) k_bo_wide(const qblas_gemm_problem *__restrict__ P, int colmajor,
+                                                 uint32_t *__restrict__ hdr, const uint32_t *__restrict__ list,
+                                                 const uint32_t *__restrict__ wq) {
+  __shared__ kul_t Kl;
+  const uint32_t nw = hdr[2];
+  if (blockIdx.x >= nw) return;
+  kul_clear_all(Kl);
+  __syncthreads();
+  for (uint32_t w = blockIdx.x; w < nw; w += gridDim.x) {
+    const uint32_t e = wq[w];
+    const int64_t p = list[4 * (int64_t)e], i = list[4 * (int64_t)e + 1], j = list[4 * (int64_t)e + 2];
+    const qblas_gemm_problem &pr = P[p];
+    const bo_geom G = bo_geometry(pr, colmajor);
+    q128 *cp = (q128 *)pr.C + i * G.csr + j * G.csc;
+    const q128 r = wide_item(Kl, G.k, (const q128 *)pr.A + i * G.asr, G.asc, (const q128 *)pr.B + j * G.bsc, G.bsr,
+                             true, q_make(pr.alpha[0], pr.alpha[1]), q_make(pr.beta[0], pr.beta[1]), ldq_c(cp), hdr);
+    if (threadIdx.x == 0) stq_c(cp, r);
   }
 }
 
@@ -1218,7 +1271,7 @@ static bo_layout bo_plan(int64_t count, const int64_t *tot, int S) {
   L.part = o;
   o = al(o + (size_t)tot[BO_PART] * OZ_PREC * 4);
   L.list = o;
-  o = al(o + (size_t)tot[BO_ENT] * 16);
+  o = al(o + (size_t)tot[BO_ENT] * 20);  // (problem, i, j) records, then the full-range queue
   L.total = o;
   return L;
 }
@@ -1258,12 +1311,14 @@ static void bo_launch(int colmajor, int64_t count, const qblas_gemm_problem *P, 
   k_bo_gemm<S><<<(unsigned)tot[BO_TILES], 256, 0, st>>>(P, count, off, colmajor, sA, sB, part);
   k_bo_finish<<<(unsigned)((tot[BO_ENT] + 255) / 256), 256, 0, st>>>(P, count, off, colmajor, S, tot[BO_ENT], part,
                                                                        eA, fA, eB, fB, hdr, list);
-  k_bo_repair<<<OZ_REPAIR_BLOCKS, OZR_BS, 0, st>>>(P, count, off, colmajor, eA, eB, hdr, list);
+  uint32_t *wq = list + 4 * tot[BO_ENT];
+  k_bo_repair<<<OZ_REPAIR_BLOCKS, OZR_BS, 0, st>>>(P, count, off, colmajor, eA, eB, hdr, list, wq);
+  k_bo_wide<<<OZ_WIDE_BLOCKS, 256, 0, st>>>(P, colmajor, hdr, list, wq);
 }
 
 // After launch_gemm_ozaki_batched_scan on the same workspace: tot = the totals it wrote (read back
-// by the caller), ws at least gemm_ozaki_batched_ws(count, tot, S) bytes.  Header word 0 of ws is
-// the status word (bit 0: a repaired entry fell outside the exact windows and was set to NaN).
+// by the caller), ws at least gemm_ozaki_batched_ws(count, tot, S) bytes.  Header words of ws: 0
+// status, 1 entries queued for the exact repair, 2 entries of them on the full-range path.
 hipError_t launch_gemm_ozaki_batched(int colmajor, int64_t count, const qblas_gemm_problem *P, int S,
                                      const int64_t *tot, void *wsv, hipStream_t st) {
   if (tot[BO_ENT] == 0) return hipSuccess;
@@ -2104,7 +2159,7 @@ static crt_layout crt_plan(int64_t m, int64_t n, int64_t k, int nq) {
   L.xrec = o;  // records of X for general alpha, beta (12 words per entry at most)
   o = al(o + (size_t)m * n * 48);
   L.list = o;
-  o = al(o + (size_t)m * n * 8);
+  o = al(o + (size_t)m * n * 12);  // (i, j) pairs, then the full-range queue
   L.bt = o;
   o = al(o + (size_t)n * k * 16);
   L.total = o;
@@ -2193,8 +2248,8 @@ hipError_t launch_gemm_crt(int colmajor, int64_t m, int64_t n, int64_t k, int t,
   }
 #undef QB_CRT_FIN
   if (ev) (void)hipEventRecord(ev[3], st);
-  return oz_repair_queue(n, k, A, asr, asc, B, bsr, bsc, eA, eB, alpha, beta, C, csr, csc, hdr,
-                         (const uint32_t *)(ws + L.list), (q128 *)(ws + L.bt), st, ev);
+  return oz_repair_queue(m, n, k, A, asr, asc, B, bsr, bsc, eA, eB, alpha, beta, C, csr, csc, hdr,
+                         (uint32_t *)(ws + L.list), (q128 *)(ws + L.bt), st, ev);
 }
 
 }  // namespace qbk

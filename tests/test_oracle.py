@@ -247,6 +247,30 @@ def test_dot_thread_chunks(O):
     assert val(O.to_pair(O.dot(1001, x, 1, y, 1, 1))) == 0
 
 
+def test_dot_strided_thread_chunks(O):
+    # level1.hpp:93-120: a strided dot with n >= 500 cuts n into T chunks of n/T elements (the
+    # last takes the remainder), each ONE serial FMA chain from +0, the partials added from +0 in
+    # chunk order.  n = 1001, T = 3: chunks [0, 333), [333, 666), [666, 1001).  y = 1.
+    #   chunk 0: x[0] = B                                   -> p0 = B
+    #   chunk 1: B, 1, -B, 1 at 333..336: B, B+1 -> B (tie), 0, 1   -> p1 = 1
+    #   chunk 2: 1 at 666 and 1 at 1000 (the remainder)     -> p2 = 2
+    #   ((0 + B) + 1) -> B (tie), + 2 -> B + 2 (exact).
+    # Each plausible slip gives another value: two lanes per chunk (p1 = 2) -> B + 4; one chain
+    # over all n -> B; partials added in reverse -> (2 + 1) + B -> B + 4 (tie to even); the
+    # remainder on the first chunk -> 2B + -(B - 2) + 1 -> B + 4; the remainder dropped -> B.
+    n, T = 1001, 3
+    v = [0] * n
+    v[0], v[333], v[334], v[335], v[336], v[666], v[1000] = B113, B113, 1, -B113, 1, 1, 1
+    xs = [0] * (2 * n)
+    xs[::2] = v
+    ys = [7] * (3 * n)          # values off the stride would show up
+    ys[::3] = [1] * n
+    x, y = qa(O, xs), qa(O, ys)
+    assert val(O.to_pair(O.dot(n, x, 2, y, 3, T))) == B113 + 2
+    # T = 1: one chain over all n: B, 2B, 2B + 1 -> 2B, B, B + 1 -> B (tie), ... -> B
+    assert val(O.to_pair(O.dot(n, x, 2, y, 3, 1))) == B113
+
+
 def test_gemv_row_vs_col_order(O):
     # level2.hpp:15-50 (row) and :53-82 (col) on the same mathematical A = [[B, 1, -B, 1]]:
     #   RowMajor: 2-lane chain over the row -> 2; y = fma(1, 2, mul(0, y)) = 2.
