@@ -212,6 +212,19 @@ int qblas_hip_gemm(int mode, int layout, int64_t m, int64_t n, int64_t k, const 
                    int64_t lda, const void *d_B, int64_t ldb, const void *beta, void *d_C, int64_t ldc,
                    void *stream);
 
+/* ---- Allocation-free dot (new; no reference counterpart) ------------------------------------
+   For callers that cannot allocate device memory (code under HIP graph capture, tight serving
+   loops, harnesses that forbid hipMalloc): qblas_hip_dot's computation (modes REFERENCE, EXACT and
+   DD, same results bit for bit) with its scratch in a static device array of
+   QBLAS_NOALLOC_SCRATCH_BYTES per GPU that the library holds until the process ends; it calls no
+   allocating routine.  The EXACT and DD forms fit any n; REFERENCE needs 32 T bytes (T <= 65536).
+   A call that needs more returns QBLAS_ERR_ARG.  Calls on one GPU share its array: calls that
+   overlap on one GPU (two streams or two threads) race and must be ordered by the caller.
+   qblas_hip_exact_status / qblas_hip_exact_wide_items do not report on these calls. */
+#define QBLAS_NOALLOC_SCRATCH_BYTES (2u << 20)
+int qblas_hip_dot_noalloc(int mode, int64_t n, const void *d_x, int64_t incx, const void *d_y, int64_t incy, int T,
+                          void *d_result, void *stream);
+
 /* ---- Batched problems (new; no reference counterpart) --------------------------------------
    Many independent problems of different sizes in one call: a device array of descriptors, each
    problem evaluated with the semantics of the single-problem routine in the given mode (REFERENCE:

@@ -1209,6 +1209,11 @@ bool is_trans_char(char c) { return c == 'T' || c == 't' || c == 'C' || c == 'c'
 
 }  // namespace
 
+namespace qbk {
+void record_error(const char *msg) { set_err_msg(msg); }
+void forget_exact_status() { t_exact_status = StatusRef(); }
+}  // namespace qbk
+
 // =============================================================================================
 // exported C ABI
 // =============================================================================================

@@ -131,4 +131,9 @@ hipError_t launch_gemm_ozaki_batched_scan(int colmajor, int64_t count, const qbl
 size_t gemm_ozaki_batched_ws(int64_t count, const int64_t *tot, int S);
 hipError_t launch_gemm_ozaki_batched(int colmajor, int64_t count, const qblas_gemm_problem *P, int S,
                                      const int64_t *tot, void *ws, hipStream_t st);
+
+// hooks into the C ABI's thread-local state (capi.hip): the last-error message, and forgetting the
+// EXACT-form status of earlier calls
+void record_error(const char *msg);
+void forget_exact_status();
 }  // namespace qbk
