@@ -33,12 +33,21 @@ Also reported:
                   the form synchronizes once per call to size its workspace).
   host_abi        the reference's C ABI with host buffers (pageable and pinned): the same qgemm,
                   qgemv RowMajor and qdot with their PCIe copies in and out, wall-clock per call.
-  ill_conditioned qgemm 8192 x 8192 x 16384 ColMajor whose entries cancel to ~1e-18 of the
-                  largest products: OZAKI (26 slices: the bound certifies all but entries within
-                  ~2^-12 of zero) and DD, with HBM in use.
+  ill_conditioned qgemm 8192 x 8192 x 16384 ColMajor whose entries cancel to ~1e-20 of the
+                  largest products: OZAKI (the library's automatic choice), DD and plain, with the
+                  call's HBM (operands + workspace) and the algorithmic HBM rate; CPU slab baseline.
+  config1_qgemv_1500  the reference's own CPU benchmark case (README.md:23, 1.6 GFLOP/s): plain
+                  qgemv RowMajor 1500 x 1500 in the REFERENCE form on the GPU (device-resident and
+                  through the C ABI with host buffers), the oracle's restatement on the host cores
+                  (oracle FMA and libquadmath fmaq), and a bit-for-bit check of both GPU results.
+  qgemm_forms[*].error  every form's max / median relative error on 4096 fixed sampled entries
+                  against the exact value and against the REFERENCE form's result; the plain form's
+                  samples checked bit for bit against the oracle's per-entry restatement.
   cpu_baseline    the oracle's restatement of the reference qgemm (level3.hpp blocking, soft
                   binary128 FMA, OpenMP over the host cores) on a 256-row slab of the same
-                  4096^3 problem, scaled to GFLOP/s (per-element cost is uniform).
+                  4096^3 problem, scaled to GFLOP/s (per-element cost is uniform); with the CPU model
+                  and a libquadmath-fmaq variant.  The batched legs get a serial loop over their
+                  first 256 problems, extrapolated.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-plain] [--size N]
 """
 import argparse
@@ -99,6 +108,25 @@ def events(torch):
     return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
 
+ERR_SAMPLES = 4096       # fixed sampled entries of C for the error statement of every qgemm form
+ERR_SEED = 2024
+
+
+def sample_index(N):
+    """The fixed sampled entries (i, j) of the N^3 qgemm (seeded), and their flat RowMajor indices."""
+    import numpy as np
+    rng = np.random.default_rng(ERR_SEED)
+    ii, jj = rng.integers(0, N, ERR_SAMPLES), rng.integers(0, N, ERR_SAMPLES)
+    return ii, jj, ii * N + jj
+
+
+def gather(torch, C, flat):
+    """binary128 entries C[flat] of a device (N*N, 2) int64 tensor, as a host (len, 2) uint64 array."""
+    import numpy as np
+    idx = torch.from_numpy(np.ascontiguousarray(flat, dtype=np.int64)).to(C.device)
+    return C.index_select(0, idx).cpu().numpy().view(np.uint64).copy()
+
+
 def bench_ozaki(Q, torch, dev, stream, N, steps, warmup, slices, method=0, crt_bits=0):
     sh = stream.cuda_stream
     A = torch.empty((N * N, 2), dtype=torch.int64, device=dev)
@@ -141,11 +169,12 @@ def bench_ozaki(Q, torch, dev, stream, N, steps, warmup, slices, method=0, crt_b
     Q.set_ozaki_crt_bits(0)
     ms = statistics.mean(step_ms)
     ph = {k: round(statistics.mean(p[k] for p in phases), 4) for k in phases[0]}
+    samples = gather(torch, C, sample_index(N)[2])
     del A, B, C
     torch.cuda.empty_cache()
     return {"ms": ms, "wall_s": wall, "method": "crt" if used == Q.OZAKI_CRT else "slices", "slices": S,
             "crt_bits": bits, "moduli": moduli, "repairs": int(repairs), "status": int(status), "phases_ms": ph,
-            "quad_GFLOPs": 2.0 * N ** 3 / (ms * 1e-3) / 1e9}
+            "quad_GFLOPs": 2.0 * N ** 3 / (ms * 1e-3) / 1e9, "samples": samples}
 
 
 def ozaki_form(oz):
@@ -154,7 +183,7 @@ def ozaki_form(oz):
         f.update(crt_bits=oz["crt_bits"], moduli=oz["moduli"])
     else:
         f["slices"] = oz["slices"]
-    f.update(repaired_entries=oz["repairs"], phases_ms=oz["phases_ms"])
+    f.update(repaired_entries=oz["repairs"], phases_ms=oz["phases_ms"], samples=oz["samples"])
     return f
 
 
@@ -178,7 +207,8 @@ def bench_form_once(Q, torch, dev, stream, N, mode):
     e_ev.record(stream)
     torch.cuda.synchronize()
     ms = s_ev.elapsed_time(e_ev)
-    return {"ms": round(ms, 3), "quad_GFLOPs": round(2.0 * N ** 3 / (ms * 1e-3) / 1e9, 2)}
+    samples = gather(torch, C, sample_index(N)[2])
+    return {"ms": round(ms, 3), "quad_GFLOPs": round(2.0 * N ** 3 / (ms * 1e-3) / 1e9, 2), "samples": samples}
 
 
 def bench_plain(Q, torch, dev, stream, N):
@@ -195,7 +225,71 @@ def bench_plain(Q, torch, dev, stream, N):
     e_ev.record(stream)
     torch.cuda.synchronize()
     ms = s_ev.elapsed_time(e_ev)
-    return {"ms": round(ms, 3), "quad_GFLOPs": round(2.0 * N ** 3 / (ms * 1e-3) / 1e9, 2)}
+    samples = gather(torch, C, sample_index(N)[2])
+    return {"ms": round(ms, 3), "quad_GFLOPs": round(2.0 * N ** 3 / (ms * 1e-3) / 1e9, 2), "samples": samples}
+
+
+def bench_config1(Q, torch, dev, stream, m, steps):
+    """The reference's own CPU benchmark case: plain qgemv RowMajor m x m, y := 1 A x + 0 y, entries
+    a + b 2^-60 (full mantissa), y0 = 0, in the REFERENCE form (the reference's order, bit-identical
+    to its result).  Device-resident calls timed back to back between events, and the same call
+    through the reference's C ABI with host buffers (quadblas_qgemv: PCIe copies in and out)."""
+    import numpy as np
+    sh = stream.cuda_stream
+    A = torch.empty((m * m, 2), dtype=torch.int64, device=dev)
+    x = torch.empty((m, 2), dtype=torch.int64, device=dev)
+    y = torch.zeros((m, 2), dtype=torch.int64, device=dev)
+    Q.dev_fill(Q.FILL_UNIFORM_FULL, 42, m * m, A.data_ptr(), stream=sh)
+    Q.dev_fill(Q.FILL_UNIFORM_FULL, 43, m, x.data_ptr(), stream=sh)
+
+    def call():
+        Q.dev_gemv(Q.MODE_REFERENCE, 0, m, m, 1.0, A.data_ptr(), m, x.data_ptr(), 1, 0.0, y.data_ptr(), 1, sh)
+
+    flops = 2.0 * m * m
+
+    def timed(mode):
+        def run():
+            Q.dev_gemv(mode, 0, m, m, 1.0, A.data_ptr(), m, x.data_ptr(), 1, 0.0, y.data_ptr(), 1, sh)
+        for _ in range(3):
+            run()
+        torch.cuda.synchronize()
+        s_ev, e_ev = events(torch)
+        s_ev.record(stream)
+        for _ in range(steps):
+            run()
+        e_ev.record(stream)
+        torch.cuda.synchronize()
+        ms_ = s_ev.elapsed_time(e_ev) / steps
+        return {"ms": round(ms_, 4), "quad_GFLOPs": round(flops / (ms_ * 1e-3) / 1e9, 2)}
+
+    other = {"exact": timed(Q.MODE_EXACT), "dd": timed(Q.MODE_DD)}
+    ms = timed(Q.MODE_REFERENCE)["ms"]
+    y.zero_()
+    call()
+    torch.cuda.synchronize()
+    y_dev = y.cpu().numpy().view(np.uint64).copy()
+    hA, hx = A.cpu().numpy().view(np.uint64).copy(), x.cpu().numpy().view(np.uint64).copy()
+    hy = np.zeros((m, 2), dtype=np.uint64)
+    saved = Q.get_mode()
+    Q.set_mode("reference")
+    Q.qgemv("R", "N", m, m, 1.0, hA, m, hx, 1, 0.0, hy, 1)
+    ts = []
+    for _ in range(7):
+        hy[:] = 0
+        t0 = time.perf_counter()
+        Q.qgemv("R", "N", m, m, 1.0, hA, m, hx, 1, 0.0, hy, 1)
+        ts.append(time.perf_counter() - t0)
+    Q.set_mode(saved)
+    host_ms = statistics.median(ts) * 1e3
+    del A, x, y
+    torch.cuda.empty_cache()
+    return {"workload": f"qgemv RowMajor {m}x{m}, alpha=1, beta=0, entries a + b 2^-60, REFERENCE order",
+            "device": {"ms": round(ms, 4), "quad_GFLOPs": round(flops / (ms * 1e-3) / 1e9, 2)},
+            "device_other_forms": other,
+            "host_abi": {"ms": round(host_ms, 3), "quad_GFLOPs": round(flops / (host_ms * 1e-3) / 1e9, 2),
+                         "pcie_bytes": 16.0 * (m * m + m + 2 * m)},
+            "reference_published_GFLOPs": 1.6, "reference_published_source": "README.md:23 (SLEEF, OpenMP CPU)",
+            "_check": (hA, hx, y_dev, hy)}
 
 
 def bench_memory(Q, torch, dev, stream, n, m, steps):
@@ -417,51 +511,60 @@ def bench_host_abi(Q, torch, dev, N, m, n, reps):
     return out
 
 
-def bench_ill_conditioned(Q, torch, dev, stream, M, N, K, slices, plain=True):
-    """qgemm M x N x K ColMajor, alpha = 1, beta = 0: rows of A carry +-1e20 u pairs along k and
-    the columns of B equal values on each pair (qblas_hip_fill_2d kinds 2 and 3), so the exact
-    entries are ~1e-18 of the largest products.  OZAKI form with the given slice count (enough
-    to certify every entry; the CRT method would need more than its 158 bits), the DD form and the
-    plain form (one call, no warm-up), one timed call each.  Only the OZAKI form meets 2^-104 here
-    (tests/test_gpu_ozaki_crt.py measures all three on sampled entries against the exact value)."""
+def bench_ill_conditioned(Q, torch, dev, M, N, K, plain=True):
+    """qgemm M x N x K ColMajor, alpha = 1, beta = 0: rows of A carry +-1e22 u pairs along k and
+    the columns of B equal values on each pair (qblas_hip_fill_2d kinds 4 and 3), so the exact
+    entries are ~1e-20 of the largest products.  OZAKI form with the library's automatic choice of
+    method and precision (the probe asks for more than the CRT method's 158 bits: the slice
+    method), the DD form and the plain form (one call, no warm-up), one timed call each, on a stream
+    of its own so that the OZAKI call's whole workspace is new (its size is what the call adds to
+    HBM in use).  Only the OZAKI form meets 2^-104 here (tests/test_gpu_ozaki_crt.py measures all
+    three on sampled entries against the exact value)."""
+    stream = torch.cuda.Stream()
     sh = stream.cuda_stream
     A = torch.empty((M * K, 2), dtype=torch.int64, device=dev)
     B = torch.empty((K * N, 2), dtype=torch.int64, device=dev)
     C = torch.zeros((M * N, 2), dtype=torch.int64, device=dev)
-    Q.dev_fill_2d(2, 5, M, K, M, 1, A.data_ptr(), stream=sh)
+    Q.dev_fill_2d(4, 5, M, K, M, 1, A.data_ptr(), stream=sh)
     Q.dev_fill_2d(3, 5, N, K, K, 0, B.data_ptr(), stream=sh)
+    torch.cuda.synchronize()
     flops = 2.0 * M * N * K
-    out = {"workload": f"qgemm {M}x{N}x{K} ColMajor alpha=1 beta=0, entries ~1e-18 of the largest products",
-           "quad_flops": flops}
+    operand_bytes = 16.0 * (M * K + K * N + M * N)
+    out = {"workload": f"qgemm {M}x{N}x{K} ColMajor alpha=1 beta=0, entries ~1e-20 of the largest products "
+                       f"(+-1e22 u pairs)", "quad_flops": flops, "operand_bytes": operand_bytes}
     for form in ("ozaki", "dd") + (("reference",) if plain else ()):
         mode = Q.MODES[form]
         if form == "ozaki":
-            Q.set_ozaki_slices(slices)
             Q.set_phase_timing(True)
         call = lambda: Q.dev_gemm(mode, 1, M, N, K, 1.0, A.data_ptr(), M, B.data_ptr(), K, 0.0, C.data_ptr(), M, sh)
+        free0, total = torch.cuda.mem_get_info()
         if form != "reference":
             call()
         torch.cuda.synchronize()
+        free1, _ = torch.cuda.mem_get_info()
         s_ev, e_ev = events(torch)
         s_ev.record(stream)
         call()
         e_ev.record(stream)
         torch.cuda.synchronize()
         ms = s_ev.elapsed_time(e_ev)
-        free, total = torch.cuda.mem_get_info()
         r = {"ms": round(ms, 2), "quad_GFLOPs": round(flops / (ms * 1e-3) / 1e9, 1),
-             "hbm_in_use_GB": round((total - free) / 1e9, 1)}
+             "hbm_GBs_algorithmic": round(operand_bytes / (ms * 1e-3) / 1e9, 1)}
+        if form != "reference":
+            ws = max(0, free0 - free1)
+            r["workspace_GB"] = round(ws / 1e9, 2)
+            r["hbm_call_GB"] = round((operand_bytes + ws) / 1e9, 2)
         if form == "ozaki":
             S = Q.ozaki_last_slices()
             ph = Q.phase_times()
             Q.set_phase_timing(False)
+            r["method"] = "crt" if Q.ozaki_last_method() == Q.OZAKI_CRT else "slices"
             r["slices"] = S
             r["repaired_entries"] = int(Q.dev_ozaki_repairs(sh))
             r["phases_ms"] = {k: round(v, 3) for k, v in ph.items()}
             ops = 2.0 * M * N * K * (S * (S + 1) // 2)
             r["mfma"] = {"int8_ops": ops, "TOP/s": round(ops / (ph["products"] * 1e-3) / 1e12, 1),
                          "frac_of_peak": round(ops / (ph["products"] * 1e-3) / 1e12 / INT8_PEAK_TOPS, 4)}
-            Q.set_ozaki_slices(0)
         out["plain" if form == "reference" else form] = r
     del A, B, C
     torch.cuda.empty_cache()
@@ -527,34 +630,61 @@ def load_traffic():
         return None
 
 
-def cpu_baseline_gemm(N, rows):
+def cpu_info():
+    """Host CPU model and the threads the OpenMP restatements use (OMP_NUM_THREADS, else all)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    return model, threads
+
+
+def cpu_baseline_gemm(N, rows, fmaq_rows):
     """The oracle's reference-order qgemm (level3.hpp blocking, kc = 126, soft binary128 FMA, OpenMP
-    over rows x columns) on rows x N of C for the same N^3 problem's distribution."""
+    over rows x columns) on rows x N of C for the same N^3 problem's distribution; also with
+    libquadmath's fmaq as the FMA (a smaller slab)."""
     import oracle as O
-    A = O.fill(1, 11, rows * N, -30, 30)
+    model, threads = cpu_info()
     B = O.fill(1, 12, N * N, -30, 30)
-    C = O.fill(0, 13, rows * N)
-    C[:] = 0
-    t0 = time.perf_counter()
-    O.gemm(0, rows, N, N, 1.0, A, N, B, N, 0.0, C, N)
-    t = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
-    return {"value": round(2.0 * rows * N * N / t / 1e9, 4), "unit": "quad-equiv GFLOP/s", "cores": cores,
-            "kind": "port",
-            "sample": f"reference-order qgemm on a {rows} x {N} slab of C ({rows}x{N}x{N} FMAs, same entry "
-                      f"distribution), oracle soft-binary128 FMA, OpenMP; {t:.1f} s",
-            "seconds": round(t, 2), "extrapolated_full_problem_s": round(t * N / rows, 1)}
+    out = None
+    for variant, r in (("oracle soft-binary128 FMA", rows), ("libquadmath fmaq", fmaq_rows)):
+        A = O.fill(1, 11, r * N, -30, 30)
+        C = O.fill(0, 13, r * N)
+        C[:] = 0
+        t0 = time.perf_counter()
+        if variant.startswith("libquadmath"):
+            with O.use_fmaq():
+                O.gemm(0, r, N, N, 1.0, A, N, B, N, 0.0, C, N)
+        else:
+            O.gemm(0, r, N, N, 1.0, A, N, B, N, 0.0, C, N)
+        t = time.perf_counter() - t0
+        v = {"value": round(2.0 * r * N * N / t / 1e9, 4), "unit": "quad-equiv GFLOP/s", "cores": threads,
+             "kind": "port", "cpu_model": model,
+             "sample": f"reference-order qgemm on a {r} x {N} slab of C ({r}x{N}x{N} FMAs, same entry distribution), "
+                       f"{variant}, OpenMP; {t:.1f} s, extrapolated to the full problem: {t * N / r:.0f} s",
+             "seconds": round(t, 2), "extrapolated_full_problem_s": round(t * N / r, 1)}
+        if out is None:
+            out = v
+        else:
+            out["fmaq_variant"] = {k: v[k] for k in ("value", "sample", "seconds", "extrapolated_full_problem_s")}
+    return out
 
 
 def cpu_baseline_memory(n, m):
     """The oracle's reference-order qdot (T chunks) + qgemv RowMajor + ColMajor, one step."""
     import oracle as O
+    model, T = cpu_info()
     x = O.fill(0, 1, n)
     y = O.fill(0, 2, n)
     A = O.fill(0, 3, m * m)
     xv = O.fill(0, 4, m)
     yv0 = O.fill(0, 5, m)
-    T = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
     t0 = time.perf_counter()
     O.dot(n, x, 1, y, 1, T)
     yv = yv0.copy()
@@ -563,9 +693,170 @@ def cpu_baseline_memory(n, m):
     O.gemv(1, m, m, 1.5, A, m, xv, 1, -0.5, yv, 1)
     t = time.perf_counter() - t0
     step_bytes = 32.0 * n + 2 * 16.0 * (m * m + m + 2 * m)
-    return {"value": round(step_bytes / t / 1e9, 3), "unit": "GB/s", "cores": T, "kind": "port",
+    return {"value": round(step_bytes / t / 1e9, 3), "unit": "GB/s", "cores": T, "kind": "port", "cpu_model": model,
             "sample": "one step: qdot 2^24 + qgemv 8192^2 RowMajor + ColMajor, reference order, oracle soft "
                       "binary128 FMA, OpenMP", "seconds": round(t, 3)}
+
+
+def cpu_config1(c1):
+    """The config-1 qgemv on the host cores: the oracle's restatement of level2.hpp:15-50 (OpenMP over
+    rows), two warm-ups then the median of 7 runs (the reference's perf_compare.cpp:32-46 protocol),
+    with the oracle's FMA and with libquadmath fmaq; and the check that the GPU result (device call
+    and C ABI call) equals the restatement bit for bit."""
+    import numpy as np
+    import oracle as O
+    model, threads = cpu_info()
+    hA, hx, y_dev, y_host = c1.pop("_check")
+    m = hx.shape[0]
+    res = {}
+    for variant in ("oracle", "fmaq"):
+        ts = []
+        want = None
+        for rep in range(9):
+            y = np.zeros((m, 2), dtype=np.uint64)
+            t0 = time.perf_counter()
+            if variant == "fmaq":
+                with O.use_fmaq():
+                    O.gemv(0, m, m, 1.0, hA, m, hx, 1, 0.0, y, 1)
+            else:
+                O.gemv(0, m, m, 1.0, hA, m, hx, 1, 0.0, y, 1)
+            if rep >= 2:
+                ts.append(time.perf_counter() - t0)
+            want = y
+        ms = statistics.median(ts) * 1e3
+        res[variant] = {"ms": round(ms, 3), "quad_GFLOPs": round(2.0 * m * m / (ms * 1e-3) / 1e9, 3), "y": want}
+    want = res["oracle"].pop("y")
+    res["fmaq"].pop("y")
+    c1["cpu_baseline"] = {"value": res["oracle"]["quad_GFLOPs"], "unit": "quad-equiv GFLOP/s", "cores": threads,
+                          "kind": "port", "cpu_model": model,
+                          "sample": f"the whole qgemv {m}x{m}, median of 7 after 2 warm-ups, OpenMP over rows",
+                          "ms": res["oracle"]["ms"], "fmaq_variant": res["fmaq"]}
+    c1["bit_exact_vs_restatement"] = {"device": bool(np.array_equal(y_dev, want)),
+                                      "host_abi": bool(np.array_equal(y_host, want))}
+    return c1
+
+
+def q_fraction(w):
+    """Exact value of a finite binary128 given as (lo, hi) uint64 words."""
+    from fractions import Fraction
+    lo, hi = int(w[0]), int(w[1])
+    be, f = (hi >> 48) & 0x7FFF, ((hi & ((1 << 48) - 1)) << 64) | lo
+    v = Fraction(f if be == 0 else f | (1 << 112)) * Fraction(2) ** ((1 if be == 0 else be) - 16383 - 112)
+    return -v if hi >> 63 else v
+
+
+def cpu_error_checks(N, forms):
+    """Error statement of every qgemm form on the fixed 4096 sampled entries: relative error
+    against the exact value (the oracle's exact accumulator, rounded once) and against the
+    REFERENCE form's (plain) result at the same entries; the plain form's samples are also checked
+    bit for bit against the oracle's per-entry restatement of the reference's blocked order
+    (level3.hpp:215-336, kc = 126: a 1 x 1 x N problem takes the same path per entry)."""
+    import numpy as np
+    import oracle as O
+    ii, jj, _ = sample_index(N)
+    A = O.fill(1, 11, N * N, -30, 30)
+    B = O.fill(1, 12, N * N, -30, 30)
+    ref = forms.get("plain", {}).get("samples")
+    for name, f in forms.items():
+        samp = f.pop("samples", None)
+        if samp is None:
+            continue
+        _, rel = O.exact_gemm_samples(ii, jj, N, 1.0, A, (N, 1), B, (N, 1), 0.0, None, (N, 1), cand=samp)
+        e = {"samples": int(len(ii)), "max_rel_vs_exact": float(rel.max()), "median_rel_vs_exact": float(np.median(rel)),
+             "log2_max_vs_exact": round(float(np.log2(max(rel.max(), 1e-300))), 2)}
+        if ref is not None:
+            rr = []
+            for a, b in zip(samp, ref):
+                fa, fb = q_fraction(a), q_fraction(b)
+                rr.append(float(abs(fa - fb) / abs(fb)) if fb != 0 else (0.0 if fa == 0 else float("inf")))
+            e["max_rel_vs_reference"] = max(rr)
+            e["median_rel_vs_reference"] = float(np.median(rr))
+        f["error"] = e
+    if ref is not None:
+        mism = 0
+        c = np.zeros((1, 2), dtype=np.uint64)
+        for s_, (i, j) in enumerate(zip(ii, jj)):
+            c[:] = 0
+            O.gemm(0, 1, 1, N, 1.0, np.ascontiguousarray(A[i * N:(i + 1) * N]), N,
+                   np.ascontiguousarray(B[j::N]), 1, 0.0, c, 1)
+            mism += int(not np.array_equal(c[0], ref[s_]))
+        forms["plain"]["error"]["restatement_mismatches"] = mism
+
+
+def cpu_baseline_batched(gemm_count, gemv_count, sample_gemm, sample_gemv, seed=77):
+    """A serial loop over the first problems of the same batches (the reference has no batched
+    routine: one restatement call per problem, OpenMP inside, not across problems -- the reference's
+    dot breaks inside an outer parallel region), extrapolated to the whole batch by its flops or bytes."""
+    import numpy as np
+    import oracle as O
+    model, threads = cpu_info()
+    rng = np.random.default_rng(seed)
+    dims = rng.integers(8, 257, size=(gemm_count, 3))
+    mn = rng.integers(32, 1025, size=(gemv_count, 2))
+    out = {}
+    # gemm batch: operands of problem p at offs[p] in the seed's kind-0 sequence (A | B | C)
+    sizes = dims[:, 0] * dims[:, 2] + dims[:, 2] * dims[:, 1] + dims[:, 0] * dims[:, 1]
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    P = min(sample_gemm, gemm_count)
+    pool = O.fill(0, seed, int(offs[P]))
+    ab = O.fill(0, seed + 1, 2 * P)
+    t0 = time.perf_counter()
+    for p in range(P):
+        m, n, k = (int(v) for v in dims[p])
+        o = int(offs[p])
+        Cp = pool[o + m * k + k * n:o + m * k + k * n + m * n]
+        O.gemm(0, m, n, k, ab[2 * p], pool[o:o + m * k], k, pool[o + m * k:o + m * k + k * n], n, ab[2 * p + 1], Cp, n)
+    t = time.perf_counter() - t0
+    fl_s = float(np.sum(2.0 * dims[:P, 0] * dims[:P, 1] * dims[:P, 2]))
+    fl = float(np.sum(2.0 * dims[:, 0] * dims[:, 1] * dims[:, 2]))
+    out["qgemm_reference"] = {"value": round(gemm_count / (t * fl / fl_s), 1), "unit": "problems/s",
+                              "quad_GFLOPs": round(fl_s / t / 1e9, 3), "cores": threads, "kind": "port",
+                              "cpu_model": model, "seconds": round(t, 2),
+                              "sample": f"serial loop over the first {P} of {gemm_count} problems (reference order), "
+                                        f"extrapolated by flops"}
+    del pool
+    asz = mn[:, 0] * mn[:, 1]
+    offs = np.concatenate([[0], np.cumsum(asz + mn[:, 1] + mn[:, 0])])
+    P = min(sample_gemv, gemv_count)
+    pool = O.fill(0, seed + 2, int(offs[P]))
+    ab = O.fill(0, seed + 3, 2 * P)
+    t0 = time.perf_counter()
+    for p in range(P):
+        m, n = int(mn[p, 0]), int(mn[p, 1])
+        o = int(offs[p])
+        y = pool[o + m * n + n:o + m * n + n + m].copy()
+        O.gemv(0, m, n, ab[2 * p], pool[o:o + m * n], n, pool[o + m * n:o + m * n + n], 1, ab[2 * p + 1], y, 1)
+    t = time.perf_counter() - t0
+    by_s = float(np.sum(16.0 * (asz[:P] + mn[:P, 1] + 2 * mn[:P, 0])))
+    by = float(np.sum(16.0 * (asz + mn[:, 1] + 2 * mn[:, 0])))
+    out["qgemv_reference"] = {"value": round(gemv_count / (t * by / by_s), 1), "unit": "problems/s",
+                              "GB/s": round(by_s / t / 1e9, 3), "cores": threads, "kind": "port", "cpu_model": model,
+                              "seconds": round(t, 2),
+                              "sample": f"serial loop over the first {P} of {gemv_count} problems (reference order), "
+                                        f"extrapolated by bytes"}
+    return out
+
+
+def cpu_baseline_ill(M, N, K, rows, cols):
+    """The ill-conditioned qgemm's restatement (the layout-correct ColMajor blocked order; the
+    reference's own ColMajor blocked path writes C row-major) on a rows x cols block of C,
+    extrapolated to M x N."""
+    import numpy as np
+    import oracle as O
+    model, threads = cpu_info()
+    Asub = np.zeros((rows * K, 2), dtype=np.uint64)
+    for r in range(rows):                       # A(r, c) = value number r K + c, stored at r + c rows
+        Asub[r::rows] = O.fill(4, 5, K, start=r * K)
+    Bsub = O.fill_pair_y(5, cols * K)           # B(l, j) = value number j K + l at l + j K
+    C = np.zeros((rows * cols, 2), dtype=np.uint64)
+    t0 = time.perf_counter()
+    O.gemm(1, rows, cols, K, 1.0, Asub, rows, Bsub, K, 0.0, C, rows)
+    t = time.perf_counter() - t0
+    scale = (M * N) / (rows * cols)
+    return {"value": round(2.0 * rows * cols * K / t / 1e9, 4), "unit": "quad-equiv GFLOP/s", "cores": threads,
+            "kind": "port", "cpu_model": model, "seconds": round(t, 2), "extrapolated_full_problem_s": round(t * scale, 1),
+            "sample": f"reference-order (layout-correct ColMajor) qgemm on a {rows} x {cols} block of C "
+                      f"({rows}x{cols}x{K} FMAs), oracle soft FMA, OpenMP, extrapolated x{scale:.0f}"}
 
 
 def main():
@@ -634,6 +925,7 @@ def main():
             if key in forms:
                 forms[key]["speedup_vs_plain"] = round(forms["plain"]["ms"] / forms[key]["ms"], 1)
     torch.cuda.empty_cache()
+    config1 = bench_config1(Q, torch, dev, stream, 1500, 200)
     # the host-buffer calls run before the legs that allocate tens of GB: on a box where 74 GB had
     # been allocated and freed, the copy of C in no longer overlapped the qgemm's kernels
     host_abi = None
@@ -641,14 +933,26 @@ def main():
         host_abi = bench_host_abi(Q, torch, dev, N, args.m, args.n_dot, 3)
     ill = None
     if not args.no_ill:
-        ill = bench_ill_conditioned(Q, torch, dev, stream, 8192, 8192, 16384, 26, plain=not args.no_ill_plain)
+        ill = bench_ill_conditioned(Q, torch, dev, 8192, 8192, 16384, plain=not args.no_ill_plain)
     batched = None
     if not args.no_batched:
         batched = bench_batched(Q, torch, dev, stream, max(args.steps, 3), args.batch_gemm, args.batch_gemv)
     cpu = cpu_mem = None
     if not args.no_cpu:
-        cpu = cpu_baseline_gemm(N, args.cpu_rows)
+        # the CPU legs: baselines on the host cores, and the oracle as the checker of the GPU
+        # results (sampled error of every qgemm form, config 1 bit for bit)
+        cpu_error_checks(N, forms)
+        cpu = cpu_baseline_gemm(N, args.cpu_rows, args.cpu_rows // 16)
         cpu_mem = cpu_baseline_memory(args.n_dot, args.m)
+        cpu_config1(config1)
+        if batched is not None:
+            batched["cpu_baseline"] = cpu_baseline_batched(args.batch_gemm, args.batch_gemv, 256, 256)
+        if ill is not None:
+            ill["cpu_baseline"] = cpu_baseline_ill(8192, 8192, 16384, 64, 1024)
+    else:
+        config1.pop("_check", None)
+    for f in forms.values():
+        f.pop("samples", None)
 
     line = {
         "metric": METRIC, "value": round(oz["quad_GFLOPs"], 2), "unit": "quad-equiv GFLOP/s", "n_gpus": 1,
@@ -661,7 +965,8 @@ def main():
                                                               if oz["method"] == "crt" else f" ({S} slices)"),
                    "error_bound": "each entry <= 2^-105.9 rel (certified) or correctly rounded (repaired)"},
         "roofline": roofline, "cpu_baseline": cpu, "qgemm_forms": forms, "memory_bound": memory,
-        "cpu_baseline_memory_bound": cpu_mem, "batched": batched, "ill_conditioned": ill, "host_abi": host_abi, "ozaki_status": oz["status"], "wall_s": round(oz["wall_s"], 3),
+        "cpu_baseline_memory_bound": cpu_mem, "config1_qgemv_1500": config1, "batched": batched,
+        "ill_conditioned": ill, "host_abi": host_abi, "ozaki_status": oz["status"], "wall_s": round(oz["wall_s"], 3),
     }
     print(json.dumps(line))
 

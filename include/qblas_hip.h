@@ -293,12 +293,14 @@ int qblas_hip_gemm_batched_select(int mode, int layout, int64_t count, int idx);
    kind 0: a + b*2^-60 summed in binary128, a, b uniform [-1, 1) doubles (full 113-bit mantissa);
    kind 1: random sign, exponent uniform in [lo_exp, hi_exp), 112 random fraction bits;
    kind 2: the qdot cancellation variant's x: +-(1e20*u) pairs on slots 0,1 of every 8, kind 0
-           elsewhere; kind 3: the matching y (equal values on each pair). */
+           elsewhere; kind 3: the matching y (equal values on each pair); kind 4: kind 2 with
+           +-(1e22*u) pairs. */
 int qblas_hip_fill(int kind, uint64_t seed, int64_t n, int lo_exp, int hi_exp, void *d_out, void *stream);
 /* The same generators over a rows x cols matrix: value number r*cols + c is stored at element
    (r, c) (layout 0: r*ld + c, 1: r + c*ld).  With kind 2 for A (rows = m, cols = k) and kind 3 for
-   B^T (rows = n, cols = k) every product pair A_il B_lj, l % 8 in {0, 1}, cancels exactly: the
-   ill-conditioned qgemm workload (exact sums ~1e-18 of the largest products). */
+   B^T (rows = n, cols = k) every product pair A_il B_lj, l % 8 in {0, 1}, cancels exactly (exact
+   sums ~1e-18 of the largest products); kind 4 instead of 2 gives ~1e-20: the ill-conditioned
+   qgemm workload. */
 int qblas_hip_fill_2d(int kind, uint64_t seed, int64_t rows, int64_t cols, int64_t ld, int layout, int lo_exp,
                       int hi_exp, void *d_out, void *stream);
 

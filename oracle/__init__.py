@@ -265,6 +265,7 @@ def abs_array(a):
 KIND_UNIFORM_FULL = 0   # a + b*2^-60, a, b uniform [-1, 1) doubles, summed in binary128
 KIND_LOG_UNIFORM = 1    # sign +-, exponent uniform in [lo, hi), 112 random fraction bits
 KIND_CANCEL_PAIRS = 2   # +-(1e20*u) pairs on every 8-block's first two slots, kind 0 elsewhere
+KIND_CANCEL_PAIRS_22 = 4  # the same with +-(1e22*u) pairs
 
 
 def fill(kind, seed, n, lo_exp=-30, hi_exp=30, start=0):

@@ -677,7 +677,9 @@ static double u11(uint64_t r) { return (double)(r >> 11) * 0x1p-52 - 1.0; }
    v_i = (1e20 * u) and v_{i+1} = -(1e20 * u) with u = 1 + (rand(seed, i) >> 12) * 2^-52
    (u in [1, 2), exact in double), formed by an exact binary128 multiply (53 x 47 bits); entries with
    i % 8 >= 2 are instead kind-0 values (the O(1) residue).  The matching y vector repeats
-   y_{i+1} = y_i on the pairs so that the pair products cancel exactly. */
+   y_{i+1} = y_i on the pairs so that the pair products cancel exactly.
+   kind 4: kind 2 with 1e22 in place of 1e20 (sums that cancel to ~1e-20 of their largest
+   products: the ill-conditioned qgemm workload). */
 void or_fill_range(int kind, uint64_t seed, size_t start, size_t n, int lo_exp, int hi_exp, q128 *out) {
     #pragma omp parallel for schedule(static)
     for (size_t t = 0; t < n; ++t) {
@@ -696,7 +698,7 @@ void or_fill_range(int kind, uint64_t seed, size_t start, size_t n, int lo_exp, 
             size_t p = i & ~(size_t)1;
             if ((p % 8) < 2) {
                 double u = 1.0 + (double)(or_rand64(seed, p) >> 12) * 0x1p-52;
-                __float128 v = (__float128)u * (__float128)1e20;
+                __float128 v = (__float128)u * (__float128)(kind == 4 ? 1e22 : 1e20);
                 *o = f2q((i & 1) ? -v : v);
             } else {
                 double a = u11(or_rand64(seed, 2 * i)), b = u11(or_rand64(seed, 2 * i + 1));

@@ -404,6 +404,7 @@ FILL_UNIFORM_FULL = 0
 FILL_LOG_UNIFORM = 1
 FILL_CANCEL_X = 2
 FILL_CANCEL_Y = 3
+FILL_CANCEL_X22 = 4   # FILL_CANCEL_X with +-(1e22*u) pairs
 
 
 def dev_fill(kind, seed, n, out, lo_exp=-30, hi_exp=30, stream=0):

@@ -1404,7 +1404,7 @@ int qblas_hip_gemm_batched_select(int mode, int layout, int64_t count, int idx) 
   return QBLAS_OK;
 }
 int qblas_hip_fill(int kind, uint64_t seed, int64_t n, int lo_exp, int hi_exp, void *d_out, void *stream) {
-  if (n < 0 || kind < 0 || kind > 3 || (kind == 1 && hi_exp <= lo_exp) || !aligned16(d_out)) {
+  if (n < 0 || kind < 0 || kind > 4 || (kind == 1 && hi_exp <= lo_exp) || !aligned16(d_out)) {
     set_err_msg("qblas_hip_fill: bad arguments");
     return QBLAS_ERR_ARG;
   }
@@ -1418,7 +1418,7 @@ int qblas_hip_fill(int kind, uint64_t seed, int64_t n, int lo_exp, int hi_exp, v
 
 int qblas_hip_fill_2d(int kind, uint64_t seed, int64_t rows, int64_t cols, int64_t ld, int layout, int lo_exp,
                       int hi_exp, void *d_out, void *stream) {
-  if (rows < 0 || cols < 0 || kind < 0 || kind > 3 || (kind == 1 && hi_exp <= lo_exp) || !aligned16(d_out) ||
+  if (rows < 0 || cols < 0 || kind < 0 || kind > 4 || (kind == 1 && hi_exp <= lo_exp) || !aligned16(d_out) ||
       ld < (layout ? rows : cols)) {
     set_err_msg("qblas_hip_fill_2d: bad arguments");
     return QBLAS_ERR_ARG;

@@ -210,7 +210,7 @@ __device__ __forceinline__ q128 full_uniform(uint64_t seed, uint64_t i) {
   return add_packed(from_double(a), mul_packed(from_double(b), from_double(0x1p-60)));
 }
 
-// value number i of generator kind (0..3)
+// value number i of generator kind (0..4)
 __device__ q128 gen_value(int kind, uint64_t seed, int64_t i, int lo_exp, int hi_exp) {
   q128 v;
   if (kind == 0) {
@@ -221,11 +221,11 @@ __device__ q128 gen_value(int kind, uint64_t seed, int64_t i, int lo_exp, int hi
     const uint32_t span = (uint32_t)(hi_exp - lo_exp);
     const int e = lo_exp + (int)((uint32_t)r0 % span);
     v = q_make(r2, (r0 & (1ull << 63)) | ((uint64_t)(e + QBIAS) << 48) | (r1 >> 16));
-  } else if (kind == 2) {
+  } else if (kind == 2 || kind == 4) {
     const int64_t p = i & ~(int64_t)1;
     if ((p % 8) < 2) {
       const double u = 1.0 + (double)(rand64(seed, (uint64_t)p) >> 12) * 0x1p-52;
-      v = mul_packed(from_double(u), from_double(1e20));
+      v = mul_packed(from_double(u), from_double(kind == 4 ? 1e22 : 1e20));
       if (i & 1) v.hi ^= 1ull << 63;
     } else {
       v = full_uniform(seed, (uint64_t)i);

@@ -251,15 +251,15 @@ def test_repair_reads_b_through_transposed_copy(O, Q, cuda, method):
 
 def test_ill_conditioned_full_size_forms(O, Q, cuda):
     # The ill-conditioned workload at full size: qgemm 8192 x 8192 x 16384 ColMajor, alpha = 1,
-    # beta = 0, entries ~1e-18 of the largest products.  The OZAKI form (automatic: the probe wants
-    # more than the CRT method's 158 bits, so the slice method) meets 2^-104 on 128 sampled entries;
-    # the DD and plain forms are run on the same data and their errors against the exact value
-    # recorded.
+    # beta = 0, entries ~1e-20 of the largest products (+-1e22 u pairs, generator kind 4).  The
+    # OZAKI form (automatic: the probe wants more than the CRT method's 158 bits, so the slice
+    # method) meets 2^-104 on 128 sampled entries; the DD and plain forms are run on the same data
+    # and their errors against the exact value recorded.
     m, n, k = 8192, 8192, 16384
     dA = cuda.empty((m * k, 2), dtype=cuda.int64, device="cuda:0")
     dB = cuda.empty((k * n, 2), dtype=cuda.int64, device="cuda:0")
     dC = cuda.zeros((m * n, 2), dtype=cuda.int64, device="cuda:0")
-    Q.dev_fill_2d(2, 5, m, k, m, 1, dA.data_ptr())
+    Q.dev_fill_2d(4, 5, m, k, m, 1, dA.data_ptr())
     Q.dev_fill_2d(3, 5, n, k, k, 0, dB.data_ptr())
     cuda.cuda.synchronize()
     A, B = from_dev(dA), from_dev(dB)

@@ -23,7 +23,7 @@ def _say(msg):
 
 
 def test_fill_matches_oracle(O, Q, cuda):
-    for kind, seed, n in ((0, 1, 10007), (1, 2, 4096), (2, 3, 999), (3, 3, 999)):
+    for kind, seed, n in ((0, 1, 10007), (1, 2, 4096), (2, 3, 999), (3, 3, 999), (4, 3, 999)):
         _say(f"fill kind={kind} seed={seed} n={n}")
         d = dev_empty(n)
         Q.dev_fill(kind, seed, n, d.data_ptr(), -30, 30)
@@ -199,3 +199,30 @@ def test_degenerate_sizes_leave_outputs(O, Q, cuda):
     Q.qgemv("R", "N", 5, 0, 1.0, A, 5, A, 1, 2.0, y, 1)
     Q.qgemm("R", "N", "N", 5, 5, 0, 1.0, A, 5, A, 5, 2.0, A.copy(), 5)
     assert np.array_equal(y, y0)
+
+
+def test_config1_gemv_rowmajor_1500(O, Q, cuda):
+    # The reference's own CPU benchmark case (README.md:23): plain qgemv RowMajor 1500 x 1500,
+    # y := 1 A x + 0 y, entries a + b 2^-60 (full mantissa), y0 = 0, REFERENCE order: bit for bit
+    # against the oracle's restatement of level2.hpp:15-50, through the device API and through the
+    # reference's C ABI (quadblas_qgemv with host buffers).
+    m = 1500
+    A = O.fill(0, 42, m * m)
+    x = O.fill(0, 43, m)
+    want = np.zeros((m, 2), dtype=np.uint64)
+    O.gemv(0, m, m, 1.0, A, m, x, 1, 0.0, want, 1)
+    _say("config 1: qgemv RowMajor 1500^2, device API")
+    dA, dx, dy = to_dev(A), to_dev(x), dev_empty(m)
+    dy.zero_()
+    Q.dev_gemv(Q.MODE_REFERENCE, 0, m, m, 1.0, dA.data_ptr(), m, dx.data_ptr(), 1, 0.0, dy.data_ptr(), 1)
+    cuda.cuda.synchronize()
+    assert np.array_equal(from_dev(dy), want)
+    _say("config 1: qgemv RowMajor 1500^2, quadblas_qgemv (host buffers)")
+    y = np.zeros((m, 2), dtype=np.uint64)
+    prev = Q.get_mode()
+    Q.set_mode(Q.MODE_REFERENCE)
+    try:
+        Q.qgemv("R", "N", m, m, 1.0, A, m, x, 1, 0.0, y, 1)
+    finally:
+        Q.set_mode(prev)
+    assert np.array_equal(y, want)

@@ -365,6 +365,27 @@ def test_exact_dot_alpha_beta(O):
     assert rounded == round_fraction(exact)
 
 
+def test_cancellation_generators(O):
+    # kinds 2 and 4: exact pairs +-(10^e u) on slots 0, 1 of every 8 (e = 20, 22), kind 0 elsewhere;
+    # the matching y repeats its value on each pair, so x . y over a block of 8 is the 6 kind-0
+    # products alone
+    n = 64
+    for kind, e in ((2, 20), (4, 22)):
+        x = O.fill(kind, 3, n)
+        y = O.fill_pair_y(3, n)
+        k0 = O.fill(0, 3, n)
+        for i, p in enumerate(rows(x)):
+            if i % 8 < 2:
+                u = 1 + Fraction(O.rand64(3, i - (i % 2)) >> 12, 2 ** 52)
+                want = u * 10 ** e * (-1 if i % 2 else 1)
+                assert p == round_fraction(want) and val(p) == want, (kind, i)
+            else:
+                assert p == pair(k0[i])
+        assert np.array_equal(y[0::8], y[1::8])
+        pairs = sum(val(pair(x[i])) * val(pair(y[i])) for i in range(n) if i % 8 < 2)
+        assert pairs == 0
+
+
 def test_generators(O):
     n = 64
     v = O.fill(0, 9, n)
