@@ -3,18 +3,19 @@
 // Computes the same routines as the reference (include/quadblas/algorithms/level1.hpp:80-137,
 // level2.hpp:15-99) but returns RN(alpha * sum_j a_j b_j + beta * y), the exact value rounded once
 // (exact.hpp).  Kernels:
-//   k_dot_exact_partial    grid-stride lanes, block-uniform window anchor, wave and block
-//                          reductions, one lo/hi window pair per block.
+//   k_dot_exact_partial    grid-stride lanes, block-uniform window anchor, products into lazy LDS
+//                          digits (exact_dev.hpp), wave and block reductions into one lo/hi window
+//                          pair per block.
+//   k_exact_dot_combine    sums the block partials on a common base in registers (column sums per
+//                          workgroup); the last workgroup ripples the carries once and rounds.
+//   k_vec_split            the gemv vector prepared once: 29-bit limbs and a tag per element.
 //   k_gemv_row_exact       one wave per (row, column chunk), wave-uniform anchor, wave reduction.
 //   k_gemv_col_exact       one lane per row, blockIdx.y = column chunk, per-lane anchor.
-//   k_exact_combine        sums the block partials on a common base (the minimum anchor) in
-//                          registers, column sums per block.
-//   k_exact_finish_dot     sums the column records, ripples carries once, rounds once.
 //   k_exact_finish_rows    one thread per row: sums the row's chunk windows (shared anchor),
 //                          forms alpha*S + beta*y exactly in registers and rounds into y.
 // Loads of the streamed operands are coalesced 128-bit loads.  A dot or row with a product outside
 // its lane windows (products more than ~2^124 from the anchor) is recomputed on the full-range
-// path (wide.hpp): the dot by the finish kernel's whole grid, rows by k_exact_wide_rows from a
+// path (wide.hpp): the dot by the combine kernel's whole grid, rows by k_exact_wide_rows from a
 // queue that the row finish fills.  Header word 2 counts the items so recomputed.
 #include "wide.hpp"
 #include "kernels.h"
@@ -30,10 +31,14 @@ constexpr int XD_BS = 256;
 
 __global__ void __launch_bounds__(XD_BS) k_dot_exact_partial(const q128 *__restrict__ x, int64_t incx,
                                                              const q128 *__restrict__ y, int64_t incy, int64_t n,
-                                                             part_t *__restrict__ parts) {
+                                                             part_t *__restrict__ parts, uint32_t *__restrict__ hdr) {
   __shared__ int s_mn[XD_BS / 64], s_mx[XD_BS / 64];
   __shared__ uint32_t s_w[XD_BS / 64][2 * WW + 1];
+  __shared__ unsigned long long s_dig[XD_BS / 64][DIG][64];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (blockIdx.x == 0 && threadIdx.x < 4) hdr[threadIdx.x] = 0;  // status, counts, ticket of the combine
+  unsigned long long *d = &s_dig[wv][0][lane];
+  dig_clear(d);
   const int64_t stride = (int64_t)gridDim.x * XD_BS;
   const int64_t i0 = (int64_t)blockIdx.x * XD_BS + threadIdx.x;
   // block-uniform anchor, centred between the smallest and largest first products of the block
@@ -52,25 +57,21 @@ __global__ void __launch_bounds__(XD_BS) k_dot_exact_partial(const q128 *__restr
     bmx = max(bmx, s_mx[w]);
   }
   const int A = (bmn == EXP_NONE) ? 0 : ((bmn + bmx) >> 1) - WIN2_CENTER;
-  uint32_t Wl[WW], Wh[WW];
-#pragma unroll
-  for (int k = 0; k < WW; ++k) {
-    Wl[k] = 0;
-    Wh[k] = 0;
-  }
   uint32_t flags = 0;
   int64_t i = i0;
   if (incx == 1 && incy == 1) {
     for (; i + stride < n; i += 2 * stride) {
       const q128 a0 = ldq_nt(x + i), b0 = ldq_nt(y + i);
       const q128 a1 = ldq_nt(x + i + stride), b1 = ldq_nt(y + i + stride);
-      accumulate(Wl, Wh, A, a0, b0, flags);
-      accumulate(Wl, Wh, A, a1, b1, flags);
+      dig_accumulate(d, A, a0, b0, flags);
+      dig_accumulate(d, A, a1, b1, flags);
     }
-    if (i < n) accumulate(Wl, Wh, A, ldq_nt(x + i), ldq_nt(y + i), flags);
+    if (i < n) dig_accumulate(d, A, ldq_nt(x + i), ldq_nt(y + i), flags);
   } else {
-    for (; i < n; i += stride) accumulate(Wl, Wh, A, ldq_nt(x + i * incx), ldq_nt(y + i * incy), flags);
+    for (; i < n; i += stride) dig_accumulate(d, A, ldq_nt(x + i * incx), ldq_nt(y + i * incy), flags);
   }
+  uint32_t Wl[WW], Wh[WW];
+  dig_to_windows(d, Wl, Wh);
   wave_sum_win(Wl);
   wave_sum_win(Wh);
   flags = wave_or(flags);
@@ -105,12 +106,28 @@ __global__ void __launch_bounds__(XD_BS) k_dot_exact_partial(const q128 *__restr
 // ------------------------------------------------------------------------------------------
 constexpr int XR_BS = 256;
 
+// The vector operand prepared once per call (exact_dev.hpp: xsplit), so the streaming kernels
+// unpack only the matrix element of each product.
+__global__ void __launch_bounds__(256) k_vec_split(int64_t n, const q128 *__restrict__ x, int64_t incx,
+                                                  uint4 *__restrict__ xl, int32_t *__restrict__ xt) {
+  for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < n; j += (int64_t)gridDim.x * 256) {
+    const xsplit v = make_xsplit(ldq_c(x + j * incx));
+    xl[j] = make_uint4(v.B[0], v.B[1], v.B[2], v.B[3]);
+    xt[j] = v.tag;
+  }
+}
+
 // RowMajor: wave = (row, chunk).  Lanes stride the chunk's columns (coalesced).
 __global__ void __launch_bounds__(XR_BS) k_gemv_row_exact(int64_t m, int64_t n, int64_t chunk, int nch,
                                                           const q128 *__restrict__ A, int64_t lda,
                                                           const q128 *__restrict__ x, int64_t incx,
-                                                          part_t *__restrict__ parts) {
+                                                          const uint4 *__restrict__ xl, const int32_t *__restrict__ xt,
+                                                          part_t *__restrict__ parts,
+                                                          uint32_t *__restrict__ hdr) {
+  __shared__ unsigned long long s_dig[XR_BS / 64][DIG][64];
   const int lane = threadIdx.x & 63;
+  if (blockIdx.x == 0 && threadIdx.x < 4) hdr[threadIdx.x] = 0;  // status and the full-range queue count
+  unsigned long long *d = &s_dig[threadIdx.x >> 6][0][lane];
   const int64_t nw = (int64_t)gridDim.x * (XR_BS / 64);
   // whole waves stride over the (row, chunk) items
   for (int64_t wid = (int64_t)blockIdx.x * (XR_BS / 64) + (threadIdx.x >> 6); wid < m * nch; wid += nw) {
@@ -123,25 +140,21 @@ __global__ void __launch_bounds__(XR_BS) k_gemv_row_exact(int64_t m, int64_t n, 
     int e0 = EXP_NONE;
     if (lane < n) e0 = pair_exp(ldq_c(r + lane), ldq_c(x + (int64_t)lane * incx));
     const int An = wave_anchor(e0);
-    uint32_t Wl[WW], Wh[WW];
-#pragma unroll
-    for (int k = 0; k < WW; ++k) {
-      Wl[k] = 0;
-      Wh[k] = 0;
-    }
+    dig_clear(d);
     uint32_t flags = 0;
     int64_t j = j0 + lane;
-    if (incx == 1) {
-      for (; j + 64 < j1; j += 128) {
-        const q128 a0 = ldq_nt(r + j), a1 = ldq_nt(r + j + 64);
-        const q128 b0 = ldq_c(x + j), b1 = ldq_c(x + j + 64);
-        accumulate(Wl, Wh, An, a0, b0, flags);
-        accumulate(Wl, Wh, An, a1, b1, flags);
-      }
-      if (j < j1) accumulate(Wl, Wh, An, ldq_nt(r + j), ldq_c(x + j), flags);
-    } else {
-      for (; j < j1; j += 64) accumulate(Wl, Wh, An, ldq_nt(r + j), ldq_c(x + j * incx), flags);
+    for (; j + 192 < j1; j += 256) {
+      const q128 a0 = ldq_nt(r + j), a1 = ldq_nt(r + j + 64), a2 = ldq_nt(r + j + 128), a3 = ldq_nt(r + j + 192);
+      const xsplit b0 = ldx(xl, xt, j), b1 = ldx(xl, xt, j + 64), b2 = ldx(xl, xt, j + 128),
+                   b3 = ldx(xl, xt, j + 192);
+      dig_accumulate_x(d, An, a0, b0, x + j * incx, flags);
+      dig_accumulate_x(d, An, a1, b1, x + (j + 64) * incx, flags);
+      dig_accumulate_x(d, An, a2, b2, x + (j + 128) * incx, flags);
+      dig_accumulate_x(d, An, a3, b3, x + (j + 192) * incx, flags);
     }
+    for (; j < j1; j += 64) dig_accumulate_x(d, An, ldq_nt(r + j), ldx(xl, xt, j), x + j * incx, flags);
+    uint32_t Wl[WW], Wh[WW];
+    dig_to_windows(d, Wl, Wh);
     wave_sum_win(Wl);
     wave_sum_win(Wh);
     flags = wave_or(flags);
@@ -154,180 +167,118 @@ constexpr int XC_BS = 256;
 __global__ void __launch_bounds__(XC_BS) k_gemv_col_exact(int64_t m, int64_t n, int64_t chunk,
                                                           const q128 *__restrict__ A, int64_t lda,
                                                           const q128 *__restrict__ x, int64_t incx,
-                                                          part_t *__restrict__ parts) {
+                                                          const uint4 *__restrict__ xl, const int32_t *__restrict__ xt,
+                                                          part_t *__restrict__ parts,
+                                                          uint32_t *__restrict__ hdr) {
+  __shared__ unsigned long long s_dig[XC_BS / 64][DIG][64];
   const int64_t i = (int64_t)blockIdx.x * XC_BS + threadIdx.x;
   const int64_t j0 = (int64_t)blockIdx.y * chunk;
   const int64_t j1 = (j0 + chunk < n) ? j0 + chunk : n;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 4) hdr[threadIdx.x] = 0;
   if (i >= m) return;
+  unsigned long long *d = &s_dig[threadIdx.x >> 6][0][threadIdx.x & 63];
+  dig_clear(d);
   // lane anchor: centred on the row's first regular product (first columns of the matrix), so
   // every column chunk of the row shares it
   int e0 = EXP_NONE;
   for (int64_t j = 0; j < n && j < 4 && e0 == EXP_NONE; ++j) e0 = pair_exp(ldq_c(A + j * lda + i), ldq_c(x + j * incx));
   const int An = (e0 == EXP_NONE) ? 0 : e0 - WIN2_CENTER;
-  uint32_t Wl[WW], Wh[WW];
-#pragma unroll
-  for (int k = 0; k < WW; ++k) {
-    Wl[k] = 0;
-    Wh[k] = 0;
-  }
   uint32_t flags = 0;
   int64_t j = j0;
-  for (; j + 1 < j1; j += 2) {
+  for (; j + 3 < j1; j += 4) {
     const q128 a0 = ldq_nt(A + j * lda + i), a1 = ldq_nt(A + (j + 1) * lda + i);
-    const q128 b0 = ldq_c(x + j * incx), b1 = ldq_c(x + (j + 1) * incx);
-    accumulate(Wl, Wh, An, a0, b0, flags);
-    accumulate(Wl, Wh, An, a1, b1, flags);
+    const q128 a2 = ldq_nt(A + (j + 2) * lda + i), a3 = ldq_nt(A + (j + 3) * lda + i);
+    const xsplit b0 = ldx(xl, xt, j), b1 = ldx(xl, xt, j + 1), b2 = ldx(xl, xt, j + 2), b3 = ldx(xl, xt, j + 3);
+    dig_accumulate_x(d, An, a0, b0, x + j * incx, flags);
+    dig_accumulate_x(d, An, a1, b1, x + (j + 1) * incx, flags);
+    dig_accumulate_x(d, An, a2, b2, x + (j + 2) * incx, flags);
+    dig_accumulate_x(d, An, a3, b3, x + (j + 3) * incx, flags);
   }
-  if (j < j1) accumulate(Wl, Wh, An, ldq_nt(A + j * lda + i), ldq_c(x + j * incx), flags);
+  for (; j < j1; ++j) dig_accumulate_x(d, An, ldq_nt(A + j * lda + i), ldx(xl, xt, j), x + j * incx, flags);
+  uint32_t Wl[WW], Wh[WW];
+  dig_to_windows(d, Wl, Wh);
   put_parts(parts + 2 * ((int64_t)blockIdx.y * m + i), An, flags, Wl, Wh);
 }
 
-// Combining the block partials, in two stages on a common base (the minimum anchor over nonzero
-// windows, minus 32):
-//   k_exact_combine: XC blocks; every thread adds its windows into a register-resident
-//     ACCW-word accumulator (select-based shifts), then lanes 0..ACCW-1 sum one word column each
-//     over the block's threads (64-bit column sums, no carry chains) into one record per block.
-//   k_exact_finish_dot: one block sums the column records, ripples the carries once and rounds.
+// Combining the dot's block partials and finishing, in one launch of XC workgroups (XC <= XF_BLOCKS):
+//   every workgroup finds the common base (the minimum anchor over nonzero windows, minus 32) and
+//   the widest anchor; each thread adds its windows into a register-resident ACCW-word accumulator
+//   (select-based shifts), lanes 0..ACCW-1 sum one word column each over the block's threads
+//   (64-bit column sums, no carry chains) into one record per workgroup, and the last workgroup to
+//   finish (a ticket in header word 3) sums the records' columns, ripples the carries once and
+//   rounds.
+//   When a product fell outside its block's windows, or the block anchors lie too far apart for the
+//   ACCW-word accumulator, the same workgroups recompute the dot on the full-range path instead:
+//   workgroup b accumulates its slice of the elements in LDS (wide.hpp), normalises it and writes
+//   its digits to record b; the last workgroup adds the records digit by digit and rounds once
+//   (header word 2 = 1: the call took the full-range path).
 constexpr int XF_BS = 256;
 constexpr int XF_BLOCKS = 32;
 struct col_rec {
   uint64_t c[ACCW];
   uint32_t flags;
-  uint32_t lost;
-  int amin;
-  uint32_t pad;
+  uint32_t pad[3];
 };
-
-__device__ __forceinline__ void parts_min_flags(const part_t *__restrict__ parts, int nparts, int &amin,
-                                                uint32_t &fl) {
-  __shared__ int s_min[XF_BS / 64];
-  __shared__ uint32_t s_fl[XF_BS / 64];
-  amin = EXP_NONE;
-  fl = 0;
-  for (int p = threadIdx.x; p < nparts; p += XF_BS) {
-    fl |= parts[p].flags;
-    if (!win_is_zero(parts[p].W)) amin = min(amin, parts[p].A);
-  }
-  amin = wave_min(amin);
-  fl = wave_or(fl);
-  if ((threadIdx.x & 63) == 0) {
-    s_min[threadIdx.x >> 6] = amin;
-    s_fl[threadIdx.x >> 6] = fl;
-  }
-  __syncthreads();
-  amin = s_min[0];
-  fl = s_fl[0];
-#pragma unroll
-  for (int w = 1; w < XF_BS / 64; ++w) {
-    amin = min(amin, s_min[w]);
-    fl |= s_fl[w];
-  }
-}
-
-__global__ void __launch_bounds__(XF_BS) k_exact_combine(const part_t *__restrict__ parts, int nparts,
-                                                         col_rec *__restrict__ recs) {
-  __shared__ uint32_t tile[XF_BS][ACCW + 1];
-  int amin;
-  uint32_t fl;
-  parts_min_flags(parts, nparts, amin, fl);
-  const int base = amin - 32;
-  uint32_t acc[ACCW];
-#pragma unroll
-  for (int k = 0; k < ACCW; ++k) acc[k] = 0;
-  uint32_t lost = 0;
-  if (amin != EXP_NONE) {
-    for (int p = blockIdx.x * XF_BS + threadIdx.x; p < nparts; p += gridDim.x * XF_BS) {
-      if (win_is_zero(parts[p].W)) continue;
-      const int off = parts[p].A - base;
-      if (off < 0 || off + WW * 32 + 32 > ACCW * 32) {
-        lost = 1;
-        continue;
-      }
-      uint32_t W[ACCW], S[ACCW];
-      const uint32_t sext = (parts[p].W[WW - 1] >> 31) ? 0xffffffffu : 0u;
-#pragma unroll
-      for (int k = 0; k < ACCW; ++k) W[k] = (k < WW) ? parts[p].W[k] : sext;
-      rshl<ACCW>(S, W, off);  // zeros enter from below, the sign extension stays above
-      uint32_t c = 0;
-#pragma unroll
-      for (int k = 0; k < ACCW; ++k) acc[k] = addc32(acc[k], S[k], c, c);
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < ACCW; ++k) tile[threadIdx.x][k] = acc[k];
-  tile[threadIdx.x][ACCW] = lost;
-  __syncthreads();
-  if (threadIdx.x < ACCW) {
-    uint64_t s = 0;
-    for (int t = 0; t < XF_BS; ++t) s += tile[t][threadIdx.x];
-    recs[blockIdx.x].c[threadIdx.x] = s;
-  } else if (threadIdx.x == ACCW) {
-    uint32_t l = 0;
-    for (int t = 0; t < XF_BS; ++t) l |= tile[t][ACCW];
-    recs[blockIdx.x].lost = l;
-    recs[blockIdx.x].flags = fl;
-    recs[blockIdx.x].amin = amin;
-  }
-}
-
-// Finish of the dot: block 0 sums the column records, ripples the carries once and rounds.  When
-// a product fell outside its block's windows, or block anchors lay too far apart for the combine,
-// the whole grid (XW_BLOCKS workgroups) recomputes the dot on the full-range path instead: block
-// b accumulates its slice of the elements in LDS (wide.hpp), normalises it and writes its digits
-// to record b; the last block to finish (a ticket in header word 3) adds the records digit by digit,
-// rounds once and writes the result (header word 2 counts the call as recomputed).
-constexpr int XW_BLOCKS = 128;
 struct dot_rec {
   int lo, t, neg;
   uint32_t flags;
   uint32_t D[KUL_N];
 };
 
-__global__ void __launch_bounds__(256) k_exact_finish_dot(const col_rec *__restrict__ recs, int nrec,
-                                                          const q128 *__restrict__ x, int64_t incx,
-                                                          const q128 *__restrict__ y, int64_t incy, int64_t n,
-                                                          dot_rec *__restrict__ wrec, q128 *__restrict__ out,
-                                                          uint32_t *__restrict__ hdr) {
-  __shared__ kul_t K;
-  __shared__ uint64_t col[ACCW];
-  __shared__ int s_last;
-  uint32_t lost = 0;
-  for (int b = 0; b < nrec; ++b) lost |= recs[b].lost;
-  if (recs[0].flags & FL_LOST) lost = 1;
-  if (!lost) {
-    if (blockIdx.x != 0) return;
-    if (threadIdx.x < ACCW) {
-      uint64_t s = 0;
-      for (int b = 0; b < nrec; ++b) s += recs[b].c[threadIdx.x];  // < 2^45 for nrec <= 32
-      col[threadIdx.x] = s;
+__device__ __forceinline__ void parts_range_flags(const part_t *__restrict__ parts, int nparts, int &amin, int &amax,
+                                                  uint32_t &fl) {
+  __shared__ int s_min[XF_BS / 64], s_max[XF_BS / 64];
+  __shared__ uint32_t s_fl[XF_BS / 64];
+  amin = EXP_NONE;
+  amax = -EXP_NONE;
+  fl = 0;
+  for (int p = threadIdx.x; p < nparts; p += XF_BS) {
+    fl |= parts[p].flags;
+    if (!win_is_zero(parts[p].W)) {
+      amin = min(amin, parts[p].A);
+      amax = max(amax, parts[p].A);
     }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    const uint32_t fl = recs[0].flags;
-    const int amin = recs[0].amin;
-    q128 r;
-    if (amin == EXP_NONE) {
-      r = (fl & (FL_NAN | FL_PINF | FL_NINF)) ? special_result(fl) : q_zero(0);
-    } else {
-      wide_acc wa;
-      wide_init(wa);
-      wa.any = true;
-      wa.base = amin - 32;
-      uint64_t carry = 0;
-      for (int k = 0; k < ACCW; ++k) {
-        const uint64_t v = col[k] + carry;  // two's complement words: the top carry is dropped
-        wa.w[k] = (uint32_t)v;
-        carry = v >> 32;
-      }
-      uint32_t M[SW], sign, sticky;
-      int lsb;
-      compact_wide(wa, M, lsb, sign, sticky);
-      r = finish_reg(M, lsb, sign, sticky, fl, false, q_one(), q_zero(0), q_zero(0));
-    }
-    stq_c(out, r);
-    return;
   }
-  // full-range path: block b takes elements [b*chunk, (b+1)*chunk)
+  amin = wave_min(amin);
+  amax = wave_max(amax);
+  fl = wave_or(fl);
+  if ((threadIdx.x & 63) == 0) {
+    s_min[threadIdx.x >> 6] = amin;
+    s_max[threadIdx.x >> 6] = amax;
+    s_fl[threadIdx.x >> 6] = fl;
+  }
+  __syncthreads();
+  amin = s_min[0];
+  amax = s_max[0];
+  fl = s_fl[0];
+#pragma unroll
+  for (int w = 1; w < XF_BS / 64; ++w) {
+    amin = min(amin, s_min[w]);
+    amax = max(amax, s_max[w]);
+    fl |= s_fl[w];
+  }
+}
+
+// last workgroup of a ticket (header word 3, cleared by the partial kernel): true for exactly one
+// workgroup, after every other one has made its record visible
+__device__ __forceinline__ bool last_block(uint32_t *__restrict__ hdr) {
+  __shared__ int s_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t ticket = __hip_atomic_fetch_add(hdr + 3, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = ticket == gridDim.x - 1;
+  }
+  __syncthreads();
+  const bool last = s_last != 0;
+  if (last) __threadfence();
+  return last;
+}
+
+// the full-range path of the dot over the whole grid (see above)
+__device__ void dot_wide_grid(kul_t &K, const q128 *__restrict__ x, int64_t incx, const q128 *__restrict__ y,
+                              int64_t incy, int64_t n, dot_rec *__restrict__ wrec, q128 *__restrict__ out,
+                              uint32_t *__restrict__ hdr) {
   kul_clear_all(K);
   __syncthreads();
   const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
@@ -383,16 +334,8 @@ __global__ void __launch_bounds__(256) k_exact_finish_dot(const col_rec *__restr
       K.L[i] = 0;
     }
   }
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint32_t ticket = __hip_atomic_fetch_add(hdr + 3, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    s_last = ticket == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  // last block: digit sums of all records (each < 2^32 * gridDim.x), then one rounding
+  if (!last_block(hdr)) return;
+  // last workgroup: digit sums of all records (each < 2^32 * gridDim.x), then one rounding
   __shared__ int s_lo, s_hi;
   __shared__ uint32_t s_fl;
   if (threadIdx.x == 0) {
@@ -403,28 +346,23 @@ __global__ void __launch_bounds__(256) k_exact_finish_dot(const col_rec *__restr
   __syncthreads();
   for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
     const dot_rec *r = wrec + b;
-    const int lo = __hip_atomic_load(&r->lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int t = __hip_atomic_load(&r->t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int ng = __hip_atomic_load(&r->neg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t f = __hip_atomic_load(&r->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int top = ng ? t + 1 : t;
-    if (lo <= top) {
-      atomicMin(&s_lo, lo);
+    const int top = r->neg ? r->t + 1 : r->t;
+    if (r->lo <= top) {
+      atomicMin(&s_lo, r->lo);
       atomicMax(&s_hi, top);
     }
-    if (f) atomicOr(&s_fl, f);
+    if (r->flags) atomicOr(&s_fl, r->flags);
   }
   __syncthreads();
   const int glo = s_lo, ghi = min(s_hi, KUL_N - 1);
   for (int i = glo + (int)threadIdx.x; i <= ghi; i += blockDim.x) {
-    int64_t s = 0;
+    int64_t sum = 0;
     for (int b = 0; b < (int)gridDim.x; ++b) {
       const dot_rec *r = wrec + b;
-      const int lo = r->lo, t = r->t;
-      if (i >= lo && i <= t) s += (int64_t)r->D[i];
-      if (r->neg && i == t + 1) s -= 1;
+      if (i >= r->lo && i <= r->t) sum += (int64_t)r->D[i];
+      if (r->neg && i == r->t + 1) sum -= 1;
     }
-    K.L[i] = (unsigned long long)s;
+    K.L[i] = (unsigned long long)sum;
   }
   if (threadIdx.x == 0) {
     K.lo = glo;
@@ -447,8 +385,108 @@ __global__ void __launch_bounds__(256) k_exact_finish_dot(const col_rec *__restr
     }
     stq_c(out, r);
     hdr[2] = 1;  // the dot took the full-range path
-    hdr[3] = 0;
   }
+}
+
+// Round a two's complement ACCW-word sum (bit 0 weighing 2^base) to binary128, in registers.
+__device__ __forceinline__ q128 round_accw(const uint32_t (&W)[ACCW], int base, uint32_t fl) {
+  const uint32_t sign = W[ACCW - 1] >> 31;
+  uint32_t M[ACCW];
+  uint32_t c = sign;
+#pragma unroll
+  for (int k = 0; k < ACCW; ++k) M[k] = sign ? addc32(~W[k], 0u, c, c) : W[k];
+  int top = -1;
+#pragma unroll
+  for (int k = 0; k < ACCW; ++k) top = M[k] ? k : top;
+  if (top < 0) return (fl & (FL_NAN | FL_PINF | FL_NINF)) ? special_result(fl) : q_zero(0);
+  const int low = top >= SW ? top - SW + 1 : 0;
+  uint32_t sticky = 0;
+#pragma unroll
+  for (int k = 0; k < ACCW; ++k) sticky |= (k < low) ? M[k] : 0u;
+  uint32_t S[ACCW];
+  (void)rshr<ACCW>(S, M, 32 * low);
+  uint32_t T[SW];
+#pragma unroll
+  for (int k = 0; k < SW; ++k) T[k] = S[k];
+  return finish_reg(T, base + 32 * low, sign, sticky ? 1u : 0u, fl, false, q_one(), q_zero(0), q_zero(0));
+}
+
+__global__ void __launch_bounds__(XF_BS) k_exact_dot_combine(const part_t *__restrict__ parts, int nparts,
+                                                             col_rec *__restrict__ recs, const q128 *__restrict__ x,
+                                                             int64_t incx, const q128 *__restrict__ y, int64_t incy,
+                                                             int64_t n, dot_rec *__restrict__ wrec,
+                                                             q128 *__restrict__ out, uint32_t *__restrict__ hdr) {
+  __shared__ uint32_t tile[XF_BS][ACCW + 1];
+  __shared__ kul_t K;
+  __shared__ uint64_t col[ACCW];
+  int amin, amax;
+  uint32_t fl;
+  parts_range_flags(parts, nparts, amin, amax, fl);
+  const int base = amin - 32;
+  const bool wide = (fl & FL_LOST) || (amin != EXP_NONE && amax - base + WW * 32 + 32 > ACCW * 32);
+  if (wide) {  // uniform over the grid
+    dot_wide_grid(K, x, incx, y, incy, n, wrec, out, hdr);
+    return;
+  }
+  uint32_t acc[ACCW];
+#pragma unroll
+  for (int k = 0; k < ACCW; ++k) acc[k] = 0;
+  if (amin != EXP_NONE) {
+    for (int p = blockIdx.x * XF_BS + threadIdx.x; p < nparts; p += gridDim.x * XF_BS) {
+      if (win_is_zero(parts[p].W)) continue;
+      const int off = parts[p].A - base;  // 32 .. ACCW * 32 - WW * 32 - 32 (checked above)
+      uint32_t W[ACCW], S[ACCW];
+      const uint32_t sext = (parts[p].W[WW - 1] >> 31) ? 0xffffffffu : 0u;
+#pragma unroll
+      for (int k = 0; k < ACCW; ++k) W[k] = (k < WW) ? parts[p].W[k] : sext;
+      rshl<ACCW>(S, W, off);  // zeros enter from below, the sign extension stays above
+      uint32_t c = 0;
+#pragma unroll
+      for (int k = 0; k < ACCW; ++k) acc[k] = addc32(acc[k], S[k], c, c);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < ACCW; ++k) tile[threadIdx.x][k] = acc[k];
+  __syncthreads();
+  // column sums: six groups of ACCW threads each sum a sixth of the rows, then one adds the six
+  constexpr int G = XF_BS / ACCW;  // 6
+  __shared__ uint64_t part6[G][ACCW];
+  if (threadIdx.x < G * ACCW) {
+    const int cidx = threadIdx.x % ACCW, g = threadIdx.x / ACCW;
+    uint64_t sum = 0;
+    for (int t = g; t < XF_BS; t += G) sum += tile[t][cidx];
+    part6[g][cidx] = sum;
+  }
+  __syncthreads();
+  if (threadIdx.x < ACCW) {
+    uint64_t sum = 0;
+#pragma unroll
+    for (int g = 0; g < G; ++g) sum += part6[g][threadIdx.x];
+    recs[blockIdx.x].c[threadIdx.x] = sum;
+  }
+  if (!last_block(hdr)) return;
+  if (threadIdx.x < ACCW) {
+    uint64_t sum = 0;
+    for (int b = 0; b < (int)gridDim.x; ++b) sum += recs[b].c[threadIdx.x];  // < 2^45 for <= 32 records
+    col[threadIdx.x] = sum;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  q128 r;
+  if (amin == EXP_NONE) {
+    r = (fl & (FL_NAN | FL_PINF | FL_NINF)) ? special_result(fl) : q_zero(0);
+  } else {
+    uint32_t W[ACCW];
+    uint64_t carry = 0;
+#pragma unroll
+    for (int k = 0; k < ACCW; ++k) {
+      const uint64_t v = col[k] + carry;  // two's complement words: the top carry is dropped
+      W[k] = (uint32_t)v;
+      carry = v >> 32;
+    }
+    r = round_accw(W, base, fl);
+  }
+  stq_c(out, r);
 }
 
 // one thread per row: sums the row's chunk windows (they share the row's anchor), then
@@ -542,7 +580,7 @@ __global__ void __launch_bounds__(256) k_exact_wide_rows(int colmajor, int64_t n
 static int dot_exact_blocks(int64_t n) {
   int64_t b = (n + XD_BS * 8 - 1) / (XD_BS * 8);
   if (b < 1) b = 1;
-  if (b > 1024) b = 1024;  // 2048 blocks: 6% slower at n = 2^24 (twice the partials to combine); 512: 10%
+  if (b > 1024) b = 1024;  // at n = 2^24: 2048 blocks 6% slower (twice the partials to combine), 512 10%
   return (int)b;
 }
 static void gemv_exact_split(int colmajor, int64_t m, int64_t n, int &nch, int64_t &chunk) {
@@ -567,34 +605,35 @@ static void gemv_exact_split(int colmajor, int64_t m, int64_t n, int &nch, int64
 static size_t dot_exact_recs_off(int64_t n) {
   return 256 + 2 * (size_t)dot_exact_blocks(n) * sizeof(part_t) + XF_BLOCKS * sizeof(col_rec);
 }
-size_t dot_exact_ws(int64_t n) { return (dot_exact_recs_off(n) + 255) / 256 * 256 + XW_BLOCKS * sizeof(dot_rec); }
-// workspace of gemv: [header 256 B | partials | queue of rows for the full-range path]
+size_t dot_exact_ws(int64_t n) { return (dot_exact_recs_off(n) + 255) / 256 * 256 + XF_BLOCKS * sizeof(dot_rec); }
+// workspace of gemv: [header 256 B | partials | queue of rows for the full-range path | the
+// prepared vector: n limb quadruples (uint4), then n tags (int32), 256-byte aligned]
 static size_t gemv_exact_queue_off(int colmajor, int64_t m, int64_t n) {
   int nch;
   int64_t chunk;
   gemv_exact_split(colmajor, m, n, nch, chunk);
   return 256 + 2 * (size_t)nch * (size_t)m * sizeof(part_t);
 }
+static size_t gemv_exact_xs_off(int colmajor, int64_t m, int64_t n) {
+  return (gemv_exact_queue_off(colmajor, m, n) + (size_t)m * sizeof(int64_t) + 255) / 256 * 256;
+}
 size_t gemv_exact_ws(int colmajor, int64_t m, int64_t n) {
-  return gemv_exact_queue_off(colmajor, m, n) + (size_t)m * sizeof(int64_t);
+  return gemv_exact_xs_off(colmajor, m, n) + (size_t)n * (sizeof(uint4) + sizeof(int32_t));
 }
 
 // workspace: [status (u32) | pad to 256 | partials]
 hipError_t launch_dot_exact(const q128 *x, int64_t incx, const q128 *y, int64_t incy, int64_t n, void *ws,
                             q128 *out, hipStream_t st) {
-  uint32_t *status = (uint32_t *)ws;
+  uint32_t *hdr = (uint32_t *)ws;
   const int blocks = dot_exact_blocks(n);
   part_t *parts = (part_t *)((char *)ws + 256);
   col_rec *recs = (col_rec *)(parts + 2 * blocks);
-  hipError_t e = launch_zero((q128 *)ws, 1, st);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_dot_exact_partial, dim3(blocks), dim3(XD_BS), 0, st, x, incx, y, incy, n, parts);
+  dot_rec *wrec = (dot_rec *)((char *)ws + (dot_exact_recs_off(n) + 255) / 256 * 256);
+  hipLaunchKernelGGL(k_dot_exact_partial, dim3(blocks), dim3(XD_BS), 0, st, x, incx, y, incy, n, parts, hdr);
   const int nparts = 2 * blocks;
   const int cblocks = nparts < XF_BLOCKS * XF_BS ? (nparts + XF_BS - 1) / XF_BS : XF_BLOCKS;
-  hipLaunchKernelGGL(k_exact_combine, dim3(cblocks), dim3(XF_BS), 0, st, parts, nparts, recs);
-  dot_rec *wrec = (dot_rec *)((char *)ws + (dot_exact_recs_off(n) + 255) / 256 * 256);
-  hipLaunchKernelGGL(k_exact_finish_dot, dim3(XW_BLOCKS), dim3(256), 0, st, recs, cblocks, x, incx, y, incy, n, wrec,
-                     out, status);
+  hipLaunchKernelGGL(k_exact_dot_combine, dim3(cblocks), dim3(XF_BS), 0, st, parts, nparts, recs, x, incx, y, incy, n,
+                     wrec, out, hdr);
   return hipGetLastError();
 }
 
@@ -606,16 +645,17 @@ hipError_t launch_gemv_exact(int colmajor, int64_t m, int64_t n, q128 alpha, con
   int nch;
   int64_t chunk;
   gemv_exact_split(colmajor, m, n, nch, chunk);
-  hipError_t e = launch_zero((q128 *)ws, 1, st);
-  if (e != hipSuccess) return e;
+  uint4 *xl = (uint4 *)((char *)ws + gemv_exact_xs_off(colmajor, m, n));
+  int32_t *xt = (int32_t *)(xl + n);
+  hipLaunchKernelGGL(k_vec_split, dim3(stride_blocks(n, 256)), dim3(256), 0, st, n, x, incx, xl, xt);
   if (!colmajor) {
     const int64_t waves = m * nch;
     hipLaunchKernelGGL(k_gemv_row_exact, dim3(stride_blocks(waves, XR_BS / 64)), dim3(XR_BS), 0,
-                       st, m, n, chunk, nch, A, lda, x, incx, parts);
+                       st, m, n, chunk, nch, A, lda, x, incx, xl, xt, parts, status);
   } else {
     if (!grid_fits((m + XC_BS - 1) / XC_BS, XC_BS)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_gemv_col_exact, dim3((unsigned)((m + XC_BS - 1) / XC_BS), (unsigned)nch), dim3(XC_BS), 0,
-                       st, m, n, chunk, A, lda, x, incx, parts);
+                       st, m, n, chunk, A, lda, x, incx, xl, xt, parts, status);
   }
   int64_t *queue = (int64_t *)((char *)ws + gemv_exact_queue_off(colmajor, m, n));
   hipLaunchKernelGGL(k_exact_finish_rows, dim3(stride_blocks(m, 64)), dim3(64), 0, st, m, nch, parts, alpha,

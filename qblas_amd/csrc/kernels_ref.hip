@@ -8,8 +8,8 @@
 //   gemv : include/quadblas/algorithms/level2.hpp:15-99
 //   gemm : include/quadblas/algorithms/level3.hpp:187-336 (kc from detail/blocking.hpp:17-67)
 // Every reference chain is inherently sequential, so one lane owns one chain; these kernels are
-// latency-bound by construction.  The fast kernels (kernels_fast.hip) trade the reference order
-// for exact accumulation.
+// latency-bound by construction.  The other forms (kernels_exact.hip, kernels_dd.hip,
+// kernels_ozaki.hip) trade the reference order for exact or wider accumulation.
 #include "q128.hpp"
 #include "kernels.h"
 
