@@ -1398,8 +1398,8 @@ struct crt_tab {
   int32_t half[64];      // (p - 1) / 2, 128 for p = 256
   int32_t off[64];       // ceil(2^19 / p) p + half: x + off is in [0, 2^20) for the residue sums
   uint32_t M[CRT_MW];    // product of the moduli
-  double xs[CRT_MW];     // 2^(32 (w - w0)) for words w >= w0, else 0 (w0: second-highest word of M)
-  double mtop;           // sum_w M[w] xs[w]
+  double msc;            // 2^(32 w0) / M
+  int32_t w0;            // lowest word whose byte sums enter the estimate of X'/M
   int32_t nq;
 };
 constexpr size_t CRT_TAB_BYTES = (sizeof(crt_tab) + 255) / 256 * 256;
@@ -1488,18 +1488,18 @@ __global__ void __launch_bounds__(64) k_crt_tables(int nq, crt_tab *__restrict__
     T->vt[b][q] = (int8_t)(p ? d : 0);
   }
   if (q == 0) {
-    int wt = 0;
-    for (int w = 0; w < CRT_MW; ++w)
-      if (sM[w]) wt = w;
-    const int w0 = wt > 0 ? wt - 1 : 0;
-    double mt = 0;
+    // the byte sums below word w0 (|s_b| <= nq 2^14 < 2^20) move X'/M by less than
+    // 2^20 2^(32 w0 - 8) / M <= 2^-14
+    int bits = 0;
+    double md = 0;
     for (int w = 0; w < CRT_MW; ++w) {
-      const double x = (w >= w0) ? __builtin_ldexp(1.0, 32 * (w - w0)) : 0.0;
       T->M[w] = sM[w];
-      T->xs[w] = x;
-      mt += (double)sM[w] * x;
+      if (sM[w]) bits = 32 * w + 32 - __builtin_clz(sM[w]);
+      md += __builtin_ldexp((double)sM[w], 32 * w);
     }
-    T->mtop = mt;
+    const int w0 = bits > 27 ? (bits - 27) / 32 : 0;
+    T->msc = __builtin_ldexp(1.0, 32 * w0) / md;
+    T->w0 = w0;
     T->nq = nq;
   }
 }
@@ -1844,6 +1844,30 @@ __device__ __forceinline__ void crt_t4(uint32_t i0, uint32_t i1, uint32_t i2, ui
   o3 = __builtin_amdgcn_perm(d, b, 0x07060302u);
 }
 
+// RN((-1)^s mag 2^lsb) for msb(mag) = msb >= 114 and a normal result (the caller checks the
+// biased exponent lsb + msb + QBIAS is in [1, 0x7ffd]); rbe receives the result's biased exponent
+template <int N>
+__device__ __forceinline__ q128 round_top(const uint32_t (&mag)[N], int msb, int lsb, uint32_t s, int &rbe) {
+  uint32_t T[N];
+  const uint32_t sticky = rshr<N>(T, mag, msb - 114);  // leading one at bit 114
+  const uint64_t lo = ((uint64_t)T[1] << 32) | T[0];
+  const uint64_t hi = ((uint64_t)T[3] << 32) | T[2];
+  const uint32_t guard = (uint32_t)(lo >> 1) & 1u;
+  const uint32_t st = ((uint32_t)lo & 1u) | sticky;
+  uint64_t sl = (lo >> 2) | (hi << 62);
+  uint64_t sh = (hi >> 2) & ((1ull << 49) - 1);
+  const uint64_t inc = guard & (st | (uint32_t)(sl & 1));
+  sl += inc;
+  sh += (sl == 0) & inc;
+  int be = lsb + msb + QBIAS;
+  if (sh >> 49) {  // rounded up to 2^113: the significand is 2^112 one binade higher (sl = 0)
+    sh >>= 1;
+    be += 1;
+  }
+  rbe = be;
+  return q_make(sl, ((uint64_t)s << 63) | ((uint64_t)be << 48) | (sh & FRAC_HI_MASK));
+}
+
 // Reconstruction and finish for one 32 x 32 block of C per wave.  The D^T = Vt R product has the
 // entries as columns: for the entries (row 8 rq + 4 h' + e, column n) of the block, lane L
 // (n = L%32, kh = L/32) holds the residues of moduli 32 kb + 16 kh + 0..15 (B operand, gathered
@@ -1855,7 +1879,7 @@ __device__ __forceinline__ void crt_t4(uint32_t i0, uint32_t i1, uint32_t i2, ui
 // FAST (alpha = +-1, beta = 0): the entry is X 2^(E_i+F_j-2t) rounded once, certified and stored
 // here.  Otherwise X is written to xrec (RW words per entry, row-major) for k_crt_finish_ab.
 template <int XW, bool FAST>
-__global__ void __launch_bounds__(256) k_crt_finish(int64_t M, int64_t N, int64_t K, int t, int kc, int64_t Mb,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) k_crt_finish(int64_t M, int64_t N, int64_t K, int t, int kc, int64_t Mb,
                                                     int64_t Nb, int64_t plane, const int8_t *__restrict__ RC,
                                                     const crt_tab *__restrict__ T, const int32_t *__restrict__ eA,
                                                     const uint32_t *__restrict__ fA, const int32_t *__restrict__ eB,
@@ -1864,7 +1888,7 @@ __global__ void __launch_bounds__(256) k_crt_finish(int64_t M, int64_t N, int64_
                                                     uint32_t *__restrict__ hdr, uint32_t *__restrict__ list,
                                                     uint32_t *__restrict__ xrec) {
   constexpr int RW = (XW + 3) & ~3;
-  const int64_t blk = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t blk = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (blk >= Mb * Nb) return;  // whole waves
   const int lane = threadIdx.x & 63, n = lane & 31, kh = lane >> 5;
   const int64_t bm = blk / Nb, bn = blk - bm * Nb;
@@ -1874,54 +1898,67 @@ __global__ void __launch_bounds__(256) k_crt_finish(int64_t M, int64_t N, int64_
   for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
     for (int kb = 0; kb < 2; ++kb) aV[mb][kb] = *reinterpret_cast<const i32x4v *>(&T->vt[n + 32 * mb][32 * kb + 16 * kh]);
-  uint32_t Mw[CRT_MW];
-  double xsc[CRT_MW];
+  const int w0 = T->w0;
+  const double msc = T->msc;
+  uint32_t Mw[XW];  // the low XW words of M (uniform)
 #pragma unroll
-  for (int w = 0; w < CRT_MW; ++w) {
-    Mw[w] = T->M[w];
-    xsc[w] = T->xs[w];
-  }
-  // X'/M is within 1/4 of the integer k (|X| <= M/4), so a product with the reciprocal rounds to
-  // the same k as the quotient: its relative error (~2^-52) times |X'/M| < 2^13 is far below 1/4
-  const double inv_mtop = 1.0 / T->mtop;
+  for (int w = 0; w < XW; ++w) Mw[w] = T->M[w];
   const uint32_t alpha_sgn = (uint32_t)(alpha.hi >> 63);
   const int64_t col = bn * 32 + n;
   i32x16v zero;
 #pragma unroll
   for (int v = 0; v < 16; ++v) zero[v] = 0;
+  // The block's residue words, a row quarter rq at a time: the wave loads plane q (q < nq, one
+  // 256-byte row per plane: lane L holds column L%32 of rows 8 rq + 4 (L/32) + 0..3, one byte per
+  // row), and one swap of the lane halves per pair of words (q, q + 16) gives lane (n, kh) the words
+  // of moduli 16 kh + 0..15 (+ 32 for the second MFMA half) for both row halves hp: Q[hp][kb][j].
+  // Moduli 48.. are never used (nq <= 47).
+  const int8_t *bbase = RC + blk * OZ_FRAG;  // uniform
 #pragma nounroll
-  for (int hp = 0; hp < 2; ++hp) {
+  for (int rq = 0; rq < 4; ++rq) {
+    // all loads first, then the swaps; a plane beyond nq - 1 reads plane nq - 1 again: its
+    // column of the digit table (the MFMA's A operand) is zero, so those words add nothing
+    uint32_t w[48];
+    const int8_t *pq = bbase + rq * 256;  // plane min(q, nq - 1), advanced a plane at a time
+    int64_t pstep = plane;
+    asm volatile("" : "+s"(pstep));  // opaque here: the 48 plane offsets are not kept across rq
+#pragma unroll
+    for (int q = 0; q < 48; ++q) {
+      w[q] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(pq) + lane);
+      if (q + 1 < nq) pq += pstep;
+    }
+    uint32_t Q[2][2][16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      // lane (n, 0) keeps q = j of row half 0 and takes q = j of half 1 from lane (n, 1), which
+      // keeps q = 16 + j of half 1 and takes q = 16 + j of half 0
+      const auto s01 = __builtin_amdgcn_permlane32_swap(w[j], w[16 + j], false, false);
+      Q[0][0][j] = s01[0];
+      Q[1][0][j] = s01[1];
+      const auto s2 = __builtin_amdgcn_permlane32_swap(w[32 + j], 0u, false, false);
+      Q[0][1][j] = s2[0];
+      Q[1][1][j] = s2[1];
+    }
 #pragma nounroll
-    for (int rq = 0; rq < 4; ++rq) {
-      // residues of the four entries (rows 8 rq + 4 hp + e), moduli 32 kb + 16 kh + j
-      uint32_t Eo[2][4][4];
-#pragma unroll
-      for (int kb = 0; kb < 2; ++kb) {
-        uint32_t P[16];
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-          const int q = 32 * kb + 16 * kh + j;
-          // an unused modulus reads plane 0 (the same addresses as q = 0) and is zeroed: a select,
-          // not a branch around each load (3% of the kernel)
-          const int qq = (q < nq) ? q : 0;
-          const uint32_t v = *reinterpret_cast<const uint32_t *>(RC + qq * plane + blk * OZ_FRAG + rq * 256 + (n + 32 * hp) * 4);
-          P[j] = (q < nq) ? v : 0u;
-        }
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-          crt_t4(P[4 * v], P[4 * v + 1], P[4 * v + 2], P[4 * v + 3], Eo[kb][0][v], Eo[kb][1][v], Eo[kb][2][v],
-                 Eo[kb][3][v]);
-      }
-#pragma nounroll
-      for (int e = 0; e < 4; e += 2) {
-        // this pair's residues, selected without indexing the register array by e
+    for (int it = 0; it < 4; ++it) {
+      const int hp = it >> 1, e = 2 * (it & 1);
+      {
+        // this pair's residues: bytes e and e + 1 of the words, transposed so that Ec[kb][x][v]
+        // holds moduli 32 kb + 16 kh + 4 v + 0..3 of entry e + x
+        const uint32_t sel = e ? 0x07030602u : 0x05010400u;
         uint32_t Ec[2][2][4];
 #pragma unroll
         for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
-          for (int x = 0; x < 2; ++x)
+          for (int v = 0; v < 4; ++v) {
+            uint32_t P[4];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) Ec[kb][x][v] = e ? Eo[kb][2 + x][v] : Eo[kb][x][v];
+            for (int i = 0; i < 4; ++i) P[i] = hp ? Q[1][kb][4 * v + i] : Q[0][kb][4 * v + i];
+            const uint32_t a = __builtin_amdgcn_perm(P[1], P[0], sel);  // i0.e i1.e i0.e+1 i1.e+1
+            const uint32_t c = __builtin_amdgcn_perm(P[3], P[2], sel);  // i2.e i3.e i2.e+1 i3.e+1
+            Ec[kb][0][v] = __builtin_amdgcn_perm(c, a, 0x05040100u);
+            Ec[kb][1][v] = __builtin_amdgcn_perm(c, a, 0x07060302u);
+          }
         i32x16v D[2][2];  // [entry e / e + 1][row block]
 #pragma unroll
         for (int x = 0; x < 2; ++x)
@@ -1939,61 +1976,49 @@ __global__ void __launch_bounds__(256) k_crt_finish(int64_t M, int64_t N, int64_
             D[x][mb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(aV[mb][0], b0, zero, 0, 0, 0);
             D[x][mb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(aV[mb][1], b1, D[x][mb], 0, 0, 0);
           }
-        // runs of this lane: index a + 4 mb (a = v/4) is run 2a + kh + 8 mb
-        int64_t own[8], send[8];
+        // Lanes kh = 0 and 1 hold the byte sums s_b of entries e and e + 1 at complementary
+        // positions b = 8 (v/4) + 4 kh + 32 mb + v%4 (D[x][mb][v] for entry e + x).  One swap of
+        // the lane halves per register pair gives every lane all sums of its own entry e + kh:
+        // then s_b = D[(b/4)%2][b/32][4 ((b%32)/8) + b%4].
 #pragma unroll
         for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-          for (int a = 0; a < 4; ++a) {
-            const int64_t r0 = (int64_t)D[0][mb][4 * a] + ((int64_t)D[0][mb][4 * a + 1] << 8) +
-                               ((int64_t)D[0][mb][4 * a + 2] << 16) + ((int64_t)D[0][mb][4 * a + 3] << 24);
-            const int64_t r1 = (int64_t)D[1][mb][4 * a] + ((int64_t)D[1][mb][4 * a + 1] << 8) +
-                               ((int64_t)D[1][mb][4 * a + 2] << 16) + ((int64_t)D[1][mb][4 * a + 3] << 24);
-            own[a + 4 * mb] = kh ? r1 : r0;   // kh 0 finishes entry e, kh 1 entry e + 1
-            send[a + 4 * mb] = kh ? r0 : r1;
+          for (int v = 0; v < 16; ++v) {
+            const auto sw = __builtin_amdgcn_permlane32_swap(D[0][mb][v], D[1][mb][v], false, false);
+            D[0][mb][v] = (int)sw[0];
+            D[1][mb][v] = (int)sw[1];
           }
-        int64_t got[8];
+#define QB_SB(b) D[((b) >> 2) & 1][(b) >> 5][(((b)&31) >> 3) * 4 + ((b)&3)]
+        // k = round(X'/M), X' = sum_b s_b 256^b: the sums of words >= w0, each word's four sums
+        // combined exactly in double (below 2^45), scaled by 2^(32 (w - w0)) and the sum by
+        // 2^(32 w0)/M, give X'/M within 2^-14
+        // (the words below w0 add less than 2^-14 M; each term is below 2^29 M: V_q < M has no
+        // digit above byte ceil(log2(M) / 8)); X'/M is within 1/4 of k (|X| <= M/4)
+        double top = 0.0;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          const uint64_t sv = (uint64_t)send[i];
-          const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)sv, 32);
-          const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(sv >> 32), 32);
-          got[i] = (int64_t)(((uint64_t)hi << 32) | lo);
+        for (int w = CRT_MW - 1; w >= 0; --w) {
+          if (w < w0) continue;  // uniform (no break: the loop must unroll, D is indexed statically)
+          const double r = __builtin_fma((double)QB_SB(4 * w + 3), 16777216.0,
+                                         __builtin_fma((double)QB_SB(4 * w + 2), 65536.0,
+                                                       __builtin_fma((double)QB_SB(4 * w + 1), 256.0,
+                                                                     (double)QB_SB(4 * w))));
+          top += __builtin_ldexp(r, 32 * (w - w0));
         }
-        // X' = sum_m run_m 2^(32 m), m = 0..10 (runs 11.. are zero: V_q < M < 2^338)
+        const int nk = -(int)__builtin_rint(top * msc);  // -k, |k| <= 128 nq < 2^13
+        // X = X' - k M (its low XW words; |X| < 2^(32 XW - 1)), a word at a time: the byte sums
+        // (|s_b| < 2^20) in pairs minus k times the 16-bit halves of M's word stay below 2^30 in 32
+        // bits; the halves are combined and carried in 64 bits
         uint32_t X[CRT_MW];
         int64_t c = 0;
 #pragma unroll
-        for (int m = 0; m < CRT_MW; ++m) {
-          const int idx = ((m >> 1) & 3) + 4 * (m >> 3);  // position of run m in its lane's list
-          const bool mine = (m & 1) == kh;
-          const int64_t rv = (m < 16) ? (mine ? own[idx] : got[idx]) : 0;
-          const int64_t s2 = rv + c;
-          X[m] = (uint32_t)s2;
-          c = s2 >> 32;
+        for (int w = 0; w < XW; ++w) {
+          const int lo = __mul24(nk, (int)(Mw[w] & 0xffffu)) + QB_SB(4 * w) + (int)((uint32_t)QB_SB(4 * w + 1) << 8);
+          const int hi = __mul24(nk, (int)(Mw[w] >> 16)) + QB_SB(4 * w + 2) + (int)((uint32_t)QB_SB(4 * w + 3) << 8);
+          const int64_t v = (int64_t)lo + (int64_t)((uint64_t)(int64_t)hi << 16) + c;
+          X[w] = (uint32_t)v;
+          c = v >> 32;
         }
-        // k = round(X'/M) from the words of |X'| at M's two highest (|X'| < 2^13 M, so the words
-        // above are zero); X = X' - k M
-        int32_t kq;  // |k| <= 128 nq < 2^13
-        {
-          const uint32_t xs_neg = X[CRT_MW - 1] >> 31;
-          uint32_t mag[CRT_MW];
-          uint32_t cc = xs_neg;
-#pragma unroll
-          for (int w = 0; w < CRT_MW; ++w) mag[w] = addc32(xs_neg ? ~X[w] : X[w], 0u, cc, cc);
-          double top = 0;
-#pragma unroll
-          for (int w = 0; w < CRT_MW; ++w) top += (double)mag[w] * xsc[w];
-          const int32_t kmag = (int32_t)__builtin_rint(top * inv_mtop);
-          kq = xs_neg ? -kmag : kmag;
-        }
-        int64_t bc = 0;
-#pragma unroll
-        for (int w = 0; w < CRT_MW; ++w) {
-          const int64_t s2 = (int64_t)X[w] - (int64_t)kq * (int64_t)Mw[w] + bc;
-          X[w] = (uint32_t)s2;
-          bc = s2 >> 32;
-        }
+#undef QB_SB
         const int64_t row = bm * 32 + 8 * rq + 4 * hp + e + kh;
         if (row < M && col < N) {
           if (!FAST) {
@@ -2008,7 +2033,7 @@ __global__ void __launch_bounds__(256) k_crt_finish(int64_t M, int64_t N, int64_
               *reinterpret_cast<i32x4v *>(xr + w) = o;
             }
           } else {
-            const uint32_t sgn = X[CRT_MW - 1] >> 31;
+            const uint32_t sgn = X[XW - 1] >> 31;
             uint32_t mag[XW];
             uint32_t cc = sgn;
 #pragma unroll
@@ -2016,14 +2041,22 @@ __global__ void __launch_bounds__(256) k_crt_finish(int64_t M, int64_t N, int64_
             q128 *cp = C + row * csr + col * csc;
             bool done = false;
             q128 res;
-            // beta = 0: beta * y is a zero unless y is Inf or NaN (then the exact repair decides)
+            // beta = 0: beta * y is a zero unless y is Inf or NaN (then the exact repair decides).
+            // The entry is certified when |alpha (AB - P)| < 2^bexp, bexp = 1 + kc + Ei + Fj - t - 2
+            // (|alpha| = 1), is at most 2^-106 of the result, whose exponent is Ei + Fj - 2t + msb
+            // (+1 when rounding carries): that needs msb >= kc + t + 104, so a smaller |X| goes to
+            // the repair without rounding, and the rounding never needs a left shift.  A result
+            // outside the normal range is left to the repair too.
+            const int msb = rmsb<XW>(mag);
             if (!((fA[row] | fB[col]) & OZF_NONFINITE) && q_bexp(ldq_c(cp)) != 0x7fffu && eA[row] != OZ_EZERO &&
-                eB[col] != OZ_EZERO && !rzero<XW>(mag)) {
+                eB[col] != OZ_EZERO && msb >= kc + t + 104) {
               const int Ei = eA[row] + 3, Fj = eB[col] + 3;
-              res = rround<XW>(mag, Ei + Fj - 2 * t, sgn ^ alpha_sgn);
-              if (q_finite_nonzero(res)) {
-                const int bexp = 1 + kc + Ei + Fj - t - 2;  // |alpha (AB - P)| < 2^bexp, |alpha| = 1
-                done = bexp <= unpack_finite(res).e - 106;
+              const int lsb = Ei + Fj - 2 * t;
+              const int be = lsb + msb + QBIAS;  // biased exponent before a rounding carry
+              if (be >= 1 && be <= 0x7ffd) {
+                int rbe;
+                res = round_top<XW>(mag, msb, lsb, sgn ^ alpha_sgn, rbe);
+                done = 1 + kc + Ei + Fj - t - 2 <= rbe - QBIAS - 106;
               }
             }
             if (done) {
