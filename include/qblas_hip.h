@@ -181,8 +181,10 @@ void qblas_clear_error(void);
 /* After a device-API call in QBLAS_MODE_EXACT or QBLAS_MODE_OZAKI on this thread (the last such
    call; any later host-pointer call replaces it): synchronizes the stream and returns 1 if a term
    fell outside the full-range accumulator (the output was set to NaN; cannot happen for binary128
-   inputs), 0 if the result is exact or certified, -1 on a HIP error or when a later call on the
-   same stream has reused the call's workspace.  Host-pointer entry points check this themselves. */
+   inputs), 0 if the result is exact or certified, -1 on a HIP error, when a later call on the
+   same stream has reused the call's workspace, or when a graph replay of a batched OZAKI call found
+   the descriptors' shapes changed (nothing computed).  Host-pointer entry points check this
+   themselves. */
 int qblas_hip_exact_status(void *stream);
 
 /* After the same kind of call: synchronizes the stream and returns how many items (dots, gemv
@@ -266,10 +268,15 @@ typedef struct qblas_gemv_problem {
    gemm in QBLAS_MODE_OZAKI: the slice method with one slice count S for the whole batch (the tuned
    or selected configuration; configuration 0 is S = 17), every entry within 2^-104 relative error
    or recomputed exactly, per problem bit-identical to qblas_hip_gemm in QBLAS_MODE_OZAKI with
-   method QBLAS_OZAKI_SLICES and slice count S.  Unlike the other modes it synchronizes the
-   stream once (the per-problem work counts are summed on the device and read back to size the
-   workspace), so it cannot be captured in a graph.  qblas_hip_ozaki_last_slices,
-   qblas_hip_ozaki_repairs and qblas_hip_exact_status report on the batch. */
+   method QBLAS_OZAKI_SLICES and slice count S.  An eager call synchronizes the stream once (the
+   per-problem work counts are summed on the device and read back to size the workspace) and
+   remembers the totals for its (descriptor array, count, layout, configuration).  A call under
+   graph capture uses the totals of the last eager call with the same descriptor array, count,
+   layout and configuration (QBLAS_ERR_ARG if there was none) and neither synchronizes nor
+   allocates; each replay checks on the device that the descriptors still give those totals, and
+   if they do not, computes nothing (C untouched) and qblas_hip_exact_status returns QBLAS_ERR_ARG.
+   qblas_hip_ozaki_last_slices, qblas_hip_ozaki_repairs and qblas_hip_exact_status report on the
+   batch. */
 int qblas_hip_gemm_batched(int mode, int layout, int64_t count, const qblas_gemm_problem *d_problems, void *stream);
 int qblas_hip_gemv_batched(int mode, int layout, int64_t count, const qblas_gemv_problem *d_problems, void *stream);
 

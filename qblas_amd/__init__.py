@@ -317,7 +317,8 @@ def dev_gemm(mode, layout, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, stream=
 def dev_exact_status(stream=0):
     """1 if a term of the last EXACT/OZAKI-form device call of this thread fell outside the
     full-range accumulator (output NaN; cannot happen for binary128 inputs), 0 if the result is
-    exact, -1 on an error (including a workspace reused by a later call on the stream)."""
+    exact, -1 on an error (including a workspace reused by a later call on the stream, and a graph
+    replay of a batched OZAKI call whose descriptors no longer have the captured shapes)."""
     return lib().qblas_hip_exact_status(stream)
 
 

@@ -16,6 +16,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <map>
@@ -316,6 +317,11 @@ int check_exact_status(hipStream_t s) {
   if (st & 1u) {
     set_err_msg("exact form: a term fell outside the full-range accumulator; the output was set to NaN");
     return QBLAS_ERR_RANGE;
+  }
+  if (st & 2u) {
+    set_err_msg("batched OZAKI form replayed from a graph: the descriptors' shapes no longer match the captured "
+                "call; nothing was computed");
+    return QBLAS_ERR_ARG;
   }
   return QBLAS_OK;
 }
@@ -1013,12 +1019,47 @@ int batched_configs(int mode) {
   return mode == QBLAS_MODE_OZAKI ? qbk::ozaki_batched_configs() : qbk::gemm_batched_configs(mode == QBLAS_MODE_DD);
 }
 
+// Totals of batched OZAKI calls by (device, descriptor array, count, layout, slices): a call under
+// graph capture cannot read its totals back, so it sizes the workspace and grids from those of the
+// last eager call on the same descriptors, and the device checks at each replay that the
+// descriptors still give them (otherwise nothing is computed and the status reports it).
+using BoKey = std::tuple<int, uintptr_t, int64_t, int, int>;
+std::mutex g_bo_mu;
+std::map<BoKey, std::array<int64_t, qbk::OZ_BATCHED_TOTALS>> g_bo_plans;
+
 // Batched OZAKI form with S slices: the per-problem counts are summed on the device and their
 // totals read back (one synchronization) to size the workspace, then the grouped launches follow.
+// Under graph capture the totals of the last eager call on the same descriptors are used instead.
 int gemm_batched_ozaki(int layout, int64_t count, const qblas_gemm_problem *P, int S, hipStream_t s) {
   void *ws, *ws2;
   const StreamWorkspaces::Entry *we = nullptr;
   int rc;
+  int dev = 0;
+  QB_CHECK(hipGetDevice(&dev));
+  const BoKey key(dev, (uintptr_t)P, count, layout != 0, S);
+  if (stream_capturing(s)) {
+    std::array<int64_t, qbk::OZ_BATCHED_TOTALS> plan;
+    {
+      std::lock_guard<std::mutex> lk(g_bo_mu);
+      auto it = g_bo_plans.find(key);
+      if (it == g_bo_plans.end()) {
+        set_err_msg("qblas_hip_gemm_batched: QBLAS_MODE_OZAKI under graph capture needs one eager call on the same "
+                    "descriptors (same count, layout and configuration) first");
+        return QBLAS_ERR_ARG;
+      }
+      plan = it->second;
+    }
+    if ((rc = batch_ws(qbk::gemm_ozaki_batched_ws(count, plan.data(), S), s, &ws2, &we))) return rc;
+    QB_CHECK(qbk::launch_gemm_ozaki_batched_scan(layout != 0, count, P, ws2, s));
+    QB_CHECK(qbk::launch_gemm_ozaki_batched(layout != 0, count, P, S, plan.data(), ws2, s, true));
+    set_status(ws2, we, true);
+    t_oz_method = QBLAS_OZAKI_SLICES;
+    t_oz_slices = S;
+    t_oz_bits = 0;
+    t_oz_moduli = 0;
+    t_oz_ev_valid = false;
+    return QBLAS_OK;
+  }
   if ((rc = batch_ws(qbk::gemm_ozaki_batched_scan_ws(count), s, &ws))) return rc;
   QB_CHECK(qbk::launch_gemm_ozaki_batched_scan(layout != 0, count, P, ws, s));
   int64_t tot[qbk::OZ_BATCHED_TOTALS];
@@ -1028,6 +1069,11 @@ int gemm_batched_ozaki(int layout, int64_t count, const qblas_gemm_problem *P, i
   if ((rc = batch_ws(qbk::gemm_ozaki_batched_ws(count, tot, S), s, &ws2, &we))) return rc;
   if (ws2 != ws) QB_CHECK(qbk::launch_gemm_ozaki_batched_scan(layout != 0, count, P, ws2, s));  // grown: new buffer
   QB_CHECK(qbk::launch_gemm_ozaki_batched(layout != 0, count, P, S, tot, ws2, s));
+  {
+    std::lock_guard<std::mutex> lk(g_bo_mu);
+    auto &pl = g_bo_plans[key];
+    for (int f = 0; f < qbk::OZ_BATCHED_TOTALS; ++f) pl[f] = tot[f];
+  }
   set_status(ws2, we, true);
   t_oz_method = QBLAS_OZAKI_SLICES;
   t_oz_slices = S;

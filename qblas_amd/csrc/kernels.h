@@ -122,6 +122,7 @@ hipError_t launch_gemv_batched(int form, int colmajor, int64_t count, const qbla
 // 9 totals at ws + gemm_ozaki_batched_totals_offset, grows ws to gemm_ozaki_batched_ws (running the
 // scan again when ws moved) and launches the rest.  ws word 0 is the status word.
 constexpr int OZ_BATCHED_TOTALS = 9;
+constexpr int OZ_HDR_STALE = 8;  // header word: planned totals did not match (nothing computed)
 int ozaki_batched_configs();
 int ozaki_batched_slices(int cfg);  // slice count of configuration cfg (0 = default)
 size_t gemm_ozaki_batched_scan_ws(int64_t count);
@@ -130,7 +131,7 @@ hipError_t launch_gemm_ozaki_batched_scan(int colmajor, int64_t count, const qbl
                                           hipStream_t st);
 size_t gemm_ozaki_batched_ws(int64_t count, const int64_t *tot, int S);
 hipError_t launch_gemm_ozaki_batched(int colmajor, int64_t count, const qblas_gemm_problem *P, int S,
-                                     const int64_t *tot, void *ws, hipStream_t st);
+                                     const int64_t *tot, void *ws, hipStream_t st, bool planned = false);
 
 // hooks into the C ABI's thread-local state (capi.hip): the last-error message, and forgetting the
 // EXACT-form status of earlier calls

@@ -1107,9 +1107,28 @@ __global__ void __launch_bounds__(1024) k_bo_scan(const qblas_gemm_problem *__re
 
 // Scaling exponents of every problem's A rows (which = 0) or B columns (which = 1): one wave per
 // contiguous row (lanes along k), or one wave per 64 strided rows (a lane per row).
+// Graph replays reuse the totals of the call that was captured: when the problems' shapes now give
+// other totals, header word OZ_HDR_STALE and status bit 1 are set and every later kernel of the
+// call returns at once (the planned workspace and grids would not fit them).
+struct bo_totals {
+  int64_t t[BO_F];
+};
+__global__ void k_bo_check(const int64_t *__restrict__ off, int64_t count, bo_totals plan, uint32_t *__restrict__ hdr) {
+  if (threadIdx.x != 0) return;
+  bool same = true;
+#pragma unroll
+  for (int f = 0; f < BO_F; ++f) same = same && off[count * BO_F + f] == plan.t[f];
+  if (!same) {
+    hdr[OZ_HDR_STALE] = 1u;
+    hdr[0] |= 2u;
+  }
+}
+
 __global__ void __launch_bounds__(256) k_bo_scale(const qblas_gemm_problem *__restrict__ P, int64_t count,
                                                   const int64_t *__restrict__ off, int colmajor, int which,
-                                                  int64_t items, int32_t *__restrict__ E, uint32_t *__restrict__ F) {
+                                                  int64_t items, int32_t *__restrict__ E, uint32_t *__restrict__ F,
+                                                  const uint32_t *__restrict__ hdr) {
+  if (hdr[OZ_HDR_STALE]) return;  // the problems differ from the plan: nothing is computed
   const int lane = threadIdx.x & 63;
   const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (w >= items) return;  // whole waves
@@ -1147,7 +1166,8 @@ template <int S>
 __global__ void __launch_bounds__(256) k_bo_slice(const qblas_gemm_problem *__restrict__ P, int64_t count,
                                                   const int64_t *__restrict__ off, int colmajor, int which,
                                                   int64_t nthreads, const int32_t *__restrict__ E,
-                                                  int8_t *__restrict__ out) {
+                                                  int8_t *__restrict__ out, const uint32_t *__restrict__ hdr) {
+  if (hdr[OZ_HDR_STALE]) return;
   const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (g >= nthreads) return;
   const int64_t chunk = g >> 6;
@@ -1165,7 +1185,9 @@ __global__ void __launch_bounds__(256) k_bo_slice(const qblas_gemm_problem *__re
 template <int S>
 __global__ void __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(1, 1)))
 k_bo_gemm(const qblas_gemm_problem *__restrict__ P, int64_t count, const int64_t *__restrict__ off, int colmajor,
-          const int8_t *__restrict__ As, const int8_t *__restrict__ Bs, uint32_t *__restrict__ part) {
+          const int8_t *__restrict__ As, const int8_t *__restrict__ Bs, uint32_t *__restrict__ part,
+          const uint32_t *__restrict__ hdr) {
+  if (hdr[OZ_HDR_STALE]) return;  // uniform: before any barrier
   const int64_t t = blockIdx.x;
   const int64_t p = bo_find(off, count, BO_TILES, t);
   const bo_geom G = bo_geometry(P[p], colmajor);
@@ -1184,6 +1206,7 @@ __global__ void __launch_bounds__(256) k_bo_finish(const qblas_gemm_problem *__r
                                                    const uint32_t *__restrict__ fA, const int32_t *__restrict__ eB,
                                                    const uint32_t *__restrict__ fB, uint32_t *__restrict__ hdr,
                                                    uint32_t *__restrict__ list) {
+  if (hdr[OZ_HDR_STALE]) return;  // nothing queued: the repair has nothing to do either
   const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (g >= total) return;
   const int64_t p = bo_find(off, count, BO_ENT, g);
@@ -1293,7 +1316,7 @@ size_t gemm_ozaki_batched_ws(int64_t count, const int64_t *tot, int S) { return 
 
 template <int S>
 static void bo_launch(int colmajor, int64_t count, const qblas_gemm_problem *P, const int64_t *tot, const int64_t *off,
-                      const bo_layout &L, char *ws, hipStream_t st) {
+                      const bo_layout &L, char *ws, hipStream_t st, bool planned) {
   int32_t *eA = (int32_t *)(ws + L.eA), *eB = (int32_t *)(ws + L.eB);
   uint32_t *fA = (uint32_t *)(ws + L.fA), *fB = (uint32_t *)(ws + L.fB);
   int8_t *sA = (int8_t *)(ws + L.sA), *sB = (int8_t *)(ws + L.sB);
@@ -1301,14 +1324,19 @@ static void bo_launch(int colmajor, int64_t count, const qblas_gemm_problem *P, 
   const int64_t ninit = tot[BO_ROWS] > tot[BO_COLS] ? tot[BO_ROWS] : tot[BO_COLS];
   const int64_t ni = ninit > OZ_HDR / 4 ? ninit : OZ_HDR / 4;
   k_oz_init<<<(unsigned)((ni + 255) / 256), 256, 0, st>>>(eA, fA, tot[BO_ROWS], eB, fB, tot[BO_COLS], hdr);
+  if (planned) {
+    bo_totals pt;
+    for (int f = 0; f < BO_F; ++f) pt.t[f] = tot[f];
+    k_bo_check<<<1, 64, 0, st>>>(off, count, pt, hdr);
+  }
   if (tot[BO_SCA] > 0)
-    k_bo_scale<<<(unsigned)((tot[BO_SCA] + 3) / 4), 256, 0, st>>>(P, count, off, colmajor, 0, tot[BO_SCA], eA, fA);
+    k_bo_scale<<<(unsigned)((tot[BO_SCA] + 3) / 4), 256, 0, st>>>(P, count, off, colmajor, 0, tot[BO_SCA], eA, fA, hdr);
   if (tot[BO_SCB] > 0)
-    k_bo_scale<<<(unsigned)((tot[BO_SCB] + 3) / 4), 256, 0, st>>>(P, count, off, colmajor, 1, tot[BO_SCB], eB, fB);
+    k_bo_scale<<<(unsigned)((tot[BO_SCB] + 3) / 4), 256, 0, st>>>(P, count, off, colmajor, 1, tot[BO_SCB], eB, fB, hdr);
   const int64_t ta = tot[BO_FRAGA] * 64, tb = tot[BO_FRAGB] * 64;
-  k_bo_slice<S><<<(unsigned)((ta + 255) / 256), 256, 0, st>>>(P, count, off, colmajor, 0, ta, eA, sA);
-  k_bo_slice<S><<<(unsigned)((tb + 255) / 256), 256, 0, st>>>(P, count, off, colmajor, 1, tb, eB, sB);
-  k_bo_gemm<S><<<(unsigned)tot[BO_TILES], 256, 0, st>>>(P, count, off, colmajor, sA, sB, part);
+  k_bo_slice<S><<<(unsigned)((ta + 255) / 256), 256, 0, st>>>(P, count, off, colmajor, 0, ta, eA, sA, hdr);
+  k_bo_slice<S><<<(unsigned)((tb + 255) / 256), 256, 0, st>>>(P, count, off, colmajor, 1, tb, eB, sB, hdr);
+  k_bo_gemm<S><<<(unsigned)tot[BO_TILES], 256, 0, st>>>(P, count, off, colmajor, sA, sB, part, hdr);
   k_bo_finish<<<(unsigned)((tot[BO_ENT] + 255) / 256), 256, 0, st>>>(P, count, off, colmajor, S, tot[BO_ENT], part,
                                                                        eA, fA, eB, fB, hdr, list);
   uint32_t *wq = list + 4 * tot[BO_ENT];
@@ -1317,10 +1345,12 @@ static void bo_launch(int colmajor, int64_t count, const qblas_gemm_problem *P, 
 }
 
 // After launch_gemm_ozaki_batched_scan on the same workspace: tot = the totals it wrote (read back
-// by the caller), ws at least gemm_ozaki_batched_ws(count, tot, S) bytes.  Header words of ws: 0
-// status, 1 entries queued for the exact repair, 2 entries of them on the full-range path.
+// by the caller), or, when planned, totals of an earlier call that the scan's are checked against
+// on the device (no synchronization: graph capture); ws at least gemm_ozaki_batched_ws(count, tot,
+// S) bytes.  Header words of ws: 0 status, 1 entries queued for the exact repair, 2 entries of them
+// on the full-range path, OZ_HDR_STALE set when planned totals did not match.
 hipError_t launch_gemm_ozaki_batched(int colmajor, int64_t count, const qblas_gemm_problem *P, int S,
-                                     const int64_t *tot, void *wsv, hipStream_t st) {
+                                     const int64_t *tot, void *wsv, hipStream_t st, bool planned) {
   if (tot[BO_ENT] == 0) return hipSuccess;
   // grids: one workgroup per tile, one thread per entry and per (fragment position, lane), one
   // wave per scaling item; queued entries are numbered in 32 bits
@@ -1335,7 +1365,7 @@ hipError_t launch_gemm_ozaki_batched(int colmajor, int64_t count, const qblas_ge
   const int64_t *off = (const int64_t *)(ws + OZ_HDR);
   switch (S) {
 #define QB_BO_CASE(s) \
-  case s: bo_launch<s>(colmajor, count, P, tot, off, L, ws, st); break;
+  case s: bo_launch<s>(colmajor, count, P, tot, off, L, ws, st, planned); break;
     QB_BO_CASE(14) QB_BO_CASE(15) QB_BO_CASE(16) QB_BO_CASE(17) QB_BO_CASE(18) QB_BO_CASE(19)
 #undef QB_BO_CASE
     default: return hipErrorInvalidValue;
@@ -1599,7 +1629,7 @@ __device__ __forceinline__ i32x4v crt_pack16(const i32x16v &d, int p, uint32_t m
 // dword of one modulus.
 // Groups 0..15 use slots 0..31 (k < 16), groups 16..31 slots 32..63.
 constexpr int CRT_RS = 17;  // LDS row stride (elements) of the residue kernel
-__global__ void __launch_bounds__(256) k_crt_residues(const q128 *__restrict__ X, int64_t R, int64_t K, int64_t sr,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) k_crt_residues(const q128 *__restrict__ X, int64_t R, int64_t K, int64_t sr,
                                                       int64_t sc, const int32_t *__restrict__ E, int t,
                                                       const crt_tab *__restrict__ T, int64_t Rb, int64_t Kb,
                                                       int64_t plane, int8_t *__restrict__ out) {
@@ -1637,21 +1667,25 @@ __global__ void __launch_bounds__(256) k_crt_residues(const q128 *__restrict__ X
   for (int v = 0; v < 16; ++v) zero[v] = 0;
   int8_t *o0 = out + (int64_t)n * plane + pos * OZ_FRAG;
   const int hh = (n >> 2) & 1, ib = 4 * (n >> 3) + (n & 3);
+  // element (r, c), c < 16 of a half: k contiguous -> lanes 16 apart read 4 rows of 16; rows
+  // contiguous -> lanes read 32 rows of 2 columns.  The second half's elements are loaded before
+  // the first half's digit sums run, so their latency is hidden behind the MFMA work.
+  q128 v[8];
+  int32_t ev[8];
+  auto load_half = [&](int half) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int r = kcontig ? 4 * i + (lane >> 4) : n, c = kcontig ? (lane & 15) : 2 * i + h;
+      const int64_t row = rb * 32 + r, col = kb * 32 + 16 * half + c;
+      const bool in = row < R && col < K;
+      v[i] = in ? ldq_c(X + row * sr + col * sc) : q_zero(0);
+      ev[i] = in ? E[row] : OZ_EZERO;
+    }
+  };
+  load_half(0);
 #pragma nounroll
   for (int half = 0; half < 2; ++half) {
     {
-      // element (r, c), c < 16 of this half: k contiguous -> lanes 16 apart read 4 rows of 16;
-      // rows contiguous -> lanes read 32 rows of 2 columns
-      q128 v[8];
-      int32_t ev[8];
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int r = kcontig ? 4 * i + (lane >> 4) : n, c = kcontig ? (lane & 15) : 2 * i + h;
-        const int64_t row = rb * 32 + r, col = kb * 32 + 16 * half + c;
-        const bool in = row < R && col < K;
-        v[i] = in ? ldq_c(X + row * sr + col * sc) : q_zero(0);
-        ev[i] = in ? E[row] : OZ_EZERO;
-      }
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
         const int r = kcontig ? 4 * i + (lane >> 4) : n, c = kcontig ? (lane & 15) : 2 * i + h;
@@ -1667,6 +1701,7 @@ __global__ void __launch_bounds__(256) k_crt_residues(const q128 *__restrict__ X
         s_hi[wave][e] = W[4] ^ 0x80808080u;
       }
     }
+    if (half == 0) load_half(1);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

@@ -199,6 +199,64 @@ def test_ozaki_batched_tuner(O, Q, cuda):
     Q.gemm_batched_select(Q.MODE_OZAKI, layout, len(dims), 0)
 
 
+def test_ozaki_batched_graph_replay(O, Q, cuda):
+    """An eager call, then the same call captured in a graph: every replay (with eager work
+    between replays) gives the eager result bit for bit; a replay after a descriptor's shape
+    changed computes nothing and reports it; restoring the shape makes replays correct again."""
+    layout = 0
+    dims, arrays, pool, ddesc = make_gemm_batch(O, cuda, layout, 40, 8, 160, 83, empties=True)
+    count = len(dims)
+    C0 = [arrays[3 * p + 2] for p in range(count)]
+    desc_host = ddesc.clone()
+    _say("batched OZAKI: eager call")
+    Q.dev_gemm_batched(Q.MODE_OZAKI, layout, count, ddesc.data_ptr())
+    cuda.cuda.synchronize()
+    assert Q.dev_exact_status() == 0
+    want = [pool.get(3 * p + 2, C0[p].shape[0]) for p in range(count)]
+    _check_oracle(O, layout, dims, arrays, C0, lambda p: want[p], np.random.default_rng(8), per_problem=60)
+    _restore(pool, cuda, C0, lambda p: 3 * p + 2)
+    cuda.cuda.synchronize()
+    _say("batched OZAKI: capture")
+    g = cuda.cuda.CUDAGraph()
+    s = cuda.cuda.Stream()
+    with cuda.cuda.stream(s):
+        with cuda.cuda.graph(g, stream=s):
+            Q.dev_gemm_batched(Q.MODE_OZAKI, layout, count, ddesc.data_ptr(), s.cuda_stream)
+    cuda.cuda.synchronize()
+    for r in range(3):
+        _say(f"batched OZAKI: replay {r}")
+        _restore(pool, cuda, C0, lambda p: 3 * p + 2)
+        cuda.cuda.synchronize()
+        g.replay()
+        cuda.cuda.synchronize()
+        assert Q.dev_exact_status() == 0
+        for p in range(count):
+            assert np.array_equal(pool.get(3 * p + 2, C0[p].shape[0]), want[p]), (r, p)
+        scratch = cuda.ones(1 << 20, device="cuda:0") * 3.0   # eager work between replays
+        assert float(scratch.sum()) == 3.0 * (1 << 20)
+    _say("batched OZAKI: replay after a shape change")
+    desc = desc_host.cpu().numpy().view(Q.GEMM_PROBLEM).copy()
+    q = next(p for p, d in enumerate(dims) if d[0] > 1 and d[1] and d[2])
+    desc[q]["m"] -= 1
+    ddesc.copy_(cuda.from_numpy(desc.view(np.uint8)).to("cuda:0"))
+    _restore(pool, cuda, C0, lambda p: 3 * p + 2)
+    cuda.cuda.synchronize()
+    g.replay()
+    cuda.cuda.synchronize()
+    assert Q.dev_exact_status() != 0
+    for p in range(count):
+        assert np.array_equal(pool.get(3 * p + 2, C0[p].shape[0]), C0[p]), ("C written by a stale replay", p)
+    ddesc.copy_(desc_host)
+    _restore(pool, cuda, C0, lambda p: 3 * p + 2)
+    cuda.cuda.synchronize()
+    g.replay()
+    cuda.cuda.synchronize()
+    assert Q.dev_exact_status() == 0
+    for p in range(count):
+        assert np.array_equal(pool.get(3 * p + 2, C0[p].shape[0]), want[p]), ("after restore", p)
+    del g
+
+
 def test_ozaki_batched_rejected_for_gemv(Q, cuda):
     desc = cuda.zeros(64, dtype=cuda.uint8, device="cuda:0")
     with pytest.raises(RuntimeError):
