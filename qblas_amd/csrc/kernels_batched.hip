@@ -307,8 +307,9 @@ __device__ __forceinline__ dd bshfl_xor_dd(dd v, int msk) {
 // lane, butterfly wave sums, and lane r finishes row r.
 // ColMajor: an item is 64 rows, one per lane, over the columns (2 accumulators).
 constexpr int BGV_CHUNK = 4;  // work items a wave takes in a row
+template <int colmajor>
 __global__ void __launch_bounds__(256) k_gemv_batched_dd(const qblas_gemv_problem *__restrict__ P, int64_t count,
-                                                         const int64_t *__restrict__ start, int colmajor) {
+                                                         const int64_t *__restrict__ start) {
   const int lane = threadIdx.x & 63;
   const int64_t total = start[count];
   const int64_t waves = (int64_t)gridDim.x * 4, w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -327,6 +328,9 @@ __global__ void __launch_bounds__(256) k_gemv_batched_dd(const qblas_gemv_proble
       if (!colmajor) {
         const int64_t i0 = ul * 4;
         const int R = (m - i0 < 4) ? (int)(m - i0) : 4;  // rows of this item (wave-uniform)
+        // y of the row this lane finishes, loaded now: its latency passes during the column loop
+        q128 *yi = y + (i0 + (lane < R ? lane : 0)) * incy;
+        const q128 yv = bld(yi);
         dd acc[4][2];
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc[r][0] = acc[r][1] = dd_zero();
@@ -361,10 +365,7 @@ __global__ void __launch_bounds__(256) k_gemv_batched_dd(const qblas_gemv_proble
             if (lane == r) mine = s;
           }
         }
-        if (lane < R) {
-          q128 *yi = y + (i0 + lane) * incy;
-          bst(yi, fma_packed(alpha, dd_to_q(mine), mul_packed(beta, bld(yi))));
-        }
+        if (lane < R) bst(yi, fma_packed(alpha, dd_to_q(mine), mul_packed(beta, yv)));
       } else {
         const int64_t i = ul * 64 + lane;
         if (i < m) {
@@ -602,7 +603,10 @@ hipError_t launch_gemv_batched(int form, int colmajor, int64_t count, const qbla
   if (form == 0) {
     k_gemv_batched_ref<<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor);
   } else if (form == 1) {
-    k_gemv_batched_dd<<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor);
+    if (colmajor)
+      k_gemv_batched_dd<1><<<BATCH_GRID, 256, 0, st>>>(P, count, start);
+    else
+      k_gemv_batched_dd<0><<<BATCH_GRID, 256, 0, st>>>(P, count, start);
   } else {
     uint32_t *status = (uint32_t *)((char *)ws + batched_status_offset(count));
     k_batch_status_zero<<<1, 64, 0, st>>>(status);
