@@ -169,28 +169,72 @@ __global__ void k_gemv_col_ref(int64_t m, int64_t n, const q128 *__restrict__ A,
 // One lane per C element.  k is processed in blocks of kc; each block is a fresh chain t and
 // C = fma(alpha, t, mul(first ? beta : 1, C)) (level3.hpp:297-300 with the micro-kernel's
 // epilogue :102-109; gemm_simple :200-210 is the single-block case).  Layout-aware indexing for
-// both layouts.  A 16x16 thread tile walks C; lanes sharing a row/column hit the same A/B lines.
+// both layouts.  A workgroup computes a 16 x 16 tile of C: per step of GR_KT values of l, its 256
+// threads load the tile's 16 x GR_KT elements of A and GR_KT x 16 of B once (coalesced along the
+// contiguous dimension), unpack them into LDS, and every lane runs its chain over the step from
+// LDS, in the same order l = kk, kk + 1, ... as the reference.  An operand that is zero,
+// subnormal, Inf or NaN is marked in LDS and taken from memory again for the packed slow path.
+constexpr int GR_KT = 32;
+constexpr int32_t GR_SPECIAL = INT32_MIN;  // uq.e of an operand the fast path does not take
+__device__ __forceinline__ uq gr_unpack(q128 v) {
+  bool sp;
+  uq u = unpack_fast(v, sp);
+  if (sp) u.e = GR_SPECIAL;
+  return u;
+}
 __global__ void __launch_bounds__(256) k_gemm_ref(int colmajor, int64_t m, int64_t n, int64_t k, int64_t kc,
                                                   q128 alpha, const q128 *__restrict__ A, int64_t lda,
                                                   const q128 *__restrict__ B, int64_t ldb, q128 beta,
                                                   q128 *__restrict__ C, int64_t ldc) {
-  const int64_t i = (int64_t)blockIdx.y * 16 + (threadIdx.x & 15);
-  const int64_t j = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
-  if (i >= m || j >= n) return;
+  __shared__ uq sA[GR_KT][16];
+  __shared__ uq sB[GR_KT][16];
+  const int ti = threadIdx.x & 15, tj = threadIdx.x >> 4;
+  const int64_t i0 = (int64_t)blockIdx.y * 16, j0 = (int64_t)blockIdx.x * 16;
+  const int64_t i = i0 + ti, j = j0 + tj;
+  const bool live = i < m && j < n;  // every thread takes part in the loads and barriers
   const int64_t as_i = colmajor ? 1 : lda, as_l = colmajor ? lda : 1;
   const int64_t bs_l = colmajor ? 1 : ldb, bs_j = colmajor ? ldb : 1;
   q128 *ci = C + (colmajor ? j * ldc + i : i * ldc + j);
-  q128 c = ld(ci);
+  q128 c = live ? ld(ci) : q_zero(0);
   const q128 *ap = A + i * as_i;
   const q128 *bp = B + j * bs_j;
   for (int64_t kk = 0; kk < k; kk += kc) {
     const int64_t ke = (kk + kc < k) ? kk + kc : k;
     chain ch;
     chain_init(ch);
-    for (int64_t l = kk; l < ke; ++l) chain_fma(ch, ld(ap + l * as_l), ld(bp + l * bs_l));
+    for (int64_t l0 = kk; l0 < ke; l0 += GR_KT) {
+      const int lt = (int)((ke - l0 < GR_KT) ? ke - l0 : GR_KT);
+      __syncthreads();  // the previous step's operands are no longer read
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int e = (int)threadIdx.x + 256 * h;  // 512 = 16 x GR_KT elements of each operand
+        // A(i0 + r, l0 + q): l contiguous (RowMajor) -> q fastest; rows contiguous -> r fastest
+        const int ra = colmajor ? (e & 15) : (e >> 5), qa = colmajor ? (e >> 4) : (e & 31);
+        if (qa < lt && i0 + ra < m) sA[qa][ra] = gr_unpack(ld(A + (i0 + ra) * as_i + (l0 + qa) * as_l));
+        // B(l0 + q, j0 + c): columns contiguous (RowMajor) -> c fastest; l contiguous -> q fastest
+        const int cb = colmajor ? (e >> 5) : (e & 15), qb = colmajor ? (e & 31) : (e >> 4);
+        if (qb < lt && j0 + cb < n) sB[qb][cb] = gr_unpack(ld(B + (l0 + qb) * bs_l + (j0 + cb) * bs_j));
+      }
+      __syncthreads();
+      if (live) {
+        for (int q = 0; q < lt; ++q) {
+          const uq ua = sA[q][ti], ub = sB[q][tj];
+          bool done = false;
+          if ((ua.e != GR_SPECIAL) & (ub.e != GR_SPECIAL) & !ch.special) {
+            bool ovf = false;
+            const uq r = fma_uq(ua, ub, ch.acc, ovf);
+            if (!ovf) {
+              ch.acc = r;
+              done = true;
+            }
+          }
+          if (!done) chain_set(ch, fma_packed(ld(ap + (l0 + q) * as_l), ld(bp + (l0 + q) * bs_l), chain_get(ch)));
+        }
+      }
+    }
     c = fma_packed(alpha, chain_get(ch), mul_packed(kk == 0 ? beta : q_one(), c));
   }
-  st(ci, c);
+  if (live) st(ci, c);
 }
 
 // ------------------------------------------------------------------------------------------
