@@ -11,8 +11,8 @@
 //   1 <= o <= 126, leaving 32 bits of headroom for carries (<= 2^32 additions per window).  Each
 //   lane keeps two such windows, 124 bits apart (win2_add_product), anchored on the centre of the
 //   first products of its wave, so products within +-124 binary orders of magnitude of that centre
-//   are exact.  A product outside both windows is not added and sets the LOST flag: the output is
-//   then NaN and the call reports QBLAS_ERR_RANGE.
+//   are exact.  A product outside both windows is not added and sets the LOST flag: the dot, row or
+//   entry is then recomputed on the full-range path (wide.hpp), which holds every binary128 product.
 // Stage 2 (combine): windows with possibly different anchors are summed exactly into a wide
 //   accumulator (ACCW words), then alpha*S + beta*y is formed exactly as big integers and rounded
 //   once.
