@@ -33,7 +33,7 @@ __global__ void __launch_bounds__(XD_BS) k_dot_exact_partial(const q128 *__restr
                                                              const q128 *__restrict__ y, int64_t incy, int64_t n,
                                                              part_t *__restrict__ parts, uint32_t *__restrict__ hdr) {
   __shared__ int s_mn[XD_BS / 64], s_mx[XD_BS / 64];
-  __shared__ uint32_t s_w[XD_BS / 64][2 * WW + 1];
+  __shared__ uint32_t s_fl[XD_BS / 64];
   __shared__ unsigned long long s_dig[XD_BS / 64][DIG][64];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (blockIdx.x == 0 && threadIdx.x < 4) hdr[threadIdx.x] = 0;  // status, counts, ticket of the combine
@@ -41,9 +41,14 @@ __global__ void __launch_bounds__(XD_BS) k_dot_exact_partial(const q128 *__restr
   dig_clear(d);
   const int64_t stride = (int64_t)gridDim.x * XD_BS;
   const int64_t i0 = (int64_t)blockIdx.x * XD_BS + threadIdx.x;
+  // the first two element pairs are loaded before the anchor is known (they set it) and are the
+  // first ones accumulated
+  const q128 zq = q_zero(0);
+  const bool h0 = i0 < n, h1 = i0 + stride < n;
+  const q128 a0 = h0 ? ldq_nt(x + i0 * incx) : zq, b0 = h0 ? ldq_nt(y + i0 * incy) : zq;
+  const q128 a1 = h1 ? ldq_nt(x + (i0 + stride) * incx) : zq, b1 = h1 ? ldq_nt(y + (i0 + stride) * incy) : zq;
   // block-uniform anchor, centred between the smallest and largest first products of the block
-  int e0 = EXP_NONE;
-  if (i0 < n) e0 = pair_exp(ldq_c(x + i0 * incx), ldq_c(y + i0 * incy));
+  const int e0 = h0 ? pair_exp(a0, b0) : EXP_NONE;
   const int mn = wave_min(e0), mx = wave_max(e0 == EXP_NONE ? -EXP_NONE : e0);
   if (lane == 0) {
     s_mn[wv] = mn;
@@ -58,46 +63,53 @@ __global__ void __launch_bounds__(XD_BS) k_dot_exact_partial(const q128 *__restr
   }
   const int A = (bmn == EXP_NONE) ? 0 : ((bmn + bmx) >> 1) - WIN2_CENTER;
   uint32_t flags = 0;
-  int64_t i = i0;
+  if (h0) dig_accumulate(d, A, a0, b0, flags);
+  if (h1) dig_accumulate(d, A, a1, b1, flags);
+  int64_t i = i0 + 2 * stride;
   if (incx == 1 && incy == 1) {
+    // two element pairs in flight per step (four: 10% slower)
     for (; i + stride < n; i += 2 * stride) {
-      const q128 a0 = ldq_nt(x + i), b0 = ldq_nt(y + i);
-      const q128 a1 = ldq_nt(x + i + stride), b1 = ldq_nt(y + i + stride);
-      dig_accumulate(d, A, a0, b0, flags);
-      dig_accumulate(d, A, a1, b1, flags);
+      const q128 u0 = ldq_nt(x + i), v0 = ldq_nt(y + i);
+      const q128 u1 = ldq_nt(x + i + stride), v1 = ldq_nt(y + i + stride);
+      dig_accumulate(d, A, u0, v0, flags);
+      dig_accumulate(d, A, u1, v1, flags);
     }
-    if (i < n) dig_accumulate(d, A, ldq_nt(x + i), ldq_nt(y + i), flags);
+    for (; i < n; i += stride) dig_accumulate(d, A, ldq_nt(x + i), ldq_nt(y + i), flags);
   } else {
     for (; i < n; i += stride) dig_accumulate(d, A, ldq_nt(x + i * incx), ldq_nt(y + i * incy), flags);
   }
-  uint32_t Wl[WW], Wh[WW];
-  dig_to_windows(d, Wl, Wh);
-  wave_sum_win(Wl);
-  wave_sum_win(Wh);
+  // block reduction in the digit domain: digit k of the block is the int64 sum of the 256 lanes'
+  // digit k (each lane's digit takes < 2^29 + 1 per product: no overflow for any practical n);
+  // DP threads per digit each add 256 / DP of them, then a shuffle reduction over the DP threads
+  constexpr int DP = 8;
+  static_assert(DIG * DP <= XD_BS && 64 % DP == 0, "digit sums: DP threads per digit within one wave");
   flags = wave_or(flags);
-  if (lane == 0) {
+  if (lane == 0) s_fl[wv] = flags;
+  __syncthreads();  // every lane's digits are final
+  if (threadIdx.x < DIG * DP) {
+    const int k = threadIdx.x / DP, q = threadIdx.x % DP;
+    int64_t sum = 0;
 #pragma unroll
-    for (int k = 0; k < WW; ++k) {
-      s_w[wv][k] = Wl[k];
-      s_w[wv][WW + k] = Wh[k];
+    for (int w = 0; w < XD_BS / 64; ++w) {
+#pragma unroll
+      for (int j = 0; j < 64 / DP; ++j) sum += (int64_t)s_dig[w][k][q + DP * j];
     }
-    s_w[wv][2 * WW] = flags;
+#pragma unroll
+    for (int m = 1; m < DP; m <<= 1) sum += __shfl_xor(sum, m);
+    if (q == 0) s_dig[0][k][0] = (unsigned long long)sum;  // only this thread reads that slot
   }
   __syncthreads();
   if (threadIdx.x == 0) {
+    uint32_t Wl[WW], Wh[WW];
+    dig_to_windows(&s_dig[0][0][0], Wl, Wh);
 #pragma unroll
-    for (int w = 1; w < XD_BS / 64; ++w) {
-      uint32_t X[WW], Y[WW];
-#pragma unroll
-      for (int k = 0; k < WW; ++k) {
-        X[k] = s_w[w][k];
-        Y[k] = s_w[w][WW + k];
-      }
-      win_add_win(Wl, X);
-      win_add_win(Wh, Y);
-      flags |= s_w[w][2 * WW];
-    }
-    put_parts(parts + 2 * (int64_t)blockIdx.x, A, flags, Wl, Wh);
+    for (int w = 1; w < XD_BS / 64; ++w) flags |= s_fl[w];
+    // a zero window is marked by the anchor EXP_NONE, so the combine reads 8 bytes per record
+    // to find the range of the nonzero ones
+    part_t *p = parts + 2 * (int64_t)blockIdx.x;
+    put_parts(p, A, flags, Wl, Wh);
+    if (win_is_zero(Wl)) p[0].A = EXP_NONE;
+    if (win_is_zero(Wh)) p[1].A = EXP_NONE;
   }
 }
 
@@ -200,24 +212,28 @@ __global__ void __launch_bounds__(XC_BS) k_gemv_col_exact(int64_t m, int64_t n, 
   put_parts(parts + 2 * ((int64_t)blockIdx.y * m + i), An, flags, Wl, Wh);
 }
 
-// Combining the dot's block partials and finishing, in one launch of XC workgroups (XC <= XF_BLOCKS):
+// Combining the dot's block partials and finishing, in one launch of XC <= XF_BLOCKS workgroups
+// (one thread per record):
 //   every workgroup finds the common base (the minimum anchor over nonzero windows, minus 32) and
-//   the widest anchor; each thread adds its windows into a register-resident ACCW-word accumulator
-//   (select-based shifts), lanes 0..ACCW-1 sum one word column each over the block's threads
-//   (64-bit column sums, no carry chains) into one record per workgroup, and the last workgroup to
-//   finish (a ticket in header word 3) sums the records' columns, ripples the carries once and
-//   rounds.
+//   the widest anchor; each thread shifts its record into a register accumulator of NW words (24
+//   when the anchors lie within 320 bits, else ACCW; select-based shifts), the workgroup sums each
+//   word column over its threads in 64 bits through LDS (no carry chains) into one record per
+//   workgroup, and the last workgroup to finish (a ticket in header word 3) adds the records'
+//   columns; its wave 0 (one lane per word) ripples the carries, takes the magnitude and finds the
+//   top word with ballots, and rounds once.
 //   When a product fell outside its block's windows, or the block anchors lie too far apart for the
 //   ACCW-word accumulator, the same workgroups recompute the dot on the full-range path instead:
 //   workgroup b accumulates its slice of the elements in LDS (wide.hpp), normalises it and writes
 //   its digits to record b; the last workgroup adds the records digit by digit and rounds once
 //   (header word 2 = 1: the call took the full-range path).
 constexpr int XF_BS = 256;
-constexpr int XF_BLOCKS = 32;
+constexpr int XF_BLOCKS = 8;
+constexpr int XF_PARTS = 2048;  // most partial records of a dot (two per block, at most 1024 blocks)
+static_assert(XF_PARTS <= XF_BLOCKS * XF_BS, "the combine has one thread per partial record");
+constexpr int XF_NARROW = 24;   // accumulator words when the anchors lie within 320 bits
+static_assert(ACCW + SW <= 64, "wave 0 holds one word per lane");
 struct col_rec {
-  uint64_t c[ACCW];
-  uint32_t flags;
-  uint32_t pad[3];
+  uint64_t c[ACCW];  // one workgroup's 64-bit word column sums
 };
 struct dot_rec {
   int lo, t, neg;
@@ -229,14 +245,25 @@ __device__ __forceinline__ void parts_range_flags(const part_t *__restrict__ par
                                                   uint32_t &fl) {
   __shared__ int s_min[XF_BS / 64], s_max[XF_BS / 64];
   __shared__ uint32_t s_fl[XF_BS / 64];
+  // nparts <= XF_PARTS: each thread loads its (up to) XF_PARTS / XF_BS {anchor, flags} pairs at
+  // once (indexes clamped into range, surplus loads ignored) instead of one dependent load per step
+  uint2 v[XF_PARTS / XF_BS];
+#pragma unroll
+  for (int u = 0; u < XF_PARTS / XF_BS; ++u) {
+    const int p = min((int)threadIdx.x + u * XF_BS, nparts - 1);
+    v[u] = *reinterpret_cast<const uint2 *>(&parts[p].A);
+  }
   amin = EXP_NONE;
   amax = -EXP_NONE;
   fl = 0;
-  for (int p = threadIdx.x; p < nparts; p += XF_BS) {
-    fl |= parts[p].flags;
-    if (!win_is_zero(parts[p].W)) {
-      amin = min(amin, parts[p].A);
-      amax = max(amax, parts[p].A);
+#pragma unroll
+  for (int u = 0; u < XF_PARTS / XF_BS; ++u) {
+    if ((int)threadIdx.x + u * XF_BS >= nparts) continue;
+    fl |= v[u].y;
+    const int a = (int)v[u].x;
+    if (a != EXP_NONE) {  // EXP_NONE marks a zero window
+      amin = min(amin, a);
+      amax = max(amax, a);
     }
   }
   amin = wave_min(amin);
@@ -388,27 +415,97 @@ __device__ void dot_wide_grid(kul_t &K, const q128 *__restrict__ x, int64_t incx
   }
 }
 
-// Round a two's complement ACCW-word sum (bit 0 weighing 2^base) to binary128, in registers.
-__device__ __forceinline__ q128 round_accw(const uint32_t (&W)[ACCW], int base, uint32_t fl) {
-  const uint32_t sign = W[ACCW - 1] >> 31;
-  uint32_t M[ACCW];
-  uint32_t c = sign;
+// One workgroup's column sums of its records in NW-word accumulators (base = the lowest anchor
+// minus 32; every nonzero record fits: anchor - base + WW * 32 + 32 <= NW * 32).
+template <int NW>
+__device__ __forceinline__ void dot_combine_cols(const uint4 (&rec)[4], bool have, int base, uint32_t *__restrict__ tile,
+                                                 col_rec *__restrict__ mine) {
+  uint32_t acc[NW];
+  const int pA = (int)rec[0].x;
+  if (have && pA != EXP_NONE) {  // EXP_NONE: a zero window
+    const uint32_t Wr[WW] = {rec[0].z, rec[0].w, rec[1].x, rec[1].y, rec[1].z, rec[1].w,
+                             rec[2].x, rec[2].y, rec[2].z, rec[2].w, rec[3].x, rec[3].y};
+    const uint32_t sext = (Wr[WW - 1] >> 31) ? 0xffffffffu : 0u;
+    uint32_t W[NW];
 #pragma unroll
-  for (int k = 0; k < ACCW; ++k) M[k] = sign ? addc32(~W[k], 0u, c, c) : W[k];
-  int top = -1;
+    for (int k = 0; k < NW; ++k) W[k] = (k < WW) ? Wr[k] : sext;
+    rshl<NW>(acc, W, pA - base);  // zeros enter from below, the sign extension stays above
+  } else {
 #pragma unroll
-  for (int k = 0; k < ACCW; ++k) top = M[k] ? k : top;
-  if (top < 0) return (fl & (FL_NAN | FL_PINF | FL_NINF)) ? special_result(fl) : q_zero(0);
-  const int low = top >= SW ? top - SW + 1 : 0;
-  uint32_t sticky = 0;
+    for (int k = 0; k < NW; ++k) acc[k] = 0;
+  }
 #pragma unroll
-  for (int k = 0; k < ACCW; ++k) sticky |= (k < low) ? M[k] : 0u;
-  uint32_t S[ACCW];
-  (void)rshr<ACCW>(S, M, 32 * low);
-  uint32_t T[SW];
+  for (int k = 0; k < NW; ++k) tile[threadIdx.x * (NW + 1) + k] = acc[k];  // the pad spreads the banks
+  __syncthreads();
+  // G groups of NW threads each sum every G-th row, then NW threads add the G sums
+  constexpr int G = XF_BS / NW;
+  __shared__ uint64_t gsum[G * NW];
+  const int cidx = threadIdx.x % NW, g = threadIdx.x / NW;
+  if (g < G) {
+    uint64_t sum = 0;
 #pragma unroll
-  for (int k = 0; k < SW; ++k) T[k] = S[k];
-  return finish_reg(T, base + 32 * low, sign, sticky ? 1u : 0u, fl, false, q_one(), q_zero(0), q_zero(0));
+    for (int i = 0; i < (XF_BS + G - 1) / G; ++i) {
+      const int t = g + i * G;
+      if (t < XF_BS) sum += tile[t * (NW + 1) + cidx];
+    }
+    gsum[g * NW + cidx] = sum;
+  }
+  __syncthreads();
+  if (threadIdx.x < NW) {
+    uint64_t sum = 0;
+#pragma unroll
+    for (int gg = 0; gg < G; ++gg) sum += gsum[gg * NW + threadIdx.x];
+    mine->c[threadIdx.x] = sum;
+  }
+}
+
+// Wave 0 of the last workgroup: lane k < NW adds column k of the nrec records (< 2^43: at most 2048
+// words of 32 bits), ripples the carries, and rounds the two's complement sum (bit 0 weighing
+// 2^base) once.
+template <int NW>
+__device__ __forceinline__ void dot_round_wave(const col_rec *__restrict__ recs, int nrec, int base, uint32_t fl,
+                                               q128 *__restrict__ out) {
+  const int lane = threadIdx.x;
+  const bool in = lane < NW;
+  uint64_t col = 0;
+  if (in) {
+    uint64_t v[XF_BLOCKS];  // every record's column loaded at once (surplus loads clamped, ignored)
+#pragma unroll
+    for (int b = 0; b < XF_BLOCKS; ++b) v[b] = recs[min(b, nrec - 1)].c[lane];
+#pragma unroll
+    for (int b = 0; b < XF_BLOCKS; ++b) col += (b < nrec) ? v[b] : 0ull;
+  }
+  // word k = low half of column k + high half of column k - 1; the carries of those 33-bit sums
+  // (generate) run through the words equal to 2^32 - 1 (propagate): found with ballots
+  const uint32_t hprev = (uint32_t)__shfl_up((int)(uint32_t)(col >> 32), 1);
+  const uint64_t s33 = (uint64_t)(uint32_t)col + (lane ? hprev : 0u);
+  uint32_t v = (uint32_t)s33;
+  const uint64_t gen = __ballot(in && (s33 >> 32) != 0) << 1;
+  const uint64_t prop = __ballot(in && v == 0xffffffffu);
+  const uint64_t cin = gen | ((prop + gen) ^ prop ^ gen);
+  v += (uint32_t)((cin >> lane) & 1u);  // word `lane` of the sum (mod 2^(32 NW))
+  const uint32_t sign = (uint32_t)__shfl((int)v, NW - 1) >> 31;
+  uint32_t M = v;
+  if (sign) {  // magnitude ~W + 1: the +1 carries through the low words that are zero
+    const uint64_t zero = __ballot(in && v == 0u);
+    const int tz = __builtin_ctzll(~zero);  // lanes >= NW are never in `zero`
+    M = ~v + (lane <= tz ? 1u : 0u);
+  }
+  if (!in) M = 0;
+  const uint64_t nz = __ballot(M != 0u);
+  q128 r;
+  if (nz == 0) {
+    r = (fl & (FL_NAN | FL_PINF | FL_NINF)) ? special_result(fl) : q_zero(0);
+  } else {
+    const int top = 63 - __builtin_clzll(nz);
+    const int low = top >= SW ? top - SW + 1 : 0;
+    const uint32_t sticky = (nz & ((1ull << low) - 1ull)) ? 1u : 0u;
+    uint32_t T[SW];
+#pragma unroll
+    for (int j = 0; j < SW; ++j) T[j] = (uint32_t)__shfl((int)M, low + j);  // low + j <= max(top, SW - 1) < 64
+    r = finish_reg(T, base + 32 * low, sign, sticky, fl, false, q_one(), q_zero(0), q_zero(0));
+  }
+  if (lane == 0) stq_c(out, r);
 }
 
 __global__ void __launch_bounds__(XF_BS) k_exact_dot_combine(const part_t *__restrict__ parts, int nparts,
@@ -416,77 +513,44 @@ __global__ void __launch_bounds__(XF_BS) k_exact_dot_combine(const part_t *__res
                                                              int64_t incx, const q128 *__restrict__ y, int64_t incy,
                                                              int64_t n, dot_rec *__restrict__ wrec,
                                                              q128 *__restrict__ out, uint32_t *__restrict__ hdr) {
-  __shared__ uint32_t tile[XF_BS][ACCW + 1];
-  __shared__ kul_t K;
-  __shared__ uint64_t col[ACCW];
+  // one LDS buffer: the column-sum tile of the common path or the full-range accumulator
+  constexpr int TILE_WORDS = XF_BS * (ACCW + 1);
+  constexpr size_t SMEM = sizeof(kul_t) > TILE_WORDS * 4 ? sizeof(kul_t) : TILE_WORDS * 4;
+  __shared__ __align__(16) unsigned char smem[SMEM];
+  // this thread's record, loaded before the range scan so that the two sets of loads overlap
+  const int pme = blockIdx.x * XF_BS + threadIdx.x;
+  uint4 rec[4];
+  {
+    const uint4 *src = reinterpret_cast<const uint4 *>(parts + min(pme, nparts - 1));
+#pragma unroll
+    for (int q = 0; q < 4; ++q) rec[q] = src[q];
+  }
   int amin, amax;
   uint32_t fl;
   parts_range_flags(parts, nparts, amin, amax, fl);
   const int base = amin - 32;
-  const bool wide = (fl & FL_LOST) || (amin != EXP_NONE && amax - base + WW * 32 + 32 > ACCW * 32);
+  const int span = amax - base + WW * 32 + 32;  // bits the accumulator needs
+  const bool wide = (fl & FL_LOST) || (amin != EXP_NONE && span > ACCW * 32);
   if (wide) {  // uniform over the grid
-    dot_wide_grid(K, x, incx, y, incy, n, wrec, out, hdr);
+    dot_wide_grid(*reinterpret_cast<kul_t *>(smem), x, incx, y, incy, n, wrec, out, hdr);
     return;
   }
-  uint32_t acc[ACCW];
-#pragma unroll
-  for (int k = 0; k < ACCW; ++k) acc[k] = 0;
-  if (amin != EXP_NONE) {
-    for (int p = blockIdx.x * XF_BS + threadIdx.x; p < nparts; p += gridDim.x * XF_BS) {
-      if (win_is_zero(parts[p].W)) continue;
-      const int off = parts[p].A - base;  // 32 .. ACCW * 32 - WW * 32 - 32 (checked above)
-      uint32_t W[ACCW], S[ACCW];
-      const uint32_t sext = (parts[p].W[WW - 1] >> 31) ? 0xffffffffu : 0u;
-#pragma unroll
-      for (int k = 0; k < ACCW; ++k) W[k] = (k < WW) ? parts[p].W[k] : sext;
-      rshl<ACCW>(S, W, off);  // zeros enter from below, the sign extension stays above
-      uint32_t c = 0;
-#pragma unroll
-      for (int k = 0; k < ACCW; ++k) acc[k] = addc32(acc[k], S[k], c, c);
-    }
+  if (amin == EXP_NONE) {  // every window zero: uniform over the grid
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+      stq_c(out, (fl & (FL_NAN | FL_PINF | FL_NINF)) ? special_result(fl) : q_zero(0));
+    return;
   }
-#pragma unroll
-  for (int k = 0; k < ACCW; ++k) tile[threadIdx.x][k] = acc[k];
-  __syncthreads();
-  // column sums: six groups of ACCW threads each sum a sixth of the rows, then one adds the six
-  constexpr int G = XF_BS / ACCW;  // 6
-  __shared__ uint64_t part6[G][ACCW];
-  if (threadIdx.x < G * ACCW) {
-    const int cidx = threadIdx.x % ACCW, g = threadIdx.x / ACCW;
-    uint64_t sum = 0;
-    for (int t = g; t < XF_BS; t += G) sum += tile[t][cidx];
-    part6[g][cidx] = sum;
-  }
-  __syncthreads();
-  if (threadIdx.x < ACCW) {
-    uint64_t sum = 0;
-#pragma unroll
-    for (int g = 0; g < G; ++g) sum += part6[g][threadIdx.x];
-    recs[blockIdx.x].c[threadIdx.x] = sum;
-  }
-  if (!last_block(hdr)) return;
-  if (threadIdx.x < ACCW) {
-    uint64_t sum = 0;
-    for (int b = 0; b < (int)gridDim.x; ++b) sum += recs[b].c[threadIdx.x];  // < 2^45 for <= 32 records
-    col[threadIdx.x] = sum;
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  q128 r;
-  if (amin == EXP_NONE) {
-    r = (fl & (FL_NAN | FL_PINF | FL_NINF)) ? special_result(fl) : q_zero(0);
-  } else {
-    uint32_t W[ACCW];
-    uint64_t carry = 0;
-#pragma unroll
-    for (int k = 0; k < ACCW; ++k) {
-      const uint64_t v = col[k] + carry;  // two's complement words: the top carry is dropped
-      W[k] = (uint32_t)v;
-      carry = v >> 32;
-    }
-    r = round_accw(W, base, fl);
-  }
-  stq_c(out, r);
+  uint32_t *tile = reinterpret_cast<uint32_t *>(smem);
+  const bool narrow = span <= XF_NARROW * 32;
+  if (narrow)
+    dot_combine_cols<XF_NARROW>(rec, pme < nparts, base, tile, recs + blockIdx.x);
+  else
+    dot_combine_cols<ACCW>(rec, pme < nparts, base, tile, recs + blockIdx.x);
+  if (!last_block(hdr) || threadIdx.x >= 64) return;
+  if (narrow)
+    dot_round_wave<XF_NARROW>(recs, gridDim.x, base, fl, out);
+  else
+    dot_round_wave<ACCW>(recs, gridDim.x, base, fl, out);
 }
 
 // one thread per row: sums the row's chunk windows (they share the row's anchor), then
@@ -580,7 +644,7 @@ __global__ void __launch_bounds__(256) k_exact_wide_rows(int colmajor, int64_t n
 static int dot_exact_blocks(int64_t n) {
   int64_t b = (n + XD_BS * 8 - 1) / (XD_BS * 8);
   if (b < 1) b = 1;
-  if (b > 1024) b = 1024;  // at n = 2^24: 2048 blocks 6% slower (twice the partials to combine), 512 10%
+  if (b > XF_PARTS / 2) b = XF_PARTS / 2;  // 1024; at n = 2^24: 2048 blocks 6% slower, 512 10%
   return (int)b;
 }
 static void gemv_exact_split(int colmajor, int64_t m, int64_t n, int &nch, int64_t &chunk) {
@@ -631,7 +695,7 @@ hipError_t launch_dot_exact(const q128 *x, int64_t incx, const q128 *y, int64_t 
   dot_rec *wrec = (dot_rec *)((char *)ws + (dot_exact_recs_off(n) + 255) / 256 * 256);
   hipLaunchKernelGGL(k_dot_exact_partial, dim3(blocks), dim3(XD_BS), 0, st, x, incx, y, incy, n, parts, hdr);
   const int nparts = 2 * blocks;
-  const int cblocks = nparts < XF_BLOCKS * XF_BS ? (nparts + XF_BS - 1) / XF_BS : XF_BLOCKS;
+  const int cblocks = (nparts + XF_BS - 1) / XF_BS;  // one thread per record (nparts <= XF_PARTS)
   hipLaunchKernelGGL(k_exact_dot_combine, dim3(cblocks), dim3(XF_BS), 0, st, parts, nparts, recs, x, incx, y, incy, n,
                      wrec, out, hdr);
   return hipGetLastError();
