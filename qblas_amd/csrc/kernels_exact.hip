@@ -656,7 +656,9 @@ static void gemv_exact_split(int colmajor, int64_t m, int64_t n, int &nch, int64
     if (want > maxc) want = maxc;
   } else {
     const int64_t rowblocks = (m + XC_BS - 1) / XC_BS;
-    want = (2048 + rowblocks - 1) / rowblocks;  // 8192^2: 39% of HBM peak; 1024 or 4096 blocks: 37%, 512: 29%
+    // 1024 workgroups: at 8192^2 the call takes 264 us against 282 us with 2048 (twice the chunk
+    // records to write and to sum in the finish), 272 with 768 or 1280, 300 with 512
+    want = (1024 + rowblocks - 1) / rowblocks;
     const int64_t maxc = (n + 63) / 64;
     if (want > maxc) want = maxc;
   }
