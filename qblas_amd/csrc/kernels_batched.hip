@@ -304,33 +304,43 @@ __device__ __forceinline__ dd bshfl_xor_dd(dd v, int msk) {
 // through linearly (one search per chunk).
 // RowMajor: an item is 4 rows; lanes stride along the columns, 2 column steps of 64 per iteration
 // (8 loads of A in flight per lane, each converted x element serving the 4 rows), dd partials per
-// lane, butterfly wave sums, and lane r finishes row r.
+// lane, one joint butterfly for the 4 rows; a wave takes 16 items and then finishes their 64 rows,
+// one per lane.
 // ColMajor: an item is 64 rows, one per lane, over the columns (2 accumulators).
-constexpr int BGV_CHUNK = 4;  // work items a wave takes in a row
+constexpr int BGV_CHUNK = 4;    // ColMajor: work items a wave takes in a row
+constexpr int BGV_CHUNK_R = 16;  // RowMajor: 16 items of 4 rows, one row per lane at the end
 template <int colmajor>
 __global__ void __launch_bounds__(256) k_gemv_batched_dd(const qblas_gemv_problem *__restrict__ P, int64_t count,
                                                          const int64_t *__restrict__ start) {
   const int lane = threadIdx.x & 63;
   const int64_t total = start[count];
   const int64_t waves = (int64_t)gridDim.x * 4, w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  for (int64_t c0 = w * BGV_CHUNK; c0 < total; c0 += waves * BGV_CHUNK) {
-    const int64_t c1 = (c0 + BGV_CHUNK < total) ? c0 + BGV_CHUNK : total;
-    int64_t p = find_problem(start, count, c0);
-    for (int64_t u = c0; u < c1; ++u) {
-      while (start[p + 1] <= u) ++p;  // the problem holding item u (empty problems are skipped)
-      const qblas_gemv_problem &v = P[p];
-      const int64_t m = v.m, n = v.n, lda = v.lda, incx = v.incx, incy = v.incy;
-      const int64_t ul = u - start[p];
-      const q128 *A = (const q128 *)v.A;
-      const q128 *x = (const q128 *)v.x;
-      q128 *y = (q128 *)v.y;
-      const q128 alpha = qv(v.alpha), beta = qv(v.beta);
-      if (!colmajor) {
-        const int64_t i0 = ul * 4;
+  if (!colmajor) {
+    // RowMajor: lanes stride along the columns of an item's 4 rows; one joint butterfly leaves
+    // row r's total in lanes 16 r .. 16 r + 15, lane 4 q + r of the wave keeps the total of row r
+    // of the chunk's item q (its y loaded when the item starts), and after the 16 items each lane
+    // finishes its row: the binary128 epilogue runs once per 64 rows instead of once per 4.
+    for (int64_t c0 = w * BGV_CHUNK_R; c0 < total; c0 += waves * BGV_CHUNK_R) {
+      const int64_t c1 = (c0 + BGV_CHUNK_R < total) ? c0 + BGV_CHUNK_R : total;
+      int64_t p = find_problem(start, count, c0);
+      dd mine = dd_zero();
+      q128 yv = q_zero(0);
+      int64_t pl = -1;  // problem of this lane's row (-1: none)
+      int64_t il = 0;  // its row index
+      for (int64_t u = c0; u < c1; ++u) {
+        while (start[p + 1] <= u) ++p;  // the problem holding item u (empty problems are skipped)
+        const qblas_gemv_problem &v = P[p];
+        const int64_t m = v.m, n = v.n, lda = v.lda, incx = v.incx, incy = v.incy;
+        const int64_t i0 = (u - start[p]) * 4;
         const int R = (m - i0 < 4) ? (int)(m - i0) : 4;  // rows of this item (wave-uniform)
-        // y of the row this lane finishes, loaded now: its latency passes during the column loop
-        q128 *yi = y + (i0 + (lane < R ? lane : 0)) * incy;
-        const q128 yv = bld(yi);
+        const q128 *A = (const q128 *)v.A;
+        const q128 *x = (const q128 *)v.x;
+        const int q = (int)(u - c0);
+        if ((lane >> 2) == q && (lane & 3) < R) {  // this lane will finish row i0 + (lane & 3)
+          pl = p;
+          il = i0 + (lane & 3);
+          yv = bld((const q128 *)v.y + il * incy);
+        }
         dd acc[4][2];
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc[r][0] = acc[r][1] = dd_zero();
@@ -355,18 +365,43 @@ __global__ void __launch_bounds__(256) k_gemv_batched_dd(const qblas_gemv_proble
             }
           }
         }
-        dd mine = dd_zero();
+        dd s[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          if (r < R) {
-            dd s = dd_add(acc[r][0], acc[r][1]);
+        for (int r = 0; r < 4; ++r) s[r] = (r < R) ? dd_add(acc[r][0], acc[r][1]) : dd_zero();
+        // mask 32: lanes with bit 5 = b keep rows 2b, 2b + 1 and send the other two
+        const bool b = (lane & 32) != 0, c = (lane & 16) != 0;
+        const dd t0 = dd_add(b ? s[2] : s[0], bshfl_xor_dd(b ? s[0] : s[2], 32));
+        const dd t1 = dd_add(b ? s[3] : s[1], bshfl_xor_dd(b ? s[1] : s[3], 32));
+        // mask 16: bit 4 = c keeps row 2b + c
+        dd tot = dd_add(c ? t1 : t0, bshfl_xor_dd(c ? t0 : t1, 16));
 #pragma unroll
-            for (int msk = 32; msk >= 1; msk >>= 1) s = dd_add(s, bshfl_xor_dd(s, msk));
-            if (lane == r) mine = s;
-          }
-        }
-        if (lane < R) bst(yi, fma_packed(alpha, dd_to_q(mine), mul_packed(beta, yv)));
-      } else {
+        for (int msk = 8; msk >= 1; msk >>= 1) tot = dd_add(tot, bshfl_xor_dd(tot, msk));
+        const int src = 16 * (lane & 3);
+        dd got;
+        got.hi = __shfl(tot.hi, src);
+        got.lo = __shfl(tot.lo, src);
+        if ((lane >> 2) == q) mine = got;
+      }
+      if (pl >= 0) {
+        const qblas_gemv_problem &v = P[pl];
+        bst((q128 *)v.y + il * v.incy, fma_packed(qv(v.alpha), dd_to_q(mine), mul_packed(qv(v.beta), yv)));
+      }
+    }
+    return;
+  }
+  for (int64_t c0 = w * BGV_CHUNK; c0 < total; c0 += waves * BGV_CHUNK) {
+    const int64_t c1 = (c0 + BGV_CHUNK < total) ? c0 + BGV_CHUNK : total;
+    int64_t p = find_problem(start, count, c0);
+    for (int64_t u = c0; u < c1; ++u) {
+      while (start[p + 1] <= u) ++p;  // the problem holding item u (empty problems are skipped)
+      const qblas_gemv_problem &v = P[p];
+      const int64_t m = v.m, n = v.n, lda = v.lda, incx = v.incx, incy = v.incy;
+      const int64_t ul = u - start[p];
+      const q128 *A = (const q128 *)v.A;
+      const q128 *x = (const q128 *)v.x;
+      q128 *y = (q128 *)v.y;
+      const q128 alpha = qv(v.alpha), beta = qv(v.beta);
+      {
         const int64_t i = ul * 64 + lane;
         if (i < m) {
           dd acc0 = dd_zero(), acc1 = dd_zero();
