@@ -17,6 +17,8 @@
 //   accumulator (ACCW words), then alpha*S + beta*y is formed exactly as big integers and rounded
 //   once.
 #pragma once
+#include <climits>
+
 #include "q128.hpp"
 
 namespace qb {
@@ -630,6 +632,55 @@ QB_HD void rmul_q(uint32_t (&r)[N + 4], const uint32_t (&x)[N], const uq &a) {
 constexpr int SW = 24;         // words of |S| handled by the register epilogue
 constexpr int FW = SW + 4 + 12;  // combine frame for alpha*S + beta*y
 
+// RN((-1)^s1 A 2^e1 + (-1)^s2 B 2^e2) for nonzero magnitudes A (NA words) and B (NB words), in an
+// FWN-word frame: the operand with the higher leading bit is placed with that bit at FWN*32-2, the
+// other one aligned to it, bits below the frame folded into a sticky bit (with one unit less for a
+// subtraction), so the sum is rounded once as the exact sum would be.
+template <int FWN, int NA, int NB>
+QB_HD q128 sum_round2(const uint32_t (&A)[NA], int e1, uint32_t s1, const uint32_t (&B)[NB], int e2, uint32_t s2) {
+  uint32_t F1[FWN], F2[FWN];
+#pragma unroll
+  for (int i = 0; i < FWN; ++i) {
+    F1[i] = (i < NA) ? A[i] : 0u;
+    F2[i] = (i < NB) ? B[i] : 0u;
+  }
+  const int m1 = rmsb<FWN>(F1), m2 = rmsb<FWN>(F2);
+  const bool one_big = (e1 + m1) >= (e2 + m2);
+  const int topL = one_big ? (e1 + m1) : (e2 + m2);
+  const int flsb = topL - (FWN * 32 - 2);
+  uint32_t L[FWN], S[FWN];
+  rshl<FWN>(L, one_big ? F1 : F2, (FWN * 32 - 2) - (one_big ? m1 : m2));
+  const int es = one_big ? e2 : e1;
+  uint32_t lost = 0;
+  if (es >= flsb) {
+    rshl<FWN>(S, one_big ? F2 : F1, es - flsb);
+  } else {
+    lost = rshr<FWN>(S, one_big ? F2 : F1, flsb - es);
+  }
+  const uint32_t sL = one_big ? s1 : s2, sSm = one_big ? s2 : s1;
+  uint32_t R[FWN];
+  uint32_t sR = sL;
+  if (sL == sSm) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < FWN; ++i) R[i] = addc32(L[i], S[i], c, c);
+  } else {
+    // R = L - S - lost (two's complement); a borrow out means |S| > |L| (only without loss)
+    uint32_t c = lost ? 0u : 1u;
+#pragma unroll
+    for (int i = 0; i < FWN; ++i) R[i] = addc32(L[i], ~S[i], c, c);
+    if (!c) {
+      uint32_t c2 = 1;
+#pragma unroll
+      for (int i = 0; i < FWN; ++i) R[i] = addc32(~R[i], 0u, c2, c2);
+      sR = sSm;
+    }
+    if (!lost && rzero<FWN>(R)) return q_zero(0);
+  }
+  R[0] |= lost;
+  return rround<FWN>(R, flsb, sR);
+}
+
 // Correctly rounded alpha*S + beta*y, with S = (-1)^sS * M * 2^lsb exact (sticky must be 0 when
 // alpha/beta are used; for the plain sum, sticky marks a nonzero tail below M, i.e. M is S
 // rounded to odd).  flags carry Inf/NaN products.  N is the word count of M; the combine frame
@@ -672,50 +723,67 @@ QB_HD q128 finish_reg_t(const uint32_t (&M)[N], int lsb, uint32_t sS, uint32_t s
   const int e2 = ub.e + uy.e - 224;
   const uint32_t s2 = ub.s ^ uy.s;
   if (z1 && z2) return q_zero(((uint32_t)(alpha.hi >> 63) ^ sS) & s2);
-  uint32_t F1[FWN], F2[FWN];
+  if (z2) {
+    uint32_t F1[FWN];
 #pragma unroll
-  for (int i = 0; i < FWN; ++i) {
-    F1[i] = (i < N + 4) ? T1[i] : 0u;
-    F2[i] = (i < 8 && !z2) ? P2[i] : 0u;
+    for (int i = 0; i < FWN; ++i) F1[i] = (i < N + 4) ? T1[i] : 0u;
+    return rround<FWN>(F1, e1, s1);
   }
-  if (z2) return rround<FWN>(F1, e1, s1);
-  if (z1) return rround<FWN>(F2, e2, s2);
-  // frame: the operand with the higher leading bit is placed with its leading bit at FWN*32-2
-  const int m1 = rmsb<FWN>(F1), m2 = rmsb<FWN>(F2);
-  const bool one_big = (e1 + m1) >= (e2 + m2);
-  const int topL = one_big ? (e1 + m1) : (e2 + m2);
-  const int flsb = topL - (FWN * 32 - 2);
-  uint32_t L[FWN], S[FWN];
-  rshl<FWN>(L, one_big ? F1 : F2, (FWN * 32 - 2) - (one_big ? m1 : m2));
-  const int es = one_big ? e2 : e1;
-  uint32_t lost = 0;
-  if (es >= flsb) {
-    rshl<FWN>(S, one_big ? F2 : F1, es - flsb);
-  } else {
-    lost = rshr<FWN>(S, one_big ? F2 : F1, flsb - es);
+  if (z1) {
+    uint32_t F2[FWN];
+#pragma unroll
+    for (int i = 0; i < FWN; ++i) F2[i] = (i < 8) ? P2[i] : 0u;
+    return rround<FWN>(F2, e2, s2);
   }
-  const uint32_t sL = one_big ? s1 : s2, sSm = one_big ? s2 : s1;
-  uint32_t R[FWN];
-  uint32_t sR = sL;
-  if (sL == sSm) {
-    uint32_t c = 0;
+  return sum_round2<FWN, N + 4, 8>(T1, e1, s1, P2, e2, s2);
+}
+
+// RN(alpha M 2^lsb (-1)^sS + beta y) for finite alpha, beta, y and M != 0 (N words), with M cut to
+// its top 128 bits before the product with alpha; the rest is exact (one rounding from a sticky
+// bit in a 12-word frame).  terr receives an exponent bounding what the cut changes in the sum
+// (|alpha| times the cut-off part of M 2^lsb), or INT_MIN when nothing was cut.
+template <int N>
+QB_HD q128 finish_reg_top(const uint32_t (&M)[N], int lsb, uint32_t sS, q128 alpha, q128 beta, q128 yv, int &terr) {
+  terr = INT_MIN;
+  const uq ua = unpack_finite(alpha), ub = unpack_finite(beta), uy = unpack_finite(yv);
+  const bool z1 = (ua.mh | ua.ml) == 0;
+  const bool z2 = ((ub.mh | ub.ml) == 0) || ((uy.mh | uy.ml) == 0);
+  // alpha = +-1 and beta * y = 0: the result is M rounded once (with alpha's sign)
+  if (z2 && ua.ml == 0 && ua.mh == HIDDEN && ua.e == 0) return rround<N>(M, lsb, sS ^ ua.s);
+  uint32_t P2[8];
+  if (!z2) mul113_w(P2, ub, uy);
+  const int e2 = ub.e + uy.e - 224;
+  const uint32_t s2 = ub.s ^ uy.s;
+  if (z1) {
+    if (z2) return q_zero(((uint32_t)(alpha.hi >> 63) ^ sS) & s2);
+    uint32_t F2[12];
 #pragma unroll
-    for (int i = 0; i < FWN; ++i) R[i] = addc32(L[i], S[i], c, c);
-  } else {
-    // R = L - S - lost (two's complement); a borrow out means |S| > |L| (only without loss)
-    uint32_t c = lost ? 0u : 1u;
-#pragma unroll
-    for (int i = 0; i < FWN; ++i) R[i] = addc32(L[i], ~S[i], c, c);
-    if (!c) {
-      uint32_t c2 = 1;
-#pragma unroll
-      for (int i = 0; i < FWN; ++i) R[i] = addc32(~R[i], 0u, c2, c2);
-      sR = sSm;
-    }
-    if (!lost && rzero<FWN>(R)) return q_zero(0);
+    for (int i = 0; i < 12; ++i) F2[i] = (i < 8) ? P2[i] : 0u;
+    return rround<12>(F2, e2, s2);
   }
-  R[0] |= lost;
-  return rround<FWN>(R, flsb, sR);
+  // H = the top 128 bits of M (leading bit 127)
+  const int sh = rmsb<N>(M) - 127;
+  uint32_t W[N];
+  uint32_t cut = 0;
+  if (sh > 0)
+    cut = rshr<N>(W, M, sh);
+  else
+    rshl<N>(W, M, -sh);
+  uint32_t H[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) H[i] = (i < N) ? W[i] : 0u;
+  if (cut) terr = lsb + sh + ua.e + 1;  // |alpha| < 2^(ua.e + 1), the cut part < 2^(lsb + sh)
+  uint32_t T1[8];
+  rmul_q<4>(T1, H, ua);
+  const int e1 = lsb + sh + ua.e - 112;
+  const uint32_t s1 = sS ^ ua.s;
+  if (z2) {
+    uint32_t F1[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) F1[i] = (i < 8) ? T1[i] : 0u;
+    return rround<12>(F1, e1, s1);
+  }
+  return sum_round2<12, 8, 8>(T1, e1, s1, P2, e2, s2);
 }
 
 QB_HD q128 finish_reg(const uint32_t (&M)[SW], int lsb, uint32_t sS, uint32_t sticky, uint32_t flags, bool has_ab,

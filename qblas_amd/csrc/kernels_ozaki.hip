@@ -550,11 +550,14 @@ __device__ __forceinline__ bool oz_finish_value(int64_t i, int64_t j, int S, int
     if (!rzero<OZ_FINW>(V)) {
       const int Ei = eA[i] + 3, Fj = eB[j] + 3;
       const int lsb = Ei + Fj - 8 * (S + 1);
-      res = finish_reg_t<OZ_FINW>(V, lsb, sgn, 0u, 0u, true, alpha, beta, yv);
+      int terr;
+      res = finish_reg_top<OZ_FINW>(V, lsb, sgn, alpha, beta, yv, terr);
       if (q_finite_nonzero(res)) {
         const int ea = unpack_finite(alpha).e;
         const int bexp = ea + 1 + kfac + Ei + Fj - 8 * S;  // |alpha (AB - P)| < 2^bexp
-        done = bexp <= unpack_finite(res).e - 106;
+        // with the epilogue's cut of the pass sum the two bounds together stay below 2^(max + 1)
+        const int eb = (terr == INT_MIN) ? bexp : max(bexp, terr) + 1;
+        done = eb <= unpack_finite(res).e - 106;
       }
     }
   }
@@ -2150,10 +2153,12 @@ __global__ void __launch_bounds__(256) k_crt_finish_ab(int64_t M, int64_t N, int
       }
       if (!rzero<XW>(X)) {
         const int Ei = eA[i] + 3, Fj = eB[j] + 3;
-        res = finish_reg_t<XW>(X, Ei + Fj - 2 * t, sgn, 0u, 0u, true, alpha, beta, yv);
+        int terr;
+        res = finish_reg_top<XW>(X, Ei + Fj - 2 * t, sgn, alpha, beta, yv, terr);
         if (q_finite_nonzero(res)) {
           const int bexp = unpack_finite(alpha).e + 1 + kc + Ei + Fj - t - 2;
-          done = bexp <= unpack_finite(res).e - 106;
+          const int eb = (terr == INT_MIN) ? bexp : max(bexp, terr) + 1;  // with the epilogue's cut
+          done = eb <= unpack_finite(res).e - 106;
         }
       }
     }
