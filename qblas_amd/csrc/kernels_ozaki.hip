@@ -1065,6 +1065,14 @@ __device__ __forceinline__ int64_t bo_find(const int64_t *__restrict__ off, int6
   }
   return lo;
 }
+// The same for item t >= t0 of a workgroup whose first item is t0: one search for t0 (workgroup
+// uniform), then a forward walk (a workgroup's items rarely span more than two problems).
+__device__ __forceinline__ int64_t bo_find_from(const int64_t *__restrict__ off, int64_t count, int f, int64_t t0,
+                                                int64_t t) {
+  int64_t p = bo_find(off, count, f, t0);
+  while (p + 1 < count && off[(p + 1) * BO_F + f] <= t) ++p;
+  return p;
+}
 
 // off[0][*] = 0, off[p + 1][f] = sum of count f over problems 0..p; one block, 1024 problems a chunk
 __global__ void __launch_bounds__(1024) k_bo_scan(const qblas_gemm_problem *__restrict__ P, int64_t count,
@@ -1136,7 +1144,7 @@ __global__ void __launch_bounds__(256) k_bo_scale(const qblas_gemm_problem *__re
   const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (w >= items) return;  // whole waves
   const int fs = which ? BO_SCB : BO_SCA;
-  const int64_t p = bo_find(off, count, fs, w);
+  const int64_t p = bo_find_from(off, count, fs, (int64_t)blockIdx.x * 4, w);
   const bo_geom G = bo_geometry(P[p], colmajor);
   const q128 *X = (const q128 *)(which ? P[p].B : P[p].A);
   const int64_t R = which ? G.n : G.m, sr = which ? G.bsc : G.asr, sc = which ? G.bsr : G.asc;
@@ -1175,7 +1183,7 @@ __global__ void __launch_bounds__(256) k_bo_slice(const qblas_gemm_problem *__re
   if (g >= nthreads) return;
   const int64_t chunk = g >> 6;
   const int fs = which ? BO_FRAGB : BO_FRAGA;
-  const int64_t p = bo_find(off, count, fs, chunk);
+  const int64_t p = bo_find_from(off, count, fs, ((int64_t)blockIdx.x * 256) >> 6, chunk);
   const bo_geom G = bo_geometry(P[p], colmajor);
   const q128 *X = (const q128 *)(which ? P[p].B : P[p].A);
   const int64_t R = which ? G.n : G.m, sr = which ? G.bsc : G.asr, sc = which ? G.bsr : G.asc;
@@ -1212,7 +1220,7 @@ __global__ void __launch_bounds__(256) k_bo_finish(const qblas_gemm_problem *__r
   if (hdr[OZ_HDR_STALE]) return;  // nothing queued: the repair has nothing to do either
   const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (g >= total) return;
-  const int64_t p = bo_find(off, count, BO_ENT, g);
+  const int64_t p = bo_find_from(off, count, BO_ENT, (int64_t)blockIdx.x * 256, g);
   const qblas_gemm_problem &pr = P[p];
   const bo_geom G = bo_geometry(pr, colmajor);
   const int64_t *o = off + p * BO_F;
