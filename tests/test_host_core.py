@@ -106,3 +106,18 @@ def test_exact_pipeline_matches_oracle(O, tmp_path):
         r = subprocess.run([str(exe), str(seed), "200"], capture_output=True, text=True)
         assert r.returncode == 0, r.stdout[-3000:]
         assert " mismatches 0" in r.stdout
+
+
+def test_ozaki_epilogue_top_bits(tmp_path):
+    """The OZAKI finish's epilogue with general alpha, beta (exact.hpp finish_reg_top: the sum cut to
+    its top 128 bits before the product with alpha) against the full-width exact epilogue
+    (finish_reg_t), on random, alpha = +-1, subnormal-alpha, beta*y = 0 and cancelling cases:
+    bit-identical when nothing was cut, else within one ulp plus the reported bound 2^terr."""
+    exe = tmp_path / "epilogue_host_check"
+    src = os.path.join(ROOT, "tests", "cpu", "epilogue_host_check.cpp")
+    subprocess.run(["g++", "-O2", "-std=gnu++17", "-o", str(exe), src, "-lquadmath"], check=True)
+    for seed in (1, 2):
+        r = subprocess.run([str(exe), str(seed), "100000"], capture_output=True, text=True)
+        assert r.returncode == 0, r.stdout[-3000:]
+        assert " mismatches 0 " in r.stdout
+
