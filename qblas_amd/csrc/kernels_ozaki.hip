@@ -1770,89 +1770,162 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
 }
 
 // One int8 GEMM per modulus: RC[q][mb][nb] = (RA[q] RB[q]^T) mod p_q (planes pA, pB, pC bytes apart)
-// for a 256 x 256 tile of (mb, nb) blocks, 8 waves of 128 x 64 (4 x 2 MFMA blocks).  A 4-slot LDS ring of stages of two
-// k blocks (2 x (8 A + 8 B) fragments, 32 KB) is filled three stages ahead with global_load_lds;
-// each wave copies 4 fragments per stage (two k blocks per barrier: fewer waits between MFMAs).
+// for a 256 x 256 tile of (mb, nb) blocks, 8 waves of 128 x 64 (4 x 2 MFMA blocks).  Stages of four
+// k blocks (4 x (8 A + 8 B) fragments, 64 KB) alternate between two LDS slots: the copies of stage
+// s + 1 (global_load_lds, 8 fragments per wave) are issued right after the barrier of stage s and
+// land while its 32 MFMAs per wave run.  Every copy issues its loads (k blocks past the end load
+// the last block again into a slot that is never read), so the wait before each barrier is the
+// same.  The fragments are software-pipelined in two register sets: a wave enters the barrier of
+// stage s still holding the last k block of stage s - 1, so after the barrier its MFMAs on those
+// run while the first k block of stage s is read from LDS, with the copies of stage s + 1 issued
+// between them; the reads of each next k block are issued between the MFMAs on the current one.
+// The order is pinned with sched_barrier.  (Products at 4096^3, 39 moduli, on one box: 2.90 ms
+// with two-block stages and reads waited for after every barrier and between MFMA pairs; 2.46 ms
+// pipelined; 2.32 ms with four-block stages, half the barriers.)
 // Workgroups of one modulus are consecutive (its residue planes stay in the caches).  Output block
 // (q, mb, nb), 1 KB: bytes v of lane L (rows 8(v/4) + 4(L/32) + v%4, column L%32 of the MFMA
 // result) at (v/4) * 256 + L * 4 + v%4, so that the reconstruction reads rows of 4 entries of 64
 // lanes contiguously.
-constexpr int CRT_WR = 2, CRT_WC = 4, CRT_FM = 4, CRT_FN = 2, CRT_KS = 2;
+constexpr int CRT_WR = 2, CRT_WC = 4, CRT_FM = 4, CRT_FN = 2, CRT_KS = 4;
 constexpr int CRT_TA = CRT_WR * CRT_FM, CRT_TB = CRT_WC * CRT_FN;
 constexpr int CRT_SLOT = (CRT_TA + CRT_TB) * OZ_FRAG * CRT_KS;
 constexpr int CRT_PER = (CRT_TA + CRT_TB) * CRT_KS / (CRT_WR * CRT_WC);
+constexpr int CRT_CHS = CRT_KCHUNK / CRT_KS;  // stages between modular reductions
+constexpr int CRT_NSLOT = 2;                 // LDS ring slots; copies run CRT_NSLOT - 1 stages ahead
 static_assert(CRT_TA * 32 == CRT_TILE && CRT_TB * 32 == CRT_TILE, "tile");
-static_assert(((CRT_TA + CRT_TB) * CRT_KS) % (CRT_WR * CRT_WC) == 0 && 2 * CRT_PER <= 15, "copy split");
-static_assert(CRT_KCHUNK % CRT_KS == 0, "reduction at stage ends");
+static_assert(((CRT_TA + CRT_TB) * CRT_KS) % (CRT_WR * CRT_WC) == 0 && (CRT_NSLOT - 2) * CRT_PER <= 15, "copy split");
+static_assert(CRT_KS % 2 == 0 && CRT_PER <= CRT_FM * CRT_FN, "two fragment sets alternate over a stage's k blocks");
 
 __global__ void __launch_bounds__(512) k_crt_gemm(const int8_t *__restrict__ RA, const int8_t *__restrict__ RB,
                                                   int64_t Mb, int64_t Nb, int64_t Kb, int64_t pA, int64_t pB,
                                                   int64_t pC, const crt_tab *__restrict__ T,
                                                   int8_t *__restrict__ RC) {
-  __shared__ __attribute__((aligned(1024))) int8_t lds[4 * CRT_SLOT];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __shared__ __attribute__((aligned(1024))) int8_t lds[CRT_NSLOT * CRT_SLOT];
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
   const int64_t tilesM = Mb / CRT_TA, tilesN = Nb / CRT_TB, per = tilesM * tilesN;
   const int64_t q = blockIdx.x / per, tt = blockIdx.x - q * per;
   const int64_t tm = tt / tilesN, tn = tt - tm * tilesN;
   const int8_t *Aq = RA + q * pA + tm * CRT_TA * Kb * OZ_FRAG;
   const int8_t *Bq = RB + q * pB + tn * CRT_TB * Kb * OZ_FRAG;
-  const int64_t ns = (Kb + CRT_KS - 1) / CRT_KS;  // stages; the last may hold one k block
+  auto slot = [](int64_t st) { return (int)((uint32_t)st % (uint32_t)CRT_NSLOT) * CRT_SLOT; };
+  const int64_t nfull = Kb / CRT_KS;  // full stages; the rest of the k blocks form a short last stage
+  // this wave's fragments of a stage: source row (or column) block and k block offset, LDS slot offset
+  const int8_t *src[CRT_PER];
+  int kko[CRT_PER], dst[CRT_PER];
+#pragma unroll
+  for (int c = 0; c < CRT_PER; ++c) {
+    const int f = wave * CRT_PER + c;  // fragment of the stage: k block kk, A 0..TA-1 then B
+    const int kk = f / (CRT_TA + CRT_TB), idx = f - kk * (CRT_TA + CRT_TB);
+    src[c] = (idx < CRT_TA) ? Aq + (int64_t)idx * Kb * OZ_FRAG : Bq + (int64_t)(idx - CRT_TA) * Kb * OZ_FRAG;
+    kko[c] = kk;
+    dst[c] = f * OZ_FRAG;
+  }
   auto copy = [&](int64_t st) {
-    if (st >= ns) return;
-    int8_t *d = &lds[(int)(st & 3) * CRT_SLOT];
+    int8_t *d = &lds[slot(st)];
 #pragma unroll
     for (int c = 0; c < CRT_PER; ++c) {
-      const int f = wave * CRT_PER + c;  // fragment of the stage: k block kk, A 0..TA-1 then B
-      const int kk = f / (CRT_TA + CRT_TB), idx = f - kk * (CRT_TA + CRT_TB);
-      const int64_t kb = st * CRT_KS + kk;
-      if (kb < Kb) {
-        const int8_t *s = (idx < CRT_TA) ? Aq + ((int64_t)idx * Kb + kb) * OZ_FRAG
-                                         : Bq + ((int64_t)(idx - CRT_TA) * Kb + kb) * OZ_FRAG;
-        __builtin_amdgcn_global_load_lds((glb_void *)(s + lane * 16), (lds_void *)(d + f * OZ_FRAG), 16, 0, 0);
-      }
+      const int64_t kb = min(st * CRT_KS + kko[c], Kb - 1);
+      __builtin_amdgcn_global_load_lds((glb_void *)(src[c] + kb * OZ_FRAG + lane * 16), (lds_void *)(d + dst[c]), 16, 0, 0);
     }
   };
+  // wait for this wave's copies of the stage about to be read (the older of the CRT_NSLOT - 1
+  // stages in flight) and for its LDS reads of the previous stage, whose slot the copy issued after
+  // the barrier overwrites; then for every wave's
+  auto waitbar = [&]() {
+    __builtin_amdgcn_s_waitcnt(((CRT_NSLOT - 2) * CRT_PER) | (7 << 4));  // vmcnt((NSLOT - 2) PER) lgkmcnt(0)
+    __builtin_amdgcn_s_barrier();
+  };
   const int wr = wave / CRT_WC, wc = wave % CRT_WC;
+  const int aoff = wr * CRT_FM * OZ_FRAG + lane * 16, boff = (CRT_TA + wc * CRT_FN) * OZ_FRAG + lane * 16;
+  auto rd = [&](int64_t st, int kk, i32x4v (&fa)[CRT_FM], i32x4v (&fb)[CRT_FN]) {
+    const int8_t *base = &lds[slot(st) + kk * (CRT_TA + CRT_TB) * OZ_FRAG];
+#pragma unroll
+    for (int a = 0; a < CRT_FM; ++a) fa[a] = *reinterpret_cast<const i32x4v *>(base + aoff + a * OZ_FRAG);
+#pragma unroll
+    for (int b = 0; b < CRT_FN; ++b) fb[b] = *reinterpret_cast<const i32x4v *>(base + boff + b * OZ_FRAG);
+  };
+  i32x16v acc[CRT_FM][CRT_FN];
+  auto mm = [&](const i32x4v (&fa)[CRT_FM], const i32x4v (&fb)[CRT_FN]) {
+#pragma unroll
+    for (int a = 0; a < CRT_FM; ++a)
+#pragma unroll
+      for (int b = 0; b < CRT_FN; ++b)
+        acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
+  };
+  // the MFMAs of (fa, fb) with k block kk of stage st read into (ga, gb) between them: the reads
+  // are issued after this wave's wait for (fa, fb) and land while the MFMAs run
+  auto mm_rd = [&](const i32x4v (&fa)[CRT_FM], const i32x4v (&fb)[CRT_FN], int64_t st, int kk, i32x4v (&ga)[CRT_FM],
+                   i32x4v (&gb)[CRT_FN]) {
+    const int8_t *base = &lds[slot(st) + kk * (CRT_TA + CRT_TB) * OZ_FRAG];
+#pragma unroll
+    for (int a = 0; a < CRT_FM; ++a)
+#pragma unroll
+      for (int b = 0; b < CRT_FN; ++b) {
+        acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        const int i = a * CRT_FN + b;  // read i of 6: the A fragments, then the B fragments
+        if (i < CRT_FM)
+          ga[i] = *reinterpret_cast<const i32x4v *>(base + aoff + i * OZ_FRAG);
+        else if (i < CRT_FM + CRT_FN)
+          gb[i - CRT_FM] = *reinterpret_cast<const i32x4v *>(base + boff + (i - CRT_FM) * OZ_FRAG);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+  };
+  // the MFMAs of (fa, fb) with the copies of stage st issued between them
+  auto mm_cp = [&](const i32x4v (&fa)[CRT_FM], const i32x4v (&fb)[CRT_FN], int64_t st) {
+    int8_t *d = &lds[slot(st)];
+#pragma unroll
+    for (int a = 0; a < CRT_FM; ++a)
+#pragma unroll
+      for (int b = 0; b < CRT_FN; ++b) {
+        acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        const int c = a * CRT_FN + b;
+        if (c < CRT_PER) {
+          const int64_t kb = min(st * CRT_KS + kko[c], Kb - 1);
+          __builtin_amdgcn_global_load_lds((glb_void *)(src[c] + kb * OZ_FRAG + lane * 16), (lds_void *)(d + dst[c]), 16, 0, 0);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+  };
   const int p = T->p[q];
   const double ipd = T->pinvd[q];
-  i32x16v acc[CRT_FM][CRT_FN];
 #pragma unroll
   for (int a = 0; a < CRT_FM; ++a)
 #pragma unroll
     for (int b = 0; b < CRT_FN; ++b)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
-  copy(0);
-  copy(1);
-  copy(2);
-  for (int64_t st = 0; st < ns; ++st) {
-    // the two newer stages' copies may stay in flight (a stage short of k blocks issues fewer,
-    // so only full stages are counted on: the last two stages wait for everything)
-    if (st + 2 < ns - 1)
-      __builtin_amdgcn_s_waitcnt((2 * CRT_PER) | (7 << 4));  // vmcnt(2 PER) lgkmcnt(0)
+#pragma unroll
+  for (int s0 = 0; s0 < CRT_NSLOT - 1; ++s0) copy(s0);
+  i32x4v fa0[CRT_FM], fb0[CRT_FN], fa1[CRT_FM], fb1[CRT_FN];
+  // one stage: (xa, xb) receives its first k block, (ya, yb) holds the previous stage's last
+  // block (none if first) and afterwards this stage's last block
+  auto stage = [&](int64_t st, bool first, i32x4v (&xa)[CRT_FM], i32x4v (&xb)[CRT_FN], i32x4v (&ya)[CRT_FM],
+                   i32x4v (&yb)[CRT_FN]) {
+    waitbar();
+    rd(st, 0, xa, xb);
+    __builtin_amdgcn_sched_barrier(0);
+    if (first)
+      copy(st + CRT_NSLOT - 1);
     else
-      __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_s_barrier();
+      mm_cp(ya, yb, st + CRT_NSLOT - 1);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int kk = 0; kk < CRT_KS; ++kk) {
-      const int64_t kb = st * CRT_KS + kk;
-      if (kb < Kb) {
-        const int8_t *base = &lds[(int)(st & 3) * CRT_SLOT + kk * (CRT_TA + CRT_TB) * OZ_FRAG + lane * 16];
-        i32x4v fa[CRT_FM], fb[CRT_FN];
-#pragma unroll
-        for (int a = 0; a < CRT_FM; ++a) fa[a] = *reinterpret_cast<const i32x4v *>(base + (wr * CRT_FM + a) * OZ_FRAG);
-#pragma unroll
-        for (int b = 0; b < CRT_FN; ++b)
-          fb[b] = *reinterpret_cast<const i32x4v *>(base + (CRT_TA + wc * CRT_FN + b) * OZ_FRAG);
-        if (kk == 0) copy(st + 3);
-#pragma unroll
-        for (int a = 0; a < CRT_FM; ++a)
-#pragma unroll
-          for (int b = 0; b < CRT_FN; ++b)
-            acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
-      }
+    for (int kk = 0; kk + 1 < CRT_KS; ++kk) {
+      if (kk & 1)
+        mm_rd(ya, yb, st, kk + 1, xa, xb);
+      else
+        mm_rd(xa, xb, st, kk + 1, ya, yb);
+      __builtin_amdgcn_sched_barrier(0);
     }
-    if (((st + 1) * CRT_KS) % CRT_KCHUNK == 0 && (st + 1) * CRT_KS < Kb) {
+  };
+  for (int64_t c0 = 0; c0 < nfull; c0 += CRT_CHS) {
+    const int64_t c1 = min(nfull, c0 + CRT_CHS);
+    stage(c0, true, fa0, fb0, fa1, fb1);
+    for (int64_t st = c0 + 1; st < c1; ++st) stage(st, false, fa0, fb0, fa1, fb1);
+    mm(fa1, fb1);
+    if (c1 == c0 + CRT_CHS && c1 * CRT_KS < Kb) {
       // |acc| <= 2048 * 32 * 2^14 = 2^30: reduce before it can overflow
 #pragma unroll
       for (int a = 0; a < CRT_FM; ++a)
@@ -1862,6 +1935,15 @@ __global__ void __launch_bounds__(512) k_crt_gemm(const int8_t *__restrict__ RA,
           for (int r = 0; r < 16; ++r) acc[a][b][r] = (int)(int8_t)crt_red_d(acc[a][b][r], p, ipd);
     }
   }
+  const int rem = (int)(Kb - nfull * CRT_KS);
+  if (rem) {  // the short last stage
+    waitbar();
+    for (int kk = 0; kk < rem; ++kk) {
+      rd(nfull, kk, fa0, fb0);
+      mm(fa0, fb0);
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(0);  // the copies past the end land before the workgroup ends
 #pragma unroll
   for (int a = 0; a < CRT_FM; ++a)
 #pragma unroll
