@@ -1781,8 +1781,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
 // between them; the reads of each next k block are issued between the MFMAs on the current one.
 // The order is pinned with sched_barrier.  (Products at 4096^3, 39 moduli, on one box: 2.90 ms
 // with two-block stages and reads waited for after every barrier and between MFMA pairs; 2.46 ms
-// pipelined; 2.32 ms with four-block stages, half the barriers.)
-// Workgroups of one modulus are consecutive (its residue planes stay in the caches).  Output block
+// pipelined; 2.32 ms with four-block stages, half the barriers; 2.29 ms persistent.)
+// The grid is one workgroup per CU; workgroup w takes tiles w, w + grid, ... in the order
+// (modulus, tile row, tile column), so the tiles of one modulus run together (its residue
+// planes stay in the caches), and the last stage of a tile copies the first stage of the next, so
+// a tile starts without waiting for a load.  Output block
 // (q, mb, nb), 1 KB: bytes v of lane L (rows 8(v/4) + 4(L/32) + v%4, column L%32 of the MFMA
 // result) at (v/4) * 256 + L * 4 + v%4, so that the reconstruction reads rows of 4 entries of 64
 // lanes contiguously.
@@ -1795,50 +1798,45 @@ constexpr int CRT_NSLOT = 2;                 // LDS ring slots; copies run CRT_N
 static_assert(CRT_TA * 32 == CRT_TILE && CRT_TB * 32 == CRT_TILE, "tile");
 static_assert(((CRT_TA + CRT_TB) * CRT_KS) % (CRT_WR * CRT_WC) == 0 && (CRT_NSLOT - 2) * CRT_PER <= 15, "copy split");
 static_assert(CRT_KS % 2 == 0 && CRT_PER <= CRT_FM * CRT_FN, "two fragment sets alternate over a stage's k blocks");
+static_assert(CRT_PER == CRT_TA && CRT_TA == CRT_TB && CRT_WR * CRT_WC == 2 * CRT_KS, "a wave copies one operand's blocks of one k block");
 
 __global__ void __launch_bounds__(512) k_crt_gemm(const int8_t *__restrict__ RA, const int8_t *__restrict__ RB,
                                                   int64_t Mb, int64_t Nb, int64_t Kb, int64_t pA, int64_t pB,
                                                   int64_t pC, const crt_tab *__restrict__ T,
-                                                  int8_t *__restrict__ RC) {
+                                                  int8_t *__restrict__ RC, int64_t ntiles) {
   __shared__ __attribute__((aligned(1024))) int8_t lds[CRT_NSLOT * CRT_SLOT];
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
-  const int64_t tilesM = Mb / CRT_TA, tilesN = Nb / CRT_TB, per = tilesM * tilesN;
-  const int64_t q = blockIdx.x / per, tt = blockIdx.x - q * per;
-  const int64_t tm = tt / tilesN, tn = tt - tm * tilesN;
-  const int8_t *Aq = RA + q * pA + tm * CRT_TA * Kb * OZ_FRAG;
-  const int8_t *Bq = RB + q * pB + tn * CRT_TB * Kb * OZ_FRAG;
-  auto slot = [](int64_t st) { return (int)((uint32_t)st % (uint32_t)CRT_NSLOT) * CRT_SLOT; };
-  const int64_t nfull = Kb / CRT_KS;  // full stages; the rest of the k blocks form a short last stage
-  // this wave's fragments of a stage: source row (or column) block and k block offset, LDS slot offset
-  const int8_t *src[CRT_PER];
-  int kko[CRT_PER], dst[CRT_PER];
-#pragma unroll
-  for (int c = 0; c < CRT_PER; ++c) {
-    const int f = wave * CRT_PER + c;  // fragment of the stage: k block kk, A 0..TA-1 then B
-    const int kk = f / (CRT_TA + CRT_TB), idx = f - kk * (CRT_TA + CRT_TB);
-    src[c] = (idx < CRT_TA) ? Aq + (int64_t)idx * Kb * OZ_FRAG : Bq + (int64_t)(idx - CRT_TA) * Kb * OZ_FRAG;
-    kko[c] = kk;
-    dst[c] = f * OZ_FRAG;
-  }
-  auto copy = [&](int64_t st) {
-    int8_t *d = &lds[slot(st)];
-#pragma unroll
-    for (int c = 0; c < CRT_PER; ++c) {
-      const int64_t kb = min(st * CRT_KS + kko[c], Kb - 1);
-      __builtin_amdgcn_global_load_lds((glb_void *)(src[c] + kb * OZ_FRAG + lane * 16), (lds_void *)(d + dst[c]), 16, 0, 0);
-    }
+  const int64_t tilesN = Nb / CRT_TB, per = (Mb / CRT_TA) * tilesN;
+  const int64_t nfull = Kb / CRT_KS;            // full stages of a tile
+  const int rem = (int)(Kb - nfull * CRT_KS);  // k blocks of its short last stage (0: none)
+  const int64_t ns = nfull + (rem ? 1 : 0);
+  auto slot = [](int64_t gs) { return (int)((uint32_t)gs % (uint32_t)CRT_NSLOT) * CRT_SLOT; };
+  // Wave w copies the 8 row blocks of A (w even) or the 8 column blocks of B (w odd) of k block
+  // w / 2 of each stage: fragment c of the stage's k block kk = w / 2 at LDS offset
+  // ((kk (TA + TB) + (w odd) TA + c) KB
+  const int kkw = wave >> 1;
+  const int dstw = (kkw * (CRT_TA + CRT_TB) + (wave & 1) * CRT_TA) * OZ_FRAG;
+  // a tile's copy source for this wave: its A row panel or B column panel
+  auto tile_src = [&](int64_t t) -> const int8_t * {
+    const int64_t q = t / per, tt = t - q * per, tm = tt / tilesN, tn = tt - tm * tilesN;
+    return (wave & 1) ? RB + q * pB + tn * CRT_TB * Kb * OZ_FRAG : RA + q * pA + tm * CRT_TA * Kb * OZ_FRAG;
   };
-  // wait for this wave's copies of the stage about to be read (the older of the CRT_NSLOT - 1
-  // stages in flight) and for its LDS reads of the previous stage, whose slot the copy issued after
-  // the barrier overwrites; then for every wave's
+  // copy fragment c of stage st (k blocks past the end load the last block again) into LDS slot d
+  auto copy1 = [&](const int8_t *s, int c, int64_t st, int8_t *d) {
+    const int64_t kb = min(st * CRT_KS + kkw, Kb - 1);
+    __builtin_amdgcn_global_load_lds((glb_void *)(s + ((int64_t)c * Kb + kb) * OZ_FRAG + lane * 16),
+                                     (lds_void *)(d + dstw + c * OZ_FRAG), 16, 0, 0);
+  };
+  // wait for this wave's copies of the stage about to be read and for its LDS reads of the
+  // previous stage, whose slot the copy issued after the barrier overwrites; then for every wave's
   auto waitbar = [&]() {
     __builtin_amdgcn_s_waitcnt(((CRT_NSLOT - 2) * CRT_PER) | (7 << 4));  // vmcnt((NSLOT - 2) PER) lgkmcnt(0)
     __builtin_amdgcn_s_barrier();
   };
   const int wr = wave / CRT_WC, wc = wave % CRT_WC;
   const int aoff = wr * CRT_FM * OZ_FRAG + lane * 16, boff = (CRT_TA + wc * CRT_FN) * OZ_FRAG + lane * 16;
-  auto rd = [&](int64_t st, int kk, i32x4v (&fa)[CRT_FM], i32x4v (&fb)[CRT_FN]) {
-    const int8_t *base = &lds[slot(st) + kk * (CRT_TA + CRT_TB) * OZ_FRAG];
+  auto rd = [&](int64_t gs, int kk, i32x4v (&fa)[CRT_FM], i32x4v (&fb)[CRT_FN]) {
+    const int8_t *base = &lds[slot(gs) + kk * (CRT_TA + CRT_TB) * OZ_FRAG];
 #pragma unroll
     for (int a = 0; a < CRT_FM; ++a) fa[a] = *reinterpret_cast<const i32x4v *>(base + aoff + a * OZ_FRAG);
 #pragma unroll
@@ -1852,11 +1850,11 @@ __global__ void __launch_bounds__(512) k_crt_gemm(const int8_t *__restrict__ RA,
       for (int b = 0; b < CRT_FN; ++b)
         acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
   };
-  // the MFMAs of (fa, fb) with k block kk of stage st read into (ga, gb) between them: the reads
-  // are issued after this wave's wait for (fa, fb) and land while the MFMAs run
-  auto mm_rd = [&](const i32x4v (&fa)[CRT_FM], const i32x4v (&fb)[CRT_FN], int64_t st, int kk, i32x4v (&ga)[CRT_FM],
+  // the MFMAs of (fa, fb) with k block kk of global stage gs read into (ga, gb) between them: the
+  // reads are issued after this wave's wait for (fa, fb) and land while the MFMAs run
+  auto mm_rd = [&](const i32x4v (&fa)[CRT_FM], const i32x4v (&fb)[CRT_FN], int64_t gs, int kk, i32x4v (&ga)[CRT_FM],
                    i32x4v (&gb)[CRT_FN]) {
-    const int8_t *base = &lds[slot(st) + kk * (CRT_TA + CRT_TB) * OZ_FRAG];
+    const int8_t *base = &lds[slot(gs) + kk * (CRT_TA + CRT_TB) * OZ_FRAG];
 #pragma unroll
     for (int a = 0; a < CRT_FM; ++a)
 #pragma unroll
@@ -1871,9 +1869,9 @@ __global__ void __launch_bounds__(512) k_crt_gemm(const int8_t *__restrict__ RA,
         __builtin_amdgcn_sched_barrier(0);
       }
   };
-  // the MFMAs of (fa, fb) with the copies of stage st issued between them
-  auto mm_cp = [&](const i32x4v (&fa)[CRT_FM], const i32x4v (&fb)[CRT_FN], int64_t st) {
-    int8_t *d = &lds[slot(st)];
+  // the MFMAs of (fa, fb) with the copies of stage st of the tile with sources s into slot d
+  // issued between them
+  auto mm_cp = [&](const i32x4v (&fa)[CRT_FM], const i32x4v (&fb)[CRT_FN], const int8_t *s, int64_t st, int8_t *d) {
 #pragma unroll
     for (int a = 0; a < CRT_FM; ++a)
 #pragma unroll
@@ -1881,81 +1879,118 @@ __global__ void __launch_bounds__(512) k_crt_gemm(const int8_t *__restrict__ RA,
         acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
         __builtin_amdgcn_sched_barrier(0);
         const int c = a * CRT_FN + b;
-        if (c < CRT_PER) {
-          const int64_t kb = min(st * CRT_KS + kko[c], Kb - 1);
-          __builtin_amdgcn_global_load_lds((glb_void *)(src[c] + kb * OZ_FRAG + lane * 16), (lds_void *)(d + dst[c]), 16, 0, 0);
-        }
+        if (c < CRT_PER) copy1(s, c, st, d);
         __builtin_amdgcn_sched_barrier(0);
       }
   };
-  const int p = T->p[q];
-  const double ipd = T->pinvd[q];
+  int64_t t = blockIdx.x;
+  if (t >= ntiles) return;
+  const int8_t *cur = tile_src(t), *nxt, *cs;
+  int64_t gs = 0;  // this workgroup's stage count over its tiles: the LDS slot of a stage
 #pragma unroll
-  for (int a = 0; a < CRT_FM; ++a)
-#pragma unroll
-    for (int b = 0; b < CRT_FN; ++b)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
-#pragma unroll
-  for (int s0 = 0; s0 < CRT_NSLOT - 1; ++s0) copy(s0);
+  for (int c = 0; c < CRT_PER; ++c) copy1(cur, c, 0, &lds[slot(gs)]);
   i32x4v fa0[CRT_FM], fb0[CRT_FN], fa1[CRT_FM], fb1[CRT_FN];
-  // one stage: (xa, xb) receives its first k block, (ya, yb) holds the previous stage's last
-  // block (none if first) and afterwards this stage's last block
-  auto stage = [&](int64_t st, bool first, i32x4v (&xa)[CRT_FM], i32x4v (&xb)[CRT_FN], i32x4v (&ya)[CRT_FM],
-                   i32x4v (&yb)[CRT_FN]) {
-    waitbar();
-    rd(st, 0, xa, xb);
-    __builtin_amdgcn_sched_barrier(0);
-    if (first)
-      copy(st + CRT_NSLOT - 1);
-    else
-      mm_cp(ya, yb, st + CRT_NSLOT - 1);
-    __builtin_amdgcn_sched_barrier(0);
+  for (; t < ntiles; t += gridDim.x) {
+    // Each stage issues the copies of the next one: stage st + 1 of this tile, or stage 0 of the
+    // workgroup's next tile (so its first stage does not wait for a load), or, after the last
+    // tile, this tile's last stage again into the free slot (never read).
+    const bool has_next = t + gridDim.x < ntiles;
+    nxt = tile_src(has_next ? t + gridDim.x : t);
+    const int64_t q = t / per, tt = t - q * per, tm = tt / tilesN, tn = tt - tm * tilesN;
+    const int p = T->p[q];
+    const double ipd = T->pinvd[q];
 #pragma unroll
-    for (int kk = 0; kk + 1 < CRT_KS; ++kk) {
-      if (kk & 1)
-        mm_rd(ya, yb, st, kk + 1, xa, xb);
-      else
-        mm_rd(xa, xb, st, kk + 1, ya, yb);
+    for (int a = 0; a < CRT_FM; ++a)
+#pragma unroll
+      for (int b = 0; b < CRT_FN; ++b)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
+    // copy sources and stage for the stage after st
+    auto next_of = [&](int64_t st, int64_t &nst) {
+      const bool in = st + 1 < ns;
+      cs = in ? cur : nxt;
+      nst = in ? st + 1 : (has_next ? 0 : ns - 1);
+    };
+    // one full stage st (global stage gs): (xa, xb) receives its first k block, (ya, yb) holds the
+    // previous stage's last block (none if first) and afterwards this stage's last block
+    auto stage = [&](int64_t st, bool first, i32x4v (&xa)[CRT_FM], i32x4v (&xb)[CRT_FN], i32x4v (&ya)[CRT_FM],
+                     i32x4v (&yb)[CRT_FN]) {
+      int64_t nst;
+      next_of(st, nst);
+      waitbar();
+      rd(gs, 0, xa, xb);
       __builtin_amdgcn_sched_barrier(0);
+      if (first) {
+#pragma unroll
+        for (int c = 0; c < CRT_PER; ++c) copy1(cs, c, nst, &lds[slot(gs + 1)]);
+      } else {
+        mm_cp(ya, yb, cs, nst, &lds[slot(gs + 1)]);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int kk = 0; kk + 1 < CRT_KS; ++kk) {
+        if (kk & 1)
+          mm_rd(ya, yb, gs, kk + 1, xa, xb);
+        else
+          mm_rd(xa, xb, gs, kk + 1, ya, yb);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      ++gs;
+    };
+    for (int64_t c0 = 0; c0 < nfull; c0 += CRT_CHS) {
+      const int64_t c1 = min(nfull, c0 + CRT_CHS);
+      stage(c0, true, fa0, fb0, fa1, fb1);
+      for (int64_t st = c0 + 1; st < c1; ++st) stage(st, false, fa0, fb0, fa1, fb1);
+      mm(fa1, fb1);
+      if (c1 == c0 + CRT_CHS && c1 * CRT_KS < Kb) {
+        // |acc| <= 2048 * 32 * 2^14 = 2^30: reduce before it can overflow
+#pragma unroll
+        for (int a = 0; a < CRT_FM; ++a)
+#pragma unroll
+          for (int b = 0; b < CRT_FN; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[a][b][r] = (int)(int8_t)crt_red_d(acc[a][b][r], p, ipd);
+      }
     }
-  };
-  for (int64_t c0 = 0; c0 < nfull; c0 += CRT_CHS) {
-    const int64_t c1 = min(nfull, c0 + CRT_CHS);
-    stage(c0, true, fa0, fb0, fa1, fb1);
-    for (int64_t st = c0 + 1; st < c1; ++st) stage(st, false, fa0, fb0, fa1, fb1);
-    mm(fa1, fb1);
-    if (c1 == c0 + CRT_CHS && c1 * CRT_KS < Kb) {
-      // |acc| <= 2048 * 32 * 2^14 = 2^30: reduce before it can overflow
+    if (rem) {  // the short last stage
+      int64_t nst;
+      next_of(nfull, nst);
+      waitbar();
 #pragma unroll
-      for (int a = 0; a < CRT_FM; ++a)
-#pragma unroll
-        for (int b = 0; b < CRT_FN; ++b)
-#pragma unroll
-          for (int r = 0; r < 16; ++r) acc[a][b][r] = (int)(int8_t)crt_red_d(acc[a][b][r], p, ipd);
+      for (int c = 0; c < CRT_PER; ++c) copy1(cs, c, nst, &lds[slot(gs + 1)]);
+      for (int kk = 0; kk < rem; ++kk) {
+        rd(gs, kk, fa0, fb0);
+        mm(fa0, fb0);
+      }
+      ++gs;
     }
+#pragma unroll
+    for (int a = 0; a < CRT_FM; ++a)
+#pragma unroll
+      for (int b = 0; b < CRT_FN; ++b) {
+        uint32_t u[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int v = 0; v < 16; ++v) u[v >> 2] |= crt_red_d(acc[a][b][v], p, ipd) << (8 * (v & 3));
+        const int64_t blk = (tm * CRT_TA + wr * CRT_FM + a) * Nb + tn * CRT_TB + wc * CRT_FN + b;
+#pragma unroll
+        for (int r4 = 0; r4 < 4; ++r4)
+          *reinterpret_cast<uint32_t *>(RC + q * pC + blk * OZ_FRAG + r4 * 256 + lane * 4) = u[r4];
+      }
+    cur = nxt;
   }
-  const int rem = (int)(Kb - nfull * CRT_KS);
-  if (rem) {  // the short last stage
-    waitbar();
-    for (int kk = 0; kk < rem; ++kk) {
-      rd(nfull, kk, fa0, fb0);
-      mm(fa0, fb0);
-    }
+  __builtin_amdgcn_s_waitcnt(0);  // the last copies land before the workgroup ends
+}
+
+static int crt_cu_count() {
+  static int cus[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cus[dev]) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    cus[dev] = n;
   }
-  __builtin_amdgcn_s_waitcnt(0);  // the copies past the end land before the workgroup ends
-#pragma unroll
-  for (int a = 0; a < CRT_FM; ++a)
-#pragma unroll
-    for (int b = 0; b < CRT_FN; ++b) {
-      uint32_t u[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-      for (int v = 0; v < 16; ++v) u[v >> 2] |= crt_red_d(acc[a][b][v], p, ipd) << (8 * (v & 3));
-      const int64_t blk = (tm * CRT_TA + wr * CRT_FM + a) * Nb + tn * CRT_TB + wc * CRT_FN + b;
-#pragma unroll
-      for (int r4 = 0; r4 < 4; ++r4)
-        *reinterpret_cast<uint32_t *>(RC + q * pC + blk * OZ_FRAG + r4 * 256 + lane * 4) = u[r4];
-    }
+  return cus[dev];
 }
 
 // 4 x 4 byte transpose: byte j of o_e = byte e of i_j (v_perm_b32 selects bytes 0-3 from its
@@ -2381,7 +2416,8 @@ hipError_t launch_gemm_crt(int colmajor, int64_t m, int64_t n, int64_t k, int t,
   k_crt_residues<<<(unsigned)((L.Mb * L.Kb + 3) / 4), 256, 0, st>>>(A, m, k, asr, asc, eA, t, tab, L.Mb, L.Kb, L.pA, RA);
   k_crt_residues<<<(unsigned)((L.Nb * L.Kb + 3) / 4), 256, 0, st>>>(B, n, k, bsc, bsr, eB, t, tab, L.Nb, L.Kb, L.pB, RB);
   if (ev) (void)hipEventRecord(ev[1], st);
-  k_crt_gemm<<<(unsigned)wgs, 512, 0, st>>>(RA, RB, L.Mb, L.Nb, L.Kb, L.pA, L.pB, L.pC, tab, RC);
+  k_crt_gemm<<<(unsigned)min(wgs, (int64_t)crt_cu_count()), 512, 0, st>>>(RA, RB, L.Mb, L.Nb, L.Kb, L.pA, L.pB, L.pC, tab,
+                                                                     RC, wgs);
   if (ev) (void)hipEventRecord(ev[2], st);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
