@@ -1594,20 +1594,25 @@ __device__ __forceinline__ uint32_t crt_red_d(int x, int p, double ip) {
   return (uint32_t)(int)r & 0xffu;
 }
 
-// 16 residue sums y (in [0, 2^20), = x + off) of one modulus -> 16 balanced residues as bytes:
-// q = floor(y / p) by a 24-bit multiply-high, r = y - q p - half in [-128, 127]
-__device__ __forceinline__ i32x4v crt_pack16(const i32x16v &d, int p, uint32_t mdiv, int hf, uint32_t off) {
+// residue of one sum x (x + off in [0, 2^20)) modulo p as a balanced byte, in four full-rate
+// instructions: Y = 16 (x + off) (one shift-add, off16 = 16 off), q = floor(Y mdiv / 2^32) =
+// floor((x + off) / p) by a 24-bit multiply-high, r = x - (q p + hmo) with hmo = half - off (mod
+// 2^32), so r = x + off - q p - half in [-128, 127]
+__device__ __forceinline__ uint32_t crt_red24(uint32_t x, uint32_t p, uint32_t mdiv, uint32_t hmo, uint32_t off16) {
+  const uint32_t y16 = (x << 4) + off16;
+  const uint32_t q = (uint32_t)(((uint64_t)(y16 & 0xffffffu) * (mdiv & 0xffffffu)) >> 32);
+  uint32_t qp;
+  // q < 2^20 / p and p <= 256: q p + hmo is one full-rate 24-bit multiply-add (the compiler
+  // otherwise picks a quarter-rate 32- or 64-bit multiply)
+  asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(qp) : "v"(q), "v"(p), "v"(hmo));
+  return x - qp;
+}
+
+// 16 residue sums x of one modulus -> 16 balanced residues as bytes
+__device__ __forceinline__ i32x4v crt_pack16(const i32x16v &d, int p, uint32_t mdiv, uint32_t hmo, uint32_t off16) {
   int r[16];
 #pragma unroll
-  for (int v = 0; v < 16; ++v) {
-    const uint32_t y = (uint32_t)d[v] + off;
-    const uint32_t q = (uint32_t)(((uint64_t)((y << 4) & 0xffffffu) * (mdiv & 0xffffffu)) >> 32);
-    // q < 2^20 / p and p <= 256: q p + half is one full-rate 24-bit multiply-add (the compiler
-    // otherwise picks the 64-bit v_mad_u64_u32)
-    uint32_t qp;
-    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(qp) : "v"(q), "v"((uint32_t)p), "v"((uint32_t)hf));
-    r[v] = (int)(y - qp);
-  }
+  for (int v = 0; v < 16; ++v) r[v] = (int)crt_red24((uint32_t)d[v], (uint32_t)p, mdiv, hmo, off16);
   i32x4v w;
   uint32_t o[4];
 #pragma unroll
@@ -1646,14 +1651,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
                                                       int64_t plane, int8_t *__restrict__ out) {
   __shared__ __attribute__((aligned(16))) uint32_t s_lo[4][32 * CRT_RS][4];
   __shared__ uint32_t s_hi[4][32 * CRT_RS];
-  __shared__ i32x4v s_cst[32];  // moduli 32..63: p, mdiv, half, off
+  __shared__ i32x4v s_cst[32];  // moduli 32..63: p, mdiv, half - off, 16 off
   if (threadIdx.x < 32) {
     const int q = 32 + (int)threadIdx.x;
     i32x4v c;
     c.x = T->p[q];
     c.y = (int)T->mdiv[q];
-    c.z = T->half[q];
-    c.w = T->off[q];
+    c.z = T->half[q] - T->off[q];
+    c.w = T->off[q] << 4;
     s_cst[threadIdx.x] = c;
   }
   __syncthreads();  // every wave of the block is still here
@@ -1667,8 +1672,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
   const i32x4v bc1 = *reinterpret_cast<const i32x4v *>(&T->cpow[32 + n][16 * h]);
   const int p0 = T->p[n];
   const uint32_t m0 = T->mdiv[n];
-  const int h0 = T->half[n];
-  const uint32_t off0 = (uint32_t)T->off[n];
+  const uint32_t hmo0 = (uint32_t)(T->half[n] - T->off[n]), off16_0 = (uint32_t)T->off[n] << 4;
   // quad byte transpose: lane j of a quad takes byte j of each quad lane's word
   const uint32_t qj = (uint32_t)(lane & 3);
   const uint32_t sel_lo = 0x0c0c0000u | ((4u + qj) << 8) | qj, sel_hi = ((4u + qj) << 24) | (qj << 16) | 0x0c0cu;
@@ -1721,17 +1725,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
       const int g = 16 * half + gg;
       const int slot = 2 * g + hh;
       const int e = (slot & 31) * CRT_RS + ib;
+      // both reads in every lane, then a select (no divergent branch)
+      const i32x4v lo = *reinterpret_cast<const i32x4v *>(&s_lo[wave][e][0]);
+      const int hi = (int)s_hi[wave][e];
       i32x4v a;
-      if (h == 0) {
-        a = *reinterpret_cast<const i32x4v *>(&s_lo[wave][e][0]);
-      } else {
-        a.x = (int)s_hi[wave][e];
-        a.y = 0;
-        a.z = 0;
-        a.w = 0;
-      }
+      a.x = h ? hi : lo.x;
+      a.y = h ? 0 : lo.y;
+      a.z = h ? 0 : lo.z;
+      a.w = h ? 0 : lo.w;
       const i32x16v d0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bc0, zero, 0, 0, 0);
-      if (n < nq) *reinterpret_cast<i32x4v *>(o0 + (2 * g + h) * 16) = crt_pack16(d0, p0, m0, h0, off0);
+      if (n < nq) *reinterpret_cast<i32x4v *>(o0 + (2 * g + h) * 16) = crt_pack16(d0, p0, m0, hmo0, off16_0);
       if (nq > 32) {  // uniform
         const i32x16v d1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(bc1, a, zero, 0, 0, 0);
         // this lane's element: slot 2g + ((n >> 2) & 1), byte 4 (n >> 3) + (n & 3)
@@ -1743,11 +1746,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
             const i32x4v c = s_cst[8 * vg + 4 * h + j];
-            const uint32_t y = (uint32_t)d1[4 * vg + j] + (uint32_t)c.w;
-            const uint32_t q = (uint32_t)(((uint64_t)((y << 4) & 0xffffffu) * ((uint32_t)c.y & 0xffffffu)) >> 32);
-            uint32_t qp;
-            asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(qp) : "v"(q), "v"((uint32_t)c.x), "v"((uint32_t)c.z));
-            r[j] = y - qp;
+            r[j] = crt_red24((uint32_t)d1[4 * vg + j], (uint32_t)c.x, (uint32_t)c.y, (uint32_t)c.z, (uint32_t)c.w);
           }
           const uint32_t t01 = __builtin_amdgcn_perm(r[1], r[0], 0x0c0c0400u);
           const uint32_t t23 = __builtin_amdgcn_perm(r[3], r[2], 0x0c0c0400u);
