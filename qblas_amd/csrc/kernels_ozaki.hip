@@ -2076,11 +2076,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   // of moduli 16 kh + 0..15 (+ 32 for the second MFMA half) for both row halves hp: Q[hp][kb][j].
   // Moduli 48.. are never used (nq <= 47).
   const int8_t *bbase = RC + blk * OZ_FRAG;  // uniform
-#pragma nounroll
-  for (int rq = 0; rq < 4; ++rq) {
-    // all loads first, then the swaps; a plane beyond nq - 1 reads plane nq - 1 again: its
-    // column of the digit table (the MFMA's A operand) is zero, so those words add nothing
-    uint32_t w[48];
+  // A row quarter's residue words: a plane beyond nq - 1 reads plane nq - 1 again: its column of
+  // the digit table (the MFMA's A operand) is zero, so those words add nothing
+  uint32_t w[48];
+  auto load_rows = [&](int rq) {
     const int8_t *pq = bbase + rq * 256;  // plane min(q, nq - 1), advanced a plane at a time
     int64_t pstep = plane;
     asm volatile("" : "+s"(pstep));  // opaque here: the 48 plane offsets are not kept across rq
@@ -2089,6 +2088,29 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       w[q] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(pq) + lane);
       if (q + 1 < nq) pq += pstep;
     }
+  };
+  const bool colin = col < N;
+  const uint32_t fBc = (FAST && colin) ? fB[col] : 0u;
+  const int32_t eBc = (FAST && colin) ? eB[col] : OZ_EZERO;
+  // a row quarter's per-entry inputs of the FAST epilogue: row exponent and flags, top word of C
+  int32_t eAr[4], eAn[4] = {OZ_EZERO, OZ_EZERO, OZ_EZERO, OZ_EZERO};
+  uint32_t fAr[4], chi[4], fAn[4] = {0u, 0u, 0u, 0u}, chn[4] = {0u, 0u, 0u, 0u};
+  auto load_entries = [&](int rq, int32_t (&ea)[4], uint32_t (&fa)[4], uint32_t (&ch)[4]) {
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+      const int64_t row = bm * 32 + 8 * rq + 4 * (it >> 1) + 2 * (it & 1) + kh;
+      const bool in = FAST && row < M && colin;
+      ea[it] = in ? eA[row] : OZ_EZERO;
+      fa[it] = in ? fA[row] : 0u;
+      ch[it] = in ? reinterpret_cast<const uint32_t *>(C + row * csr + col * csc)[3] : 0u;
+    }
+  };
+  load_entries(0, eAr, fAr, chi);
+  load_rows(0);
+  // Each quarter waits once, at its top, for everything loaded so far; its swaps free w, and the
+  // next quarter's loads are issued before its reconstruction, so they land meanwhile
+#pragma nounroll
+  for (int rq = 0; rq < 4; ++rq) {
     uint32_t Q[2][2][16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
@@ -2100,6 +2122,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       const auto s2 = __builtin_amdgcn_permlane32_swap(w[32 + j], 0u, false, false);
       Q[0][1][j] = s2[0];
       Q[1][1][j] = s2[1];
+    }
+    if (rq < 3) {
+      load_entries(rq + 1, eAn, fAn, chn);
+      load_rows(rq + 1);
     }
 #pragma nounroll
     for (int it = 0; it < 4; ++it) {
@@ -2210,9 +2236,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
             // the repair without rounding, and the rounding never needs a left shift.  A result
             // outside the normal range is left to the repair too.
             const int msb = rmsb<XW>(mag);
-            if (!((fA[row] | fB[col]) & OZF_NONFINITE) && q_bexp(ldq_c(cp)) != 0x7fffu && eA[row] != OZ_EZERO &&
-                eB[col] != OZ_EZERO && msb >= kc + t + 104) {
-              const int Ei = eA[row] + 3, Fj = eB[col] + 3;
+            int32_t eAi = eAr[0];
+            uint32_t fAi = fAr[0], chii = chi[0];
+#pragma unroll
+            for (int i = 1; i < 4; ++i)
+              if (it == i) {
+                eAi = eAr[i];
+                fAi = fAr[i];
+                chii = chi[i];
+              }
+            if (!((fAi | fBc) & OZF_NONFINITE) && ((chii >> 16) & 0x7fffu) != 0x7fffu && eAi != OZ_EZERO &&
+                eBc != OZ_EZERO && msb >= kc + t + 104) {
+              const int Ei = eAi + 3, Fj = eBc + 3;
               const int lsb = Ei + Fj - 2 * t;
               const int be = lsb + msb + QBIAS;  // biased exponent before a rounding carry
               if (be >= 1 && be <= 0x7ffd) {
@@ -2231,6 +2266,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
           }
         }
       }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      eAr[i] = eAn[i];
+      fAr[i] = fAn[i];
+      chi[i] = chn[i];
     }
   }
 }
