@@ -438,40 +438,41 @@ __device__ __forceinline__ void oz_gemm_tile(const int8_t *__restrict__ As, cons
       }
     }
   }
-  // V = sum_{d=D0..D1} acc_d 256^(S-1-d): Horner over the range in OZ_PW signed words, then a
-  // shift by 8 (S-1-D1); result element r of this lane is row 8(r/4) + 4(lane/32) + r%4,
-  // column lane%32 of the wave's block
+  // V = sum_{d=D0..D1} acc_d 256^(S-1-d) in OZ_PW two's complement words: diagonal d sits at bit
+  // off = 8 (S-1-d), in word w = off / 32 shifted by b = off mod 32.  Its low part (acc << b) is
+  // added into lo[w] (carries counted in cy[w]) and its high part (acc >> (32 - b), arithmetic;
+  // the sign for b = 0) into hi[w + 1]; one signed carry pass over the words then gives V.  Result
+  // element r of this lane is row 8(r/4) + 4(lane/32) + r%4, column lane%32 of the wave's block
   const int64_t row0 = tm * OZ_TILE + wr * 32 + 4 * (lane >> 5);
   const int64_t col = tn * OZ_TILE + wc * 32 + (lane & 31);
   uint32_t *pz = part + (zbase + kp) * Mp * Np * OZ_PREC;
-  constexpr int SH = 8 * (S - 1 - D1);
+  static_assert(8 * (S - 1 - D0) + 31 + 3 < 32 * OZ_PW, "V fits its words");
 #pragma clang loop unroll(full)
   for (int r = 0; r < 16; ++r) {
-    uint32_t V[OZ_PW];
+    uint32_t lo[OZ_PW], cy[OZ_PW];
+    int32_t hi[OZ_PW];
 #pragma unroll
-    for (int i = 0; i < OZ_PW; ++i) V[i] = 0;
+    for (int i = 0; i < OZ_PW; ++i) {
+      lo[i] = 0u;
+      cy[i] = 0u;
+      hi[i] = 0;
+    }
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
-#pragma unroll
-      for (int i = OZ_PW - 1; i > 0; --i) V[i] = (V[i] << 8) | (V[i - 1] >> 24);
-      V[0] <<= 8;
-      const uint32_t a = (uint32_t)acc[d][r], ext = (acc[d][r] < 0) ? 0xffffffffu : 0u;
-      uint32_t c = 0;
-#pragma unroll
-      for (int i = 0; i < OZ_PW; ++i) V[i] = addc32(V[i], i == 0 ? a : ext, c, c);
+      const int off = 8 * (S - 1 - (D0 + d)), w = off >> 5, bs = off & 31;
+      const int32_t x = acc[d][r];
+      const uint64_t sum = (uint64_t)lo[w] + (uint32_t)(bs ? ((uint32_t)x << bs) : (uint32_t)x);
+      lo[w] = (uint32_t)sum;
+      cy[w] += (uint32_t)(sum >> 32);
+      if (w + 1 < OZ_PW) hi[w + 1] += bs ? (x >> (32 - bs)) : (x >> 31);
     }
-    if (SH > 0) {
-      // left shift of the two's complement value by SH bits (SH < 32 * OZ_PW)
-      constexpr int WS = SH / 32, BS = SH % 32;
-      uint32_t T[OZ_PW];
+    uint32_t V[OZ_PW];
+    int64_t c = 0;
 #pragma unroll
-      for (int i = OZ_PW - 1; i >= 0; --i) {
-        const uint32_t hi = (i - WS >= 0) ? V[i - WS] : 0u;
-        const uint32_t lo = (i - WS - 1 >= 0) ? V[i - WS - 1] : 0u;
-        T[i] = BS ? ((hi << BS) | (lo >> (32 - BS))) : hi;
-      }
-#pragma unroll
-      for (int i = 0; i < OZ_PW; ++i) V[i] = T[i];
+    for (int i = 0; i < OZ_PW; ++i) {
+      const int64_t tv = c + (int64_t)lo[i] + ((int64_t)cy[i] << 32) + (int64_t)hi[i];
+      V[i] = (uint32_t)tv;
+      c = tv >> 32;
     }
     const int64_t row = row0 + 8 * (r >> 2) + (r & 3);
     uint32_t *p = pz + (row * Np + col) * OZ_PREC;
@@ -486,6 +487,7 @@ __device__ __forceinline__ void oz_gemm_tile(const int8_t *__restrict__ As, cons
     v1.w = (int)V[7];
     *reinterpret_cast<i32x4v *>(p) = v0;
     *reinterpret_cast<i32x4v *>(p + 4) = v1;
+    __builtin_amdgcn_sched_barrier(0);  // one entry at a time: its word sums stay in few registers
   }
 }
 
