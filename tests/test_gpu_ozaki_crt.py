@@ -8,7 +8,8 @@ certified value is the correctly rounded alpha*X*2^(E_i+F_j-2t) + beta*c of the 
 the scaled integers; the tolerance covers the operands' rounding to t bits.  Cases: small and ragged
 shapes in both layouts, more than one 256 x 256 product tile, k long enough for the in-loop modular
 reduction (k > 65536), the fewest and most bits (13 and 46 moduli), zero rows, Inf operands,
-alpha = 0, the automatic choice of method and bits, and the 4096^3 workload on 4096 sampled
+alpha = 0, the automatic choice of method and bits, more tiles than workgroups of the persistent
+product kernel with a short last stage of k blocks, and the 4096^3 workload on 4096 sampled
 entries.
 """
 import numpy as np
@@ -223,6 +224,27 @@ def test_crt_4096_cube_sampled(O, Q, cuda):
     rounded, rel, cand = _exact(O, 0, N, 1.0, A, N, B, N, 0.0, C0, N, got, ii, jj)
     _say(f"  max rel {rel.max():.3e} (2^{np.log2(max(rel.max(), 1e-300)):.1f}), "
          f"median 2^{np.log2(max(np.median(rel), 1e-300)):.1f}")
+    assert rel.max() <= TOL
+
+
+@pytest.mark.parametrize("layout,k", [(0, 4100), (1, 4160), (0, 4192)])
+def test_crt_tiles_per_workgroup_short_last_stage(O, Q, cuda, layout, k):
+    # 1024 x 1024 at t = 136: 16 tiles x about 40 moduli, more than one tile per workgroup of the
+    # persistent product kernel (one workgroup per CU), so each workgroup copies its next tile's
+    # first stage during the last stage of the current one; k = 4100, 4160, 4192 leave 1, 2, 3 k
+    # blocks in the short last stage of four
+    m = n = 1024
+    t = 136
+    _say(f"ozaki crt persistent tiles layout={layout} m=n={m} k={k} t={t}")
+    A, lda, B, ldb, C0, ldc = _operands(O, layout, m, n, k, "log30", 3100 + k)
+    got, status, repairs, info = _run(Q, cuda, layout, m, n, k, 1.5, -0.5, A, lda, B, ldb, C0, ldc, t)
+    assert info == (Q.OZAKI_CRT, t, _moduli(k, t)) and 16 * info[2] > 256
+    assert status == 0
+    rng = np.random.default_rng(k)
+    ii = np.concatenate([rng.integers(0, m, 700), np.full(16, m - 1), np.arange(16)])
+    jj = np.concatenate([rng.integers(0, n, 700), np.arange(n - 16, n), np.full(16, n - 1)])
+    rounded, rel, cand = _exact(O, layout, k, 1.5, A, lda, B, ldb, -0.5, C0, ldc, got, ii, jj)
+    _say(f"  moduli={info[2]} repairs={repairs} max rel={rel.max():.3e}")
     assert rel.max() <= TOL
 
 
