@@ -235,7 +235,8 @@ __global__ void __launch_bounds__(256) k_gemv_batched_ref(const qblas_gemv_probl
   const int lane = threadIdx.x & 63;
   const int64_t total = start[count];
   const int64_t waves = (int64_t)gridDim.x * 4;
-  for (int64_t u = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); u < total; u += waves) {
+  for (int64_t u = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); u < total;
+       u += waves) {
     const int64_t p = find_problem(start, count, u);
     const qblas_gemv_problem &v = P[p];
     const int64_t m = v.m, n = v.n, lda = v.lda, incx = v.incx, incy = v.incy;
@@ -314,7 +315,9 @@ __global__ void __launch_bounds__(256) k_gemv_batched_dd(const qblas_gemv_proble
                                                          const int64_t *__restrict__ start) {
   const int lane = threadIdx.x & 63;
   const int64_t total = start[count];
-  const int64_t waves = (int64_t)gridDim.x * 4, w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  // the wave index as a scalar: item and problem searches run in scalar registers
+  const int64_t waves = (int64_t)gridDim.x * 4,
+                w = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   if (!colmajor) {
     // RowMajor: lanes stride along the columns of an item's 4 rows; one joint butterfly leaves
     // row r's total in lanes 16 r .. 16 r + 15, lane 4 q + r of the wave keeps the total of row r
@@ -438,7 +441,8 @@ __global__ void __launch_bounds__(256) k_gemv_batched_exact(const qblas_gemv_pro
   const int lane = threadIdx.x & 63;
   const int64_t total = start[count];
   const int64_t waves = (int64_t)gridDim.x * 4;
-  for (int64_t u = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); u < total; u += waves) {
+  for (int64_t u = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); u < total;
+       u += waves) {
     const int64_t p = find_problem(start, count, u);
     const qblas_gemv_problem &v = P[p];
     const int64_t m = v.m, n = v.n, lda = v.lda, incx = v.incx, incy = v.incy;
@@ -510,7 +514,7 @@ __global__ void __launch_bounds__(256) k_gemv_batched_exact_wide(const qblas_gem
   __shared__ int64_t s_p[256], s_i[256];
   __shared__ int s_n;
   if (status[1] == 0) return;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int64_t total = start[count];
   kul_clear_all(K);
   if (threadIdx.x == 0) s_n = 0;

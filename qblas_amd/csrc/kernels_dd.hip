@@ -185,7 +185,9 @@ __global__ void __launch_bounds__(GV_BS) k_gemv_row_dd(int64_t m, int64_t n, q12
                                                        q128 *__restrict__ y, int64_t incy) {
   const int64_t W = (int64_t)gridDim.x * (GV_BS / 64);
   const int lane = threadIdx.x & 63;
-  for (int64_t row0 = ((int64_t)blockIdx.x * (GV_BS / 64) + (threadIdx.x >> 6)) * 2; row0 < m; row0 += 2 * W) {
+  // the wave index as a scalar: the row pair's addresses stay in scalar registers
+  for (int64_t row0 = ((int64_t)blockIdx.x * (GV_BS / 64) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6))) * 2;
+       row0 < m; row0 += 2 * W) {
     if (row0 + 1 < m)  // uniform per wave
       gemv_rows_dd<2>(row0, n, alpha, A, lda, xs, beta, y, incy, lane);
     else

@@ -142,7 +142,9 @@ __global__ void __launch_bounds__(XR_BS) k_gemv_row_exact(int64_t m, int64_t n, 
   unsigned long long *d = &s_dig[threadIdx.x >> 6][0][lane];
   const int64_t nw = (int64_t)gridDim.x * (XR_BS / 64);
   // whole waves stride over the (row, chunk) items
-  for (int64_t wid = (int64_t)blockIdx.x * (XR_BS / 64) + (threadIdx.x >> 6); wid < m * nch; wid += nw) {
+  // the wave index as a scalar: the item's row, chunk and row address stay in scalar registers
+  for (int64_t wid = (int64_t)blockIdx.x * (XR_BS / 64) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+       wid < m * nch; wid += nw) {
     const int64_t row = wid / nch;
     const int c = (int)(wid - row * nch);
     const int64_t j0 = (int64_t)c * chunk;

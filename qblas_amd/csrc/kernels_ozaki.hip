@@ -132,7 +132,7 @@ __global__ void __launch_bounds__(256) k_oz_scale(const q128 *__restrict__ X, in
   int e = OZ_EZERO;
   uint32_t fl = 0;
   if (contig) {
-    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t r = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
     if (r >= R) return;  // whole waves
     const q128 *row = X + r * sr;
@@ -347,7 +347,7 @@ __device__ __forceinline__ void oz_gemm_tile(const int8_t *__restrict__ As, cons
   constexpr int ND = D1 - D0 + 1;
   constexpr int TMAX = (D1 < S - 1) ? D1 : S - 1;  // highest slice any pair of the range uses
   __shared__ __attribute__((aligned(1024))) int8_t lds[4 * SLOT];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
   const int64_t kb0 = kp * OZ_KPASS_BLOCKS;
   const int64_t nkb = ((kb0 + OZ_KPASS_BLOCKS < Kb) ? kb0 + OZ_KPASS_BLOCKS : Kb) - kb0;
   const int64_t nst = NG * nkb;
@@ -1143,7 +1143,7 @@ __global__ void __launch_bounds__(256) k_bo_scale(const qblas_gemm_problem *__re
                                                   const uint32_t *__restrict__ hdr) {
   if (hdr[OZ_HDR_STALE]) return;  // the problems differ from the plan: nothing is computed
   const int lane = threadIdx.x & 63;
-  const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t w = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   if (w >= items) return;  // whole waves
   const int fs = which ? BO_SCB : BO_SCA;
   const int64_t p = bo_find_from(off, count, fs, (int64_t)blockIdx.x * 4, w);
@@ -1664,7 +1664,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
     s_cst[threadIdx.x] = c;
   }
   __syncthreads();  // every wave of the block is still here
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, n = lane & 31, h = lane >> 5;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63, n = lane & 31,
+            h = lane >> 5;
   const int64_t pos = (int64_t)blockIdx.x * 4 + wave;
   if (pos >= Rb * Kb) return;  // whole waves; only wave-local LDS below
   const int64_t rb = pos / Kb, kb = pos - rb * Kb;
