@@ -1137,39 +1137,60 @@ __global__ void k_bo_check(const int64_t *__restrict__ off, int64_t count, bo_to
   }
 }
 
+constexpr int64_t BO_SCALE_BLOCKS = 4096;  // k_bo_scale's grid at most: 16384 waves, each a range of items
 __global__ void __launch_bounds__(256) k_bo_scale(const qblas_gemm_problem *__restrict__ P, int64_t count,
                                                   const int64_t *__restrict__ off, int colmajor, int which,
                                                   int64_t items, int32_t *__restrict__ E, uint32_t *__restrict__ F,
                                                   const uint32_t *__restrict__ hdr) {
   if (hdr[OZ_HDR_STALE]) return;  // the problems differ from the plan: nothing is computed
   const int lane = threadIdx.x & 63;
+  // each wave takes a contiguous range of items: one problem search, then a forward walk
+  const int64_t W = (int64_t)gridDim.x * 4;
   const int64_t w = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  if (w >= items) return;  // whole waves
+  const int64_t per_w = (items + W - 1) / W, i0 = w * per_w, i1 = (i0 + per_w < items) ? i0 + per_w : items;
+  if (i0 >= i1) return;  // whole waves
   const int fs = which ? BO_SCB : BO_SCA;
-  const int64_t p = bo_find_from(off, count, fs, (int64_t)blockIdx.x * 4, w);
-  const bo_geom G = bo_geometry(P[p], colmajor);
-  const q128 *X = (const q128 *)(which ? P[p].B : P[p].A);
-  const int64_t R = which ? G.n : G.m, sr = which ? G.bsc : G.asr, sc = which ? G.bsr : G.asc;
-  const int64_t ro = off[p * BO_F + (which ? BO_COLS : BO_ROWS)];
-  const int64_t u = w - off[p * BO_F + fs];
-  const bool contig = which ? colmajor : !colmajor;
-  int e = OZ_EZERO;
-  uint32_t fl = 0;
-  if (contig) {
-    const q128 *row = X + u * sr;
-    for (int64_t c = lane; c < G.k; c += 64) e = max(e, oz_ilogb(ldq_c(row + c * sc), fl));
-    e = wave_max(e);
-    fl = wave_or(fl);
-    if (lane == 0) {
-      E[ro + u] = e;
-      F[ro + u] = fl;
-    }
-  } else {
-    const int64_t r = u * 64 + lane;
-    if (r < R) {
-      for (int64_t c = 0; c < G.k; ++c) e = max(e, oz_ilogb(ldq_c(X + r * sr + c * sc), fl));
-      E[ro + r] = e;
-      F[ro + r] = fl;
+  int64_t p = bo_find(off, count, fs, i0);
+  for (int64_t it = i0; it < i1; ++it) {
+    while (p + 1 < count && off[(p + 1) * BO_F + fs] <= it) ++p;
+    const bo_geom G = bo_geometry(P[p], colmajor);
+    const q128 *X = (const q128 *)(which ? P[p].B : P[p].A);
+    const int64_t R = which ? G.n : G.m, sr = which ? G.bsc : G.asr, sc = which ? G.bsr : G.asc;
+    const int64_t ro = off[p * BO_F + (which ? BO_COLS : BO_ROWS)];
+    const int64_t u = it - off[p * BO_F + fs];
+    const bool contig = which ? colmajor : !colmajor;
+    int e = OZ_EZERO;
+    uint32_t fl = 0;
+    // four (contiguous: one row per wave) or eight (one row per lane) loads in flight per lane
+    // before their exponents are taken: the rows are short (k <= 256 takes one round per wave)
+    if (contig) {
+      const q128 *row = X + u * sr;
+      for (int64_t c0 = lane; c0 < G.k; c0 += 256) {
+        q128 v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = (c0 + 64 * i < G.k) ? ldq_c(row + (c0 + 64 * i) * sc) : q_zero(0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) e = max(e, oz_ilogb(v[i], fl));
+      }
+      e = wave_max(e);
+      fl = wave_or(fl);
+      if (lane == 0) {
+        E[ro + u] = e;
+        F[ro + u] = fl;
+      }
+    } else {
+      const int64_t r = u * 64 + lane;
+      if (r < R) {
+        for (int64_t c0 = 0; c0 < G.k; c0 += 8) {
+          q128 v[8];
+#pragma unroll
+          for (int i = 0; i < 8; ++i) v[i] = (c0 + i < G.k) ? ldq_c(X + r * sr + (c0 + i) * sc) : q_zero(0);
+#pragma unroll
+          for (int i = 0; i < 8; ++i) e = max(e, oz_ilogb(v[i], fl));
+        }
+        E[ro + r] = e;
+        F[ro + r] = fl;
+      }
     }
   }
 }
@@ -1343,9 +1364,11 @@ static void bo_launch(int colmajor, int64_t count, const qblas_gemm_problem *P, 
     k_bo_check<<<1, 64, 0, st>>>(off, count, pt, hdr);
   }
   if (tot[BO_SCA] > 0)
-    k_bo_scale<<<(unsigned)((tot[BO_SCA] + 3) / 4), 256, 0, st>>>(P, count, off, colmajor, 0, tot[BO_SCA], eA, fA, hdr);
+    k_bo_scale<<<(unsigned)min((tot[BO_SCA] + 3) / 4, (int64_t)BO_SCALE_BLOCKS), 256, 0, st>>>(P, count, off, colmajor, 0,
+                                                                                           tot[BO_SCA], eA, fA, hdr);
   if (tot[BO_SCB] > 0)
-    k_bo_scale<<<(unsigned)((tot[BO_SCB] + 3) / 4), 256, 0, st>>>(P, count, off, colmajor, 1, tot[BO_SCB], eB, fB, hdr);
+    k_bo_scale<<<(unsigned)min((tot[BO_SCB] + 3) / 4, (int64_t)BO_SCALE_BLOCKS), 256, 0, st>>>(P, count, off, colmajor, 1,
+                                                                                           tot[BO_SCB], eB, fB, hdr);
   const int64_t ta = tot[BO_FRAGA] * 64, tb = tot[BO_FRAGB] * 64;
   k_bo_slice<S><<<(unsigned)((ta + 255) / 256), 256, 0, st>>>(P, count, off, colmajor, 0, ta, eA, sA, hdr);
   k_bo_slice<S><<<(unsigned)((tb + 255) / 256), 256, 0, st>>>(P, count, off, colmajor, 1, tb, eB, sB, hdr);
