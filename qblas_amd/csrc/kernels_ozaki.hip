@@ -408,32 +408,47 @@ __device__ __forceinline__ void oz_gemm_tile(const int8_t *__restrict__ As, cons
         __builtin_amdgcn_s_waitcnt(0);
       }
       __builtin_amdgcn_s_barrier();
-      {
-        const int8_t *base = &lds[(int)(h & 3) * SLOT + lane * 16];
-#pragma unroll
-        for (int t = 0; t < G; ++t) {
-          const int sl = g * G + t;
-          if (sl < S && sl <= TMAX) {
-            fa[sl] = *reinterpret_cast<const i32x4v *>(base + (wr * G + t) * OZ_FRAG);
-            fb[sl] = *reinterpret_cast<const i32x4v *>(base + ((2 + wc) * G + t) * OZ_FRAG);
-          }
-        }
-      }
       copy(h + 3);
+      // The stage's fragment pairs (A and B of slice g G + t) are read one group ahead: pair 0 now,
+      // pair t + 1 right after the first MFMA of group t, so that every wait before a group (the
+      // copies in flight make each wait cover all LDS reads issued so far) finds its pair read
+      // during the previous group's MFMAs.  The order is pinned with sched_barrier.
+      const int8_t *base = &lds[(int)(h & 3) * SLOT + lane * 16];
+      auto rd_pair = [&](int t) {
+        const int sl = g * G + t;
+        if (t < G && sl < S && sl <= TMAX) {
+          fa[sl] = *reinterpret_cast<const i32x4v *>(base + (wr * G + t) * OZ_FRAG);
+          fb[sl] = *reinterpret_cast<const i32x4v *>(base + ((2 + wc) * G + t) * OZ_FRAG);
+        }
+      };
+      rd_pair(0);
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int t2 = 0; t2 < G; ++t2) {
         const int mm = g * G + t2;
         if (mm < S) {
+          bool next = true;  // pair t2 + 1 still to be read (folds at compile time)
+          auto mfma = [&](int d, const i32x4v &x, const i32x4v &y) {
+            acc[d - D0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(x, y, acc[d - D0], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+            if (next) {
+              rd_pair(t2 + 1);
+              next = false;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+          };
 #pragma unroll
           for (int u = 0; u <= mm; ++u) {
             const int d = mm + u;
-            if (d >= D0 && d <= D1) acc[d - D0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[mm], fb[u], acc[d - D0], 0, 0, 0);
+            if (d >= D0 && d <= D1) mfma(d, fa[mm], fb[u]);
           }
 #pragma unroll
           for (int t = 0; t < mm; ++t) {
             const int d = t + mm;
-            if (d >= D0 && d <= D1) acc[d - D0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[t], fb[mm], acc[d - D0], 0, 0, 0);
+            if (d >= D0 && d <= D1) mfma(d, fa[t], fb[mm]);
           }
+          if (next) rd_pair(t2 + 1);  // the group had no MFMA in the diagonal range
+          __builtin_amdgcn_sched_barrier(0);
         }
       }
     }
