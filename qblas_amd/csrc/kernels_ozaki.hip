@@ -2078,6 +2078,10 @@ __device__ __forceinline__ q128 round_top(const uint32_t (&mag)[N], int msb, int
 // s_b = sum_q R_q v_qb for b = 8(v/4) + 4 kh + v%4 (+ 32 for the second row block): 4-byte runs
 // m = 2(v/4) + kh (+ 8).  Lanes L and L + 32 swap half their runs so that lane kh = 0 holds all
 // runs of entry e and lane kh = 1 all runs of entry e + 1.
+// the smallest xbits (|X| < 2^xbits) for which the launcher picks XW words for X: XW = 6 below
+// 192 bits, 9 below 288, CRT_MW from there
+constexpr int crt_xbits_min(int XW) { return XW == 9 ? 192 : XW == CRT_MW ? 288 : 0; }
+
 // XW: words of X = sum_l N_il N'_lj as two's complement (|X| < K 2^(2t-4) < 2^(32 XW - 1)).
 // FAST (alpha = +-1, beta = 0): the entry is X 2^(E_i+F_j-2t) rounded once, certified and stored
 // here.  Otherwise X is written to xrec (RW words per entry, row-major) for k_crt_finish_ab.
@@ -2218,15 +2222,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
             D[1][mb][v] = (int)sw[1];
           }
 #define QB_SB(b) D[((b) >> 2) & 1][(b) >> 5][(((b)&31) >> 3) * 4 + ((b)&3)]
-        // k = round(X'/M), X' = sum_b s_b 256^b: the sums of words >= w0, each word's four sums
+        // k = round(X'/M), X' = sum_b s_b 256^b: the sums of words >= WLO, each word's four sums
         // combined exactly in double (below 2^45), scaled by 2^(32 (w - w0)) and the sum by
         // 2^(32 w0)/M, give X'/M within 2^-14
-        // (the words below w0 add less than 2^-14 M; each term is below 2^29 M: V_q < M has no
-        // digit above byte ceil(log2(M) / 8)); X'/M is within 1/4 of k (|X| <= M/4)
+        // (the words below w0 add less than 2^-14 M, so starting at WLO <= w0 only adds
+        // precision; each term is below 2^29 M: V_q < M has no digit above byte
+        // ceil(log2(M) / 8)); X'/M is within 1/4 of k (|X| <= M/4).  WLO is a lower bound of w0
+        // for the XW bucket (a static word range, no per-word select): M >= K 2^(2t-2) gives
+        // bits(M) >= xbits + 1, so w0 = (bits(M) - 27) / 32 >= (crt_xbits_min(XW) + 1 - 27) / 32
+        constexpr int WLO = (crt_xbits_min(XW) + 1 - 27) / 32;
         double top = 0.0;
 #pragma unroll
-        for (int w = CRT_MW - 1; w >= 0; --w) {
-          if (w < w0) continue;  // uniform (no break: the loop must unroll, D is indexed statically)
+        for (int w = CRT_MW - 1; w >= WLO; --w) {
           const double r = __builtin_fma((double)QB_SB(4 * w + 3), 16777216.0,
                                          __builtin_fma((double)QB_SB(4 * w + 2), 65536.0,
                                                        __builtin_fma((double)QB_SB(4 * w + 1), 256.0,
@@ -2519,9 +2526,9 @@ hipError_t launch_gemm_crt(int colmajor, int64_t m, int64_t n, int64_t k, int t,
     k_crt_finish_ab<XW><<<stride_blocks(mn, 256), 256, 0, st>>>(m, n, t, kc, xrec, eA, fA, eB, fB, alpha,    \
                                                                       beta, C, csr, csc, hdr, list);              \
   }
-  if (xbits < 192) {
+  if (xbits < crt_xbits_min(9)) {
     QB_CRT_FIN(6)
-  } else if (xbits < 288) {
+  } else if (xbits < crt_xbits_min(CRT_MW)) {
     QB_CRT_FIN(9)
   } else {
     QB_CRT_FIN(CRT_MW)
