@@ -78,12 +78,17 @@ __device__ __forceinline__ int64_t batch_items(const void *P, int kind, int64_t 
   return colmajor ? cdiv(v.m, 64) : cdiv(v.m, 4);
 }
 
-// start[0] = 0, start[p + 1] = items of problems 0..p; one block, chunks of 1024 problems
+// start[0] = 0, start[p + 1] = items of problems 0..p; one block, chunks of 1024 problems.  A
+// non-null ctr (the chunk queue of the kernel that follows) is set to zero.
 __global__ void __launch_bounds__(1024) k_batch_scan(const void *__restrict__ P, int kind, int64_t count, int tm,
-                                                     int tn, int colmajor, int64_t *__restrict__ start) {
+                                                     int tn, int colmajor, int64_t *__restrict__ start,
+                                                     uint32_t *__restrict__ ctr) {
   __shared__ int64_t s[1024];
   int64_t base = 0;
-  if (threadIdx.x == 0) start[0] = 0;
+  if (threadIdx.x == 0) {
+    start[0] = 0;
+    if (ctr) *ctr = 0u;
+  }
   for (int64_t b = 0; b < count; b += 1024) {
     const int64_t p = b + threadIdx.x;
     s[threadIdx.x] = (p < count) ? batch_items(P, kind, p, tm, tn, colmajor) : 0;
@@ -300,9 +305,9 @@ __device__ __forceinline__ dd bshfl_xor_dd(dd v, int msk) {
   return r;
 }
 
-// Waves take chunks of BGV_CHUNK consecutive work items, chunk c by wave c mod waves, so that the
-// waves running at one time work on neighbouring memory; within a chunk the problems are stepped
-// through linearly (one search per chunk).
+// Waves take chunks of consecutive work items in order (ColMajor: chunk c by wave c mod waves;
+// RowMajor: from a queue), so that the waves running at one time work on neighbouring memory;
+// within a chunk the problems are stepped through linearly (one search per chunk).
 // RowMajor: an item is 4 rows; lanes stride along the columns, 2 column steps of 64 per iteration
 // (8 loads of A in flight per lane, each converted x element serving the 4 rows), dd partials per
 // lane, one joint butterfly for the 4 rows; a wave takes 16 items and then finishes their 64 rows,
@@ -312,7 +317,8 @@ constexpr int BGV_CHUNK = 4;    // ColMajor: work items a wave takes in a row
 constexpr int BGV_CHUNK_R = 16;  // RowMajor: 16 items of 4 rows, one row per lane at the end
 template <int colmajor>
 __global__ void __launch_bounds__(256) k_gemv_batched_dd(const qblas_gemv_problem *__restrict__ P, int64_t count,
-                                                         const int64_t *__restrict__ start) {
+                                                         const int64_t *__restrict__ start,
+                                                         uint32_t *__restrict__ queue) {
   const int lane = threadIdx.x & 63;
   const int64_t total = start[count];
   // the wave index as a scalar: item and problem searches run in scalar registers
@@ -323,8 +329,14 @@ __global__ void __launch_bounds__(256) k_gemv_batched_dd(const qblas_gemv_proble
     // row r's total in lanes 16 r .. 16 r + 15, lane 4 q + r of the wave keeps the total of row r
     // of the chunk's item q (its y loaded when the item starts), and after the 16 items each lane
     // finishes its row: the binary128 epilogue runs once per 64 rows instead of once per 4.
-    for (int64_t c0 = w * BGV_CHUNK_R; c0 < total; c0 += waves * BGV_CHUNK_R) {
+    // chunk w first, then chunks waves, waves + 1, ... in the order the waves ask for them
+    // (queue word zeroed by the scan): the items' costs differ up to 32x with n, and a fixed
+    // round-robin share left the kernel waiting on its slowest waves
+    for (int64_t cw = w; cw * BGV_CHUNK_R < total;) {
+      const int64_t c0 = cw * BGV_CHUNK_R;
       const int64_t c1 = (c0 + BGV_CHUNK_R < total) ? c0 + BGV_CHUNK_R : total;
+      uint32_t ticket = 0;
+      if (lane == 0) ticket = atomicAdd(queue, 1u);
       int64_t p = find_problem(start, count, c0);
       dd mine = dd_zero();
       q128 yv = q_zero(0);
@@ -347,19 +359,29 @@ __global__ void __launch_bounds__(256) k_gemv_batched_dd(const qblas_gemv_proble
         dd acc[4][2];
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc[r][0] = acc[r][1] = dd_zero();
-        for (int64_t j = lane; j < n; j += 128) {
+        // the next column step's A and x are loaded before the current step is summed, so that
+        // each lane keeps two steps of loads in flight (every load index stays below n)
+        q128 a0[4], a1[4], xq0, xq1;
+        auto load_step = [&](int64_t j, q128(&b0)[4], q128(&b1)[4], q128 &xb0, q128 &xb1) {
           const bool h1 = j + 64 < n;
-          q128 a0[4], a1[4];
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             if (r < R) {
               const q128 *row = A + (i0 + r) * lda;
-              a0[r] = bld(row + j);
-              a1[r] = h1 ? bld(row + j + 64) : q_zero(0);
+              b0[r] = bld(row + j);
+              b1[r] = h1 ? bld(row + j + 64) : q_zero(0);
             }
           }
-          const dd x0 = q_to_dd(bld(x + j * incx));
-          const dd x1 = h1 ? q_to_dd(bld(x + (j + 64) * incx)) : dd_zero();
+          xb0 = bld(x + j * incx);
+          xb1 = h1 ? bld(x + (j + 64) * incx) : q_zero(0);
+        };
+        if (lane < n) load_step(lane, a0, a1, xq0, xq1);
+        for (int64_t j = lane; j < n; j += 128) {
+          q128 b0[4], b1[4], xb0, xb1;
+          if (j + 128 < n) load_step(j + 128, b0, b1, xb0, xb1);
+          const bool h1 = j + 64 < n;
+          const dd x0 = q_to_dd(xq0);
+          const dd x1 = h1 ? q_to_dd(xq1) : dd_zero();
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             if (r < R) {
@@ -367,6 +389,13 @@ __global__ void __launch_bounds__(256) k_gemv_batched_dd(const qblas_gemv_proble
               acc[r][1] = dd_fma(q_to_dd(a1[r]), x1, acc[r][1]);
             }
           }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            a0[r] = b0[r];
+            a1[r] = b1[r];
+          }
+          xq0 = xb0;
+          xq1 = xb1;
         }
         dd s[4];
 #pragma unroll
@@ -389,6 +418,7 @@ __global__ void __launch_bounds__(256) k_gemv_batched_dd(const qblas_gemv_proble
         const qblas_gemv_problem &v = P[pl];
         bst((q128 *)v.y + il * v.incy, fma_packed(qv(v.alpha), dd_to_q(mine), mul_packed(qv(v.beta), yv)));
       }
+      cw = waves + (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)ticket);  // lane 0's ticket
     }
     return;
   }
@@ -614,7 +644,7 @@ hipError_t launch_gemm_batched(int dd_mode, int colmajor, int64_t count, const q
   if (cfg < 0 || cfg >= gemm_batched_configs(dd_mode)) return hipErrorInvalidValue;
   int64_t *start = (int64_t *)ws;
   const gemm_cfg c = dd_mode ? DD_CFG[cfg] : REF_CFG[cfg];
-  k_batch_scan<<<1, 1024, 0, st>>>(P, 0, count, c.tm, c.tn, colmajor, start);
+  k_batch_scan<<<1, 1024, 0, st>>>(P, 0, count, c.tm, c.tn, colmajor, start, nullptr);
   if (!dd_mode) {
     switch (cfg) {
       case 0: k_gemm_batched_ref<16><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, kc); break;
@@ -638,14 +668,16 @@ hipError_t launch_gemv_batched(int form, int colmajor, int64_t count, const qbla
                                hipStream_t st) {
   int64_t *start = (int64_t *)ws;
   const int kind = form == 1 ? 2 : (form == 2 ? 3 : 1);
-  k_batch_scan<<<1, 1024, 0, st>>>(P, kind, count, 0, 0, colmajor, start);
+  // the DD RowMajor kernel's chunk queue: a word of the status block, past the status words
+  uint32_t *queue = (uint32_t *)((char *)ws + batched_status_offset(count)) + 32;
+  k_batch_scan<<<1, 1024, 0, st>>>(P, kind, count, 0, 0, colmajor, start, (form == 1 && !colmajor) ? queue : nullptr);
   if (form == 0) {
     k_gemv_batched_ref<<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor);
   } else if (form == 1) {
     if (colmajor)
-      k_gemv_batched_dd<1><<<BATCH_GRID, 256, 0, st>>>(P, count, start);
+      k_gemv_batched_dd<1><<<BATCH_GRID, 256, 0, st>>>(P, count, start, nullptr);
     else
-      k_gemv_batched_dd<0><<<BATCH_GRID, 256, 0, st>>>(P, count, start);
+      k_gemv_batched_dd<0><<<BATCH_GRID, 256, 0, st>>>(P, count, start, queue);
   } else {
     uint32_t *status = (uint32_t *)((char *)ws + batched_status_offset(count));
     k_batch_status_zero<<<1, 64, 0, st>>>(status);
