@@ -2,9 +2,10 @@
 //
 // A device array of descriptors (qblas_gemm_problem / qblas_gemv_problem) is turned into a flat
 // work list by k_batch_scan (work items per problem, exclusive prefix sums, all on the device),
-// then one grouped persistent kernel walks the list: every workgroup (or wave) takes items
-// blockIdx.x, blockIdx.x + gridDim.x, ..., finds its problem by binary search in the prefix sums
-// and computes one tile (gemm) or one group of rows (gemv).  Nothing is synchronized or copied
+// then one grouped persistent kernel walks the list: every workgroup (or wave) takes item
+// blockIdx.x (or its wave index) first and later items from a queue word that the scan zeroes
+// (gemm, DD RowMajor gemv) or in steps of the grid (the other gemv kernels), finds its problem by
+// binary search in the prefix sums and computes one tile (gemm) or one group of rows (gemv).  Nothing is synchronized or copied
 // between the two launches, so the sequence can be captured in a hipGraph.
 //
 // Per problem the semantics are those of the single-problem routines:
@@ -108,40 +109,52 @@ __global__ void __launch_bounds__(1024) k_batch_scan(const void *__restrict__ P,
 // ------------------------------------------------------------------------------------------
 // gemm, REFERENCE order: one lane per C entry, tile TM x (256 / TM), rows fastest
 // ------------------------------------------------------------------------------------------
+// one C entry of a REFERENCE batched gemm problem (level3.hpp:215-336 order)
+__device__ __forceinline__ void ref_gemm_entry(const qblas_gemm_problem &g, int64_t i, int64_t j, int colmajor,
+                                               int64_t kc_default) {
+  const int64_t m = g.m, n = g.n, k = g.k;
+  const int64_t kc = (m <= 64 && n <= 64 && k <= 64) ? k : kc_default;
+  const q128 *A = (const q128 *)g.A;
+  const q128 *B = (const q128 *)g.B;
+  q128 *C = (q128 *)g.C;
+  const int64_t as_i = colmajor ? 1 : g.lda, as_l = colmajor ? g.lda : 1;
+  const int64_t bs_l = colmajor ? 1 : g.ldb, bs_j = colmajor ? g.ldb : 1;
+  const q128 alpha = qv(g.alpha), beta = qv(g.beta);
+  q128 *ci = C + (colmajor ? j * g.ldc + i : i * g.ldc + j);
+  q128 c = bld(ci);
+  const q128 *ap = A + i * as_i;
+  const q128 *bp = B + j * bs_j;
+  for (int64_t kk = 0; kk < k; kk += kc) {
+    const int64_t ke = (kk + kc < k) ? kk + kc : k;
+    chain ch;
+    chain_init(ch);
+    for (int64_t l = kk; l < ke; ++l) chain_fma(ch, bld(ap + l * as_l), bld(bp + l * bs_l));
+    c = fma_packed(alpha, chain_get(ch), mul_packed(kk == 0 ? beta : q_one(), c));
+  }
+  bst(ci, c);
+}
+
 template <int TM>
 __global__ void __launch_bounds__(256) k_gemm_batched_ref(const qblas_gemm_problem *__restrict__ P, int64_t count,
                                                           const int64_t *__restrict__ start, int colmajor,
-                                                          int64_t kc_default) {
+                                                          int64_t kc_default, uint32_t *__restrict__ queue) {
   constexpr int TN = 256 / TM;
+  __shared__ uint32_t s_ticket;
   const int64_t total = start[count];
-  for (int64_t t = blockIdx.x; t < total; t += gridDim.x) {
+  // tile blockIdx.x first, then tiles gridDim.x, gridDim.x + 1, ... from the queue (zeroed by the
+  // scan): a tile's cost grows with its problem's k
+  for (int64_t t = blockIdx.x; t < total;) {
+    if (threadIdx.x == 0) s_ticket = atomicAdd(queue, 1u);
     const int64_t p = find_problem(start, count, t);
     const qblas_gemm_problem &g = P[p];
-    const int64_t m = g.m, n = g.n, k = g.k;
     const int64_t tl = t - start[p];
-    const int64_t tiles_n = cdiv(n, TN);
+    const int64_t tiles_n = cdiv(g.n, TN);
     const int64_t ti = tl / tiles_n, tj = tl - ti * tiles_n;
     const int64_t i = ti * TM + (threadIdx.x % TM), j = tj * TN + (threadIdx.x / TM);
-    if (i >= m || j >= n) continue;
-    const int64_t kc = (m <= 64 && n <= 64 && k <= 64) ? k : kc_default;
-    const q128 *A = (const q128 *)g.A;
-    const q128 *B = (const q128 *)g.B;
-    q128 *C = (q128 *)g.C;
-    const int64_t as_i = colmajor ? 1 : g.lda, as_l = colmajor ? g.lda : 1;
-    const int64_t bs_l = colmajor ? 1 : g.ldb, bs_j = colmajor ? g.ldb : 1;
-    const q128 alpha = qv(g.alpha), beta = qv(g.beta);
-    q128 *ci = C + (colmajor ? j * g.ldc + i : i * g.ldc + j);
-    q128 c = bld(ci);
-    const q128 *ap = A + i * as_i;
-    const q128 *bp = B + j * bs_j;
-    for (int64_t kk = 0; kk < k; kk += kc) {
-      const int64_t ke = (kk + kc < k) ? kk + kc : k;
-      chain ch;
-      chain_init(ch);
-      for (int64_t l = kk; l < ke; ++l) chain_fma(ch, bld(ap + l * as_l), bld(bp + l * bs_l));
-      c = fma_packed(alpha, chain_get(ch), mul_packed(kk == 0 ? beta : q_one(), c));
-    }
-    bst(ci, c);
+    if (i < g.m && j < g.n) ref_gemm_entry(g, i, j, colmajor, kc_default);
+    __syncthreads();  // the ticket is written
+    t = gridDim.x + (int64_t)s_ticket;
+    __syncthreads();  // every thread has read it before the next tile writes it
   }
 }
 
@@ -158,13 +171,18 @@ __device__ __attribute__((noinline)) q128 bdd_out(double hi, double lo, q128 alp
 
 template <int TM, int TN, int RM, int RN>
 __global__ void __launch_bounds__(256) k_gemm_batched_dd(const qblas_gemm_problem *__restrict__ P, int64_t count,
-                                                         const int64_t *__restrict__ start, int colmajor) {
+                                                         const int64_t *__restrict__ start, int colmajor,
+                                                         uint32_t *__restrict__ queue) {
   constexpr int KS = 16, CT = TN / RN;
   static_assert((TM / RM) * (TN / RN) == 256, "256 threads");
   __shared__ double sAh[KS][TM], sAl[KS][TM], sBh[KS][TN], sBl[KS][TN];
+  __shared__ uint32_t s_ticket;
   const int tr = threadIdx.x / CT, tc = threadIdx.x % CT;
   const int64_t total = start[count];
-  for (int64_t t = blockIdx.x; t < total; t += gridDim.x) {
+  // tile blockIdx.x first, then tiles gridDim.x, gridDim.x + 1, ... from the queue (zeroed by the
+  // scan): a tile's cost grows with its problem's k
+  for (int64_t t = blockIdx.x; t < total;) {
+    if (threadIdx.x == 0) s_ticket = atomicAdd(queue, 1u);
     const int64_t p = find_problem(start, count, t);
     const qblas_gemm_problem &g = P[p];
     const int64_t m = g.m, n = g.n, k = g.k;
@@ -229,6 +247,9 @@ __global__ void __launch_bounds__(256) k_gemm_batched_dd(const qblas_gemm_proble
           bst(ci, bdd_out(acc[r][c].hi, acc[r][c].lo, alpha, beta, bld(ci)));
         }
       }
+    __syncthreads();  // the ticket is written
+    t = gridDim.x + (int64_t)s_ticket;
+    __syncthreads();  // every thread has read it before the next tile writes it
   }
 }
 
@@ -644,21 +665,23 @@ hipError_t launch_gemm_batched(int dd_mode, int colmajor, int64_t count, const q
   if (cfg < 0 || cfg >= gemm_batched_configs(dd_mode)) return hipErrorInvalidValue;
   int64_t *start = (int64_t *)ws;
   const gemm_cfg c = dd_mode ? DD_CFG[cfg] : REF_CFG[cfg];
-  k_batch_scan<<<1, 1024, 0, st>>>(P, 0, count, c.tm, c.tn, colmajor, start, nullptr);
+  // the kernels' tile queue: a word of the status block, past the status words
+  uint32_t *queue = (uint32_t *)((char *)ws + batched_status_offset(count)) + 32;
+  k_batch_scan<<<1, 1024, 0, st>>>(P, 0, count, c.tm, c.tn, colmajor, start, queue);
   if (!dd_mode) {
     switch (cfg) {
-      case 0: k_gemm_batched_ref<16><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, kc); break;
-      case 1: k_gemm_batched_ref<8><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, kc); break;
-      case 2: k_gemm_batched_ref<32><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, kc); break;
-      case 3: k_gemm_batched_ref<4><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, kc); break;
-      default: k_gemm_batched_ref<64><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, kc); break;
+      case 0: k_gemm_batched_ref<16><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, kc, queue); break;
+      case 1: k_gemm_batched_ref<8><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, kc, queue); break;
+      case 2: k_gemm_batched_ref<32><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, kc, queue); break;
+      case 3: k_gemm_batched_ref<4><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, kc, queue); break;
+      default: k_gemm_batched_ref<64><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, kc, queue); break;
     }
   } else {
     switch (cfg) {
-      case 0: k_gemm_batched_dd<32, 32, 2, 2><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor); break;
-      case 1: k_gemm_batched_dd<64, 64, 4, 4><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor); break;
-      case 2: k_gemm_batched_dd<16, 16, 1, 1><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor); break;
-      default: k_gemm_batched_dd<64, 32, 4, 2><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor); break;
+      case 0: k_gemm_batched_dd<32, 32, 2, 2><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, queue); break;
+      case 1: k_gemm_batched_dd<64, 64, 4, 4><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, queue); break;
+      case 2: k_gemm_batched_dd<16, 16, 1, 1><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, queue); break;
+      default: k_gemm_batched_dd<64, 32, 4, 2><<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, queue); break;
     }
   }
   return hipGetLastError();
