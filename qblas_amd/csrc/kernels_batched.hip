@@ -257,12 +257,16 @@ __global__ void __launch_bounds__(256) k_gemm_batched_dd(const qblas_gemm_proble
 // gemv, REFERENCE order: units of 64 lanes (one wave) of one problem
 // ------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_gemv_batched_ref(const qblas_gemv_problem *__restrict__ P, int64_t count,
-                                                          const int64_t *__restrict__ start, int colmajor) {
+                                                          const int64_t *__restrict__ start, int colmajor,
+                                                          uint32_t *__restrict__ queue) {
   const int lane = threadIdx.x & 63;
   const int64_t total = start[count];
   const int64_t waves = (int64_t)gridDim.x * 4;
-  for (int64_t u = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); u < total;
-       u += waves) {
+  // item = the wave index first, then items waves, waves + 1, ... from the queue (zeroed by the
+  // scan): an item's cost grows with its problem's n
+  for (int64_t u = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); u < total;) {
+    uint32_t ticket = 0;
+    if (lane == 0) ticket = atomicAdd(queue, 1u);
     const int64_t p = find_problem(start, count, u);
     const qblas_gemv_problem &v = P[p];
     const int64_t m = v.m, n = v.n, lda = v.lda, incx = v.incx, incy = v.incy;
@@ -311,6 +315,7 @@ __global__ void __launch_bounds__(256) k_gemv_batched_ref(const qblas_gemv_probl
         bst(yi, chain_get(ch));
       }
     }
+    u = waves + (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)ticket);  // lane 0's ticket
   }
 }
 
@@ -691,11 +696,12 @@ hipError_t launch_gemv_batched(int form, int colmajor, int64_t count, const qbla
                                hipStream_t st) {
   int64_t *start = (int64_t *)ws;
   const int kind = form == 1 ? 2 : (form == 2 ? 3 : 1);
-  // the DD RowMajor kernel's chunk queue: a word of the status block, past the status words
+  // the REFERENCE and DD RowMajor kernels' queue: a word of the status block, past the status words
   uint32_t *queue = (uint32_t *)((char *)ws + batched_status_offset(count)) + 32;
-  k_batch_scan<<<1, 1024, 0, st>>>(P, kind, count, 0, 0, colmajor, start, (form == 1 && !colmajor) ? queue : nullptr);
+  k_batch_scan<<<1, 1024, 0, st>>>(P, kind, count, 0, 0, colmajor, start,
+                                   (form == 0 || (form == 1 && !colmajor)) ? queue : nullptr);
   if (form == 0) {
-    k_gemv_batched_ref<<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor);
+    k_gemv_batched_ref<<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, queue);
   } else if (form == 1) {
     if (colmajor)
       k_gemv_batched_dd<1><<<BATCH_GRID, 256, 0, st>>>(P, count, start, nullptr);
