@@ -2,11 +2,11 @@
 //
 // A device array of descriptors (qblas_gemm_problem / qblas_gemv_problem) is turned into a flat
 // work list by k_batch_scan (work items per problem, exclusive prefix sums, all on the device),
-// then one grouped persistent kernel walks the list: every workgroup (or wave) takes item
-// blockIdx.x (or its wave index) first and later items from a queue word that the scan zeroes
-// (gemm, DD RowMajor gemv) or in steps of the grid (the other gemv kernels), finds its problem by
-// binary search in the prefix sums and computes one tile (gemm) or one group of rows (gemv).  Nothing is synchronized or copied
-// between the two launches, so the sequence can be captured in a hipGraph.
+// then one grouped persistent kernel walks the list: every workgroup (or wave) takes item (or
+// chunk) blockIdx.x (or its wave index) first and later ones from a queue word that the scan
+// zeroes (all but the DD ColMajor gemv, which steps through the grid), finds its problem by binary
+// search in the prefix sums and computes one tile (gemm) or one group of rows (gemv).  Nothing is
+// synchronized or copied between the two launches, so the sequence can be captured in a hipGraph.
 //
 // Per problem the semantics are those of the single-problem routines:
 //   REFERENCE gemm  level3.hpp:215-336: per entry, k in blocks of kc (kc = k when m, n, k <= 64,
@@ -493,60 +493,71 @@ __device__ __forceinline__ q128 exact_row_out(const uint32_t (&Wl)[WW], const ui
 // status[1] != 0 after this kernel: some row was left for the full-range path (its y unwritten)
 __global__ void __launch_bounds__(256) k_gemv_batched_exact(const qblas_gemv_problem *__restrict__ P, int64_t count,
                                                             const int64_t *__restrict__ start, int colmajor,
-                                                            uint32_t *__restrict__ status) {
+                                                            uint32_t *__restrict__ status,
+                                                            uint32_t *__restrict__ queue) {
   const int lane = threadIdx.x & 63;
   const int64_t total = start[count];
   const int64_t waves = (int64_t)gridDim.x * 4;
-  for (int64_t u = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); u < total;
-       u += waves) {
-    const int64_t p = find_problem(start, count, u);
-    const qblas_gemv_problem &v = P[p];
-    const int64_t m = v.m, n = v.n, lda = v.lda, incx = v.incx, incy = v.incy;
-    const int64_t ul = u - start[p];
-    const q128 *A = (const q128 *)v.A;
-    const q128 *x = (const q128 *)v.x;
-    q128 *y = (q128 *)v.y;
-    const q128 alpha = qv(v.alpha), beta = qv(v.beta);
-    uint32_t Wl[WW], Wh[WW];
-#pragma unroll
-    for (int k = 0; k < WW; ++k) {
-      Wl[k] = 0;
-      Wh[k] = 0;
-    }
-    uint32_t flags = 0;
-    if (!colmajor) {
-      const int64_t i = ul;
-      const q128 *row = A + i * lda;
-      int e0 = EXP_NONE;
-      if (lane < n) e0 = pair_exp(ldq_c(row + lane), ldq_c(x + (int64_t)lane * incx));
-      const int An = wave_anchor(e0);
-      for (int64_t j = lane; j < n; j += 64) accumulate(Wl, Wh, An, ldq_c(row + j), ldq_c(x + j * incx), flags);
-      wave_sum_win(Wl);
-      wave_sum_win(Wh);
-      flags = wave_or(flags);
-      if (lane == 0) {
-        q128 *yi = y + i * incy;
-        if (flags & FL_LOST) {
-          atomicOr(status + 1, 1u);
-        } else {
-          stq_c(yi, exact_row_out(Wl, Wh, An, flags, alpha, beta, ldq_c(yi)));
-        }
+  // chunks of XB_CHUNK items: chunk = the wave index first, then chunks waves, waves + 1, ... from
+  // the queue (zeroed by the scan); an item's cost grows with its problem's n.  One ticket per
+  // item (a row, RowMajor) made the queue word's atomics the bound.
+  constexpr int64_t XB_CHUNK = 16;
+  for (int64_t cw = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+       cw * XB_CHUNK < total;) {
+    uint32_t ticket = 0;
+    if (lane == 0) ticket = atomicAdd(queue, 1u);
+    const int64_t cend = ((cw + 1) * XB_CHUNK < total) ? (cw + 1) * XB_CHUNK : total;
+    for (int64_t u = cw * XB_CHUNK; u < cend; ++u) {
+      const int64_t p = find_problem(start, count, u);
+      const qblas_gemv_problem &v = P[p];
+      const int64_t m = v.m, n = v.n, lda = v.lda, incx = v.incx, incy = v.incy;
+      const int64_t ul = u - start[p];
+      const q128 *A = (const q128 *)v.A;
+      const q128 *x = (const q128 *)v.x;
+      q128 *y = (q128 *)v.y;
+      const q128 alpha = qv(v.alpha), beta = qv(v.beta);
+      uint32_t Wl[WW], Wh[WW];
+  #pragma unroll
+      for (int k = 0; k < WW; ++k) {
+        Wl[k] = 0;
+        Wh[k] = 0;
       }
-    } else {
-      const int64_t i = ul * 64 + lane;
-      if (i < m) {
+      uint32_t flags = 0;
+      if (!colmajor) {
+        const int64_t i = ul;
+        const q128 *row = A + i * lda;
         int e0 = EXP_NONE;
-        for (int64_t j = 0; j < n && j < 4 && e0 == EXP_NONE; ++j) e0 = pair_exp(ldq_c(A + j * lda + i), ldq_c(x + j * incx));
-        const int An = (e0 == EXP_NONE) ? 0 : e0 - WIN2_CENTER;
-        for (int64_t j = 0; j < n; ++j) accumulate(Wl, Wh, An, ldq_c(A + j * lda + i), ldq_c(x + j * incx), flags);
-        q128 *yi = y + i * incy;
-        if (flags & FL_LOST) {
-          atomicOr(status + 1, 1u);
-        } else {
-          stq_c(yi, exact_row_out(Wl, Wh, An, flags, alpha, beta, ldq_c(yi)));
+        if (lane < n) e0 = pair_exp(ldq_c(row + lane), ldq_c(x + (int64_t)lane * incx));
+        const int An = wave_anchor(e0);
+        for (int64_t j = lane; j < n; j += 64) accumulate(Wl, Wh, An, ldq_c(row + j), ldq_c(x + j * incx), flags);
+        wave_sum_win(Wl);
+        wave_sum_win(Wh);
+        flags = wave_or(flags);
+        if (lane == 0) {
+          q128 *yi = y + i * incy;
+          if (flags & FL_LOST) {
+            atomicOr(status + 1, 1u);
+          } else {
+            stq_c(yi, exact_row_out(Wl, Wh, An, flags, alpha, beta, ldq_c(yi)));
+          }
+        }
+      } else {
+        const int64_t i = ul * 64 + lane;
+        if (i < m) {
+          int e0 = EXP_NONE;
+          for (int64_t j = 0; j < n && j < 4 && e0 == EXP_NONE; ++j) e0 = pair_exp(ldq_c(A + j * lda + i), ldq_c(x + j * incx));
+          const int An = (e0 == EXP_NONE) ? 0 : e0 - WIN2_CENTER;
+          for (int64_t j = 0; j < n; ++j) accumulate(Wl, Wh, An, ldq_c(A + j * lda + i), ldq_c(x + j * incx), flags);
+          q128 *yi = y + i * incy;
+          if (flags & FL_LOST) {
+            atomicOr(status + 1, 1u);
+          } else {
+            stq_c(yi, exact_row_out(Wl, Wh, An, flags, alpha, beta, ldq_c(yi)));
+          }
         }
       }
     }
+    cw = waves + (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)ticket);  // lane 0's ticket
   }
 }
 
@@ -696,10 +707,10 @@ hipError_t launch_gemv_batched(int form, int colmajor, int64_t count, const qbla
                                hipStream_t st) {
   int64_t *start = (int64_t *)ws;
   const int kind = form == 1 ? 2 : (form == 2 ? 3 : 1);
-  // the REFERENCE and DD RowMajor kernels' queue: a word of the status block, past the status words
+  // the REFERENCE, DD RowMajor and EXACT kernels' queue: a word of the status block, past the status words
   uint32_t *queue = (uint32_t *)((char *)ws + batched_status_offset(count)) + 32;
   k_batch_scan<<<1, 1024, 0, st>>>(P, kind, count, 0, 0, colmajor, start,
-                                   (form == 0 || (form == 1 && !colmajor)) ? queue : nullptr);
+                                   (form != 1 || !colmajor) ? queue : nullptr);
   if (form == 0) {
     k_gemv_batched_ref<<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, queue);
   } else if (form == 1) {
@@ -710,7 +721,7 @@ hipError_t launch_gemv_batched(int form, int colmajor, int64_t count, const qbla
   } else {
     uint32_t *status = (uint32_t *)((char *)ws + batched_status_offset(count));
     k_batch_status_zero<<<1, 64, 0, st>>>(status);
-    k_gemv_batched_exact<<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, status);
+    k_gemv_batched_exact<<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, status, queue);
     k_gemv_batched_exact_wide<<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, status);
   }
   return hipGetLastError();
