@@ -4,9 +4,9 @@
 // work list by k_batch_scan (work items per problem, exclusive prefix sums, all on the device),
 // then one grouped persistent kernel walks the list: every workgroup (or wave) takes item (or
 // chunk) blockIdx.x (or its wave index) first and later ones from a queue word that the scan
-// zeroes (all but the DD ColMajor gemv, which steps through the grid), finds its problem by binary
-// search in the prefix sums and computes one tile (gemm) or one group of rows (gemv).  Nothing is
-// synchronized or copied between the two launches, so the sequence can be captured in a hipGraph.
+// zeroes, finds its problem by binary search in the prefix sums and computes one tile (gemm) or
+// one group of rows (gemv).  Nothing is synchronized or copied between the two launches, so the
+// sequence can be captured in a hipGraph.
 //
 // Per problem the semantics are those of the single-problem routines:
 //   REFERENCE gemm  level3.hpp:215-336: per entry, k in blocks of kc (kc = k when m, n, k <= 64,
@@ -331,8 +331,8 @@ __device__ __forceinline__ dd bshfl_xor_dd(dd v, int msk) {
   return r;
 }
 
-// Waves take chunks of consecutive work items in order (ColMajor: chunk c by wave c mod waves;
-// RowMajor: from a queue), so that the waves running at one time work on neighbouring memory;
+// Waves take chunks of consecutive work items in order (the first by wave index, later ones from
+// the queue), so that the waves running at one time work on neighbouring memory;
 // within a chunk the problems are stepped through linearly (one search per chunk).
 // RowMajor: an item is 4 rows; lanes stride along the columns, 2 column steps of 64 per iteration
 // (8 loads of A in flight per lane, each converted x element serving the 4 rows), dd partials per
@@ -448,8 +448,11 @@ __global__ void __launch_bounds__(256) k_gemv_batched_dd(const qblas_gemv_proble
     }
     return;
   }
-  for (int64_t c0 = w * BGV_CHUNK; c0 < total; c0 += waves * BGV_CHUNK) {
+  for (int64_t cw = w; cw * BGV_CHUNK < total;) {  // chunk w, then chunks from the queue
+    const int64_t c0 = cw * BGV_CHUNK;
     const int64_t c1 = (c0 + BGV_CHUNK < total) ? c0 + BGV_CHUNK : total;
+    uint32_t ticket = 0;
+    if (lane == 0) ticket = atomicAdd(queue, 1u);
     int64_t p = find_problem(start, count, c0);
     for (int64_t u = c0; u < c1; ++u) {
       while (start[p + 1] <= u) ++p;  // the problem holding item u (empty problems are skipped)
@@ -475,6 +478,7 @@ __global__ void __launch_bounds__(256) k_gemv_batched_dd(const qblas_gemv_proble
         }
       }
     }
+    cw = waves + (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)ticket);  // lane 0's ticket
   }
 }
 
@@ -707,15 +711,14 @@ hipError_t launch_gemv_batched(int form, int colmajor, int64_t count, const qbla
                                hipStream_t st) {
   int64_t *start = (int64_t *)ws;
   const int kind = form == 1 ? 2 : (form == 2 ? 3 : 1);
-  // the REFERENCE, DD RowMajor and EXACT kernels' queue: a word of the status block, past the status words
+  // the kernels' item queue: a word of the status block, past the status words
   uint32_t *queue = (uint32_t *)((char *)ws + batched_status_offset(count)) + 32;
-  k_batch_scan<<<1, 1024, 0, st>>>(P, kind, count, 0, 0, colmajor, start,
-                                   (form != 1 || !colmajor) ? queue : nullptr);
+  k_batch_scan<<<1, 1024, 0, st>>>(P, kind, count, 0, 0, colmajor, start, queue);
   if (form == 0) {
     k_gemv_batched_ref<<<BATCH_GRID, 256, 0, st>>>(P, count, start, colmajor, queue);
   } else if (form == 1) {
     if (colmajor)
-      k_gemv_batched_dd<1><<<BATCH_GRID, 256, 0, st>>>(P, count, start, nullptr);
+      k_gemv_batched_dd<1><<<BATCH_GRID, 256, 0, st>>>(P, count, start, queue);
     else
       k_gemv_batched_dd<0><<<BATCH_GRID, 256, 0, st>>>(P, count, start, queue);
   } else {
